@@ -1,0 +1,158 @@
+/*
+ * fem_gpu.h -- C ABI of the MI355X (gfx950) Fem3D implicit-step library (libfem_gpu.so).
+ *
+ * The reference exposes this path as C++ virtual interfaces, not as a C ABI. Each entry
+ * point below names the reference interface it replaces (paths relative to the reference
+ * root). Conventions:
+ *   - plain pointers and sizes only; int status codes (FEM_OK = 0), message via fem_last_error;
+ *   - one context = one representation = one device + one HIP stream; not thread-safe per
+ *     context (the reference updates one representation per thread-pool task,
+ *     Physics/FreeMotion.cpp:52-63);
+ *   - DOF layout is node-major interleaved, dof = 3 * node + axis (Math/OdeState.h:38-207);
+ *   - host-pointer entry points copy synchronously; *_device entry points take device pointers
+ *     and are ordered on the context stream.
+ */
+#ifndef FEM_GPU_H
+#define FEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_ctx fem_ctx;
+
+/* Status codes. Negative values are errors (the reference throws
+ * SurgSim::Framework::AssertionFailure, Framework/Assert.h:77-95). */
+#define FEM_OK 0
+#define FEM_INACTIVE 1          /* state became non-finite; representation deactivated
+                                   (Physics/FemRepresentation.cpp:285-293) */
+#define FEM_ERR_INVALID (-1)    /* invalid argument or precondition */
+#define FEM_ERR_HIP (-2)        /* HIP runtime error */
+#define FEM_ERR_ORDER (-3)      /* call out of order (e.g. step before upload) */
+#define FEM_ERR_NOT_CONVERGED (-4) /* solver did not reach its tolerance (reported, not thrown,
+                                      by Eigen's CG; here only when options ask for it) */
+
+/* Element kinds: the factory names of Physics/Fem3DElementTetrahedron.cpp:47,
+ * Physics/Fem3DElementCorotationalTetrahedron.cpp:29-30, Physics/Fem3DElementCube.cpp:32. */
+enum {
+    FEM_ELEMENT_TET_LINEAR = 0,       /* "SurgSim::Physics::Fem3DElementTetrahedron" */
+    FEM_ELEMENT_TET_COROTATIONAL = 1, /* "SurgSim::Physics::Fem3DElementCorotationalTetrahedron" */
+    FEM_ELEMENT_HEX = 2               /* "SurgSim::Physics::Fem3DElementCube" */
+};
+
+/* Integration schemes (Math/OdeSolver.h:36-49). */
+enum {
+    FEM_SCHEME_EULER_IMPLICIT = 0,        /* INTEGRATIONSCHEME_EULER_IMPLICIT */
+    FEM_SCHEME_LINEAR_EULER_IMPLICIT = 1  /* INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT */
+};
+
+/* Arithmetic of the element kernels, assembly and CG. */
+enum {
+    FEM_PRECISION_FP64 = 0,         /* everything in FP64 */
+    FEM_PRECISION_FP32_REFINED = 1  /* FP32 elements/SpMV/CG, FP64 iterative refinement of the
+                                       implicit-step residual; FP64 polar decomposition for
+                                       near-degenerate or inverted elements */
+};
+
+typedef struct fem_options {
+    double dt;                   /* time step (s) */
+    double gravity[3];           /* default (0, -9.81, 0), Physics/Representation.cpp:36 */
+    int32_t gravity_enabled;     /* default 1, Physics/Representation.cpp:38 */
+    int32_t scheme;              /* FEM_SCHEME_* */
+    int32_t precision;           /* FEM_PRECISION_* */
+    int32_t newton_max_iterations; /* default 1, Math/OdeSolverEulerImplicit.cpp:26 */
+    double cg_tolerance;         /* relative residual tolerance (Eigen default: machine epsilon) */
+    int64_t cg_max_iterations;   /* < 0: 2 * ndof (Eigen default) */
+    double inner_tolerance;      /* FP32 mode: relative tolerance of each inner FP32 CG solve */
+    int32_t max_refinements;     /* FP32 mode: maximum FP64 refinement sweeps per step */
+    int32_t fail_on_nonconvergence; /* 0: silent like Eigen's CG (default) */
+} fem_options;
+
+typedef struct fem_stats {
+    int64_t steps;                 /* implicit steps taken */
+    int64_t last_cg_iterations;    /* CG iterations of the last linear solve (sum of inner solves) */
+    double last_cg_error;          /* sqrt(|r|^2 / |b|^2) at exit of the last solve */
+    int64_t total_cg_iterations;
+    int64_t last_refinements;      /* FP32 mode: refinement sweeps of the last step */
+    int64_t det_r_invalid;         /* Physics/Fem3DElementCorotationalTetrahedron.cpp:161-166 */
+    int64_t g_singular;            /* Physics/Fem3DElementCorotationalTetrahedron.cpp:211-216 */
+    int64_t fp64_polar_fallbacks;  /* FP32 mode: elements whose rotation was redone in FP64 */
+    int32_t active;                /* 0 once the state became non-finite */
+    int32_t converged;             /* last solve reached its tolerance */
+} fem_stats;
+
+/* Context lifetime. Replaces constructing a Fem3DRepresentation (Physics/Fem3DRepresentation.h:54-121). */
+int fem_create(int device, fem_ctx** out);
+void fem_destroy(fem_ctx* ctx);
+const char* fem_last_error(const fem_ctx* ctx);
+/* Run the context's work on an existing hipStream_t (NULL: the context's own stream). */
+int fem_set_stream(fem_ctx* ctx, void* hip_stream);
+
+/* Mesh + materials. Replaces Fem3DRepresentation::setFem + FemRepresentation::setFemElementType +
+ * FemRepresentation::doInitialize (Physics/Fem3DRepresentation.cpp:77-126, 274-284;
+ * Physics/FemRepresentation.cpp:85-168): element initialisation (with the reference's material
+ * and degeneracy asserts, Physics/FemElement.cpp:33-43), lumped node masses and the sparsity
+ * pattern. connectivity: num_elements x (4 | 8) node ids. young/poisson/density: per element. */
+int fem_upload_mesh(fem_ctx* ctx, int element_kind, int64_t num_nodes, const double* x0,
+                    int64_t num_elements, const int32_t* connectivity, const double* young,
+                    const double* poisson, const double* density);
+
+/* Fixed DOFs. Replaces OdeState::addBoundaryCondition(nodeId[, nodeDofId]) (Math/OdeState.cpp:115-142);
+ * duplicates are ignored as in the reference. */
+int fem_add_boundary_condition_node(fem_ctx* ctx, int64_t node);
+int fem_add_boundary_condition_dof(fem_ctx* ctx, int64_t dof);
+
+/* Options. fem_default_options fills the reference defaults. Replaces the IntegrationScheme /
+ * LinearSolver selection of DeformableRepresentation::doInitialize (Physics/DeformableRepresentation.cpp:333-385)
+ * and LinearSparseSolveAndInverseCG::setTolerance / setMaxIterations (Math/LinearSparseSolveAndInverse.cpp:99-117). */
+void fem_default_options(fem_options* opt);
+int fem_set_options(fem_ctx* ctx, const fem_options* opt);
+
+/* External generalized force (ndof values), kept until replaced; NULL clears it.
+ * Replaces DeformableRepresentation::addExternalGeneralizedForce (Physics/Fem3DRepresentation.cpp:133-198)
+ * for constant nodal loads. */
+int fem_set_external_force(fem_ctx* ctx, const double* force);
+
+/* State transfer (x, v of ndof values each; NULL leaves one unchanged). Replaces
+ * DeformableRepresentation::setInitialState / getCurrentState (Physics/DeformableRepresentation.cpp:86-119). */
+int fem_set_state(fem_ctx* ctx, const double* x, const double* v);
+int fem_get_state(fem_ctx* ctx, double* x, double* v);
+int fem_set_state_device(fem_ctx* ctx, const double* x, const double* v);
+int fem_get_state_device(fem_ctx* ctx, double* x, double* v);
+
+/* Advance nsteps implicit steps. Replaces FemRepresentation::update(dt) ->
+ * OdeSolver::solve(dt, current, &new, computeCompliance = false) (Physics/FemRepresentation.cpp:232-294,
+ * Math/OdeSolverEulerImplicit.cpp:55-161, Math/OdeSolverLinearEulerImplicit.cpp:43-64).
+ * Returns FEM_INACTIVE once the state becomes non-finite (further calls do nothing). */
+int fem_step(fem_ctx* ctx, int32_t nsteps);
+int fem_synchronize(fem_ctx* ctx);
+int fem_query(fem_ctx* ctx, fem_stats* stats);
+
+/* Sparsity pattern as scalar CSR (int32). The pattern is structurally symmetric, so these arrays
+ * equal the reference's column-major (outerIndex, innerIndex) of Eigen::SparseMatrix<double>
+ * built at Physics/FemRepresentation.cpp:121-132. */
+int64_t fem_num_nonzeros(fem_ctx* ctx);
+int fem_get_pattern(fem_ctx* ctx, int32_t* rowptr, int32_t* colidx);
+
+/* Last assembled system matrix as the CG operator sees it: the lower-triangle self-adjoint view of
+ * A = M/dt + D + dt K with boundary conditions (Math/OdeSolverEulerImplicit.cpp:148-152,
+ * Math/LinearSparseSolveAndInverse.h:122), values in the pattern's CSR order. Last right-hand side
+ * f - dt K v with boundary conditions (Math/OdeSolverEulerImplicit.cpp:155-160). */
+int fem_get_system_matrix(fem_ctx* ctx, double* values);
+int fem_get_rhs(fem_ctx* ctx, double* rhs);
+
+/* Element evaluation at positions x (ndof, host) for elements [first, first + count): rotation
+ * (9, row-major), element stiffness K (row-major, (3n)^2) and force f (3n) as
+ * Fem3DElement*::doUpdateFMDK(state, FMDK) leaves m_R, m_K, m_f
+ * (Physics/Fem3DElementCorotationalTetrahedron.cpp:77-288, Physics/Fem3DElementTetrahedron.cpp:77-90,
+ * Physics/Fem3DElementCube.cpp:129-141). Any output may be NULL. */
+int fem_evaluate_elements(fem_ctx* ctx, const double* x, int64_t first, int64_t count, double* rotation,
+                          double* stiffness, double* force);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM_GPU_H */

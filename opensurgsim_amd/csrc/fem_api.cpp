@@ -1,0 +1,886 @@
+// C ABI of libfem_gpu.so: context, mesh set-up (pattern, assembly maps) and the implicit step.
+//
+// Host set-up restates (paths relative to the reference root):
+//   FemElement::initialize material asserts         Physics/FemElement.cpp:33-43
+//   corotational V invertibility assert             Physics/Fem3DElementCorotationalTetrahedron.cpp:57-72
+//   lumped node masses                              Physics/FemRepresentation.cpp:102-117
+//   sparsity pattern (full 3x3 node-pair blocks)    Physics/FemRepresentation.cpp:121-132
+//   boundary conditions                             Math/OdeState.cpp:115-218
+//   implicit Euler orchestration                    Math/OdeSolverEulerImplicit.cpp:55-161,
+//                                                   Math/OdeSolverLinearEulerImplicit.cpp:43-64
+#include "../../include/fem_gpu.h"
+#include "fem_kernels.hpp"
+#include "fem_hex.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace femgpu;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArg : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count) HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct fem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    fem_options opt;
+    bool mesh_ready = false;
+    int kind = -1;
+    int64_t num_nodes = 0, num_elements = 0, ndof = 0;
+    int nn = 4;
+
+    // host copies needed after set-up
+    std::vector<int32_t> h_rowptr_b, h_colidx_b;   // block CSR
+    std::vector<double> h_mass_per_node;
+    std::vector<uint8_t> h_bcmask;
+    std::vector<int64_t> h_bc_dofs;
+    bool bc_dirty = true;
+    bool fg_dirty = true;
+
+    // device data
+    DevBuf<double> x0, x, v, xs, vs, fg, fext, b, dv, dvprev, r64;
+    DevBuf<int4> conn4;
+    DevBuf<double4> mat;
+    DevBuf<unsigned char> rec;  // element records (TK)
+    DevBuf<double> rhs_rec;
+    DevBuf<int> lower_blk, mirror_blk, blk_row, colidx_b, rowptr_b, ctr_ptr, ctr_list, node_ptr, node_list, diagblk;
+    DevBuf<unsigned char> S;    // operator values (TK)
+    DevBuf<uint8_t> bcmask;
+    DevBuf<unsigned char> cg_x, cg_r, cg_p, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
+    DevBuf<double> partial;
+    DevBuf<CgScalars> scal;    // [0] inner/FP64 solve, [1] outer refinement
+    DevBuf<int> counters;      // [0..2] element flags, [3] invalid state, [4] Newton max-norm bits (int pair)
+    DevBuf<unsigned long long> nrm;
+    int num_lower = 0;
+    int64_t num_blocks = 0;
+    bool have_fext = false;
+
+    // hexahedra (element-by-element path)
+    HexData hex;
+
+    // host-pinned readback
+    CgScalars* h_sc = nullptr;
+    int* h_counters = nullptr;
+
+    // state
+    bool linear_matrix_ready = false;
+    double matrix_dt = 0.0;
+    int last_precision = -1;
+    int predicted_iters = 16;
+    fem_stats stats;
+};
+
+namespace {
+
+int fail(fem_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(fem_ctx* c, F&& f) {
+    if (!c) return FEM_ERR_INVALID;
+    try {
+        return f();
+    } catch (const InvalidArg& e) {
+        return fail(c, FEM_ERR_INVALID, e.what());
+    } catch (const HipError& e) {
+        return fail(c, FEM_ERR_HIP, e.what());
+    } catch (const std::exception& e) {
+        return fail(c, FEM_ERR_INVALID, e.what());
+    }
+}
+
+// Signed tet volume as Physics/Fem3DElementTetrahedron.cpp:218-235.
+double det3v(const double* a, const double* b, const double* c) {
+    return a[0] * b[1] * c[2] + a[2] * b[0] * c[1] + a[1] * b[2] * c[0] - a[2] * b[1] * c[0] - a[1] * b[0] * c[2] -
+           a[0] * b[2] * c[1];
+}
+double tet_volume(const double* p0, const double* p1, const double* p2, const double* p3) {
+    return (det3v(p1, p2, p3) - det3v(p0, p2, p3) + det3v(p0, p1, p3) - det3v(p0, p1, p2)) / 6.0;
+}
+
+void check_material(double E, double nu, double rho, int64_t e) {
+    auto bad = [&](const char* m) { throw InvalidArg(std::string(m) + " (element " + std::to_string(e) + ")"); };
+    if (rho == 0.0) bad("Mass density is not set");
+    if (nu == 0.0) bad("Poisson ratio is not set");
+    if (E == 0.0) bad("Young modulus is not set");
+    if (!(rho > 0.0)) bad("Mass density is invalid, it should be positive");
+    if (!(nu > 0.0 && nu < 0.5)) bad("Poisson ratio is invalid, it should be within [0 0.5)");
+    if (!(E > 0.0)) bad("Young modulus is invalid, it should be positive");
+}
+
+void ensure_bc(fem_ctx* c) {
+    if (c->bc_dirty) {
+        c->bcmask.upload(c->h_bcmask.data(), c->h_bcmask.size(), c->stream);
+        c->bc_dirty = false;
+    }
+    if (c->fg_dirty) {
+        // gravity force m_i g (Physics/FemRepresentation.cpp:641-662)
+        std::vector<double> fg(c->ndof, 0.0);
+        if (c->opt.gravity_enabled)
+            for (int64_t n = 0; n < c->num_nodes; ++n)
+                for (int k = 0; k < 3; ++k) fg[3 * n + k] = c->opt.gravity[k] * (1.0 * c->h_mass_per_node[n]);
+        c->fg.upload(fg.data(), fg.size(), c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->fg_dirty = false;
+    }
+}
+
+void alloc_precision_buffers(fem_ctx* c) {
+    int prec = c->opt.precision;
+    if (prec == c->last_precision) return;
+    size_t tk = prec == FEM_PRECISION_FP32_REFINED ? 4 : 8;
+    size_t n = (size_t)c->ndof;
+    if (c->kind != FEM_ELEMENT_HEX) {
+        c->rec.alloc((size_t)c->num_elements * kTetRec * tk);
+        c->S.alloc((size_t)c->num_blocks * 9 * tk);
+    }
+    c->cg_x.alloc(n * tk);
+    c->cg_r.alloc(n * tk);
+    c->cg_p.alloc(n * tk);
+    c->cg_q.alloc(n * tk);
+    c->cg_dinv.alloc(n * tk);
+    c->cg_b.alloc(n * tk);
+    c->last_precision = prec;
+    c->linear_matrix_ready = false;
+}
+
+CgArgs cg_args(fem_ctx* c, bool fp32, double tol) {
+    CgArgs a;
+    a.num_nodes = (int)c->num_nodes;
+    a.n = (int)c->ndof;
+    a.rowptr = c->rowptr_b.p;
+    a.colidx = c->colidx_b.p;
+    a.diagblk = c->diagblk.p;
+    a.S = c->S.p;
+    a.fp32 = fp32;
+    a.x = c->cg_x.p;
+    a.r = c->cg_r.p;
+    a.p = c->cg_p.p;
+    a.q = c->cg_q.p;
+    a.dinv = c->cg_dinv.p;
+    a.b = c->cg_b.p;
+    a.partial = c->partial.p;
+    a.sc = c->scal.p;
+    a.tol = tol;
+    return a;
+}
+
+int64_t max_iters(fem_ctx* c) {
+    int64_t m = c->opt.cg_max_iterations < 0 ? 2 * c->ndof : c->opt.cg_max_iterations;
+    return std::min<int64_t>(m, std::numeric_limits<int>::max() - 1);
+}
+
+// Runs the CG iterations of an initialised solve: launches a predicted number of iterations, then
+// polls the device 'done' flag between chunks. Returns the iteration count.
+int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
+    int k = 0;
+    int first = std::max(1, std::min(c->predicted_iters, max_iter));
+    for (; k < first; ++k) launch_cg_iteration(a, k, max_iter, c->stream);
+    for (;;) {
+        HIPCHECK(hipMemcpyAsync(c->h_sc, a.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->h_sc->done || k >= max_iter) break;
+        int chunk = std::min(std::max(2, c->predicted_iters / 8), max_iter - k);
+        for (int j = 0; j < chunk; ++j, ++k) launch_cg_iteration(a, k, max_iter, c->stream);
+    }
+    int it = c->h_sc->iters;
+    c->predicted_iters = it + 1;
+    return it;
+}
+
+// Linear solve S dv = b (b in c->b, FP64). FP64: Eigen-CG on the FP64 operator. FP32: FP32 CG inside
+// FP64 iterative refinement on the implicit-step residual. Result in c->dv.
+void linear_solve(fem_ctx* c) {
+    const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
+    const int max_iter = (int)max_iters(c);
+    size_t n = (size_t)c->ndof;
+    if (!fp32) {
+        CgArgs a = cg_args(c, false, c->opt.cg_tolerance);
+        a.b = c->b.p;
+        a.x = c->dv.p;
+        launch_cg_init(a, c->stream);
+        int it = run_cg_iterations(c, a, max_iter);
+        c->stats.last_cg_iterations = it;
+        c->stats.total_cg_iterations += it;
+        c->stats.last_cg_error = c->h_sc->err;
+        c->stats.converged = c->h_sc->converged;
+        c->stats.last_refinements = 0;
+        return;
+    }
+    HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+    CgArgs a = cg_args(c, true, c->opt.inner_tolerance);
+    CgScalars* outer = c->scal.p + 1;
+    int total = 0, sweeps = 0;
+    for (int sweep = 0; sweep <= c->opt.max_refinements; ++sweep) {
+        if (sweep == 0) launch_residual64(a, c->b.p, nullptr, nullptr, c->stream);
+        else launch_residual64(a, c->b.p, c->dv.p, c->r64.p, c->stream);
+        launch_refine_prepare(a, sweep == 0 ? c->b.p : c->r64.p, sweep == 0, c->opt.cg_tolerance, outer, c->stream);
+        if (sweep == c->opt.max_refinements) {
+            // final residual test only
+            HIPCHECK(hipMemcpyAsync(c->h_sc, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            break;
+        }
+        // inner FP32 solve (its init is skipped through the done flag set by the check)
+        HIPCHECK(hipMemcpyAsync(c->h_sc, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->h_sc->done) break;
+        launch_cg_init(a, c->stream);
+        int it = run_cg_iterations(c, a, max_iter);
+        total += it + 1;
+        launch_refine_accumulate((int)n, (const float*)c->cg_x.p, c->dv.p, outer, c->stream);
+        sweeps++;
+    }
+    CgScalars o;
+    HIPCHECK(hipMemcpyAsync(&o, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->stats.last_cg_iterations = total;
+    c->stats.total_cg_iterations += total;
+    c->stats.last_cg_error = o.err;
+    c->stats.converged = o.converged;
+    c->stats.last_refinements = sweeps;
+}
+
+void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs) {
+    TetElementArgs a;
+    a.num_elements = (int)c->num_elements;
+    a.conn = c->conn4.p;
+    a.x0 = c->x0.p;
+    a.xn = c->x.p;
+    a.vn = c->v.p;
+    a.xs = xs;
+    a.vs = vs;
+    a.mat = c->mat.p;
+    a.dt = c->opt.dt;
+    a.write_matrix = write_matrix ? 1 : 0;
+    a.rec = c->rec.p;
+    a.rhs_rec = c->rhs_rec.p;
+    a.flag_counts = c->counters.p;
+    launch_tet_elements(c->kind, c->opt.precision == FEM_PRECISION_FP32_REFINED, a, c->stream);
+}
+
+void assemble_tet(fem_ctx* c) {
+    launch_assemble(c->opt.precision == FEM_PRECISION_FP32_REFINED, c->num_lower, c->lower_blk.p, c->mirror_blk.p,
+                    c->blk_row.p, c->colidx_b.p, c->ctr_ptr.p, c->ctr_list.p, c->rec.p, c->bcmask.p, c->S.p,
+                    c->stream);
+}
+
+void rhs_tet(fem_ctx* c) {
+    launch_rhs((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->rhs_rec.p, c->fg.p,
+               c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream);
+}
+
+// One implicit step (OdeSolverEulerImplicit::solve with computeCompliance = false, or
+// OdeSolverLinearEulerImplicit::solve). Returns FEM_OK or FEM_INACTIVE.
+int step_once(fem_ctx* c) {
+    const double dt = c->opt.dt;
+    const size_t n = (size_t)c->ndof;
+    const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
+    const int newton = std::max(1, c->opt.newton_max_iterations);
+    if (c->kind == FEM_ELEMENT_HEX) {
+        hex_step(c->hex, c->opt, c->x0.p, c->x.p, c->v.p, c->fg.p, c->have_fext ? c->fext.p : nullptr, c->bcmask.p,
+                 c->partial.p, c->scal.p, c->h_sc, c->counters.p + 3, c->stream, c->predicted_iters, c->stats,
+                 (int)max_iters(c));
+    } else if (linear_scheme && c->linear_matrix_ready && c->matrix_dt == dt) {
+        // Steps >= 2 of OdeSolverLinearEulerImplicit: element forces only, constant operator.
+        tet_elements(c, false, c->x.p, c->v.p);
+        rhs_tet(c);
+        linear_solve(c);
+        launch_update((int)n, c->dv.p, c->x.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+    } else if (newton == 1 || linear_scheme) {
+        tet_elements(c, true, c->x.p, c->v.p);
+        assemble_tet(c);
+        rhs_tet(c);
+        linear_solve(c);
+        launch_update((int)n, c->dv.p, c->x.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+        if (linear_scheme) {
+            c->linear_matrix_ready = true;
+            c->matrix_dt = dt;
+        }
+    } else {
+        // Newton-Raphson (Math/OdeSolverEulerImplicit.cpp:84-115): newState starts as the current state.
+        HIPCHECK(hipMemcpyAsync(c->xs.p, c->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipMemcpyAsync(c->vs.p, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipMemsetAsync(c->dvprev.p, 0, n * sizeof(double), c->stream));
+        for (int it = 0; it < newton; ++it) {
+            tet_elements(c, true, c->xs.p, c->vs.p);
+            assemble_tet(c);
+            rhs_tet(c);
+            linear_solve(c);
+            launch_update((int)n, c->dv.p, c->xs.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+            // ||dv - dv_prev||_inf < epsilon (1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112)
+            std::vector<double> h_dv(n), h_prev(n);
+            HIPCHECK(hipMemcpyAsync(h_dv.data(), c->dv.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipMemcpyAsync(h_prev.data(), c->dvprev.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            double var = 0.0;
+            for (size_t i = 0; i < n; ++i) var = std::max(var, std::fabs(h_dv[i] - h_prev[i]));
+            if (var < 1e-5) break;
+            HIPCHECK(hipMemcpyAsync(c->dvprev.p, c->dv.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    c->stats.steps++;
+    return FEM_OK;
+}
+
+void build_tet_maps(fem_ctx* c, const int32_t* conn) {
+    const int64_t N = c->num_nodes, E = c->num_elements;
+    // Node adjacency (every node pair sharing an element, both orders, plus the diagonal).
+    std::vector<int64_t> cnt(N + 1, 0);
+    for (int64_t e = 0; e < E; ++e)
+        for (int a = 0; a < 4; ++a) cnt[conn[4 * e + a] + 1] += 4;
+    for (int64_t i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> adj(cnt[N]);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) adj[pos[conn[4 * e + a]]++] = conn[4 * e + b];
+    c->h_rowptr_b.assign(N + 1, 0);
+    c->h_colidx_b.clear();
+    c->h_colidx_b.reserve(cnt[N] / 3);
+    for (int64_t i = 0; i < N; ++i) {
+        auto b = adj.begin() + cnt[i], en = adj.begin() + cnt[i + 1];
+        std::sort(b, en);
+        auto last = std::unique(b, en);
+        c->h_colidx_b.insert(c->h_colidx_b.end(), b, last);
+        if (c->h_colidx_b.size() > (size_t)std::numeric_limits<int32_t>::max() / 9)
+            throw InvalidArg("pattern too large for int32 scalar indices");
+        c->h_rowptr_b[i + 1] = (int32_t)c->h_colidx_b.size();
+    }
+    adj.clear();
+    adj.shrink_to_fit();
+    const auto& rp = c->h_rowptr_b;
+    const auto& ci = c->h_colidx_b;
+    c->num_blocks = (int64_t)ci.size();
+    auto find_blk = [&](int32_t I, int32_t J) -> int32_t {
+        auto b = ci.begin() + rp[I], en = ci.begin() + rp[I + 1];
+        auto it = std::lower_bound(b, en, J);
+        return (int32_t)(it - ci.begin());
+    };
+    std::vector<int32_t> blk_row(c->num_blocks), diag(N);
+    std::vector<int32_t> lower_of_blk(c->num_blocks, -1);
+    std::vector<int32_t> lower_blk, mirror;
+    for (int32_t I = 0; I < (int32_t)N; ++I)
+        for (int32_t k = rp[I]; k < rp[I + 1]; ++k) {
+            blk_row[k] = I;
+            int32_t J = ci[k];
+            if (J == I) diag[I] = k;
+            if (J <= I) {
+                lower_of_blk[k] = (int32_t)lower_blk.size();
+                lower_blk.push_back(k);
+                mirror.push_back(J == I ? k : find_blk(J, I));
+            }
+        }
+    c->num_lower = (int)lower_blk.size();
+    // Contributions per lower block, in element order (counting sort).
+    std::vector<int32_t> cptr(c->num_lower + 1, 0);
+    std::vector<int32_t> codes_t(10 * E);
+    std::vector<int32_t> codes(10 * E);
+    for (int64_t e = 0; e < E; ++e) {
+        const int32_t* id = conn + 4 * e;
+        int k = 0;
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) {
+                int32_t I = std::max(id[a], id[b]), J = std::min(id[a], id[b]);
+                int ps = a * (7 - a) / 2 + (b - a - 1);
+                int32_t t = lower_of_blk[find_blk(I, J)];
+                codes_t[10 * e + k] = t;
+                codes[10 * e + k] = (int32_t)(e << 4) | ps;
+                cptr[t + 1]++;
+                k++;
+            }
+        for (int a = 0; a < 4; ++a) {
+            int32_t t = lower_of_blk[diag[id[a]]];
+            codes_t[10 * e + k] = t;
+            codes[10 * e + k] = (int32_t)(e << 4) | (6 + a);
+            cptr[t + 1]++;
+            k++;
+        }
+    }
+    for (int t = 0; t < c->num_lower; ++t) cptr[t + 1] += cptr[t];
+    std::vector<int32_t> clist(10 * E);
+    {
+        std::vector<int32_t> p(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < 10 * E; ++i) clist[p[codes_t[i]]++] = codes[i];
+    }
+    // Node rhs contributions, element order.
+    std::vector<int32_t> nptr(N + 1, 0), nlist(4 * E);
+    for (int64_t e = 0; e < E; ++e)
+        for (int a = 0; a < 4; ++a) nptr[conn[4 * e + a] + 1]++;
+    for (int64_t i = 0; i < N; ++i) nptr[i + 1] += nptr[i];
+    {
+        std::vector<int32_t> p(nptr.begin(), nptr.end() - 1);
+        for (int64_t e = 0; e < E; ++e)
+            for (int a = 0; a < 4; ++a) nlist[p[conn[4 * e + a]]++] = (int32_t)(e << 2) | a;
+    }
+    hipStream_t s = c->stream;
+    c->rowptr_b.alloc(N + 1);
+    c->rowptr_b.upload(rp.data(), N + 1, s);
+    c->colidx_b.alloc(c->num_blocks);
+    c->colidx_b.upload(ci.data(), c->num_blocks, s);
+    c->blk_row.alloc(c->num_blocks);
+    c->blk_row.upload(blk_row.data(), c->num_blocks, s);
+    c->diagblk.alloc(N);
+    c->diagblk.upload(diag.data(), N, s);
+    c->lower_blk.alloc(c->num_lower);
+    c->lower_blk.upload(lower_blk.data(), c->num_lower, s);
+    c->mirror_blk.alloc(c->num_lower);
+    c->mirror_blk.upload(mirror.data(), c->num_lower, s);
+    c->ctr_ptr.alloc(c->num_lower + 1);
+    c->ctr_ptr.upload(cptr.data(), c->num_lower + 1, s);
+    c->ctr_list.alloc(10 * E);
+    c->ctr_list.upload(clist.data(), 10 * E, s);
+    c->node_ptr.alloc(N + 1);
+    c->node_ptr.upload(nptr.data(), N + 1, s);
+    c->node_list.alloc(4 * E);
+    c->node_list.upload(nlist.data(), 4 * E, s);
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+void fem_default_options(fem_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->dt = 1e-3;
+    o->gravity[0] = 0.0;
+    o->gravity[1] = -9.81;
+    o->gravity[2] = 0.0;
+    o->gravity_enabled = 1;
+    o->scheme = FEM_SCHEME_EULER_IMPLICIT;
+    o->precision = FEM_PRECISION_FP64;
+    o->newton_max_iterations = 1;
+    o->cg_tolerance = std::numeric_limits<double>::epsilon();
+    o->cg_max_iterations = -1;
+    o->inner_tolerance = 1e-4;
+    o->max_refinements = 8;
+    o->fail_on_nonconvergence = 0;
+}
+
+int fem_create(int device, fem_ctx** out) {
+    if (!out) return FEM_ERR_INVALID;
+    *out = nullptr;
+    fem_ctx* c = new fem_ctx();
+    int rc = guarded(c, [&] {
+        int count = 0;
+        HIPCHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw InvalidArg("invalid device ordinal");
+        c->device = device;
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+        HIPCHECK(hipHostMalloc((void**)&c->h_sc, 2 * sizeof(CgScalars), hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void**)&c->h_counters, 8 * sizeof(int), hipHostMallocDefault));
+        c->partial.alloc(4 * kCgBlocks);
+        c->scal.alloc(2);
+        c->counters.alloc(8);
+        HIPCHECK(hipMemset(c->counters.p, 0, 8 * sizeof(int)));
+        HIPCHECK(hipMemset(c->scal.p, 0, 2 * sizeof(CgScalars)));
+        fem_default_options(&c->opt);
+        std::memset(&c->stats, 0, sizeof(c->stats));
+        c->stats.active = 1;
+        return FEM_OK;
+    });
+    if (rc != FEM_OK) {
+        fem_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FEM_OK;
+}
+
+void fem_destroy(fem_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->h_sc) (void)hipHostFree(c->h_sc);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* fem_last_error(const fem_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fem_set_stream(fem_ctx* c, void* s) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (s) {
+            if (c->own_stream) HIPCHECK(hipStreamDestroy(c->stream));
+            c->stream = (hipStream_t)s;
+            c->own_stream = false;
+        } else if (!c->own_stream) {
+            HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, int64_t num_elements,
+                    const int32_t* conn, const double* young, const double* poisson, const double* density) {
+    return guarded(c, [&] {
+        if (c->mesh_ready) throw InvalidArg("You must set the FemElement type before initializing (mesh already uploaded)");
+        if (kind != FEM_ELEMENT_TET_LINEAR && kind != FEM_ELEMENT_TET_COROTATIONAL && kind != FEM_ELEMENT_HEX)
+            throw InvalidArg("unknown element kind");
+        if (num_nodes <= 0 || num_elements <= 0 || !x0 || !conn || !young || !poisson || !density)
+            throw InvalidArg("empty mesh or null pointer");
+        if (3 * num_nodes > std::numeric_limits<int32_t>::max() / 2 || num_elements >= (1LL << 27))
+            throw InvalidArg("mesh too large for int32 indexing");
+        HIPCHECK(hipSetDevice(c->device));
+        const int nn = kind == FEM_ELEMENT_HEX ? 8 : 4;
+        c->kind = kind;
+        c->nn = nn;
+        c->num_nodes = num_nodes;
+        c->num_elements = num_elements;
+        c->ndof = 3 * num_nodes;
+        for (int64_t e = 0; e < num_elements; ++e) {
+            for (int a = 0; a < nn; ++a) {
+                int32_t id = conn[nn * e + a];
+                if (id < 0 || id >= num_nodes)
+                    throw InvalidArg("Invalid nodeId " + std::to_string(id) + " expected in range [0.." +
+                                     std::to_string(num_nodes - 1) + "]");
+            }
+            check_material(young[e], poisson[e], density[e], e);
+        }
+        c->h_mass_per_node.assign(num_nodes, 0.0);
+        std::vector<double> mat(4 * num_elements);
+        for (int64_t e = 0; e < num_elements; ++e) {
+            double E = young[e], nu = poisson[e], rho = density[e];
+            mat[4 * e] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+            mat[4 * e + 1] = E / (2.0 * (1 + nu));
+            mat[4 * e + 2] = rho;
+            mat[4 * e + 3] = E;
+        }
+        if (kind != FEM_ELEMENT_HEX) {
+            for (int64_t e = 0; e < num_elements; ++e) {
+                const int32_t* id = conn + 4 * e;
+                double vol = tet_volume(x0 + 3 * id[0], x0 + 3 * id[1], x0 + 3 * id[2], x0 + 3 * id[3]);
+                if (kind == FEM_ELEMENT_TET_COROTATIONAL && !(std::fabs(6.0 * vol) > 1e-12))
+                    throw InvalidArg("Trying to initialize an invalid co-rotational tetrahedron. Matrix V not "
+                                     "invertible (element " + std::to_string(e) + ")");
+                double mass = vol * density[e];
+                for (int a = 0; a < 4; ++a) c->h_mass_per_node[id[a]] += mass / 4;
+            }
+            build_tet_maps(c, conn);
+            std::vector<int4> c4(num_elements);
+            for (int64_t e = 0; e < num_elements; ++e)
+                c4[e] = make_int4(conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]);
+            c->conn4.alloc(num_elements);
+            c->conn4.upload(c4.data(), num_elements, c->stream);
+            c->rhs_rec.alloc((size_t)num_elements * 12);
+        } else {
+            hex_setup(c->hex, num_nodes, x0, num_elements, conn, mat.data(), c->h_mass_per_node, c->h_rowptr_b,
+                      c->h_colidx_b, c->stream);
+            c->num_blocks = (int64_t)c->h_colidx_b.size();
+        }
+        c->mat.alloc(num_elements);
+        c->mat.upload(reinterpret_cast<const double4*>(mat.data()), num_elements, c->stream);
+        size_t n = (size_t)c->ndof;
+        c->x0.alloc(n);
+        c->x0.upload(x0, n, c->stream);
+        c->x.alloc(n);
+        c->x.upload(x0, n, c->stream);
+        c->v.alloc(n);
+        HIPCHECK(hipMemsetAsync(c->v.p, 0, n * sizeof(double), c->stream));
+        c->xs.alloc(n);
+        c->vs.alloc(n);
+        c->fg.alloc(n);
+        c->fext.alloc(n);
+        c->b.alloc(n);
+        c->dv.alloc(n);
+        c->dvprev.alloc(n);
+        c->r64.alloc(n);
+        c->bcmask.alloc(n);
+        c->h_bcmask.assign(n, 0);
+        c->h_bc_dofs.clear();
+        c->bc_dirty = true;
+        c->fg_dirty = true;
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->mesh_ready = true;
+        c->last_precision = -1;
+        alloc_precision_buffers(c);
+        return FEM_OK;
+    });
+}
+
+int fem_add_boundary_condition_dof(fem_ctx* c, int64_t dof) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("Number of dof per node = 0. Make sure to call setNumDof() first");
+        if (dof < 0 || dof >= c->ndof) throw InvalidArg("Invalid dof " + std::to_string(dof));
+        if (!c->h_bcmask[dof]) {
+            c->h_bcmask[dof] = 1;
+            c->h_bc_dofs.push_back(dof);
+            c->bc_dirty = true;
+            c->linear_matrix_ready = false;
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_add_boundary_condition_node(fem_ctx* c, int64_t node) {
+    if (!c) return FEM_ERR_INVALID;
+    if (c->mesh_ready && (node < 0 || node >= c->num_nodes))
+        return fail(c, FEM_ERR_INVALID, "Invalid nodeId " + std::to_string(node));
+    for (int k = 0; k < 3; ++k) {
+        int rc = fem_add_boundary_condition_dof(c, 3 * node + k);
+        if (rc) return rc;
+    }
+    return FEM_OK;
+}
+
+int fem_set_options(fem_ctx* c, const fem_options* o) {
+    return guarded(c, [&] {
+        if (!o) throw InvalidArg("null options");
+        if (!(o->dt > 0.0)) throw InvalidArg("dt must be positive");
+        if (o->scheme != FEM_SCHEME_EULER_IMPLICIT && o->scheme != FEM_SCHEME_LINEAR_EULER_IMPLICIT)
+            throw InvalidArg("unknown integration scheme");
+        if (o->precision != FEM_PRECISION_FP64 && o->precision != FEM_PRECISION_FP32_REFINED)
+            throw InvalidArg("unknown precision");
+        if (o->newton_max_iterations < 1) throw InvalidArg("The maximum iteration needs to be at least 1");
+        if (!(o->cg_tolerance >= 0.0)) throw InvalidArg("negative tolerance");
+        bool grav = o->gravity_enabled != c->opt.gravity_enabled || o->gravity[0] != c->opt.gravity[0] ||
+                    o->gravity[1] != c->opt.gravity[1] || o->gravity[2] != c->opt.gravity[2];
+        if (o->scheme != c->opt.scheme || o->dt != c->opt.dt) c->linear_matrix_ready = false;
+        c->opt = *o;
+        if (grav) c->fg_dirty = true;
+        if (c->mesh_ready) {
+            HIPCHECK(hipSetDevice(c->device));
+            alloc_precision_buffers(c);
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_set_external_force(fem_ctx* c, const double* f) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        HIPCHECK(hipSetDevice(c->device));
+        if (f) {
+            c->fext.upload(f, (size_t)c->ndof, c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            c->have_fext = true;
+        } else {
+            c->have_fext = false;
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_set_state(fem_ctx* c, const double* x, const double* v) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        HIPCHECK(hipSetDevice(c->device));
+        if (x) c->x.upload(x, (size_t)c->ndof, c->stream);
+        if (v) c->v.upload(v, (size_t)c->ndof, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_get_state(fem_ctx* c, double* x, double* v) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        HIPCHECK(hipSetDevice(c->device));
+        size_t bytes = (size_t)c->ndof * sizeof(double);
+        if (x) HIPCHECK(hipMemcpyAsync(x, c->x.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (v) HIPCHECK(hipMemcpyAsync(v, c->v.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_set_state_device(fem_ctx* c, const double* x, const double* v) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        HIPCHECK(hipSetDevice(c->device));
+        size_t bytes = (size_t)c->ndof * sizeof(double);
+        if (x) HIPCHECK(hipMemcpyAsync(c->x.p, x, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (v) HIPCHECK(hipMemcpyAsync(c->v.p, v, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_get_state_device(fem_ctx* c, double* x, double* v) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        HIPCHECK(hipSetDevice(c->device));
+        size_t bytes = (size_t)c->ndof * sizeof(double);
+        if (x) HIPCHECK(hipMemcpyAsync(x, c->x.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (v) HIPCHECK(hipMemcpyAsync(v, c->v.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_step(fem_ctx* c, int32_t nsteps) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "Initial state has not been set yet");
+        if (nsteps < 0) throw InvalidArg("negative step count");
+        if (!c->stats.active) return FEM_INACTIVE;
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_bc(c);
+        alloc_precision_buffers(c);
+        for (int i = 0; i < nsteps; ++i) {
+            step_once(c);
+            if (c->opt.fail_on_nonconvergence && !c->stats.converged)
+                return fail(c, FEM_ERR_NOT_CONVERGED, "linear solve did not converge");
+        }
+        HIPCHECK(hipMemcpyAsync(c->h_counters, c->counters.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->stats.det_r_invalid = c->h_counters[0];
+        c->stats.g_singular = c->h_counters[1];
+        c->stats.fp64_polar_fallbacks = c->h_counters[2];
+        if (c->h_counters[3]) {
+            c->stats.active = 0;
+            return FEM_INACTIVE;
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_synchronize(fem_ctx* c) {
+    return guarded(c, [&] {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_query(fem_ctx* c, fem_stats* s) {
+    if (!c || !s) return FEM_ERR_INVALID;
+    *s = c->stats;
+    return FEM_OK;
+}
+
+int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
+
+int fem_get_pattern(fem_ctx* c, int32_t* rowptr, int32_t* colidx) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        const auto& rp = c->h_rowptr_b;
+        const auto& ci = c->h_colidx_b;
+        int64_t pos = 0;
+        for (int64_t I = 0; I < c->num_nodes; ++I) {
+            int64_t len = rp[I + 1] - rp[I];
+            for (int r = 0; r < 3; ++r) {
+                if (rowptr) rowptr[3 * I + r] = (int32_t)pos;
+                if (colidx)
+                    for (int64_t k = rp[I]; k < rp[I + 1]; ++k)
+                        for (int cc = 0; cc < 3; ++cc) colidx[pos + 3 * (k - rp[I]) + cc] = 3 * ci[k] + cc;
+                pos += 3 * len;
+            }
+        }
+        if (rowptr) rowptr[c->ndof] = (int32_t)pos;
+        return FEM_OK;
+    });
+}
+
+int fem_get_system_matrix(fem_ctx* c, double* values) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready || !values) throw InvalidArg("mesh not uploaded or null output");
+        if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("hexahedral meshes use the element-by-element operator");
+        HIPCHECK(hipSetDevice(c->device));
+        const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
+        std::vector<double> S(9 * c->num_blocks);
+        if (fp32) {
+            std::vector<float> Sf(9 * c->num_blocks);
+            HIPCHECK(hipMemcpy(Sf.data(), c->S.p, Sf.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < Sf.size(); ++i) S[i] = Sf[i];
+        } else {
+            HIPCHECK(hipMemcpy(S.data(), c->S.p, S.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        const auto& rp = c->h_rowptr_b;
+        int64_t pos = 0;
+        for (int64_t I = 0; I < c->num_nodes; ++I)
+            for (int r = 0; r < 3; ++r)
+                for (int64_t k = rp[I]; k < rp[I + 1]; ++k)
+                    for (int cc = 0; cc < 3; ++cc) values[pos++] = S[9 * k + 3 * r + cc];
+        return FEM_OK;
+    });
+}
+
+int fem_get_rhs(fem_ctx* c, double* rhs) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready || !rhs) throw InvalidArg("mesh not uploaded or null output");
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipMemcpy(rhs, c->b.p, (size_t)c->ndof * sizeof(double), hipMemcpyDeviceToHost));
+        return FEM_OK;
+    });
+}
+
+int fem_evaluate_elements(fem_ctx* c, const double* x, int64_t first, int64_t count, double* R, double* K,
+                          double* f) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready || !x) throw InvalidArg("mesh not uploaded or null positions");
+        if (first < 0 || count < 0 || first + count > c->num_elements) throw InvalidArg("element range out of bounds");
+        HIPCHECK(hipSetDevice(c->device));
+        const int nn3 = 3 * c->nn;
+        DevBuf<double> dx, dR, dK, df;
+        dx.alloc((size_t)c->ndof);
+        dx.upload(x, (size_t)c->ndof, c->stream);
+        dR.alloc((size_t)count * 9);
+        dK.alloc((size_t)count * nn3 * nn3);
+        df.alloc((size_t)count * nn3);
+        if (c->kind == FEM_ELEMENT_HEX)
+            hex_eval_elements(c->hex, (int)first, (int)count, c->x0.p, dx.p, dR.p, dK.p, df.p, c->stream);
+        else
+            launch_eval_elements(c->kind, (int)first, (int)count, c->conn4.p, c->x0.p, dx.p, c->mat.p, dR.p, dK.p,
+                                 df.p, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        HIPCHECK(hipGetLastError());
+        if (R) HIPCHECK(hipMemcpy(R, dR.p, (size_t)count * 9 * sizeof(double), hipMemcpyDeviceToHost));
+        if (K) HIPCHECK(hipMemcpy(K, dK.p, (size_t)count * nn3 * nn3 * sizeof(double), hipMemcpyDeviceToHost));
+        if (f) HIPCHECK(hipMemcpy(f, df.p, (size_t)count * nn3 * sizeof(double), hipMemcpyDeviceToHost));
+        return FEM_OK;
+    });
+}
+
+}  // extern "C"

