@@ -151,6 +151,21 @@ int fem_get_rhs(fem_ctx* ctx, double* rhs);
 int fem_evaluate_elements(fem_ctx* ctx, const double* x, int64_t first, int64_t count, double* rotation,
                           double* stiffness, double* force);
 
+/* Per-phase HIP-event timers on the context stream. No counterpart in the reference's FEM path; they
+ * stand in for its Framework::Timer instrumentation around update(dt) (Framework/Timer.h:31-93,
+ * Physics/PhysicsManager.cpp:130-152). spmv_* count only the CG operator applications that did work
+ * (launches skipped after convergence are reported separately). */
+enum {
+    FEM_PHASE_ELEMENT = 0, FEM_PHASE_ASSEMBLE, FEM_PHASE_RHS, FEM_PHASE_CG_INIT, FEM_PHASE_SPMV,
+    FEM_PHASE_CG_VECTOR, FEM_PHASE_REFINE, FEM_PHASE_UPDATE, FEM_PHASE_SPMV_SKIPPED, FEM_NUM_PHASES
+};
+typedef struct fem_phase_times {
+    double ms[FEM_NUM_PHASES];       /* summed kernel time per phase */
+    int64_t launches[FEM_NUM_PHASES];
+} fem_phase_times;
+int fem_enable_phase_timers(fem_ctx* ctx, int every);   /* instrument one step in `every` (0: off); resets totals */
+int fem_get_phase_times(fem_ctx* ctx, fem_phase_times* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,77 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// HIP-event timers per phase (resolved at host synchronisation points).
+struct PhaseTimer {
+    bool enabled = false;
+    struct Pair {
+        hipEvent_t a, b;
+        int phase, tag;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Pair> pending;
+    hipEvent_t open_ev = nullptr;
+    int open_phase = 0, open_tag = 0;
+    double ms[FEM_NUM_PHASES] = {};
+    int64_t launches[FEM_NUM_PHASES] = {};
+    hipEvent_t get() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    void begin(hipStream_t s, int phase, int tag = 0) {
+        open_ev = get();
+        open_phase = phase;
+        open_tag = tag;
+        HIPCHECK(hipEventRecord(open_ev, s));
+    }
+    void end(hipStream_t s) {
+        hipEvent_t b = get();
+        HIPCHECK(hipEventRecord(b, s));
+        pending.push_back({open_ev, b, open_phase, open_tag});
+    }
+    // Call after the stream has been synchronised. SpMV pairs with tag > spmv_last were launched after
+    // convergence (no work) and are booked separately.
+    void resolve(int spmv_last = 1 << 30) {
+        for (auto& p : pending) {
+            float t = 0.f;
+            HIPCHECK(hipEventElapsedTime(&t, p.a, p.b));
+            int ph = p.phase;
+            if (ph == FEM_PHASE_SPMV && p.tag > spmv_last) ph = FEM_PHASE_SPMV_SKIPPED;
+            ms[ph] += t;
+            launches[ph]++;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (int i = 0; i < FEM_NUM_PHASES; ++i) {
+            ms[i] = 0.0;
+            launches[i] = 0;
+        }
+    }
+    ~PhaseTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+    }
+};
+
+#define TIMED(ctx, phase, tag, stmt)                                   \
+    do {                                                               \
+        if ((ctx)->timer.enabled) (ctx)->timer.begin((ctx)->stream, phase, tag); \
+        stmt;                                                          \
+        if ((ctx)->timer.enabled) (ctx)->timer.end((ctx)->stream);     \
+    } while (0)
+
 }  // namespace
 
 struct fem_ctx {
@@ -86,13 +157,14 @@ struct fem_ctx {
     DevBuf<double> x0, x, v, xs, vs, fg, fext, b, dv, dvprev, r64;
     DevBuf<int4> conn4;
     DevBuf<double4> mat;
-    DevBuf<unsigned char> rec;  // element records (TK)
+    DevBuf<double> rec;         // element records
     DevBuf<double> rhs_rec;
     DevBuf<int> lower_blk, mirror_blk, blk_row, colidx_b, rowptr_b, ctr_ptr, ctr_list, node_ptr, node_list, diagblk;
-    DevBuf<unsigned char> S;    // operator values (TK)
+    DevBuf<double> S64;         // operator values (FP64, 9 per block)
+    DevBuf<float> S32;          // FP32 copy (FP32 mode)
     DevBuf<uint8_t> bcmask;
-    DevBuf<unsigned char> cg_x, cg_r, cg_p, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
-    DevBuf<double> partial;
+    DevBuf<unsigned char> cg_x, cg_r, cg_z, cg_p0, cg_p1, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
+    DevBuf<double> partial;    // [4 * kCgBlocks] block sums of the grid reductions
     DevBuf<CgScalars> scal;    // [0] inner/FP64 solve, [1] outer refinement
     DevBuf<int> counters;      // [0..2] element flags, [3] invalid state, [4] Newton max-norm bits (int pair)
     DevBuf<unsigned long long> nrm;
@@ -111,8 +183,12 @@ struct fem_ctx {
     bool linear_matrix_ready = false;
     double matrix_dt = 0.0;
     int last_precision = -1;
-    int predicted_iters = 16;
+    int predicted_iters = 16;     // FP64 solve / current inner solve
+    int predicted_sweep[16];       // FP32 inner solves, per refinement sweep
     fem_stats stats;
+    PhaseTimer timer;
+    int timer_every = 0;           // instrument one step in timer_every (0: off)
+    int64_t timer_count = 0;
 };
 
 namespace {
@@ -175,37 +251,44 @@ void ensure_bc(fem_ctx* c) {
 void alloc_precision_buffers(fem_ctx* c) {
     int prec = c->opt.precision;
     if (prec == c->last_precision) return;
-    size_t tk = prec == FEM_PRECISION_FP32_REFINED ? 4 : 8;
+    const bool fp32 = prec == FEM_PRECISION_FP32_REFINED;
+    size_t tv = fp32 ? 4 : 8;
     size_t n = (size_t)c->ndof;
     if (c->kind != FEM_ELEMENT_HEX) {
-        c->rec.alloc((size_t)c->num_elements * kTetRec * tk);
-        c->S.alloc((size_t)c->num_blocks * 9 * tk);
+        if (!c->rec.p) c->rec.alloc((size_t)c->num_elements * kTetRec);
+        if (!c->S64.p) c->S64.alloc((size_t)c->num_blocks * 9);
+        if (fp32) c->S32.alloc((size_t)c->num_blocks * 9);
+        else c->S32.release();
     }
-    c->cg_x.alloc(n * tk);
-    c->cg_r.alloc(n * tk);
-    c->cg_p.alloc(n * tk);
-    c->cg_q.alloc(n * tk);
-    c->cg_dinv.alloc(n * tk);
-    c->cg_b.alloc(n * tk);
+    for (auto* b : {&c->cg_x, &c->cg_r, &c->cg_z, &c->cg_p0, &c->cg_p1, &c->cg_q, &c->cg_dinv, &c->cg_b})
+        b->alloc(n * tv);
     c->last_precision = prec;
     c->linear_matrix_ready = false;
 }
 
+Operator op_of(fem_ctx* c) {
+    Operator o;
+    o.num_nodes = (int)c->num_nodes;
+    o.plane = c->num_blocks;
+    o.rowptr = c->rowptr_b.p;
+    o.colidx = c->colidx_b.p;
+    o.diagblk = c->diagblk.p;
+    return o;
+}
+
 CgArgs cg_args(fem_ctx* c, bool fp32, double tol) {
     CgArgs a;
-    a.num_nodes = (int)c->num_nodes;
+    a.op = op_of(c);
     a.n = (int)c->ndof;
-    a.rowptr = c->rowptr_b.p;
-    a.colidx = c->colidx_b.p;
-    a.diagblk = c->diagblk.p;
-    a.S = c->S.p;
+    a.S = fp32 ? (const void*)c->S32.p : (const void*)c->S64.p;
     a.fp32 = fp32;
     a.x = c->cg_x.p;
     a.r = c->cg_r.p;
-    a.p = c->cg_p.p;
+    a.z = c->cg_z.p;
+    a.p[0] = c->cg_p0.p;
+    a.p[1] = c->cg_p1.p;
     a.q = c->cg_q.p;
     a.dinv = c->cg_dinv.p;
-    a.b = c->cg_b.p;
     a.partial = c->partial.p;
     a.sc = c->scal.p;
     a.tol = tol;
@@ -221,16 +304,24 @@ int64_t max_iters(fem_ctx* c) {
 // polls the device 'done' flag between chunks. Returns the iteration count.
 int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
     int k = 0;
-    int first = std::max(1, std::min(c->predicted_iters, max_iter));
-    for (; k < first; ++k) launch_cg_iteration(a, k, max_iter, c->stream);
+    // iteration k = convergence test of k-1 + p update + operator application, then vector updates;
+    // the last launch (k == max_iter) only tests.
+    auto iteration = [&](int kk) {
+        TIMED(c, FEM_PHASE_SPMV, kk, launch_cg_apply(a, kk, max_iter, c->stream));
+        if (kk < max_iter) TIMED(c, FEM_PHASE_CG_VECTOR, kk, launch_cg_vector(a, kk, c->stream));
+    };
+    // iteration k tests iteration k - 1: a converged solve with iters = i needs launches 0 .. i + 1
+    int first = std::max(1, std::min(c->predicted_iters + 1, max_iter + 1));
+    for (; k < first; ++k) iteration(k);
     for (;;) {
         HIPCHECK(hipMemcpyAsync(c->h_sc, a.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
-        if (c->h_sc->done || k >= max_iter) break;
-        int chunk = std::min(std::max(2, c->predicted_iters / 8), max_iter - k);
-        for (int j = 0; j < chunk; ++j, ++k) launch_cg_iteration(a, k, max_iter, c->stream);
+        if (c->h_sc->done || k > max_iter) break;
+        int chunk = std::min(std::max(2, c->predicted_iters / 8), max_iter + 1 - k);
+        for (int j = 0; j < chunk; ++j, ++k) iteration(k);
     }
     int it = c->h_sc->iters;
+    if (c->timer.enabled) c->timer.resolve(it);
     c->predicted_iters = it + 1;
     return it;
 }
@@ -243,9 +334,8 @@ void linear_solve(fem_ctx* c) {
     size_t n = (size_t)c->ndof;
     if (!fp32) {
         CgArgs a = cg_args(c, false, c->opt.cg_tolerance);
-        a.b = c->b.p;
         a.x = c->dv.p;
-        launch_cg_init(a, c->stream);
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_cg_init(a, c->b.p, nullptr, c->stream));
         int it = run_cg_iterations(c, a, max_iter);
         c->stats.last_cg_iterations = it;
         c->stats.total_cg_iterations += it;
@@ -259,23 +349,21 @@ void linear_solve(fem_ctx* c) {
     CgScalars* outer = c->scal.p + 1;
     int total = 0, sweeps = 0;
     for (int sweep = 0; sweep <= c->opt.max_refinements; ++sweep) {
-        if (sweep == 0) launch_residual64(a, c->b.p, nullptr, nullptr, c->stream);
-        else launch_residual64(a, c->b.p, c->dv.p, c->r64.p, c->stream);
-        launch_refine_prepare(a, sweep == 0 ? c->b.p : c->r64.p, sweep == 0, c->opt.cg_tolerance, outer, c->stream);
-        if (sweep == c->opt.max_refinements) {
-            // final residual test only
-            HIPCHECK(hipMemcpyAsync(c->h_sc, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
-            HIPCHECK(hipStreamSynchronize(c->stream));
-            break;
-        }
-        // inner FP32 solve (its init is skipped through the done flag set by the check)
-        HIPCHECK(hipMemcpyAsync(c->h_sc, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        if (c->h_sc->done) break;
-        launch_cg_init(a, c->stream);
+        TIMED(c, FEM_PHASE_REFINE, 0,
+              launch_residual64(a.op, c->S64.p, c->b.p, c->dv.p, sweep == 0, c->r64.p, c->partial.p, outer, a.sc,
+                                c->opt.cg_tolerance, c->stream));
+        if (sweep == c->opt.max_refinements) break;  // final residual test only
+        // inner FP32 solve on the FP64 residual (skipped on the device once the outer test passed)
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_cg_init(a, c->r64.p, outer, c->stream));
+        const int ps = std::min(sweep, 15);
+        c->predicted_iters = c->predicted_sweep[ps];
         int it = run_cg_iterations(c, a, max_iter);
+        HIPCHECK(hipMemcpyAsync(c->h_sc + 1, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->h_sc[1].done) break;
+        c->predicted_sweep[ps] = c->predicted_iters;
         total += it + 1;
-        launch_refine_accumulate((int)n, (const float*)c->cg_x.p, c->dv.p, outer, c->stream);
+        TIMED(c, FEM_PHASE_REFINE, 0, launch_refine_accumulate((int)n, (const float*)c->cg_x.p, c->dv.p, outer, c->stream));
         sweeps++;
     }
     CgScalars o;
@@ -300,21 +388,29 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.mat = c->mat.p;
     a.dt = c->opt.dt;
     a.write_matrix = write_matrix ? 1 : 0;
+    a.general = (xs != c->x.p || vs != c->v.p) ? 1 : 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
     a.flag_counts = c->counters.p;
-    launch_tet_elements(c->kind, c->opt.precision == FEM_PRECISION_FP32_REFINED, a, c->stream);
+    TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
 }
 
 void assemble_tet(fem_ctx* c) {
-    launch_assemble(c->opt.precision == FEM_PRECISION_FP32_REFINED, c->num_lower, c->lower_blk.p, c->mirror_blk.p,
-                    c->blk_row.p, c->colidx_b.p, c->ctr_ptr.p, c->ctr_list.p, c->rec.p, c->bcmask.p, c->S.p,
-                    c->stream);
+    if (c->timer.enabled) c->timer.begin(c->stream, FEM_PHASE_ASSEMBLE);
+    launch_assemble(c->num_lower, c->lower_blk.p, c->mirror_blk.p, c->blk_row.p, c->colidx_b.p, c->ctr_ptr.p,
+                    c->ctr_list.p, c->rec.p, c->bcmask.p, c->num_blocks, c->S64.p,
+                    c->opt.precision == FEM_PRECISION_FP32_REFINED ? c->S32.p : nullptr, c->stream);
+    if (c->timer.enabled) c->timer.end(c->stream);
 }
 
 void rhs_tet(fem_ctx* c) {
-    launch_rhs((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->rhs_rec.p, c->fg.p,
-               c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream);
+    TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->rhs_rec.p, c->fg.p,
+                                          c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream));
+}
+
+void state_update(fem_ctx* c, const double* xs) {
+    TIMED(c, FEM_PHASE_UPDATE, 0,
+          launch_update((int)c->ndof, c->dv.p, xs, c->x.p, c->v.p, c->opt.dt, c->counters.p + 3, c->stream));
 }
 
 // One implicit step (OdeSolverEulerImplicit::solve with computeCompliance = false, or
@@ -333,13 +429,13 @@ int step_once(fem_ctx* c) {
         tet_elements(c, false, c->x.p, c->v.p);
         rhs_tet(c);
         linear_solve(c);
-        launch_update((int)n, c->dv.p, c->x.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+        state_update(c, c->x.p);
     } else if (newton == 1 || linear_scheme) {
         tet_elements(c, true, c->x.p, c->v.p);
         assemble_tet(c);
         rhs_tet(c);
         linear_solve(c);
-        launch_update((int)n, c->dv.p, c->x.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+        state_update(c, c->x.p);
         if (linear_scheme) {
             c->linear_matrix_ready = true;
             c->matrix_dt = dt;
@@ -354,7 +450,7 @@ int step_once(fem_ctx* c) {
             assemble_tet(c);
             rhs_tet(c);
             linear_solve(c);
-            launch_update((int)n, c->dv.p, c->xs.p, c->x.p, c->v.p, dt, c->counters.p + 3, c->stream);
+            state_update(c, c->xs.p);
             // ||dv - dv_prev||_inf < epsilon (1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112)
             std::vector<double> h_dv(n), h_prev(n);
             HIPCHECK(hipMemcpyAsync(h_dv.data(), c->dv.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -526,6 +622,7 @@ int fem_create(int device, fem_ctx** out) {
         HIPCHECK(hipMemset(c->counters.p, 0, 8 * sizeof(int)));
         HIPCHECK(hipMemset(c->scal.p, 0, 2 * sizeof(CgScalars)));
         fem_default_options(&c->opt);
+        for (int i = 0; i < 16; ++i) c->predicted_sweep[i] = 8;
         std::memset(&c->stats, 0, sizeof(c->stats));
         c->stats.active = 1;
         return FEM_OK;
@@ -770,12 +867,15 @@ int fem_step(fem_ctx* c, int32_t nsteps) {
         ensure_bc(c);
         alloc_precision_buffers(c);
         for (int i = 0; i < nsteps; ++i) {
+            c->timer.enabled = c->timer_every > 0 && (c->timer_count++ % c->timer_every) == 0;
             step_once(c);
             if (c->opt.fail_on_nonconvergence && !c->stats.converged)
                 return fail(c, FEM_ERR_NOT_CONVERGED, "linear solve did not converge");
         }
         HIPCHECK(hipMemcpyAsync(c->h_counters, c->counters.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
+        c->timer.resolve();
+        c->timer.enabled = false;
         c->stats.det_r_invalid = c->h_counters[0];
         c->stats.g_singular = c->h_counters[1];
         c->stats.fp64_polar_fallbacks = c->h_counters[2];
@@ -798,6 +898,32 @@ int fem_query(fem_ctx* c, fem_stats* s) {
     if (!c || !s) return FEM_ERR_INVALID;
     *s = c->stats;
     return FEM_OK;
+}
+
+int fem_enable_phase_timers(fem_ctx* c, int enable) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->timer.resolve();
+        c->timer.reset();
+        c->timer_every = enable > 0 ? enable : 0;
+        c->timer_count = 0;
+        c->timer.enabled = false;
+        return FEM_OK;
+    });
+}
+
+int fem_get_phase_times(fem_ctx* c, fem_phase_times* out) {
+    return guarded(c, [&] {
+        if (!out) throw InvalidArg("null output");
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->timer.resolve();
+        for (int i = 0; i < FEM_NUM_PHASES; ++i) {
+            out->ms[i] = c->timer.ms[i];
+            out->launches[i] = c->timer.launches[i];
+        }
+        return FEM_OK;
+    });
 }
 
 int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
@@ -828,15 +954,8 @@ int fem_get_system_matrix(fem_ctx* c, double* values) {
         if (!c->mesh_ready || !values) throw InvalidArg("mesh not uploaded or null output");
         if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("hexahedral meshes use the element-by-element operator");
         HIPCHECK(hipSetDevice(c->device));
-        const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
         std::vector<double> S(9 * c->num_blocks);
-        if (fp32) {
-            std::vector<float> Sf(9 * c->num_blocks);
-            HIPCHECK(hipMemcpy(Sf.data(), c->S.p, Sf.size() * sizeof(float), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < Sf.size(); ++i) S[i] = Sf[i];
-        } else {
-            HIPCHECK(hipMemcpy(S.data(), c->S.p, S.size() * sizeof(double), hipMemcpyDeviceToHost));
-        }
+        HIPCHECK(hipMemcpy(S.data(), c->S64.p, S.size() * sizeof(double), hipMemcpyDeviceToHost));
         const auto& rp = c->h_rowptr_b;
         int64_t pos = 0;
         for (int64_t I = 0; I < c->num_nodes; ++I)

@@ -1,5 +1,5 @@
 // gfx950 kernels of the Fem3D implicit step: tetrahedron element kernels, deterministic gather
-// assembly into a 3x3-block CSR operator, right-hand side, Jacobi-preconditioned CG and the
+// assembly into a 3x3-block operator, right-hand side, Jacobi-preconditioned CG and the
 // implicit-Euler state update.
 //
 // Reference semantics (paths relative to the reference root):
@@ -13,6 +13,7 @@
 #include "fem_device.hpp"
 #include "fem_kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace femgpu {
@@ -24,6 +25,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// Sums of two values over the block with a single barrier; results valid in thread 0.
+// lds must hold 2 * blockDim/64 doubles.
+__device__ __forceinline__ void block_sum2(double& v0, double& v1, double* lds) {
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[w] = v0;
+        lds[nw + w] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        v0 = wave_sum(threadIdx.x < (unsigned)nw ? lds[threadIdx.x] : 0.0);
+        v1 = wave_sum(threadIdx.x < (unsigned)nw ? lds[nw + threadIdx.x] : 0.0);
+    }
 }
 
 // Sum over the block; result valid in thread 0. lds must hold blockDim/64 doubles.
@@ -42,31 +60,21 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return t;
 }
 
-// Sum of kCgBlocks partials, broadcast to every thread of the block.
-__device__ __forceinline__ double reduce_partials(const double* part, double* lds) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < kCgBlocks; i += blockDim.x) v += part[i];
-    v = block_sum(v, lds);
-    if (threadIdx.x == 0) lds[63] = v;
-    __syncthreads();
-    double r = lds[63];
-    __syncthreads();
-    return r;
-}
-
 // ------------------------------------------------------------------------------------------
-// Tetrahedron element kernels.
+// Tetrahedron element kernel (FP64). One thread per element: rotation, force, the 16 stiffness
+// blocks, the right-hand-side contribution f + K (xn - xs - vn dt) - M (vn - vs) / dt and the
+// lower-triangle record of A_e = M_e / dt + dt K_e.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ int pair_slot(int a, int b) { return a * (7 - a) / 2 + (b - a - 1); }
+__device__ __forceinline__ constexpr int pair_slot(int a, int b) { return a * (7 - a) / 2 + (b - a - 1); }
 
-template <typename TK, bool COROT>
+template <bool COROT, bool GENERAL>
 __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= A.num_elements) return;
     int4 id4 = A.conn[e];
-    int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-    double x0[12], xn[12], d1[12], d2[12];
+    const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
     const double dt = A.dt;
+    double x0[12], xn[12], d1[12], d2[12];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -76,28 +84,35 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
             double xv = A.xn[i];
             double vv = A.vn[i];
             xn[3 * a + c] = xv;
-            d1[3 * a + c] = (xv - A.xs[i]) - vv * dt;
-            d2[3 * a + c] = vv - A.vs[i];
+            if (GENERAL) {
+                d1[3 * a + c] = (xv - A.xs[i]) - vv * dt;
+                d2[3 * a + c] = vv - A.vs[i];
+            } else {
+                d1[3 * a + c] = -(vv * dt);
+                d2[3 * a + c] = 0.0;
+            }
         }
     }
     double4 m = A.mat[e];
     const double lam = m.x, mu = m.y, rho = m.z;
-    double f[12];
-    double V;
-    int flags = 0;
-    CoroFactors<TK> cf;
     double g[4][3];
-    V = tet_rest_gradients(x0, g);
+    const double V = tet_rest_gradients(x0, g);
+    double f[12];
+    CoroFactors<double> cf;
+    int flags = 0;
     if (COROT) {
-        flags = coro_factors<TK>(x0, xn, g, V, lam, mu, f, cf);
+        flags = coro_factors<double>(x0, xn, g, V, lam, mu, f, cf);
     } else {
         // f = -K0 (x - x0)  (Physics/Fem3DElementTetrahedron.cpp:77-90)
+#pragma unroll
         for (int a = 0; a < 4; ++a) {
             double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
             for (int b = 0; b < 4; ++b) {
                 double Kb[9];
                 linear_block<double>(g, V, lam, mu, a, b, Kb);
                 double u[3] = {xn[3 * b] - x0[3 * b], xn[3 * b + 1] - x0[3 * b + 1], xn[3 * b + 2] - x0[3 * b + 2]};
+#pragma unroll
                 for (int i = 0; i < 3; ++i) t[i] += Kb[3 * i] * u[0] + Kb[3 * i + 1] * u[1] + Kb[3 * i + 2] * u[2];
             }
             f[3 * a] = -t[0];
@@ -113,42 +128,41 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
     // Mass: M_e = R M0 R^T = rho V / 20 (2 on the diagonal blocks, 1 off) x I3.
     const double mc = rho * V / 20.0;
     const double idt = 1.0 / dt;
-    TK* rec = reinterpret_cast<TK*>(A.rec) + (size_t)e * kTetRec;
+    double* rec = A.rec + (size_t)e * kTetRec;
     double* rr = A.rhs_rec + (size_t)e * 12;
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < 4; ++a) {
         double ra[3] = {f[3 * a], f[3 * a + 1], f[3 * a + 2]};
+#pragma unroll
         for (int b = 0; b < 4; ++b) {
-            TK K[9];
-            if (COROT) coro_block<TK>(cf, a, b, K);
-            else linear_block<TK>(g, V, lam, mu, a, b, K);
+            double K[9];
+            if (COROT) coro_block<double>(cf, a, b, K);
+            else linear_block<double>(g, V, lam, mu, a, b, K);
             // rhs = f + K (xn - xs - vn dt) - M (vn - vs) / dt   (Math/OdeSolverEulerImplicit.cpp:157-158)
             const double mab = (a == b) ? 2.0 * mc : mc;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                double kd = (double)K[3 * i] * d1[3 * b] + (double)K[3 * i + 1] * d1[3 * b + 1] +
-                            (double)K[3 * i + 2] * d1[3 * b + 2];
-                ra[i] += kd - (mab * d2[3 * b + i]) / dt;
+                ra[i] += K[3 * i] * d1[3 * b] + K[3 * i + 1] * d1[3 * b + 1] + K[3 * i + 2] * d1[3 * b + 2];
+                if (GENERAL) ra[i] -= (mab * d2[3 * b + i]) / dt;
             }
             if (!A.write_matrix) continue;
             // A_ab = M_ab * (1/dt) + K_ab * dt   (D = 0 for 3D elements, Physics/FemElement.cpp:131-145)
             const double mdt = mab * idt;
             if (a == b) {
-                TK* o = rec + kTetPairSlots + 6 * a;
-                o[0] = (TK)(mdt + (double)K[0] * dt);
-                o[1] = (TK)((double)K[3] * dt);
-                o[2] = (TK)(mdt + (double)K[4] * dt);
-                o[3] = (TK)((double)K[6] * dt);
-                o[4] = (TK)((double)K[7] * dt);
-                o[5] = (TK)(mdt + (double)K[8] * dt);
+                double* o = rec + kTetPairSlots + 6 * a;
+                o[0] = mdt + K[0] * dt;
+                o[1] = K[3] * dt;
+                o[2] = mdt + K[4] * dt;
+                o[3] = K[6] * dt;
+                o[4] = K[7] * dt;
+                o[5] = mdt + K[8] * dt;
             } else if (ids[a] > ids[b]) {
-                int ps = (a < b) ? pair_slot(a, b) : pair_slot(b, a);
-                TK* o = rec + 9 * ps;
+                const int ps = (a < b) ? pair_slot(a, b) : pair_slot(b, a);
+                double* o = rec + 9 * ps;
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        o[3 * i + j] = (TK)(((i == j) ? mdt : 0.0) + (double)K[3 * i + j] * dt);
+                    for (int j = 0; j < 3; ++j) o[3 * i + j] = ((i == j) ? mdt : 0.0) + K[3 * i + j] * dt;
             }
         }
         rr[3 * a] = ra[0];
@@ -157,15 +171,15 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
     }
 }
 
-void launch_tet_elements(int kind, bool fp32, const TetElementArgs& a, hipStream_t s) {
-    dim3 grid((a.num_elements + 255) / 256), block(256);
+void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     if (a.num_elements == 0) return;
+    dim3 grid((a.num_elements + 255) / 256), block(256);
     if (kind == 1) {
-        if (fp32) hipLaunchKernelGGL((tet_element_kernel<float, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((tet_element_kernel<double, true>), grid, block, 0, s, a);
+        if (a.general) hipLaunchKernelGGL((tet_element_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((tet_element_kernel<true, false>), grid, block, 0, s, a);
     } else {
-        if (fp32) hipLaunchKernelGGL((tet_element_kernel<float, false>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((tet_element_kernel<double, false>), grid, block, 0, s, a);
+        if (a.general) hipLaunchKernelGGL((tet_element_kernel<false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((tet_element_kernel<false, false>), grid, block, 0, s, a);
     }
 }
 
@@ -175,40 +189,41 @@ void launch_tet_elements(int kind, bool fp32, const TetElementArgs& a, hipStream
 // diagonal, Math/OdeState.cpp:197-218) and writes the block and its transposed mirror, giving the
 // lower-triangle self-adjoint view that Eigen's CG applies (Math/LinearSparseSolveAndInverse.h:122).
 // ------------------------------------------------------------------------------------------
-template <typename TK>
 __global__ __launch_bounds__(256) void assemble_kernel(int num_lower, const int* __restrict__ lower_blk,
                                                        const int* __restrict__ mirror_blk,
                                                        const int* __restrict__ blk_row,
                                                        const int* __restrict__ blk_col,
                                                        const int* __restrict__ ctr_ptr,
-                                                       const int* __restrict__ ctr_list, const TK* __restrict__ rec,
-                                                       const uint8_t* __restrict__ bcmask, TK* __restrict__ S) {
+                                                       const int* __restrict__ ctr_list,
+                                                       const double* __restrict__ rec,
+                                                       const uint8_t* __restrict__ bcmask, int64_t plane,
+                                                       double* __restrict__ S64, float* __restrict__ S32) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_lower) return;
-    int blk = lower_blk[t];
-    int I = blk_row[blk];
-    int J = blk_col[blk];
+    const int blk = lower_blk[t];
+    const int I = blk_row[blk];
+    const int J = blk_col[blk];
     double acc[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-    int c0 = ctr_ptr[t], c1 = ctr_ptr[t + 1];
+    const int c0 = ctr_ptr[t], c1 = ctr_ptr[t + 1];
     if (I != J) {
         for (int c = c0; c < c1; ++c) {
             int code = ctr_list[c];
-            const TK* r = rec + (size_t)(code >> 4) * kTetRec + 9 * (code & 15);
+            const double* r = rec + (size_t)(code >> 4) * kTetRec + 9 * (code & 15);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k] += (double)r[k];
+            for (int k = 0; k < 9; ++k) acc[k] += r[k];
         }
     } else {
         for (int c = c0; c < c1; ++c) {
             int code = ctr_list[c];
-            const TK* r = rec + (size_t)(code >> 4) * kTetRec + kTetPairSlots + 6 * ((code & 15) - 6);
-            acc[0] += (double)r[0];
-            acc[3] += (double)r[1];
-            acc[4] += (double)r[2];
-            acc[6] += (double)r[3];
-            acc[7] += (double)r[4];
-            acc[8] += (double)r[5];
+            const double* r = rec + (size_t)(code >> 4) * kTetRec + kTetPairSlots + 6 * ((code & 15) - 6);
+            acc[0] += r[0];
+            acc[3] += r[1];
+            acc[4] += r[2];
+            acc[6] += r[3];
+            acc[7] += r[4];
+            acc[8] += r[5];
         }
         acc[1] = acc[3];
         acc[2] = acc[6];
@@ -220,38 +235,28 @@ __global__ __launch_bounds__(256) void assemble_kernel(int num_lower, const int*
         br[k] = bcmask[3 * I + k];
         bcol[k] = bcmask[3 * J + k];
     }
-    TK out[9];
+    const int mb = mirror_blk[t];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             double v = acc[3 * i + j];
             if (br[i] | bcol[j]) v = (I == J && i == j) ? 1.0 : 0.0;
-            out[3 * i + j] = (TK)v;
+            S64[9 * (int64_t)blk + 3 * i + j] = v;
+            if (S32) S32[9 * (int64_t)blk + 3 * i + j] = (float)v;
+            if (I != J) {
+                S64[9 * (int64_t)mb + 3 * j + i] = v;
+                if (S32) S32[9 * (int64_t)mb + 3 * j + i] = (float)v;
+            }
         }
-    TK* o = S + (size_t)blk * 9;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) o[k] = out[k];
-    if (I != J) {
-        TK* om = S + (size_t)mirror_blk[t] * 9;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) om[3 * j + i] = out[3 * i + j];
-    }
 }
 
-void launch_assemble(bool fp32, int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
-                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const void* rec,
-                     const uint8_t* bcmask, void* S, hipStream_t s) {
+void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
+                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
+                     const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s) {
     if (num_lower == 0) return;
-    dim3 grid((num_lower + 255) / 256), block(256);
-    if (fp32)
-        hipLaunchKernelGGL((assemble_kernel<float>), grid, block, 0, s, num_lower, lower_blk, mirror_blk, blk_row,
-                           blk_col, ctr_ptr, ctr_list, (const float*)rec, bcmask, (float*)S);
-    else
-        hipLaunchKernelGGL((assemble_kernel<double>), grid, block, 0, s, num_lower, lower_blk, mirror_blk, blk_row,
-                           blk_col, ctr_ptr, ctr_list, (const double*)rec, bcmask, (double*)S);
+    hipLaunchKernelGGL(assemble_kernel, dim3((num_lower + 255) / 256), dim3(256), 0, s, num_lower, lower_blk,
+                       mirror_blk, blk_row, blk_col, ctr_ptr, ctr_list, rec, bcmask, plane, S64, S32);
 }
 
 // Right-hand side per node: element contributions in element order, gravity m_i g
@@ -288,56 +293,131 @@ void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const 
 }
 
 // ------------------------------------------------------------------------------------------
-// Conjugate gradient (Eigen 3.3 internal::conjugate_gradient recurrence, x0 = 0).
-// Kernels run on a fixed grid of kCgBlocks blocks; dot products are reduced per block into
-// partials that the next kernel reduces in a fixed order (bit-reproducible runs).
+// 3x3-block SpMV core: a group of G lanes per block row; lane l takes blocks rowptr[I] + l, + G, ...
+// (each block is 9 contiguous values, so the group reads one contiguous span of the row); the row sum
+// is a butterfly over the group, so every lane of the group ends with the full sum.
+// fetch(J, x0, x1, x2) supplies the input vector at block column J; epi(I, lane, y0, y1, y2) consumes
+// row I on every lane of its group.
 // ------------------------------------------------------------------------------------------
-template <typename TM, typename TV>
-__global__ __launch_bounds__(kCgThreads) void cg_init_kernel(int num_nodes, const int* __restrict__ diagblk,
-                                                             const TM* __restrict__ S, const TV* __restrict__ b,
-                                                             TV* __restrict__ x, TV* __restrict__ r,
-                                                             TV* __restrict__ p, TV* __restrict__ dinv,
-                                                             double* __restrict__ partial) {
-    __shared__ double lds[64];
-    int n = 3 * num_nodes;
-    double bb = 0.0, rz = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int I = i / 3, c = i - 3 * I;
-        TM s = S[(size_t)diagblk[I] * 9 + 4 * c];
-        // DiagonalPreconditioner::factorize: 1/diag, 1 where the diagonal is zero
-        TV d = (s != TM(0)) ? TV(1) / (TV)s : TV(1);
-        TV bi = b[i];
-        x[i] = TV(0);
-        r[i] = bi;
-        dinv[i] = d;
-        TV pi = d * bi;
-        p[i] = pi;
-        bb += (double)bi * (double)bi;
-        rz += (double)bi * (double)pi;
-    }
-    double s0 = block_sum(bb, lds);
-    double s1 = block_sum(rz, lds);
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s0;
-        partial[kCgBlocks + blockIdx.x] = s1;
+template <typename TM, typename TX, int G, typename Fetch, typename Epi>
+__device__ __forceinline__ void group_spmv(const Operator& op, const TM* __restrict__ S, Fetch fetch, Epi epi) {
+    const int lane = threadIdx.x & (G - 1);
+    const int gpb = blockDim.x / G;                 // groups per block
+    const int gib = threadIdx.x / G;                // group in block
+    // XCD-aware rows: workgroups are dispatched round-robin over the 8 XCDs, so logical block
+    // L = (b % 8) * (grid / 8) + b / 8 gives each XCD one contiguous slab of rows (and of the columns
+    // its gathers touch), keeping those gathers in its own L2.
+    const int nb = gridDim.x;
+    const int L = (nb % kXcds == 0) ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+    const int per = (op.num_nodes + nb - 1) / nb;
+    const int r0 = L * per;
+    const int r1 = min(op.num_nodes, r0 + per);
+    for (int I = r0 + gib; I < r1; I += gpb) {
+        const int k0 = op.rowptr[I], k1 = op.rowptr[I + 1];
+        TX y0 = 0, y1 = 0, y2 = 0;
+        for (int k = k0 + lane; k < k1; k += G) {
+            const int J = op.colidx[k];
+            TX p0, p1, p2;
+            fetch(J, p0, p1, p2);
+            const TM* B = S + 9 * (int64_t)k;
+            y0 += (TX)B[0] * p0 + (TX)B[1] * p1 + (TX)B[2] * p2;
+            y1 += (TX)B[3] * p0 + (TX)B[4] * p1 + (TX)B[5] * p2;
+            y2 += (TX)B[6] * p0 + (TX)B[7] * p1 + (TX)B[8] * p2;
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            y0 += __shfl_xor(y0, o, G);
+            y1 += __shfl_xor(y1, o, G);
+            y2 += __shfl_xor(y2, o, G);
+        }
+        epi(I, lane, y0, y1, y2);
     }
 }
 
-__global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __restrict__ partial, CgScalars* sc,
-                                                              double tol) {
+// ------------------------------------------------------------------------------------------
+// Conjugate gradient (Eigen 3.3 internal::conjugate_gradient recurrence, x0 = 0), two kernels per
+// iteration on a fixed grid of kCgBlocks x kCgThreads (full occupancy). Each kernel publishes its
+// block sums; the next kernel reduces them in block order in every block (deterministic, no atomics)
+// and block 0 records the scalars that later kernels need.
+// Partial slots: [0, B) |r|^2 or |b|^2, [B, 2B) r.z, [2B, 3B) p.q, [3B, 4B) FP64 residual.
+// ------------------------------------------------------------------------------------------
+// Two sums of kCgBlocks partials each, formed by wave 0 in a fixed order and broadcast through LDS.
+__device__ __forceinline__ void reduce_partials2(const double* pa, const double* pb, double& a, double& b,
+                                                 double* lds) {
+    if (threadIdx.x < 64) {
+        double va = 0.0, vb = 0.0;
+        for (int i = threadIdx.x; i < kCgBlocks; i += 64) {
+            va += pa[i];
+            if (pb) vb += pb[i];
+        }
+        va = wave_sum(va);
+        vb = wave_sum(vb);
+        if (threadIdx.x == 0) {
+            lds[62] = va;
+            lds[63] = vb;
+        }
+    }
+    __syncthreads();
+    a = lds[62];
+    b = lds[63];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double reduce_partials(const double* part, double* lds) {
+    double a, b;
+    reduce_partials2(part, nullptr, a, b, lds);
+    return a;
+}
+
+// x = 0, r = (TV) b, dinv = 1/diag (1 where the diagonal is 0, DiagonalPreconditioner), z = dinv r;
+// partial |r|^2 and r.z. Skipped when skip && skip->done (FP64 refinement already converged).
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) void cg_init_kernel(Operator op, const TV* __restrict__ S,
+                                                             const double* __restrict__ b, TV* __restrict__ x,
+                                                             TV* __restrict__ r, TV* __restrict__ z,
+                                                             TV* __restrict__ dinv, double* __restrict__ partial,
+                                                             const CgScalars* __restrict__ skip) {
     __shared__ double lds[64];
-    double bb = reduce_partials(partial, lds);
-    double rz = reduce_partials(partial + kCgBlocks, lds);
+    if (skip && skip->done) return;
+    const int n = 3 * op.num_nodes;
+    double bb = 0.0, rz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int I = i / 3, c = i - 3 * I;
+        const TV s = S[9 * (int64_t)op.diagblk[I] + 4 * c];
+        const TV d = (s != TV(0)) ? TV(1) / s : TV(1);
+        const TV bi = (TV)b[i];
+        x[i] = TV(0);
+        r[i] = bi;
+        dinv[i] = d;
+        const TV zi = d * bi;
+        z[i] = zi;
+        bb += (double)bi * (double)bi;
+        rz += (double)bi * (double)zi;
+    }
+    block_sum2(bb, rz, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bb;
+        partial[kCgBlocks + blockIdx.x] = rz;
+    }
+}
+
+// |b|^2 and r.z -> threshold max(tol^2 |b|^2, realmin) and Eigen's early exits.
+__global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __restrict__ partial, CgScalars* sc,
+                                                              const CgScalars* __restrict__ skip, double tol) {
+    __shared__ double lds[64];
+    if (skip && skip->done) return;
+    double bb, rz;
+    reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
     if (threadIdx.x == 0) {
         sc->bb = bb;
         sc->rz[0] = rz;
         sc->rz[1] = 0.0;
         sc->threshold = fmax(tol * tol * bb, DBL_MIN);
+        sc->rr = bb;
         sc->iters = 0;
-        sc->last_rr = bb;
-        sc->done = 0;
-        sc->converged = 0;
         sc->err = 0.0;
+        sc->converged = 0;
+        sc->done = 0;
         if (bb == 0.0) {
             sc->done = 1;
             sc->converged = 1;
@@ -349,203 +429,177 @@ __global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __re
     }
 }
 
-// q = S p (one group of G lanes per block row) and partial p . q.
-template <typename TM, typename TV, int G>
-__global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(int num_nodes, const int* __restrict__ rowptr,
-                                                             const int* __restrict__ colidx,
-                                                             const TM* __restrict__ S, const TV* __restrict__ p,
-                                                             TV* __restrict__ q, double* __restrict__ partial,
-                                                             const CgScalars* __restrict__ sc) {
+// Iteration k, part 1. k > 0: |r|^2 of iteration k-1 below the threshold ends the solve with
+// iters = k - 1 (Eigen breaks before incrementing), k == max_iter ends it unconverged; otherwise
+// beta = rz_k / rz_{k-1}. Then p_k = z + beta p_{k-1}, formed on the fly for every column the row group
+// needs and stored for its own row; q = S p_k; partial p_k . q.
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) void cg_apply_kernel(Operator op, const TV* __restrict__ S, int k,
+                                                              int max_iter, const TV* __restrict__ z,
+                                                              const TV* __restrict__ p_old, TV* __restrict__ p_new,
+                                                              TV* __restrict__ q, double* __restrict__ partial,
+                                                              CgScalars* __restrict__ sc) {
     __shared__ double lds[64];
     if (sc->done) return;
-    const int lane = threadIdx.x & (G - 1);
-    const int group = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int ngroups = gridDim.x * blockDim.x / G;
-    double acc = 0.0;
-    for (int I = group; I < num_nodes; I += ngroups) {
-        TV y0 = 0, y1 = 0, y2 = 0;
-        int k1 = rowptr[I + 1];
-        for (int k = rowptr[I] + lane; k < k1; k += G) {
-            int J = colidx[k];
-            const TM* B = S + (size_t)k * 9;
-            TV p0 = p[3 * J], p1 = p[3 * J + 1], p2 = p[3 * J + 2];
-            y0 += (TV)B[0] * p0 + (TV)B[1] * p1 + (TV)B[2] * p2;
-            y1 += (TV)B[3] * p0 + (TV)B[4] * p1 + (TV)B[5] * p2;
-            y2 += (TV)B[6] * p0 + (TV)B[7] * p1 + (TV)B[8] * p2;
+    double beta = 0.0;
+    if (k > 0) {
+        double rr, rzk;
+        reduce_partials2(partial, partial + kCgBlocks, rr, rzk, lds);
+        const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+        if (rr < sc->threshold || k >= max_iter) {
+            if (first) {
+                const bool conv = rr < sc->threshold;
+                sc->rr = rr;
+                sc->err = sqrt(rr / sc->bb);
+                sc->iters = conv ? k - 1 : k;
+                sc->converged = conv ? 1 : 0;
+                sc->done = 1;
+            }
+            return;
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y0 += __shfl_xor(y0, o, G);
-            y1 += __shfl_xor(y1, o, G);
-            y2 += __shfl_xor(y2, o, G);
-        }
-        if (lane == 0) {
-            q[3 * I] = y0;
-            q[3 * I + 1] = y1;
-            q[3 * I + 2] = y2;
-            acc += (double)p[3 * I] * (double)y0 + (double)p[3 * I + 1] * (double)y1 + (double)p[3 * I + 2] * (double)y2;
+        beta = rzk / sc->rz[(k - 1) & 1];
+        if (first) {
+            sc->rz[k & 1] = rzk;
+            sc->rr = rr;
         }
     }
-    double s = block_sum(acc, lds);
+    const TV bt = (TV)beta;
+    const bool k0 = (k == 0);
+    double acc = 0.0;
+    group_spmv<TV, TV, kRowGroup>(
+        op, S,
+        [&](int J, TV& a0, TV& a1, TV& a2) {
+            if (k0) {
+                a0 = z[3 * J];
+                a1 = z[3 * J + 1];
+                a2 = z[3 * J + 2];
+            } else {
+                a0 = z[3 * J] + bt * p_old[3 * J];
+                a1 = z[3 * J + 1] + bt * p_old[3 * J + 1];
+                a2 = z[3 * J + 2] + bt * p_old[3 * J + 2];
+            }
+        },
+        [&](int I, int lane, TV y0, TV y1, TV y2) {
+            if (lane < 3) {
+                const int i = 3 * I + lane;
+                const TV y = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+                const TV a = k0 ? z[i] : z[i] + bt * p_old[i];
+                p_new[i] = a;
+                q[i] = y;
+                acc += (double)a * (double)y;
+            }
+        });
+    const double s = block_sum(acc, lds);
     if (threadIdx.x == 0) partial[2 * kCgBlocks + blockIdx.x] = s;
 }
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; partial r.r and r.z (z = dinv r).
+// Iteration k, part 2: alpha = rz_k / (p . q); x += alpha p; r -= alpha q; z = dinv r; partials.
 template <typename TV>
-__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(int n, int k, TV* __restrict__ x, TV* __restrict__ r,
-                                                               const TV* __restrict__ p, const TV* __restrict__ q,
+__global__ __launch_bounds__(kCgThreads) void cg_vector_kernel(int n, int k, TV* __restrict__ x, TV* __restrict__ r,
+                                                               TV* __restrict__ z, const TV* __restrict__ p,
+                                                               const TV* __restrict__ q,
                                                                const TV* __restrict__ dinv,
                                                                double* __restrict__ partial,
                                                                const CgScalars* __restrict__ sc) {
     __shared__ double lds[64];
     if (sc->done) return;
-    double pq = reduce_partials(partial + 2 * kCgBlocks, lds);
-    double alpha = sc->rz[k & 1] / pq;
-    TV a = (TV)alpha;
+    const double pq = reduce_partials(partial + 2 * kCgBlocks, lds);
+    const TV a = (TV)(sc->rz[k & 1] / pq);
     double rr = 0.0, rz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += a * p[i];
-        TV ri = r[i] - a * q[i];
+        const TV ri = r[i] - a * q[i];
         r[i] = ri;
+        const TV zi = dinv[i] * ri;
+        z[i] = zi;
         rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)(dinv[i] * ri);
+        rz += (double)ri * (double)zi;
     }
-    double s0 = block_sum(rr, lds);
-    double s1 = block_sum(rz, lds);
+    block_sum2(rr, rz, lds);
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s0;
-        partial[kCgBlocks + blockIdx.x] = s1;
+        partial[blockIdx.x] = rr;
+        partial[kCgBlocks + blockIdx.x] = rz;
     }
 }
 
-// Convergence test on |r|^2, then beta = rz_new / rz; p = z + beta p.
-template <typename TV>
-__global__ __launch_bounds__(kCgThreads) void cg_pupdate_kernel(int n, int k, int max_iter, const TV* __restrict__ r,
-                                                                TV* __restrict__ p, const TV* __restrict__ dinv,
-                                                                const double* __restrict__ partial,
-                                                                CgScalars* __restrict__ sc) {
-    __shared__ double lds[64];
-    if (sc->done) return;
-    double rr = reduce_partials(partial, lds);
-    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-    if (rr < sc->threshold) {
-        if (first) {
-            sc->last_rr = rr;
-            sc->err = sqrt(rr / sc->bb);
-            sc->iters = k;
-            sc->converged = 1;
-            sc->done = 1;
-        }
-        return;
-    }
-    double rzn = reduce_partials(partial + kCgBlocks, lds);
-    double beta = rzn / sc->rz[k & 1];
-    TV bt = (TV)beta;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = dinv[i] * r[i] + bt * p[i];
-    if (first) {
-        sc->rz[(k + 1) & 1] = rzn;
-        sc->last_rr = rr;
-        if (k + 1 >= max_iter) {
-            sc->err = sqrt(rr / sc->bb);
-            sc->iters = k + 1;
-            sc->converged = 0;
-            sc->done = 1;
-        }
-    }
-}
-
-template <typename TM, typename TV>
-static void cg_iteration_t(const CgArgs& a, int k, int max_iter, hipStream_t s) {
-    dim3 grid(kCgBlocks), block(kCgThreads);
-    hipLaunchKernelGGL((cg_spmv_kernel<TM, TV, 4>), grid, block, 0, s, a.num_nodes, a.rowptr, a.colidx,
-                       (const TM*)a.S, (const TV*)a.p, (TV*)a.q, a.partial, a.sc);
-    hipLaunchKernelGGL((cg_update_kernel<TV>), grid, block, 0, s, a.n, k, (TV*)a.x, (TV*)a.r, (const TV*)a.p,
-                       (const TV*)a.q, (const TV*)a.dinv, a.partial, a.sc);
-    hipLaunchKernelGGL((cg_pupdate_kernel<TV>), grid, block, 0, s, a.n, k, max_iter, (const TV*)a.r, (TV*)a.p,
-                       (const TV*)a.dinv, a.partial, a.sc);
-}
-
-void launch_cg_iteration(const CgArgs& a, int k, int max_iter, hipStream_t s) {
-    if (a.fp32) cg_iteration_t<float, float>(a, k, max_iter, s);
-    else cg_iteration_t<double, double>(a, k, max_iter, s);
-}
-
-void launch_cg_init(const CgArgs& a, hipStream_t s) {
+void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {
     dim3 grid(kCgBlocks), block(kCgThreads);
     if (a.fp32)
-        hipLaunchKernelGGL((cg_init_kernel<float, float>), grid, block, 0, s, a.num_nodes, a.diagblk,
-                           (const float*)a.S, (const float*)a.b, (float*)a.x, (float*)a.r, (float*)a.p,
-                           (float*)a.dinv, a.partial);
+        hipLaunchKernelGGL((cg_init_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, b, (float*)a.x,
+                           (float*)a.r, (float*)a.z, (float*)a.dinv, a.partial, skip);
     else
-        hipLaunchKernelGGL((cg_init_kernel<double, double>), grid, block, 0, s, a.num_nodes, a.diagblk,
-                           (const double*)a.S, (const double*)a.b, (double*)a.x, (double*)a.r, (double*)a.p,
-                           (double*)a.dinv, a.partial);
-    hipLaunchKernelGGL(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, a.tol);
+        hipLaunchKernelGGL((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
+                           (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial, skip);
+    hipLaunchKernelGGL(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
+}
+
+void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
+    dim3 grid(kCgBlocks), block(kCgThreads);
+    void* pn = a.p[k & 1];
+    const void* po = a.p[(k + 1) & 1];
+    if (a.fp32)
+        hipLaunchKernelGGL((cg_apply_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, k, max_iter,
+                           (const float*)a.z, (const float*)po, (float*)pn, (float*)a.q, a.partial, a.sc);
+    else
+        hipLaunchKernelGGL((cg_apply_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
+                           (const double*)a.z, (const double*)po, (double*)pn, (double*)a.q, a.partial, a.sc);
+}
+
+void launch_cg_vector(const CgArgs& a, int k, hipStream_t s) {
+    dim3 grid(kCgBlocks), block(kCgThreads);
+    const void* p = a.p[k & 1];
+    if (a.fp32)
+        hipLaunchKernelGGL((cg_vector_kernel<float>), grid, block, 0, s, a.n, k, (float*)a.x, (float*)a.r,
+                           (float*)a.z, (const float*)p, (const float*)a.q, (const float*)a.dinv, a.partial, a.sc);
+    else
+        hipLaunchKernelGGL((cg_vector_kernel<double>), grid, block, 0, s, a.n, k, (double*)a.x, (double*)a.r,
+                           (double*)a.z, (const double*)p, (const double*)a.q, (const double*)a.dinv, a.partial,
+                           a.sc);
 }
 
 // ------------------------------------------------------------------------------------------
-// FP64 iterative refinement around the FP32 CG.
+// FP64 iterative refinement around the FP32 CG: r64 = b - S64 dv in FP64 (the first sweep takes
+// r64 = b) with partial |r64|^2; the check kernel then applies the outer test
+// |r64|^2 < max(tol^2 |b|^2, realmin) and marks the inner solve done when it passes.
 // ------------------------------------------------------------------------------------------
-// r64 = b - S dv with S in FP32 promoted to FP64; partial |r64|^2 into partial[3 * kCgBlocks + blk].
-template <int G>
-__global__ __launch_bounds__(kCgThreads) void residual64_kernel(int num_nodes, const int* __restrict__ rowptr,
-                                                                const int* __restrict__ colidx,
-                                                                const float* __restrict__ S,
+__global__ __launch_bounds__(kCgThreads) void residual64_kernel(Operator op, const double* __restrict__ S,
                                                                 const double* __restrict__ b,
-                                                                const double* __restrict__ dv,
-                                                                double* __restrict__ r64, double* __restrict__ partial) {
+                                                                const double* __restrict__ dv, int first,
+                                                                double* __restrict__ r64,
+                                                                double* __restrict__ partial) {
     __shared__ double lds[64];
-    const int lane = threadIdx.x & (G - 1);
-    const int group = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int ngroups = gridDim.x * blockDim.x / G;
     double acc = 0.0;
-    for (int I = group; I < num_nodes; I += ngroups) {
-        double y0 = 0, y1 = 0, y2 = 0;
-        int k1 = rowptr[I + 1];
-        for (int k = rowptr[I] + lane; k < k1; k += G) {
-            int J = colidx[k];
-            const float* B = S + (size_t)k * 9;
-            double p0 = dv[3 * J], p1 = dv[3 * J + 1], p2 = dv[3 * J + 2];
-            y0 += (double)B[0] * p0 + (double)B[1] * p1 + (double)B[2] * p2;
-            y1 += (double)B[3] * p0 + (double)B[4] * p1 + (double)B[5] * p2;
-            y2 += (double)B[6] * p0 + (double)B[7] * p1 + (double)B[8] * p2;
+    if (first) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * op.num_nodes; i += gridDim.x * blockDim.x) {
+            const double e = b[i];
+            r64[i] = e;
+            acc += e * e;
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y0 += __shfl_xor(y0, o, G);
-            y1 += __shfl_xor(y1, o, G);
-            y2 += __shfl_xor(y2, o, G);
-        }
-        if (lane == 0) {
-            double r0 = b[3 * I] - y0, r1 = b[3 * I + 1] - y1, r2 = b[3 * I + 2] - y2;
-            r64[3 * I] = r0;
-            r64[3 * I + 1] = r1;
-            r64[3 * I + 2] = r2;
-            acc += r0 * r0 + r1 * r1 + r2 * r2;
-        }
+    } else {
+        group_spmv<double, double, kRowGroup>(
+            op, S,
+            [&](int J, double& a0, double& a1, double& a2) {
+                a0 = dv[3 * J];
+                a1 = dv[3 * J + 1];
+                a2 = dv[3 * J + 2];
+            },
+            [&](int I, int lane, double y0, double y1, double y2) {
+                if (lane < 3) {
+                    const int i = 3 * I + lane;
+                    const double e = b[i] - (lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+                    r64[i] = e;
+                    acc += e * e;
+                }
+            });
     }
-    double s = block_sum(acc, lds);
+    const double s = block_sum(acc, lds);
     if (threadIdx.x == 0) partial[3 * kCgBlocks + blockIdx.x] = s;
 }
 
-// |b|^2 partials for the first sweep.
-__global__ __launch_bounds__(kCgThreads) void norm2_kernel(int n, const double* __restrict__ v,
-                                                           double* __restrict__ partial) {
-    __shared__ double lds[64];
-    double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += v[i] * v[i];
-    double s = block_sum(acc, lds);
-    if (threadIdx.x == 0) partial[3 * kCgBlocks + blockIdx.x] = s;
-}
-
-// Outer test and the inner right-hand side. outer->bb holds |b|^2 after the first sweep; outer->done
-// is set once |b - S dv|^2 < max(tol^2 |b|^2, realmin). The inner CG's scalars are marked done as well
-// so that its kernels skip.
 __global__ __launch_bounds__(kCgThreads) void refine_check_kernel(const double* __restrict__ partial, int first,
                                                                   double tol, CgScalars* outer, CgScalars* inner) {
     __shared__ double lds[64];
-    double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
+    const double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
     if (threadIdx.x == 0) {
         if (first) {
             outer->bb = rr;
@@ -554,23 +608,14 @@ __global__ __launch_bounds__(kCgThreads) void refine_check_kernel(const double* 
             outer->done = 0;
             outer->converged = 0;
         }
-        outer->last_rr = rr;
+        outer->rr = rr;
         outer->err = outer->bb > 0 ? sqrt(rr / outer->bb) : 0.0;
         if (outer->bb == 0.0 || rr < outer->threshold) {
             outer->done = 1;
             outer->converged = 1;
             inner->done = 1;
-        } else {
-            inner->done = 0;
         }
     }
-}
-
-__global__ __launch_bounds__(kCgThreads) void to_float_kernel(int n, const double* __restrict__ src,
-                                                              float* __restrict__ dst,
-                                                              const CgScalars* __restrict__ outer) {
-    if (outer->done) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = (float)src[i];
 }
 
 __global__ __launch_bounds__(kCgThreads) void accumulate_kernel(int n, const float* __restrict__ xf,
@@ -580,19 +625,12 @@ __global__ __launch_bounds__(kCgThreads) void accumulate_kernel(int n, const flo
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dv[i] += (double)xf[i];
 }
 
-void launch_residual64(const CgArgs& a, const double* b, const double* dv, double* r64, hipStream_t s) {
-    if (dv == nullptr) {
-        hipLaunchKernelGGL(norm2_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.n, b, a.partial);
-        return;
-    }
-    hipLaunchKernelGGL((residual64_kernel<4>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.num_nodes, a.rowptr,
-                       a.colidx, (const float*)a.S, b, dv, r64, a.partial);
-}
-
-void launch_refine_prepare(const CgArgs& a, const double* r64, int first, double tol, CgScalars* outer,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, first, tol, outer, a.sc);
-    hipLaunchKernelGGL(to_float_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.n, r64, (float*)a.b, outer);
+void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
+                       double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(residual64_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, op, S64, b, dv, first, r64,
+                       partial);
+    hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, first, tol, outer, inner);
 }
 
 void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s) {

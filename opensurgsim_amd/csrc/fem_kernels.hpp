@@ -6,22 +6,25 @@
 
 namespace femgpu {
 
-// Per-element record written by the element kernels and read by the assembly kernel:
+// Per-element record written by the tet element kernel and read by the assembly kernel (FP64):
 // slots [0, 54): the 6 node-pair blocks (pair order (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) of
 // A_e = M_e / dt + dt K_e, each oriented so that its row node has the larger global id, row-major;
 // slots [54, 78): the lower triangles (00,10,11,20,21,22) of the 4 diagonal blocks.
-constexpr int kTetRec = 80;   // padded
+constexpr int kTetRec = 80;   // padded to a multiple of 16 bytes
 constexpr int kTetPairSlots = 54;
 
-// Number of blocks of the fixed CG grid (partials per reduction).
+// CG launch geometry: a fixed grid of kCgBlocks blocks of kCgThreads threads (8 waves per SIMD on
+// 256 CUs); grid-stride loops cover any size, and every grid reduction has kCgBlocks partials.
 constexpr int kCgBlocks = 512;
-constexpr int kCgThreads = 256;
+constexpr int kCgThreads = 1024;
+constexpr int kRowGroup = 8;      // lanes per block row in the SpMV
+constexpr int kXcds = 8;          // accelerator complex dies per MI355X
 
 struct CgScalars {
     double rz[2];       // r . z, ping-ponged by iteration parity
     double bb;          // |b|^2
     double threshold;   // max(tol^2 |b|^2, realmin)
-    double last_rr;     // |r|^2 at the last convergence test
+    double rr;          // |r|^2 at the last test
     double err;         // sqrt(|r|^2 / |b|^2) at exit
     int done;           // solve finished (converged or capped)
     int converged;
@@ -37,60 +40,69 @@ struct TetElementArgs {
     const double* vn;
     const double* xs;     // start-of-step state
     const double* vs;
-    const double4* mat;   // lambda, mu, rho, unused
+    const double4* mat;   // lambda, mu, rho, E
     double dt;
     int write_matrix;     // 0: only the rhs record
-    void* rec;            // TK[num_elements][kTetRec]
-    double* rhs_rec;      // double[num_elements][12]
+    int general;          // 0: xn == xs and vn == vs (first Newton iteration)
+    double* rec;          // [num_elements][kTetRec]
+    double* rhs_rec;      // [num_elements][12]
     int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
 };
 
-void launch_tet_elements(int kind, bool fp32, const TetElementArgs& a, hipStream_t s);
+void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s);
 
-// Assembles the symmetric (lower-mirrored) BSR operator from the element records.
-void launch_assemble(bool fp32, int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
-                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const void* rec,
-                     const uint8_t* bcmask, void* S, hipStream_t s);
+// The operator: 3x3-block CSR over the symmetric pattern; block k holds its 9 values row-major at
+// [9 k, 9 k + 9).
+struct Operator {
+    int num_nodes;
+    int64_t plane;       // number of blocks
+    const int* rowptr;   // block CSR
+    const int* colidx;
+    const int* diagblk;
+};
+
+// Gather assembly from the element records into the FP64 operator (and the FP32 copy when S32 != null).
+void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
+                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
+                     const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s);
 
 void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const double* rhs_rec, const double* fg,
                 const double* fext, const uint8_t* bcmask, double* b, hipStream_t s);
 
-// CG on the BSR operator.
+// Jacobi-preconditioned CG (Eigen recurrence) on the operator; two kernels per iteration.
 struct CgArgs {
-    int num_nodes;
+    Operator op;
     int n;              // 3 * num_nodes
-    const int* rowptr;  // block CSR
-    const int* colidx;
-    const int* diagblk;
-    const void* S;      // TM[9 * nblocks]
-    bool fp32;          // TM = TV = float, FP64 reductions
-    void* x;            // TV
+    const void* S;      // TV operator values
+    bool fp32;          // TV = float (FP64 reductions)
+    void* x;
     void* r;
-    void* p;
+    void* z;
+    void* p[2];         // p of even / odd iterations
     void* q;
     void* dinv;
-    const void* b;      // TV right-hand side
     double* partial;    // [4 * kCgBlocks]
     CgScalars* sc;
     double tol;
 };
-void launch_cg_init(const CgArgs& a, hipStream_t s);
-void launch_cg_iteration(const CgArgs& a, int k, int max_iter, hipStream_t s);
+// x = 0, r = (TV) b, z = dinv r and the solve's scalars; skipped when skip && skip->done.
+void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);
+// Iteration k: convergence test of k - 1, p = z + beta p, q = S p, p . q.
+void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s);
+// Iteration k: alpha, x += alpha p, r -= alpha q, z = dinv r, r . r and r . z.
+void launch_cg_vector(const CgArgs& a, int k, hipStream_t s);
 
-// FP64 refinement helpers (FP32 mode).
-// r64 = b - S dv (S in float, FP64 arithmetic); partial |r64|^2.
-void launch_residual64(const CgArgs& a, const double* b, const double* dv, double* r64, hipStream_t s);
-// Sets up an inner solve: rf = (float) r64 (sweep > 0) or (float) b (sweep 0); outer scalars.
-void launch_refine_prepare(const CgArgs& a, const double* r64, int first, double tol, CgScalars* outer,
-                           hipStream_t s);
-// dv += (double) xf
+// FP64 iterative refinement around the FP32 CG: r64 = b - S64 dv (first: r64 = b) and the outer test.
+void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
+                       double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
+                       hipStream_t s);
 void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s);
 
 // State update: v += dv; x = xs + dt v (Math/OdeSolverEulerImplicit.cpp:98-99). Sets *invalid on non-finite.
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
                    hipStream_t s);
 
-// Full element outputs for tests: R (9), K ((3nn)^2) row-major, f (3nn); double.
+// Full element outputs for tests: R (9), K ((3nn)^2) row-major, f (3nn); FP64.
 void launch_eval_elements(int kind, int first, int count, const int4* conn, const double* x0, const double* x,
                           const double4* mat, double* R, double* K, double* f, hipStream_t s);
 

@@ -23,8 +23,11 @@ EXPORTS = [
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
     "fem_set_options", "fem_set_external_force", "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
-    "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements",
+    "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
+    "fem_get_phase_times",
 ]
+
+PHASES = ["element", "assemble", "rhs", "cg_init", "spmv", "cg_vector", "refine", "update", "spmv_skipped"]
 
 
 class FemOptions(ctypes.Structure):
@@ -59,6 +62,10 @@ class FemStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FemPhaseTimes(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 9), ("launches", ctypes.c_int64 * 9)]
 
 
 _lib = None
@@ -102,6 +109,8 @@ def load():
     L.fem_get_system_matrix.argtypes = [_vp, _dp]
     L.fem_get_rhs.argtypes = [_vp, _dp]
     L.fem_evaluate_elements.argtypes = [_vp, _dp, _i64, _i64, _dp, _dp, _dp]
+    L.fem_enable_phase_timers.argtypes = [_vp, ctypes.c_int]
+    L.fem_get_phase_times.argtypes = [_vp, ctypes.POINTER(FemPhaseTimes)]
     _lib = L
     return L
 
@@ -222,6 +231,15 @@ class FemGpu:
         s = FemStats()
         self._check(self._L.fem_query(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def enable_phase_timers(self, every=1):
+        """Bracket every kernel of one step in `every` with HIP events (0 turns them off)."""
+        self._check(self._L.fem_enable_phase_timers(self.h, int(every)))
+
+    def phase_times(self):
+        t = FemPhaseTimes()
+        self._check(self._L.fem_get_phase_times(self.h, ctypes.byref(t)))
+        return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(PHASES)}
 
     def pattern(self):
         nnz = self._L.fem_num_nonzeros(self.h)
