@@ -1,0 +1,517 @@
+"""Pins the FP64 CPU oracle (oracle/fem_oracle.cpp) against the reference's own known-answer tests,
+closed forms, invariants and finite differences (CPU only).
+
+Each test names the reference test or source lines it restates (paths relative to the reference root).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from opensurgsim_amd import meshgen as mg
+
+
+def isapprox(a, b, prec=1e-12):
+    """Eigen's isApprox: |a - b| <= prec * min(|a|, |b|) (Frobenius)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.linalg.norm(a - b) <= prec * min(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def rotation(angle, axis):
+    axis = np.asarray(axis, dtype=float)
+    axis = axis / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+# ------------------------------------------------------------------------------------------
+# Linear tetrahedron (Physics/UnitTests/Fem3DElementTetrahedronTests.cpp:109-248)
+# ------------------------------------------------------------------------------------------
+TET_X0 = np.array([[0.1, 1.2, 2.3], [1.1, 1.2, 2.3], [0.1, 2.2, 2.3], [0.1, 1.2, 3.3]])
+RHO, E, NU = 1000.0, 1e6, 0.45
+
+
+def expected_tet_K_from_BE():
+    """K = V B^T E B with the hand-derived B of the reference test (:169-209)."""
+    V = 1.0 / 6.0
+    B = np.zeros((6, 12))
+    B[0, 0] = -1; B[0, 3] = 1; B[1, 1] = -1; B[1, 7] = 1; B[2, 2] = -1; B[2, 11] = 1
+    B[3, 0] = -1; B[3, 1] = -1; B[3, 4] = 1; B[3, 6] = 1
+    B[4, 1] = -1; B[4, 2] = -1; B[4, 8] = 1; B[4, 10] = 1
+    B[5, 0] = -1; B[5, 2] = -1; B[5, 5] = 1; B[5, 9] = 1
+    B *= 1.0 / (6.0 * V)
+    Em = np.zeros((6, 6))
+    Em[:3, :3] = NU
+    np.fill_diagonal(Em[:3, :3], 1.0 - NU)
+    np.fill_diagonal(Em[3:, 3:], 0.5 - NU)
+    Em *= E / ((1.0 + NU) * (1.0 - 2.0 * NU))
+    return V * B.T @ Em @ B
+
+
+def expected_tet_K_felippa():
+    """Felippa's explicit matrix (AFEM Ch. 9), as restated by the reference test (:212-247)."""
+    Ec = E / (12.0 * (1.0 - 2.0 * NU) * (1.0 + NU))
+    n0 = 1.0 - 2.0 * NU
+    n1 = 1.0 - NU
+    K = np.zeros((12, 12))
+    K[0, 1] = K[0, 2] = K[1, 2] = 1.0
+    K[0, 3] = -2.0 * n1; K[0, 4] = -n0; K[0, 5] = -n0
+    K[1, 3] = -2.0 * NU; K[1, 4] = -n0
+    K[2, 3] = -2.0 * NU; K[2, 5] = -n0
+    K[0, 6] = -n0; K[0, 7] = -2.0 * NU
+    K[1, 6] = -n0; K[1, 7] = -2.0 * n1; K[1, 8] = -n0
+    K[2, 7] = -2.0 * NU; K[2, 8] = -n0
+    K[0, 9] = -n0; K[0, 11] = -2.0 * NU
+    K[1, 10] = -n0; K[1, 11] = -2.0 * NU
+    K[2, 9] = -n0; K[2, 10] = -n0; K[2, 11] = -2.0 * n1
+    K[3, 7] = K[3, 11] = 2.0 * NU
+    K[4, 6] = n0
+    K[5, 9] = n0
+    K[7, 11] = 2.0 * NU
+    K[8, 10] = n0
+    K = K + K.T
+    for i in range(3):
+        K[i, i] = 4.0 - 6.0 * NU
+    for i in range(3, 12):
+        K[i, i] = n0
+    K[3, 3] = K[7, 7] = K[11, 11] = 2.0 * n1
+    return K * Ec
+
+
+def test_tet_stiffness_matches_both_closed_forms(oracle_lib):
+    K, M, vol = oracle.tet_linear_km(TET_X0, E, NU, RHO)
+    assert vol == pytest.approx(1.0 / 6.0, rel=1e-14)
+    assert isapprox(K, expected_tet_K_from_BE())
+    assert isapprox(K, expected_tet_K_felippa())
+    assert np.array_equal(K, K.T)  # symmetrized at :215
+
+
+def test_tet_mass_closed_form(oracle_lib):
+    _, M, vol = oracle.tet_linear_km(TET_X0, E, NU, RHO)
+    Me = np.zeros((12, 12))
+    for a in range(4):
+        for b in range(4):
+            Me[3 * a:3 * a + 3, 3 * b:3 * b + 3] = np.eye(3) * (2.0 if a == b else 1.0)
+    Me *= RHO * (1.0 / 6.0) / 20.0
+    assert isapprox(M, Me)
+
+
+def test_tet_stiffness_rigid_modes_and_rest_force(oracle_lib):
+    K, _, _ = oracle.tet_linear_km(TET_X0, E, NU, RHO)
+    for c in range(3):
+        t = np.zeros(12)
+        t[c::3] = 1.0
+        assert np.abs(K @ t).max() < 1e-9 * np.abs(K).max()
+    # infinitesimal rotations about the origin
+    for w in np.eye(3):
+        u = np.cross(np.broadcast_to(w, (4, 3)), TET_X0).reshape(-1)
+        assert np.abs(K @ u).max() < 1e-9 * np.abs(K).max()
+
+
+def test_material_asserts(oracle_lib):
+    # Physics/FemElement.cpp:36-42
+    for E_, nu_, rho_ in [(0.0, 0.3, 1.0), (1.0, 0.0, 1.0), (1.0, 0.3, 0.0), (-1.0, 0.3, 1.0), (1.0, 0.5, 1.0),
+                          (1.0, 0.3, -1.0)]:
+        with pytest.raises(RuntimeError):
+            oracle.tet_linear_km(TET_X0, E_, nu_, rho_)
+
+
+# ------------------------------------------------------------------------------------------
+# Corotational tetrahedron (Physics/UnitTests/Fem3DElementCorotationalTetrahedronTests.cpp:106-336)
+# ------------------------------------------------------------------------------------------
+CORO_X0 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+CORO_ROT = rotation(4.1415, [1.0, 0.2, -0.3])
+CORO_T = np.array([1.2, 2.3, 3.4])
+
+
+def blockdiag4(R):
+    R12 = np.zeros((12, 12))
+    for a in range(4):
+        R12[3 * a:3 * a + 3, 3 * a:3 * a + 3] = R
+    return R12
+
+
+def test_coro_rest_and_rigid_motions(oracle_lib):
+    K0, M0, _ = oracle.tet_linear_km(CORO_X0, E, NU, RHO)
+    R12 = blockdiag4(CORO_ROT)
+    cases = [
+        ("none", np.eye(3), np.zeros(3), True),
+        ("translation", np.eye(3), CORO_T, True),
+        ("rotation", CORO_ROT, np.zeros(3), True),
+        ("rotation+translation", CORO_ROT, CORO_T, True),
+        ("rotation*scaling+translation", CORO_ROT * 45.3, CORO_T, False),
+    ]
+    for name, L, t, rigid in cases:
+        x = CORO_X0 @ L.T + t
+        R, M, K, f, flags = oracle.coro_element(CORO_X0, x, E, NU, RHO)
+        assert flags == 0, name
+        Rexp = L / np.cbrt(np.linalg.det(L))
+        assert isapprox(R, Rexp), name
+        assert isapprox(M, blockdiag4(Rexp) @ M0 @ blockdiag4(Rexp).T), name
+        if rigid:
+            assert isapprox(K, blockdiag4(Rexp) @ K0 @ blockdiag4(Rexp).T), name
+            assert np.abs(f).max() < 1e-8 * np.abs(K).max(), name
+        else:
+            assert not isapprox(K, R12 @ K0 @ R12.T), name
+        if name in ("rotation", "rotation+translation"):
+            assert not isapprox(K, K0), name
+
+
+def test_coro_force_is_rotated_linear_force(oracle_lib):
+    """f = -R K0 R^T (x - R x0) (Fem3DElementCorotationalTetrahedronTests.cpp:445-520)."""
+    rng = np.random.default_rng(0)
+    K0, _, _ = oracle.tet_linear_km(CORO_X0, E, NU, RHO)
+    for _ in range(5):
+        L = rotation(rng.uniform(0, 6), rng.normal(size=3)) @ (np.eye(3) + 0.05 * rng.normal(size=(3, 3)))
+        x = CORO_X0 @ L.T + rng.normal(size=3)
+        R, M, K, f, flags = oracle.coro_element(CORO_X0, x, E, NU, RHO)
+        R12 = blockdiag4(R)
+        fexp = -R12 @ K0 @ R12.T @ (x.reshape(-1) - R12 @ CORO_X0.reshape(-1))
+        assert np.abs(f - fexp).max() <= 1e-8 * np.abs(fexp).max()
+
+
+def fd_rotation(x0, x, h=1e-6):
+    """Central finite differences of the oracle's rotation wrt the 12 positions."""
+    dR = []
+    for l in range(12):
+        xp = x.reshape(-1).copy()
+        xm = x.reshape(-1).copy()
+        xp[l] += h
+        xm[l] -= h
+        Rp = oracle.coro_element(x0, xp, E, NU, RHO)[0]
+        Rm = oracle.coro_element(x0, xm, E, NU, RHO)[0]
+        dR.append((Rp - Rm) / (2 * h))
+    return dR
+
+
+def test_coro_stiffness_is_fd_jacobian_plus_documented_quirk(oracle_lib):
+    """The exact corotational stiffness is -df/dx (Barbic 2012). The reference evaluates the first
+    correction term with m_K already overwritten by R K0 R^T (Fem3DElementCorotationalTetrahedron.cpp:203, 276),
+    so its K differs from -df/dx by  sum_l [dR/dx_l (R K0 R^T - K0)(R^T x - x0)]_l  (a column per DOF).
+    Both -df/dx and dR/dx come from central differences of the oracle, independent of its K code."""
+    rng = np.random.default_rng(1)
+    K0, _, _ = oracle.tet_linear_km(CORO_X0, E, NU, RHO)
+    L = rotation(0.7, [0.3, -1.0, 0.4]) @ (np.eye(3) + np.array([[0.04, 0.01, -0.02], [0.0, -0.03, 0.02],
+                                                                  [0.01, 0.0, 0.05]]))
+    x = (CORO_X0 @ L.T + np.array([0.1, 0.2, -0.3]) + 0.01 * rng.normal(size=(4, 3))).reshape(-1)
+    R, M, K, f, flags = oracle.coro_element(CORO_X0, x, E, NU, RHO)
+    h = 1e-6
+    Kfd = np.zeros((12, 12))
+    for l in range(12):
+        xp = x.copy()
+        xm = x.copy()
+        xp[l] += h
+        xm[l] -= h
+        fp = oracle.coro_element(CORO_X0, xp, E, NU, RHO)[3]
+        fm = oracle.coro_element(CORO_X0, xm, E, NU, RHO)[3]
+        Kfd[:, l] = -(fp - fm) / (2 * h)
+    dR = fd_rotation(CORO_X0, x, h)
+    R12 = blockdiag4(R)
+    u = R12.T @ x - CORO_X0.reshape(-1)
+    w = (R12 @ K0 @ R12.T - K0) @ u
+    quirk = np.zeros((12, 12))
+    for l in range(12):
+        quirk[:, l] = blockdiag4(dR[l]) @ w
+    scale = np.abs(K).max()
+    assert np.abs(K - (Kfd + quirk)).max() < 2e-6 * scale
+    # and the quirk term is not negligible here (so the test would catch its omission)
+    assert np.abs(quirk).max() > 1e-3 * scale
+
+
+def test_coro_degenerate_element_asserts(oracle_lib):
+    """A tet degenerated to a triangle cannot be initialised (Fem3DElementCorotationalTetrahedronTests.cpp:225-229)."""
+    bad = CORO_X0.copy()
+    bad[3] = bad[2]
+    with pytest.raises(RuntimeError):
+        oracle.coro_element(bad, bad, E, NU, RHO)
+
+
+def eigen_rotation_scaling(F):
+    """computeRotationScaling from numpy's SVD: flip the LARGEST singular direction when det < 0."""
+    U, s, Vt = np.linalg.svd(F)
+    x = np.linalg.det(U @ Vt)
+    s = s.copy()
+    s[0] *= x
+    S = Vt.T @ np.diag(s) @ Vt
+    U2 = U.copy()
+    U2[:, 0] /= x
+    return U2 @ Vt, S
+
+
+def test_polar_decomposition_eigen_convention(oracle_lib):
+    rng = np.random.default_rng(2)
+    for trial in range(200):
+        F = rng.normal(size=(3, 3))
+        if trial % 3 == 0:
+            F = rotation(rng.uniform(0, 6), rng.normal(size=3)) @ (np.eye(3) + 0.1 * rng.normal(size=(3, 3)))
+        R, S = oracle.rotation_scaling(F)
+        Re, Se = eigen_rotation_scaling(F)
+        assert np.abs(R - Re).max() < 1e-10
+        assert np.abs(S - Se).max() < 1e-10 * max(1.0, np.abs(Se).max())
+        assert abs(np.linalg.det(R) - 1.0) < 1e-12
+        assert np.abs(R @ S - F).max() < 1e-12 * max(1.0, np.abs(F).max())
+
+
+# ------------------------------------------------------------------------------------------
+# Hexahedron (Physics/UnitTests/Fem3DElementCubeTests.cpp:102-470, 561-648)
+# ------------------------------------------------------------------------------------------
+CUBE_POS = np.array([[-0.5, -0.5, -0.5], [0.5, -0.5, -0.5], [-0.5, 0.5, -0.5], [0.5, 0.5, -0.5],
+                     [-0.5, -0.5, 0.5], [0.5, -0.5, 0.5], [-0.5, 0.5, 0.5], [0.5, 0.5, 0.5]])
+CUBE_IDS = [0, 1, 3, 2, 4, 5, 7, 6]
+
+
+def hex_K_independent(X, E_, nu_, order=3):
+    """24x24 stiffness of a trilinear hex by Gauss-Legendre quadrature of the given order, written
+    independently of the oracle (tensor-product shape functions from the node signs)."""
+    sg = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                   [-1, 1, 1]], dtype=float)
+    pts, wts = np.polynomial.legendre.leggauss(order)
+    lam = E_ * nu_ / ((1 + nu_) * (1 - 2 * nu_))
+    mu = E_ / (2 * (1 + nu_))
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] = lam + 2 * mu
+    C[3:, 3:] = np.eye(3) * mu
+    K = np.zeros((24, 24))
+    for i, a in enumerate(pts):
+        for j, b in enumerate(pts):
+            for k, c in enumerate(pts):
+                xi = np.array([a, b, c])
+                dN = np.zeros((8, 3))
+                for n in range(8):
+                    f = 1 + sg[n] * xi
+                    dN[n, 0] = sg[n, 0] * f[1] * f[2] / 8
+                    dN[n, 1] = f[0] * sg[n, 1] * f[2] / 8
+                    dN[n, 2] = f[0] * f[1] * sg[n, 2] / 8
+                J = dN.T @ X  # J[r, axis] = sum_n dN[n, r] X[n, axis]
+                G = dN @ np.linalg.inv(J).T  # dN/dx
+                B = np.zeros((6, 24))
+                for n in range(8):
+                    gx, gy, gz = G[n]
+                    B[0, 3 * n] = gx; B[1, 3 * n + 1] = gy; B[2, 3 * n + 2] = gz
+                    B[3, 3 * n] = gy; B[3, 3 * n + 1] = gx
+                    B[4, 3 * n + 1] = gz; B[4, 3 * n + 2] = gy
+                    B[5, 3 * n] = gz; B[5, 3 * n + 2] = gx
+                K += wts[i] * wts[j] * wts[k] * np.linalg.det(J) * B.T @ C @ B
+    return K
+
+
+def expected_cube_mass():
+    """Closed-form consistent mass of the unit cube (Fem3DElementCubeTests.cpp:374-420)."""
+    a = 1.0 / 27.0
+    b = a / 2.0
+    c = a / 4.0
+    d = a / 8.0
+    M = np.zeros((24, 24))
+    np.fill_diagonal(M, a)
+
+    def diag(r0, c0, n, v):
+        for t in range(n):
+            M[r0 + t, c0 + t] = v
+
+    diag(0, 3, 21, b)
+    diag(9, 12, 3, c)
+    diag(0, 6, 18, c)
+    diag(6, 12, 6, d)
+    diag(0, 9, 15, c)
+    diag(0, 9, 3, b)
+    diag(12, 21, 3, b)
+    diag(0, 12, 12, b)
+    diag(0, 15, 9, c)
+    diag(0, 18, 6, d)
+    diag(0, 21, 3, c)
+    M = np.triu(M) + np.triu(M, 1).T
+    return M * RHO
+
+
+def test_cube_matrices(oracle_lib):
+    X = CUBE_POS[CUBE_IDS]
+    K, M, vol = oracle.cube_km(X, E, NU, RHO)
+    assert vol == pytest.approx(1.0, rel=1e-14)
+    assert isapprox(K, hex_K_independent(X, E, NU, order=3), 1e-12)
+    assert isapprox(M, expected_cube_mass(), 1e-12)
+    # closed-form entries of Muller-Teschner-Gross (the diagonal d and the K00 coupling e)
+    a = E * (1 - NU) / ((1 + NU) * (1 - 2 * NU))
+    b = E * NU / ((1 + NU) * (1 - 2 * NU))
+    c = E / (2 * (1 + NU))
+    assert np.allclose(np.diag(K), (a + 2 * c) / 9.0, rtol=1e-12)
+    assert K[0, 1] == pytest.approx((b + c) / 12.0, rel=1e-12)
+    # rigid modes
+    for cc in range(3):
+        t = np.zeros(24)
+        t[cc::3] = 1
+        assert np.abs(K @ t).max() < 1e-9 * np.abs(K).max()
+
+
+def test_cube_distorted_and_invalid(oracle_lib):
+    rng = np.random.default_rng(4)
+    X = CUBE_POS[CUBE_IDS] * np.array([0.01, 0.02, 0.015]) + 1e-3 * rng.normal(size=(8, 3))
+    K, M, vol = oracle.cube_km(X, 2e4, 0.48, 1050.0)
+    # the reference's 2x2x2 rule (exact only for affine cubes) reproduced independently
+    assert isapprox(K, hex_K_independent(X, 2e4, 0.48, order=2), 1e-12)
+    assert isapprox(M.sum(), 1050.0 * vol * 3, 1e-12)
+    with pytest.raises(RuntimeError):
+        oracle.cube_km(CUBE_POS[[0, 1, 2, 3, 4, 5, 6, 7]], E, NU, RHO)  # wrong ordering: non-positive volume
+
+
+# ------------------------------------------------------------------------------------------
+# CG (Math/UnitTests/LinearSparseSolveAndInverseTests.cpp:33-193) and the lower-triangle view
+# ------------------------------------------------------------------------------------------
+def cg_fixture():
+    n = 18
+    A = np.array([[math.fmod(10.3 * math.cos(float(r * c)) + 3.24, 10.0) for c in range(n)] for r in range(n)])
+    b = np.array([math.fmod(-4.1 * r * r + 3.46, 5.0) for r in range(n)])
+    return A, b
+
+
+def csc_of_dense(A):
+    M = sp.csc_matrix(A)
+    M.sort_indices()
+    return M.indptr, M.indices, M.data
+
+
+def test_cg_matches_reference_behaviour(oracle_lib):
+    A, b = cg_fixture()
+    xexp = np.linalg.solve(A, b)
+    outer, inner, vals = csc_of_dense(A)
+    x, it, err = oracle.cg_solve(outer, inner, vals, b, 2.220446049250313e-16, -1)
+    assert isapprox(x, xexp, 1e-5)
+    x, it, err = oracle.cg_solve(outer, inner, vals, b, 1e-10, 18)
+    assert it == 18
+    assert isapprox(x, xexp, 1e-5)
+    assert not isapprox(x, xexp, 1e-10)
+    x, it, err = oracle.cg_solve(outer, inner, vals, b, 1e-10, 50)
+    assert isapprox(x, xexp, 1e-10)
+
+
+def test_cg_uses_lower_triangle_view(oracle_lib):
+    """Eigen's ConjugateGradient<SparseMatrix> (UpLo = Lower) solves with the lower triangle mirrored."""
+    rng = np.random.default_rng(5)
+    n = 30
+    B = rng.normal(size=(n, n))
+    A = B @ B.T + n * np.eye(n)
+    Anon = A + np.triu(0.3 * rng.normal(size=(n, n)), 1)  # perturb the strict upper triangle only
+    b = rng.normal(size=n)
+    outer, inner, vals = csc_of_dense(Anon)
+    x, it, err = oracle.cg_solve(outer, inner, vals, b, 1e-14, 200)
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) < 1e-10 * np.linalg.norm(x)
+    assert np.linalg.norm(x - np.linalg.solve(Anon, b)) > 1e-3 * np.linalg.norm(x)
+
+
+def test_cg_zero_rhs_and_identity(oracle_lib):
+    outer, inner, vals = csc_of_dense(np.eye(6))
+    x, it, err = oracle.cg_solve(outer, inner, vals, np.zeros(6), 1e-8)
+    assert it == 0 and err == 0 and not x.any()
+    b = np.arange(1.0, 7.0)
+    x, it, err = oracle.cg_solve(outer, inner, vals, b, 1e-8)
+    assert np.array_equal(x, b) and it == 0
+
+
+# ------------------------------------------------------------------------------------------
+# Representation-level semantics
+# ------------------------------------------------------------------------------------------
+def test_pattern_is_node_pair_adjacency(oracle_lib):
+    """Physics/FemRepresentation.cpp:121-132: full 3x3 blocks for every node pair sharing an element."""
+    cfg = mg.config_corotational((3, 4, 2))
+    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    outer, inner = s.pattern()
+    N = cfg.x0.shape[0]
+    adj = np.zeros((N, N), dtype=bool)
+    for t in cfg.conn:
+        for a in t:
+            for b_ in t:
+                adj[a, b_] = True
+    dense = np.kron(adj, np.ones((3, 3), dtype=bool))
+    P = sp.csc_matrix(dense.astype(float))
+    P.sort_indices()
+    assert np.array_equal(outer, P.indptr) and np.array_equal(inner, P.indices)
+
+
+def test_block_meshes_are_conforming_and_positive():
+    for n in [(1, 1, 1), (2, 3, 4), (5, 5, 5)]:
+        x0, tets, _ = mg.block_tet_mesh(*n)
+        vol = mg.tet_volumes(x0, tets)
+        assert (vol > 0).all()
+        assert vol.sum() == pytest.approx(np.prod(n) * 1e-6, rel=1e-12)
+        # conforming: every interior triangle is shared by exactly two tets
+        faces = {}
+        for t in tets:
+            for f in ([0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]):
+                key = tuple(sorted(t[f]))
+                faces[key] = faces.get(key, 0) + 1
+        nx, ny, nz = n
+        boundary = 2 * 2 * (nx * ny + ny * nz + nx * nz)
+        assert sum(1 for c in faces.values() if c == 1) == boundary
+        assert all(c <= 2 for c in faces.values())
+
+
+def test_gravity_lumped_masses_and_free_fall(oracle_lib):
+    """Gravity f_i = m_i g with m_i = sum_e rho V_e / 4 (Physics/FemRepresentation.cpp:102-117, 641-662);
+    a free mesh at rest falls exactly as the mass point of OdeSolverEulerImplicitTests.cpp:56-137:
+    v1 = v0 + g dt, x1 = x0 + v1 dt."""
+    cfg = mg.config_corotational((3, 2, 2))
+    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    s.set_solver(tol=1e-14)
+    m = s.mass_per_node()
+    assert m.sum() == pytest.approx(1000.0 * 12 * 1e-6, rel=1e-12)
+    dt = 1e-3
+    s.step(dt, 1)
+    x, v = s.state()
+    g = np.array([0.0, -9.81, 0.0])
+    assert np.abs(v.reshape(-1, 3) - g * dt).max() < 1e-12
+    assert np.abs(x - (cfg.x0.reshape(-1) + dt * v)).max() < 1e-15
+
+
+def test_boundary_conditions_faithful_equals_efficient(oracle_lib):
+    cfg = mg.config_corotational((3, 3, 3))
+    runs = []
+    for faithful in (False, True):
+        s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+        for nd in cfg.bc_nodes:
+            s.add_bc_node(nd)
+        s.set_solver(tol=1e-10, faithful_bc=faithful)
+        s.set_state(v=cfg.v0)
+        s.step(cfg.dt, 3)
+        runs.append((s.matrix_values(2), *s.state()))
+    for a, b_ in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b_)
+    A = runs[0][0]
+    # BC rows/columns are zero except the unit diagonal (Math/OdeState.cpp:197-218)
+    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    outer, inner = s.pattern()
+    n = len(outer) - 1
+    Ad = sp.csc_matrix((A, inner, outer), shape=(n, n)).toarray()
+    bc = np.zeros(n, dtype=bool)
+    for nd in cfg.bc_nodes:
+        bc[3 * nd:3 * nd + 3] = True
+    assert not Ad[bc][:, ~bc].any() and not Ad[~bc][:, bc].any()
+    assert np.array_equal(Ad[bc][:, bc], np.eye(bc.sum()))
+
+
+def test_linear_scheme_equals_implicit_euler_for_linear_tets(oracle_lib):
+    """For linear elements one Newton iteration is exact, so LinearEulerImplicit and EulerImplicit agree."""
+    cfg = mg.config_linear_cpu((4, 4, 4))
+    out = []
+    for scheme in (oracle.EULER_IMPLICIT, oracle.LINEAR_EULER_IMPLICIT):
+        s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+        for nd in cfg.bc_nodes:
+            s.add_bc_node(nd)
+        s.set_solver(scheme=scheme, linsolver=oracle.LU)
+        s.step(cfg.dt, 20)
+        out.append(s.state())
+    assert np.linalg.norm(out[0][1] - out[1][1]) < 1e-12 * np.linalg.norm(out[0][1])
+
+
+def test_nonfinite_state_deactivates(oracle_lib):
+    """Physics/DeformableRepresentationTest.cpp:309-322: +inf velocity -> update() deactivates."""
+    cfg = mg.config_corotational((2, 2, 2))
+    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    v = np.zeros(cfg.x0.size)
+    v[5] = np.inf
+    s.set_state(v=v)
+    s.set_solver(tol=1e-8, max_iter=50)
+    assert s.step(cfg.dt, 1) == 1
+    assert not s.valid()
