@@ -106,6 +106,64 @@ def cpu_baseline(cfg, sample_steps):
     return x0.size * sample_steps / t, t, err
 
 
+def bench_hex(args):
+    """Hexahedral block (Fem3DElementCube), 128 x 64 x 128 cubes by default: FP32 element-by-element CG with
+    cached element matrices inside FP64 refinement to 1e-6, against the FP64 matrix-free path at 1e-10."""
+    n = (args.hex_n, args.hex_n // 2, args.hex_n)
+    cfg = mg.config_hex(n)
+    N = cfg.x0.shape[0]
+    D = 3 * N
+    steps = args.steps if args.steps_given else cfg.steps
+    log("workload: hexahedra %s, %d elements, %d nodes, %d DOF, %d steps" % (n, cfg.conn.shape[0], N, D, steps))
+
+    def run(precision, tol):
+        g = femgpu.FemGpu(0)
+        t0 = time.perf_counter()
+        g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+        g.add_bc_nodes(cfg.bc_nodes)
+        g.set_options(dt=cfg.dt, cg_tolerance=tol, precision=precision, gravity=cfg.gravity)
+        g.set_external_force(cfg.fext)
+        log("  set-up %.1f s" % (time.perf_counter() - t0))
+        if args.warmup:
+            g.step(args.warmup)
+        device_sync(g)
+        g.enable_phase_timers(args.timer_every)
+        it0 = g.stats()["total_cg_iterations"]
+        t0 = time.perf_counter()
+        rc = g.step(steps)
+        device_sync(g)
+        t = time.perf_counter() - t0
+        assert rc == femgpu.FEM_OK
+        st = g.stats()
+        x, v = g.state()
+        ph = g.phase_times()
+        g.close()
+        return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, stats=st, phases=ph)
+
+    r32 = run(femgpu.FP32_REFINED, 1e-6)
+    log("fp32+refine: %.3f s, %d CG iterations, last sweeps %d" % (r32["seconds"], r32["cg"],
+                                                                    r32["stats"]["last_refinements"]))
+    r64 = run(femgpu.FP64, 1e-10)
+    log("fp64 (1e-10): %.3f s, %d CG iterations" % (r64["seconds"], r64["cg"]))
+    x0 = cfg.x0.reshape(-1)
+    err = dict(u=float(np.linalg.norm(r32["x"] - r64["x"]) / np.linalg.norm(r64["x"] - x0)),
+               v=float(np.linalg.norm(r32["v"] - r64["v"]) / np.linalg.norm(r64["v"])))
+    log("fp32+refine vs fp64 rel err", err)
+    out = {"metric": METRIC, "value": D * steps / r32["seconds"], "unit": "DOF-steps/s", "n_gpus": 1,
+           "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",
+           "data": "synthetic",
+           "config": {"workload": "hex-block %dx%dx%d Fem3DElementCube, EbE FP32 cached Ke + FP64 refinement 1e-6, "
+                                  "dt=1ms" % n, "elements": int(cfg.conn.shape[0]), "nodes": int(N), "dof": int(D),
+                      "parallelism": "single-gpu"},
+           "rel_err_vs_fp64": err,
+           "fp64": {"value": D * steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / steps,
+                    "cg_iters_per_step": r64["cg"] / steps},
+           "fp32_refined": {"cg_iters_per_step": r32["cg"] / steps},
+           "phases_fp32": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in r32["phases"].items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,11 +174,17 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--timer-every", type=int, default=10,
                     help="bracket every kernel of one step in N with HIP events (0: no events)")
+    ap.add_argument("--workload", default="corotational", choices=["corotational", "hex"])
+    ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
     args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design (no --gpus > 1).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "hex":
+        bench_hex(args)
+        return
 
     n = (args.n, args.n, args.n)
     cfg = mg.config_corotational(n)

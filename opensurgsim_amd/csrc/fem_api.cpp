@@ -185,6 +185,7 @@ struct fem_ctx {
     int last_precision = -1;
     int predicted_iters = 16;     // FP64 solve / current inner solve
     int predicted_sweep[16];       // FP32 inner solves, per refinement sweep
+    int predicted_hex[17];         // hexahedral path: FP64 solve, then per refinement sweep
     fem_stats stats;
     PhaseTimer timer;
     int timer_every = 0;           // instrument one step in timer_every (0: off)
@@ -421,9 +422,32 @@ int step_once(fem_ctx* c) {
     const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
     const int newton = std::max(1, c->opt.newton_max_iterations);
     if (c->kind == FEM_ELEMENT_HEX) {
-        hex_step(c->hex, c->opt, c->x0.p, c->x.p, c->v.p, c->fg.p, c->have_fext ? c->fext.p : nullptr, c->bcmask.p,
-                 c->partial.p, c->scal.p, c->h_sc, c->counters.p + 3, c->stream, c->predicted_iters, c->stats,
-                 (int)max_iters(c));
+        HexStepArgs a;
+        a.opt = &c->opt;
+        a.x0 = c->x0.p;
+        a.x = c->x.p;
+        a.v = c->v.p;
+        a.fg = c->fg.p;
+        a.fext = c->have_fext ? c->fext.p : nullptr;
+        a.bcmask = c->bcmask.p;
+        a.mat = c->mat.p;
+        a.b = c->b.p;
+        a.dv = c->dv.p;
+        a.r64 = c->r64.p;
+        a.vec[0] = c->cg_x.p;
+        a.vec[1] = c->cg_r.p;
+        a.vec[2] = c->cg_z.p;
+        a.vec[3] = c->cg_p0.p;
+        a.vec[4] = c->cg_q.p;
+        a.vec[5] = c->cg_dinv.p;
+        a.vec[6] = c->dvprev.p;
+        a.partial = c->partial.p;
+        a.sc = c->scal.p;
+        a.h_sc = c->h_sc;
+        a.invalid = c->counters.p + 3;
+        a.max_iter = (int)max_iters(c);
+        a.predicted = c->predicted_hex;
+        hex_step(c->hex, a, c->stats, c->stream);
     } else if (linear_scheme && c->linear_matrix_ready && c->matrix_dt == dt) {
         // Steps >= 2 of OdeSolverLinearEulerImplicit: element forces only, constant operator.
         tet_elements(c, false, c->x.p, c->v.p);
@@ -623,6 +647,7 @@ int fem_create(int device, fem_ctx** out) {
         HIPCHECK(hipMemset(c->scal.p, 0, 2 * sizeof(CgScalars)));
         fem_default_options(&c->opt);
         for (int i = 0; i < 16; ++i) c->predicted_sweep[i] = 8;
+        for (int i = 0; i < 17; ++i) c->predicted_hex[i] = 8;
         std::memset(&c->stats, 0, sizeof(c->stats));
         c->stats.active = 1;
         return FEM_OK;
@@ -641,6 +666,7 @@ void fem_destroy(fem_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    hex_release(c->hex);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -989,7 +1015,7 @@ int fem_evaluate_elements(fem_ctx* c, const double* x, int64_t first, int64_t co
         dK.alloc((size_t)count * nn3 * nn3);
         df.alloc((size_t)count * nn3);
         if (c->kind == FEM_ELEMENT_HEX)
-            hex_eval_elements(c->hex, (int)first, (int)count, c->x0.p, dx.p, dR.p, dK.p, df.p, c->stream);
+            hex_eval_elements(c->hex, (int)first, (int)count, c->x0.p, dx.p, c->mat.p, dR.p, dK.p, df.p, c->stream);
         else
             launch_eval_elements(c->kind, (int)first, (int)count, c->conn4.p, c->x0.p, dx.p, c->mat.p, dR.p, dK.p,
                                  df.p, c->stream);

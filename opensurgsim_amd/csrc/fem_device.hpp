@@ -22,20 +22,20 @@
 namespace femgpu {
 
 template <typename T>
-__device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
+__host__ __device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
 template <typename T>
-__device__ __forceinline__ T dot3(const T* a, const T* b) {
+__host__ __device__ __forceinline__ T dot3(const T* a, const T* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
 // C = A B (3x3 row-major)
 template <typename T>
-__device__ __forceinline__ void mm3(const T* A, const T* B, T* C) {
+__host__ __device__ __forceinline__ void mm3(const T* A, const T* B, T* C) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -43,7 +43,7 @@ __device__ __forceinline__ void mm3(const T* A, const T* B, T* C) {
 }
 // C = A^T B
 template <typename T>
-__device__ __forceinline__ void mtm3(const T* A, const T* B, T* C) {
+__host__ __device__ __forceinline__ void mtm3(const T* A, const T* B, T* C) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -51,18 +51,18 @@ __device__ __forceinline__ void mtm3(const T* A, const T* B, T* C) {
 }
 // y = A x
 template <typename T>
-__device__ __forceinline__ void mv3(const T* A, const T* x, T* y) {
+__host__ __device__ __forceinline__ void mv3(const T* A, const T* x, T* y) {
     y[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
     y[1] = A[3] * x[0] + A[4] * x[1] + A[5] * x[2];
     y[2] = A[6] * x[0] + A[7] * x[1] + A[8] * x[2];
 }
 template <typename T>
-__device__ __forceinline__ T det3(const T* A) {
+__host__ __device__ __forceinline__ T det3(const T* A) {
     return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 // Cofactor inverse; returns det. inv untouched when det == 0.
 template <typename T>
-__device__ __forceinline__ T inv3(const T* A, T* inv) {
+__host__ __device__ __forceinline__ T inv3(const T* A, T* inv) {
     T c00 = A[4] * A[8] - A[5] * A[7];
     T c01 = A[5] * A[6] - A[3] * A[8];
     T c02 = A[3] * A[7] - A[4] * A[6];

@@ -1,22 +1,691 @@
-// Hexahedral (Fem3DElementCube) path.
+// Hexahedral (Fem3DElementCube) path on gfx950: element-by-element operators over a greedy element
+// colouring (no two elements of one colour share a node, so the scatter needs no atomics and the
+// summation order is fixed).
+//
+//   FP32 inner CG:  cached A_e = M_e / dt + dt K_e as 36 packed 3x3 blocks (a <= b), 8 lanes per
+//                   element (lane a owns node a's three rows; the transposed halves of the
+//                   off-diagonal blocks go to the other lanes through shuffles).
+//   FP64 pieces:    matrix-free Gauss-point operator: (A p)_a = sum_g w_g detJ_g [rho N_a (sum_b N_b p_b) / dt
+//                   + dt sigma(p) g_a], sigma = lambda tr(eps) I + 2 mu eps, eps = sym(sum_b p_b g_b^T);
+//                   sigma g_a is the B_a^T C B p of Physics/Fem3DElementCube.cpp:213-288.
 #include "fem_hex.hpp"
+#include "fem_device.hpp"
+#include "fem_reduce.hpp"
 
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
 #include <stdexcept>
+#include <string>
 
 namespace femgpu {
 
-void hex_setup(HexData&, int64_t, const double*, int64_t, const int32_t*, const double*, std::vector<double>&,
-               std::vector<int32_t>&, std::vector<int32_t>&, hipStream_t) {
-    throw std::runtime_error("hexahedral elements are not available in this build");
+namespace {
+
+#define HCHECK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Node sign table of Physics/Fem3DElementCube.cpp:72-77 and the 2-point rule of
+// Math/GaussLegendreQuadrature.cpp:30-34 (+1/sqrt(3) first; weights 1).
+__host__ __device__ __forceinline__ double hex_sign(int i, int axis) {
+    const int s[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    return (double)s[i][axis];
+}
+__host__ __device__ __forceinline__ double gauss2(int i) {
+    return i == 0 ? 0.57735026918962576451 : -0.57735026918962576451;
 }
 
-void hex_step(HexData&, const fem_options&, const double*, double*, double*, const double*, const double*,
-              const uint8_t*, double*, CgScalars*, CgScalars*, int*, hipStream_t, int&, fem_stats&, int) {
-    throw std::runtime_error("hexahedral elements are not available in this build");
+// Shape functions, their physical gradients and det J at Gauss point (gi, gj, gk).
+// J(r, c) = dx_c / dxi_r; dN/dx = J^-1 dN/dxi (Physics/Fem3DElementCube.cpp:169-251).
+__host__ __device__ __forceinline__ double hex_gp(const double* X, int gi, int gj, int gk, double N[8],
+                                                  double g[8][3]) {
+    const double xi[3] = {gauss2(gi), gauss2(gj), gauss2(gk)};
+    double dN[8][3];
+    for (int i = 0; i < 8; ++i) {
+        const double f0 = 1 + xi[0] * hex_sign(i, 0), f1 = 1 + xi[1] * hex_sign(i, 1), f2 = 1 + xi[2] * hex_sign(i, 2);
+        N[i] = 0.125 * f0 * f1 * f2;
+        dN[i][0] = 0.125 * hex_sign(i, 0) * f1 * f2;
+        dN[i][1] = 0.125 * f0 * hex_sign(i, 1) * f2;
+        dN[i][2] = 0.125 * f0 * f1 * hex_sign(i, 2);
+    }
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 8; ++i)
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) J[3 * r + c] += X[3 * i + c] * dN[i][r];
+    double Ji[9];
+    const double d = inv3(J, Ji);
+    for (int i = 0; i < 8; ++i)
+        for (int r = 0; r < 3; ++r) g[i][r] = Ji[3 * r] * dN[i][0] + Ji[3 * r + 1] * dN[i][1] + Ji[3 * r + 2] * dN[i][2];
+    return d;
 }
 
-void hex_eval_elements(HexData&, int, int, const double*, const double*, double*, double*, double*, hipStream_t) {
-    throw std::runtime_error("hexahedral elements are not available in this build");
+__device__ __forceinline__ constexpr int blk_index(int a, int b) { return 8 * a - a * (a - 1) / 2 + (b - a); }
+
+// ------------------------------------------------------------------------------------------
+// Set-up: packed FP32 A_e (one thread per (element, block)) and the FP64 diagonal (coloured).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void hex_prepare_kernel(int E, const int* __restrict__ conn,
+                                                          const double* __restrict__ x0, const double4* __restrict__ mat,
+                                                          double dt, float* __restrict__ A32) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)E * 36) return;
+    const int e = (int)(t / 36), kb = (int)(t % 36);
+    int a = 0, rem = kb;
+    while (rem >= 8 - a) {
+        rem -= 8 - a;
+        ++a;
+    }
+    const int b = a + rem;
+    double X[24];
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 3; ++c) X[3 * i + c] = x0[3 * (size_t)conn[8 * e + i] + c];
+    const double4 m = mat[e];
+    double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int gi = 0; gi < 2; ++gi)
+        for (int gj = 0; gj < 2; ++gj)
+            for (int gk = 0; gk < 2; ++gk) {
+                double N[8], g[8][3];
+                const double w = hex_gp(X, gi, gj, gk, N, g);
+                const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+                const double ms = w * m.z * N[a] * N[b] / dt;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        A[3 * i + j] += dt * w * (m.x * g[a][i] * g[b][j] + m.y * g[b][i] * g[a][j] +
+                                                  (i == j ? m.y * gab : 0.0)) +
+                                        (i == j ? ms : 0.0);
+            }
+    float* o = A32 + ((size_t)e * 36 + kb) * 9;
+    for (int i = 0; i < 9; ++i) o[i] = (float)A[i];
+}
+
+// FP64 element operator y = alpha_m M p + alpha_k K p for one element (matrix-free).
+__device__ __forceinline__ void hex_apply64(const double* X, const double4 m, const double* p, double am, double ak,
+                                            double* y) {
+    for (int i = 0; i < 24; ++i) y[i] = 0.0;
+    for (int gi = 0; gi < 2; ++gi)
+        for (int gj = 0; gj < 2; ++gj)
+            for (int gk = 0; gk < 2; ++gk) {
+                double N[8], g[8][3];
+                const double w = hex_gp(X, gi, gj, gk, N, g);
+                double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                double np[3] = {0, 0, 0};
+                for (int b = 0; b < 8; ++b)
+                    for (int i = 0; i < 3; ++i) {
+                        np[i] += N[b] * p[3 * b + i];
+                        for (int j = 0; j < 3; ++j) H[3 * i + j] += p[3 * b + i] * g[b][j];
+                    }
+                const double tr = H[0] + H[4] + H[8];
+                double sig[9];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        sig[3 * i + j] = m.y * (H[3 * i + j] + H[3 * j + i]) + (i == j ? m.x * tr : 0.0);
+                const double wk = ak * w, wm = am * w * m.z;
+                for (int a = 0; a < 8; ++a)
+                    for (int i = 0; i < 3; ++i)
+                        y[3 * a + i] += wk * (sig[3 * i] * g[a][0] + sig[3 * i + 1] * g[a][1] + sig[3 * i + 2] * g[a][2]) +
+                                        wm * N[a] * np[i];
+            }
+}
+
+// y[nodes] += sign * (alpha_m M + alpha_k K) w_e over the elements of one colour (w masked on BC DOFs
+// when mask != null; w = (x - x0) + dt v when v != null, giving the right-hand side).
+__global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* __restrict__ order,
+                                                          const int* __restrict__ conn, const double* __restrict__ x0,
+                                                          const double4* __restrict__ mat, const double* __restrict__ p,
+                                                          const double* __restrict__ xcur, const double* __restrict__ v,
+                                                          double dt, const uint8_t* __restrict__ mask, double am,
+                                                          double ak, double sign, double* __restrict__ y) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int e = order[t];
+    int ids[8];
+    for (int i = 0; i < 8; ++i) ids[i] = conn[8 * e + i];
+    double X[24], w[24], out[24];
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const size_t d = 3 * (size_t)ids[i] + c;
+            X[3 * i + c] = x0[d];
+            if (v) w[3 * i + c] = (xcur[d] - x0[d]) + dt * v[d];
+            else w[3 * i + c] = (mask && mask[d]) ? 0.0 : p[d];
+        }
+    hex_apply64(X, mat[e], w, am, ak, out);
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 3; ++c) y[3 * (size_t)ids[i] + c] += sign * out[3 * i + c];
+}
+
+// FP32 cached operator: q[nodes] += A_e p_e (p masked on BC DOFs), 8 lanes per element.
+__global__ __launch_bounds__(256) void hex_apply32_kernel(int count, const int* __restrict__ order,
+                                                          const int* __restrict__ conn, const float* __restrict__ A32,
+                                                          const float* __restrict__ p, const uint8_t* __restrict__ mask,
+                                                          float* __restrict__ q) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int el = t >> 3, a = t & 7;
+    const bool active = el < count;
+    const int e = active ? order[el] : 0;
+    const int node = active ? conn[8 * e + a] : 0;
+    float pa[3] = {0.f, 0.f, 0.f};
+    if (active)
+        for (int c = 0; c < 3; ++c) pa[c] = mask[3 * node + c] ? 0.f : p[3 * node + c];
+    float y[3] = {0.f, 0.f, 0.f};
+    const float* Ae = A32 + (size_t)e * 36 * 9;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        float pb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pb[c] = __shfl(pa[c], b, 8);
+        float tb[3] = {0.f, 0.f, 0.f};
+        if (active && b >= a) {
+            const float* B = Ae + 9 * blk_index(a, b);
+            float Bv[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Bv[i] = B[i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) y[i] += Bv[3 * i] * pb[0] + Bv[3 * i + 1] * pb[1] + Bv[3 * i + 2] * pb[2];
+            if (b > a) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) tb[j] = Bv[j] * pa[0] + Bv[3 + j] * pa[1] + Bv[6 + j] * pa[2];
+            }
+        }
+        // sum over the lanes a < b of B_ab^T p_a; lane b keeps it
+#pragma unroll
+        for (int o = 4; o > 0; o >>= 1)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tb[j] += __shfl_xor(tb[j], o, 8);
+        if (a == b)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) y[j] += tb[j];
+    }
+    if (active)
+        for (int c = 0; c < 3; ++c) q[3 * node + c] += y[c];
+}
+
+// FP64 diagonal of A over one colour.
+__global__ __launch_bounds__(128) void hex_diag_kernel(int count, const int* __restrict__ order,
+                                                       const int* __restrict__ conn, const double* __restrict__ x0,
+                                                       const double4* __restrict__ mat, double dt,
+                                                       double* __restrict__ diag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int e = order[t];
+    int ids[8];
+    double X[24], d[24];
+    for (int i = 0; i < 8; ++i) {
+        ids[i] = conn[8 * e + i];
+        for (int c = 0; c < 3; ++c) X[3 * i + c] = x0[3 * (size_t)ids[i] + c];
+    }
+    for (int i = 0; i < 24; ++i) d[i] = 0.0;
+    const double4 m = mat[e];
+    for (int gi = 0; gi < 2; ++gi)
+        for (int gj = 0; gj < 2; ++gj)
+            for (int gk = 0; gk < 2; ++gk) {
+                double N[8], g[8][3];
+                const double w = hex_gp(X, gi, gj, gk, N, g);
+                for (int a = 0; a < 8; ++a) {
+                    const double gg = g[a][0] * g[a][0] + g[a][1] * g[a][1] + g[a][2] * g[a][2];
+                    for (int i = 0; i < 3; ++i)
+                        d[3 * a + i] += dt * w * (m.x * g[a][i] * g[a][i] + m.y * g[a][i] * g[a][i] + m.y * gg) +
+                                        w * m.z * N[a] * N[a] / dt;
+                }
+            }
+    for (int a = 0; a < 8; ++a)
+        for (int c = 0; c < 3; ++c) diag[3 * (size_t)ids[a] + c] += d[3 * a + c];
+}
+
+__global__ __launch_bounds__(256) void hex_cast_diag_kernel(int n, const double* __restrict__ d64,
+                                                            float* __restrict__ d32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d32[i] = (float)d64[i];
+}
+
+// b = fg + fext before the element forces are subtracted; BC DOFs zeroed afterwards.
+__global__ __launch_bounds__(256) void hex_rhs_init_kernel(int n, const double* __restrict__ fg,
+                                                           const double* __restrict__ fext, double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = fg[i] + (fext ? fext[i] : 0.0);
+}
+__global__ __launch_bounds__(256) void hex_rhs_bc_kernel(int n, const uint8_t* __restrict__ mask,
+                                                         double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mask[i]) b[i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// CG pieces for the element-by-element operator (Eigen recurrence, as in fem_kernels.hip).
+// ------------------------------------------------------------------------------------------
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) void hex_cg_init_kernel(int n, const double* __restrict__ b,
+                                                                 const TV* __restrict__ diag, TV* __restrict__ x,
+                                                                 TV* __restrict__ r, TV* __restrict__ z,
+                                                                 TV* __restrict__ dinv, const uint8_t* __restrict__ mask,
+                                                                 double* __restrict__ partial,
+                                                                 const CgScalars* __restrict__ skip) {
+    __shared__ double lds[64];
+    if (skip && skip->done) return;
+    double bb = 0.0, rz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const TV s = mask[i] ? TV(1) : diag[i];  // BC rows of the system matrix are the identity
+        const TV d = (s != TV(0)) ? TV(1) / s : TV(1);
+        const TV bi = (TV)b[i];
+        x[i] = TV(0);
+        r[i] = bi;
+        dinv[i] = d;
+        const TV zi = d * bi;
+        z[i] = zi;
+        bb += (double)bi * (double)bi;
+        rz += (double)bi * (double)zi;
+    }
+    block_sum2(bb, rz, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bb;
+        partial[kCgBlocks + blockIdx.x] = rz;
+    }
+}
+
+// Iteration k, part 1: convergence test of k - 1, beta, p = z + beta p, q = 0.
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) void hex_cg_pupdate_kernel(int n, int k, int max_iter,
+                                                                    const TV* __restrict__ z, TV* __restrict__ p,
+                                                                    TV* __restrict__ q, double* __restrict__ partial,
+                                                                    CgScalars* __restrict__ sc) {
+    __shared__ double lds[64];
+    if (sc->done) return;
+    double beta = 0.0;
+    if (k > 0) {
+        double rr, rzk;
+        reduce_partials2(partial, partial + kCgBlocks, rr, rzk, lds);
+        const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+        if (rr < sc->threshold || k >= max_iter) {
+            if (first) {
+                const bool conv = rr < sc->threshold;
+                sc->rr = rr;
+                sc->err = sqrt(rr / sc->bb);
+                sc->iters = conv ? k - 1 : k;
+                sc->converged = conv ? 1 : 0;
+                sc->done = 1;
+            }
+            return;
+        }
+        beta = rzk / sc->rz[(k - 1) & 1];
+        if (first) {
+            sc->rz[k & 1] = rzk;
+            sc->rr = rr;
+        }
+    }
+    const TV bt = (TV)beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        p[i] = (k == 0) ? z[i] : z[i] + bt * p[i];
+        q[i] = TV(0);
+    }
+}
+
+// BC rows of the operator are the identity: q_i = p_i there; partial p . q.
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) void hex_cg_bc_pq_kernel(int n, const uint8_t* __restrict__ mask,
+                                                                  const TV* __restrict__ p, TV* __restrict__ q,
+                                                                  double* __restrict__ partial,
+                                                                  const CgScalars* __restrict__ sc) {
+    __shared__ double lds[64];
+    if (sc->done) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        TV qi = q[i];
+        if (mask[i]) {
+            qi = p[i];
+            q[i] = qi;
+        }
+        acc += (double)p[i] * (double)qi;
+    }
+    double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[2 * kCgBlocks + blockIdx.x] = s;
+}
+
+// FP64 residual r = b - S dv from q64 = A (masked dv); partial |r|^2 for the refinement test.
+__global__ __launch_bounds__(kCgThreads) void hex_residual_kernel(int n, const uint8_t* __restrict__ mask,
+                                                                  const double* __restrict__ b,
+                                                                  const double* __restrict__ dv,
+                                                                  const double* __restrict__ q64,
+                                                                  double* __restrict__ r64,
+                                                                  double* __restrict__ partial) {
+    __shared__ double lds[64];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double e = b[i] - (mask[i] ? dv[i] : q64[i]);
+        r64[i] = e;
+        acc += e * e;
+    }
+    double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[3 * kCgBlocks + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kCgThreads) void hex_refine_check_kernel(const double* __restrict__ partial, int first,
+                                                                      double tol, CgScalars* outer, CgScalars* inner) {
+    __shared__ double lds[64];
+    const double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
+    if (threadIdx.x == 0) {
+        if (first) {
+            outer->bb = rr;
+            outer->threshold = fmax(tol * tol * rr, DBL_MIN);
+            outer->iters = 0;
+            outer->done = 0;
+            outer->converged = 0;
+        }
+        outer->rr = rr;
+        outer->err = outer->bb > 0 ? sqrt(rr / outer->bb) : 0.0;
+        if (outer->bb == 0.0 || rr < outer->threshold) {
+            outer->done = 1;
+            outer->converged = 1;
+            inner->done = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kCgThreads) void hex_accumulate_kernel(int n, const float* __restrict__ xf,
+                                                                    double* __restrict__ dv,
+                                                                    const CgScalars* __restrict__ outer) {
+    if (outer->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dv[i] += (double)xf[i];
+}
+
+__global__ void hex_eval_kernel(int first, int count, const int* conn, const double* x0all, const double* xall,
+                                const double4* mat, double* Kout, double* fout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int e = first + t;
+    double X[24], u[24];
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const size_t d = 3 * (size_t)conn[8 * e + i] + c;
+            X[3 * i + c] = x0all[d];
+            u[3 * i + c] = xall[d] - x0all[d];
+        }
+    const double4 m = mat[e];
+    double f[24];
+    for (int i = 0; i < 24; ++i) f[i] = 0.0;
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) {
+            double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int gi = 0; gi < 2; ++gi)
+                for (int gj = 0; gj < 2; ++gj)
+                    for (int gk = 0; gk < 2; ++gk) {
+                        double N[8], g[8][3];
+                        const double w = hex_gp(X, gi, gj, gk, N, g);
+                        const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                K[3 * i + j] += w * (m.x * g[a][i] * g[b][j] + m.y * g[b][i] * g[a][j] +
+                                                     (i == j ? m.y * gab : 0.0));
+                    }
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j) {
+                    if (Kout) Kout[(size_t)t * 576 + (3 * a + i) * 24 + 3 * b + j] = K[3 * i + j];
+                    f[3 * a + i] -= K[3 * i + j] * u[3 * b + j];
+                }
+            }
+        }
+    if (fout)
+        for (int i = 0; i < 24; ++i) fout[(size_t)t * 24 + i] = f[i];
+}
+
+inline dim3 blocks_for(int64_t n, int threads) { return dim3((unsigned)((n + threads - 1) / threads)); }
+
+// Host-side: element data for validity and masses (Physics/Fem3DElementCube.cpp:321-366: V > 1e-12;
+// evaluateJ asserts invertibility with the Eigen threshold 1e-12 on det J).
+double host_hex_volume(const double* X, int64_t e) {
+    double v = 0.0;
+    for (int gi = 0; gi < 2; ++gi)
+        for (int gj = 0; gj < 2; ++gj)
+            for (int gk = 0; gk < 2; ++gk) {
+                double N[8], g[8][3];
+                const double d = hex_gp(X, gi, gj, gk, N, g);
+                if (!(std::fabs(d) > 1e-12))
+                    throw std::invalid_argument("Found a non invertible matrix J while computing Fem3DElementCube "
+                                                "stiffness matrix (element " + std::to_string(e) + ")");
+                v += d;
+            }
+    if (!(v > 1e-12))
+        throw std::invalid_argument("Fem3DElementCube ill-defined, its volume is not positive (element " +
+                                    std::to_string(e) + ")");
+    return v;
+}
+
+}  // namespace
+
+void hex_release(HexData& h) {
+    for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.A32, (void*)h.diag64, (void*)h.diag32})
+        if (p) (void)hipFree(p);
+    h.conn = h.order = nullptr;
+    h.A32 = nullptr;
+    h.diag64 = nullptr;
+    h.diag32 = nullptr;
+    h.ready = false;
+}
+
+void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elements, const int32_t* conn,
+               const double* mat, std::vector<double>& mass_per_node, std::vector<int32_t>& rowptr_b,
+               std::vector<int32_t>& colidx_b, hipStream_t s) {
+    hex_release(h);
+    h.num_nodes = num_nodes;
+    h.num_elements = num_elements;
+    mass_per_node.assign(num_nodes, 0.0);
+    for (int64_t e = 0; e < num_elements; ++e) {
+        double X[24];
+        for (int i = 0; i < 8; ++i)
+            for (int c = 0; c < 3; ++c) X[3 * i + c] = x0[3 * (size_t)conn[8 * e + i] + c];
+        const double vol = host_hex_volume(X, e);
+        const double mass = vol * mat[4 * e + 2];  // FemElement::getMass (Physics/FemElement.cpp:106-109)
+        for (int i = 0; i < 8; ++i) mass_per_node[conn[8 * e + i]] += mass / 8;
+    }
+    // node-pair block pattern (Physics/FemRepresentation.cpp:121-132)
+    std::vector<int64_t> cnt(num_nodes + 1, 0);
+    for (int64_t e = 0; e < num_elements; ++e)
+        for (int a = 0; a < 8; ++a) cnt[conn[8 * e + a] + 1] += 8;
+    for (int64_t i = 0; i < num_nodes; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> adj(cnt[num_nodes]);
+    {
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int64_t e = 0; e < num_elements; ++e)
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b) adj[pos[conn[8 * e + a]]++] = conn[8 * e + b];
+    }
+    rowptr_b.assign(num_nodes + 1, 0);
+    colidx_b.clear();
+    for (int64_t i = 0; i < num_nodes; ++i) {
+        auto b = adj.begin() + cnt[i], en = adj.begin() + cnt[i + 1];
+        std::sort(b, en);
+        auto last = std::unique(b, en);
+        colidx_b.insert(colidx_b.end(), b, last);
+        if (colidx_b.size() > (size_t)INT32_MAX / 9) throw std::invalid_argument("pattern too large for int32 indices");
+        rowptr_b[i + 1] = (int32_t)colidx_b.size();
+    }
+    // greedy element colouring: lowest colour unused by any element sharing a node
+    std::vector<uint32_t> used(num_nodes, 0);
+    std::vector<int> color(num_elements);
+    int ncol = 0;
+    for (int64_t e = 0; e < num_elements; ++e) {
+        uint32_t m = 0;
+        for (int a = 0; a < 8; ++a) m |= used[conn[8 * e + a]];
+        int c = 0;
+        while (c < 32 && (m >> c) & 1u) ++c;
+        if (c == 32) throw std::invalid_argument("element colouring needs more than 32 colours");
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < 8; ++a) used[conn[8 * e + a]] |= 1u << c;
+    }
+    h.num_colors = ncol;
+    h.color_ptr.assign(ncol + 1, 0);
+    for (int64_t e = 0; e < num_elements; ++e) h.color_ptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; ++c) h.color_ptr[c + 1] += h.color_ptr[c];
+    std::vector<int> order(num_elements);
+    {
+        std::vector<int> p(h.color_ptr.begin(), h.color_ptr.end() - 1);
+        for (int64_t e = 0; e < num_elements; ++e) order[p[color[e]]++] = (int)e;
+    }
+    HCHECK(hipMalloc(&h.conn, (size_t)num_elements * 8 * sizeof(int)));
+    HCHECK(hipMalloc(&h.order, (size_t)num_elements * sizeof(int)));
+    HCHECK(hipMalloc(&h.A32, (size_t)num_elements * 36 * 9 * sizeof(float)));
+    HCHECK(hipMalloc(&h.diag64, (size_t)num_nodes * 3 * sizeof(double)));
+    HCHECK(hipMalloc(&h.diag32, (size_t)num_nodes * 3 * sizeof(float)));
+    HCHECK(hipMemcpyAsync(h.conn, conn, (size_t)num_elements * 8 * sizeof(int), hipMemcpyHostToDevice, s));
+    HCHECK(hipMemcpyAsync(h.order, order.data(), (size_t)num_elements * sizeof(int), hipMemcpyHostToDevice, s));
+    HCHECK(hipStreamSynchronize(s));
+    h.a32_dt = 0.0;
+    h.ready = true;
+}
+
+namespace {
+
+void hex_prepare(HexData& h, const double* x0, const double4* mat, double dt, hipStream_t s) {
+    const int64_t E = h.num_elements;
+    hipLaunchKernelGGL(hex_prepare_kernel, blocks_for(E * 36, 256), dim3(256), 0, s, (int)E, h.conn, x0, mat, dt,
+                       h.A32);
+    const int n = (int)(3 * h.num_nodes);
+    HCHECK(hipMemsetAsync(h.diag64, 0, (size_t)n * sizeof(double), s));
+    for (int c = 0; c < h.num_colors; ++c) {
+        const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
+        hipLaunchKernelGGL(hex_diag_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
+                           h.conn, x0, mat, dt, h.diag64);
+    }
+    hipLaunchKernelGGL(hex_cast_diag_kernel, blocks_for(n, 256), dim3(256), 0, s, n, h.diag64, h.diag32);
+    h.a32_dt = dt;
+}
+
+// y += sign (am M + ak K) w over all colours (FP64, matrix-free).
+void apply64(HexData& h, const double* x0, const double4* mat, const double* p, const double* xcur, const double* v,
+             double dt, const uint8_t* mask, double am, double ak, double sign, double* y, hipStream_t s) {
+    for (int c = 0; c < h.num_colors; ++c) {
+        const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
+        hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
+                           h.conn, x0, mat, p, xcur, v, dt, mask, am, ak, sign, y);
+    }
+}
+
+// Runs CG iterations with the element-by-element operator until the device reports done.
+template <typename TV>
+int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgScalars* h_sc, int& predicted,
+               TV* x, TV* r, TV* z, TV* p, TV* q, TV* dinv, hipStream_t s) {
+    const int n = (int)(3 * h.num_nodes);
+    const double dt = a.opt->dt;
+    auto iteration = [&](int k) {
+        hipLaunchKernelGGL((hex_cg_pupdate_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, k, a.max_iter, z,
+                           p, q, a.partial, sc);
+        if (k >= a.max_iter) return;
+        for (int c = 0; c < h.num_colors; ++c) {
+            const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
+            if (fp32)
+                hipLaunchKernelGGL(hex_apply32_kernel, blocks_for((int64_t)cnt * 8, 256), dim3(256), 0, s, cnt,
+                                   h.order + h.color_ptr[c], h.conn, h.A32, (const float*)p, a.bcmask, (float*)q);
+            else
+                hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt,
+                                   h.order + h.color_ptr[c], h.conn, a.x0, a.mat, (const double*)p, nullptr, nullptr,
+                                   dt, a.bcmask, 1.0 / dt, dt, 1.0, (double*)q);
+        }
+        hipLaunchKernelGGL((hex_cg_bc_pq_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, p, q,
+                           a.partial, sc);
+        launch_cg_vector_raw(fp32, n, k, x, r, z, p, q, dinv, a.partial, sc, s);
+    };
+    int k = 0;
+    const int first = std::max(1, std::min(predicted + 1, a.max_iter + 1));
+    for (; k < first; ++k) iteration(k);
+    for (;;) {
+        HCHECK(hipMemcpyAsync(h_sc, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+        HCHECK(hipStreamSynchronize(s));
+        if (h_sc->done || k > a.max_iter) break;
+        const int chunk = std::min(std::max(2, predicted / 8), a.max_iter + 1 - k);
+        for (int j = 0; j < chunk; ++j, ++k) iteration(k);
+    }
+    predicted = h_sc->iters + 1;
+    return h_sc->iters;
+}
+
+}  // namespace
+
+void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s) {
+    const fem_options& o = *a.opt;
+    const double dt = o.dt;
+    const int n = (int)(3 * h.num_nodes);
+    const bool fp32 = o.precision == FEM_PRECISION_FP32_REFINED;
+    if (h.a32_dt != dt) hex_prepare(h, a.x0, a.mat, dt, s);
+    // rhs = f - dt K v + gravity + fext = -K ((x - x0) + dt v) + fg + fext, BC DOFs zeroed
+    hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.b);
+    apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt, nullptr, 0.0, 1.0, -1.0, a.b, s);
+    hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, a.b);
+    CgScalars* sc = a.sc;
+    if (!fp32) {
+        double* x = a.dv;
+        hipLaunchKernelGGL((hex_cg_init_kernel<double>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.b, h.diag64,
+                           x, (double*)a.vec[1], (double*)a.vec[2], (double*)a.vec[5], a.bcmask, a.partial,
+                           (const CgScalars*)nullptr);
+        launch_cg_init2(a.partial, sc, nullptr, o.cg_tolerance, s);
+        const int it = run_hex_cg<double>(h, a, false, sc, a.h_sc, a.predicted[0], x, (double*)a.vec[1],
+                                          (double*)a.vec[2], (double*)a.vec[3], (double*)a.vec[4],
+                                          (double*)a.vec[5], s);
+        stats.last_cg_iterations = it;
+        stats.total_cg_iterations += it;
+        stats.last_cg_error = a.h_sc->err;
+        stats.converged = a.h_sc->converged;
+        stats.last_refinements = 0;
+    } else {
+        CgScalars* outer = sc + 1;
+        HCHECK(hipMemsetAsync(a.dv, 0, (size_t)n * sizeof(double), s));
+        int total = 0, sweeps = 0;
+        for (int sweep = 0; sweep <= o.max_refinements; ++sweep) {
+            if (sweep == 0) {
+                hipLaunchKernelGGL(hex_residual_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, a.b,
+                                   a.dv, a.dv, a.r64, a.partial);  // dv = 0: r = b
+            } else {
+                double* q64 = (double*)a.vec[6];
+                HCHECK(hipMemsetAsync(q64, 0, (size_t)n * sizeof(double), s));
+                apply64(h, a.x0, a.mat, a.dv, nullptr, nullptr, dt, a.bcmask, 1.0 / dt, dt, 1.0, q64, s);
+                hipLaunchKernelGGL(hex_residual_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, a.b,
+                                   a.dv, q64, a.r64, a.partial);
+            }
+            hipLaunchKernelGGL(hex_refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, sweep == 0,
+                               o.cg_tolerance, outer, sc);
+            if (sweep == o.max_refinements) break;
+            hipLaunchKernelGGL((hex_cg_init_kernel<float>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.r64,
+                               h.diag32, (float*)a.vec[0], (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[5],
+                               a.bcmask, a.partial, (const CgScalars*)outer);
+            launch_cg_init2(a.partial, sc, outer, o.inner_tolerance, s);
+            const int ps = 1 + std::min(sweep, 15);
+            const int it = run_hex_cg<float>(h, a, true, sc, a.h_sc, a.predicted[ps], (float*)a.vec[0],
+                                             (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[3], (float*)a.vec[4],
+                                             (float*)a.vec[5], s);
+            HCHECK(hipMemcpyAsync(a.h_sc + 1, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+            HCHECK(hipStreamSynchronize(s));
+            if (a.h_sc[1].done) break;
+            total += it + 1;
+            hipLaunchKernelGGL(hex_accumulate_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n,
+                               (const float*)a.vec[0], a.dv, outer);
+            sweeps++;
+        }
+        CgScalars oh;
+        HCHECK(hipMemcpyAsync(&oh, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+        HCHECK(hipStreamSynchronize(s));
+        stats.last_cg_iterations = total;
+        stats.total_cg_iterations += total;
+        stats.last_cg_error = oh.err;
+        stats.converged = oh.converged;
+        stats.last_refinements = sweeps;
+    }
+    launch_update(n, a.dv, a.x, a.x, a.v, dt, a.invalid, s);
+}
+
+void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, const double4* mat,
+                       double* R, double* K, double* f, hipStream_t s) {
+    if (count <= 0) return;
+    if (R) {
+        // rotation of a linear element: identity
+        std::vector<double> I(9 * (size_t)count, 0.0);
+        for (int t = 0; t < count; ++t) I[9 * t] = I[9 * t + 4] = I[9 * t + 8] = 1.0;
+        HCHECK(hipMemcpyAsync(R, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HCHECK(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(hex_eval_kernel, blocks_for(count, 64), dim3(64), 0, s, first, count, h.conn, x0, x, mat, K, f);
 }
 
 }  // namespace femgpu

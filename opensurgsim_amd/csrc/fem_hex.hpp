@@ -1,5 +1,11 @@
-// Hexahedral (Fem3DElementCube) path: element-by-element operator (Physics/Fem3DElementCube.cpp,
-// Physics/FemElement.cpp:168-210).
+// Hexahedral (Fem3DElementCube) path: element-by-element operator.
+//
+// Reference semantics (paths relative to the reference root):
+//   element K, M (2x2x2 Gauss)    Physics/Fem3DElementCube.cpp:65-366, Math/GaussLegendreQuadrature.cpp:30-34
+//   force f = -K (x - x0)         Physics/Fem3DElementCube.cpp:129-141
+//   element-by-element products   Physics/FemElement.cpp:168-210 (addMatVec)
+//   implicit step                 Math/OdeSolverEulerImplicit.cpp:55-161
+// K and M of a trilinear hex are constant, so the system matrix A = M/dt + dt K is constant too.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,20 +20,50 @@ namespace femgpu {
 struct HexData {
     int64_t num_nodes = 0;
     int64_t num_elements = 0;
+    int num_colors = 0;
+    std::vector<int> color_ptr;   // elements of color c: order[color_ptr[c] .. color_ptr[c + 1])
+    int* conn = nullptr;          // [E][8] node ids
+    int* order = nullptr;         // elements sorted by color
+    float* A32 = nullptr;         // [E][36 blocks][9] packed upper block triangle of M/dt + dt K (FP32)
+    double* diag64 = nullptr;     // diagonal of A (per DOF)
+    float* diag32 = nullptr;
+    double a32_dt = 0.0;          // dt the cached A32 / diagonal were built for
     bool ready = false;
 };
 
-// Element initialisation (volume/J asserts, lumped masses) and the block pattern of the would-be
-// assembled matrix (for the pattern query). Throws std::runtime_error on invalid elements.
+// Host set-up: element validity (positive volume, invertible J, Physics/Fem3DElementCube.cpp:169-211, 321-366),
+// lumped node masses (Physics/FemRepresentation.cpp:102-117), the node-pair block pattern and the
+// element colouring. mat: per element (lambda, mu, rho, E).
 void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elements, const int32_t* conn,
                const double* mat, std::vector<double>& mass_per_node, std::vector<int32_t>& rowptr_b,
                std::vector<int32_t>& colidx_b, hipStream_t s);
+void hex_release(HexData& h);
 
-void hex_step(HexData& h, const fem_options& opt, const double* x0, double* x, double* v, const double* fg,
-              const double* fext, const uint8_t* bcmask, double* partial, CgScalars* sc, CgScalars* h_sc,
-              int* invalid, hipStream_t s, int& predicted_iters, fem_stats& stats, int max_iter);
+// One implicit step on the hexahedral mesh. Work buffers come from the caller.
+struct HexStepArgs {
+    const fem_options* opt;
+    const double* x0;
+    double* x;
+    double* v;
+    const double* fg;       // gravity m_i g
+    const double* fext;     // may be null
+    const uint8_t* bcmask;
+    const double4* mat;
+    double* b;              // rhs
+    double* dv;             // solution
+    double* r64;
+    void* vec[7];           // x, r, z, p, q, dinv, spare (TV = float or double)
+    double* partial;
+    CgScalars* sc;          // [2]
+    CgScalars* h_sc;        // pinned host copies [2]
+    int* invalid;
+    int max_iter;
+    int* predicted;         // [17]: FP64 solve, then per refinement sweep
+};
+void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
 
-void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, double* R, double* K,
-                       double* f, hipStream_t s);
+// FP64 element outputs for tests: K (24x24 row-major) and f = -K (x - x0).
+void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, const double4* mat,
+                       double* R, double* K, double* f, hipStream_t s);
 
 }  // namespace femgpu

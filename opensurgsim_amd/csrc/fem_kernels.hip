@@ -12,53 +12,12 @@
 //   update           Math/OdeSolverEulerImplicit.cpp:98-99
 #include "fem_device.hpp"
 #include "fem_kernels.hpp"
+#include "fem_reduce.hpp"
 
 #include <algorithm>
 #include <cfloat>
 
 namespace femgpu {
-
-// ------------------------------------------------------------------------------------------
-// Reductions (deterministic: fixed lane/wave order).
-// ------------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Sums of two values over the block with a single barrier; results valid in thread 0.
-// lds must hold 2 * blockDim/64 doubles.
-__device__ __forceinline__ void block_sum2(double& v0, double& v1, double* lds) {
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        lds[w] = v0;
-        lds[nw + w] = v1;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        v0 = wave_sum(threadIdx.x < (unsigned)nw ? lds[threadIdx.x] : 0.0);
-        v1 = wave_sum(threadIdx.x < (unsigned)nw ? lds[nw + threadIdx.x] : 0.0);
-    }
-}
-
-// Sum over the block; result valid in thread 0. lds must hold blockDim/64 doubles.
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-    v = wave_sum(v);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    int nw = blockDim.x >> 6;
-    double t = 0.0;
-    if (threadIdx.x < 64) {
-        t = (threadIdx.x < (unsigned)nw) ? lds[threadIdx.x] : 0.0;
-        t = wave_sum(t);
-    }
-    __syncthreads();
-    return t;
-}
 
 // ------------------------------------------------------------------------------------------
 // Tetrahedron element kernel (FP64). One thread per element: rotation, force, the 16 stiffness
@@ -341,34 +300,6 @@ __device__ __forceinline__ void group_spmv(const Operator& op, const TM* __restr
 // and block 0 records the scalars that later kernels need.
 // Partial slots: [0, B) |r|^2 or |b|^2, [B, 2B) r.z, [2B, 3B) p.q, [3B, 4B) FP64 residual.
 // ------------------------------------------------------------------------------------------
-// Two sums of kCgBlocks partials each, formed by wave 0 in a fixed order and broadcast through LDS.
-__device__ __forceinline__ void reduce_partials2(const double* pa, const double* pb, double& a, double& b,
-                                                 double* lds) {
-    if (threadIdx.x < 64) {
-        double va = 0.0, vb = 0.0;
-        for (int i = threadIdx.x; i < kCgBlocks; i += 64) {
-            va += pa[i];
-            if (pb) vb += pb[i];
-        }
-        va = wave_sum(va);
-        vb = wave_sum(vb);
-        if (threadIdx.x == 0) {
-            lds[62] = va;
-            lds[63] = vb;
-        }
-    }
-    __syncthreads();
-    a = lds[62];
-    b = lds[63];
-    __syncthreads();
-}
-
-__device__ __forceinline__ double reduce_partials(const double* part, double* lds) {
-    double a, b;
-    reduce_partials2(part, nullptr, a, b, lds);
-    return a;
-}
-
 // x = 0, r = (TV) b, dinv = 1/diag (1 where the diagonal is 0, DiagonalPreconditioner), z = dinv r;
 // partial |r|^2 and r.z. Skipped when skip && skip->done (FP64 refinement already converged).
 template <typename TV>
@@ -543,6 +474,21 @@ void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
     else
         hipLaunchKernelGGL((cg_apply_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
                            (const double*)a.z, (const double*)po, (double*)pn, (double*)a.q, a.partial, a.sc);
+}
+
+void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s) {
+    hipLaunchKernelGGL(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sc, skip, tol);
+}
+
+void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, const void* p, const void* q,
+                          const void* dinv, double* partial, const CgScalars* sc, hipStream_t s) {
+    dim3 grid(kCgBlocks), block(kCgThreads);
+    if (fp32)
+        hipLaunchKernelGGL((cg_vector_kernel<float>), grid, block, 0, s, n, k, (float*)x, (float*)r, (float*)z,
+                           (const float*)p, (const float*)q, (const float*)dinv, partial, sc);
+    else
+        hipLaunchKernelGGL((cg_vector_kernel<double>), grid, block, 0, s, n, k, (double*)x, (double*)r, (double*)z,
+                           (const double*)p, (const double*)q, (const double*)dinv, partial, sc);
 }
 
 void launch_cg_vector(const CgArgs& a, int k, hipStream_t s) {

@@ -92,6 +92,12 @@ void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s);
 // Iteration k: alpha, x += alpha p, r -= alpha q, z = dinv r, r . r and r . z.
 void launch_cg_vector(const CgArgs& a, int k, hipStream_t s);
 
+// Scalars of a solve from the partials of an init kernel (|b|^2 in [0, B), r.z in [B, 2B)).
+void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s);
+// The vector part of iteration k on raw buffers (used by the element-by-element operators).
+void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, const void* p, const void* q,
+                          const void* dinv, double* partial, const CgScalars* sc, hipStream_t s);
+
 // FP64 iterative refinement around the FP32 CG: r64 = b - S64 dv (first: r64 = b) and the outer test.
 void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
