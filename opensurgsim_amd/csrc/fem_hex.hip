@@ -2,9 +2,8 @@
 // colouring (no two elements of one colour share a node, so the scatter needs no atomics and the
 // summation order is fixed).
 //
-//   FP32 inner CG:  cached A_e = M_e / dt + dt K_e as 36 packed 3x3 blocks (a <= b), 8 lanes per
-//                   element (lane a owns node a's three rows; the transposed halves of the
-//                   off-diagonal blocks go to the other lanes through shuffles).
+//   FP32 inner CG:  cached A_e = M_e / dt + dt K_e as 36 packed 3x3 blocks (a <= b), one element per
+//                   lane, stored interleaved by 64-element tiles so every load is wave-contiguous.
 //   FP64 pieces:    matrix-free Gauss-point operator: (A p)_a = sum_g w_g detJ_g [rho N_a (sum_b N_b p_b) / dt
 //                   + dt sigma(p) g_a], sigma = lambda tr(eps) I + 2 mu eps, eps = sym(sum_b p_b g_b^T);
 //                   sigma g_a is the B_a^T C B p of Physics/Fem3DElementCube.cpp:213-288.
@@ -66,21 +65,42 @@ __host__ __device__ __forceinline__ double hex_gp(const double* X, int gi, int g
 
 __device__ __forceinline__ constexpr int blk_index(int a, int b) { return 8 * a - a * (a - 1) / 2 + (b - a); }
 
-// ------------------------------------------------------------------------------------------
-// Set-up: packed FP32 A_e (one thread per (element, block)) and the FP64 diagonal (coloured).
-// ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void hex_prepare_kernel(int E, const int* __restrict__ conn,
-                                                          const double* __restrict__ x0, const double4* __restrict__ mat,
-                                                          double dt, float* __restrict__ A32) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= (int64_t)E * 36) return;
-    const int e = (int)(t / 36), kb = (int)(t % 36);
-    int a = 0, rem = kb;
-    while (rem >= 8 - a) {
-        rem -= 8 - a;
+// Packed block k of the upper block triangle (a <= b), row-major over a: k = blk_index(a, b).
+__host__ __device__ constexpr int blk_a(int k) {
+    int a = 0;
+    while (k >= 8 - a) {
+        k -= 8 - a;
         ++a;
     }
-    const int b = a + rem;
+    return a;
+}
+__host__ __device__ constexpr int blk_b(int k) {
+    int a = 0;
+    while (k >= 8 - a) {
+        k -= 8 - a;
+        ++a;
+    }
+    return a + k;
+}
+
+// Cached FP32 element matrices, interleaved for one element per lane: position t of the padded colour
+// order (tile T = t / 64, lane l = t % 64) keeps its 324 floats (36 blocks x 9, row-major within a
+// block) as 81 float4 chunks at A4[(81 T + k) 64 + l], so each wave-wide load reads 1 KiB contiguous.
+constexpr int kHexChunks = 81;
+
+// ------------------------------------------------------------------------------------------
+// Set-up: A_e = M_e / dt + dt K_e in FP32, one thread per (element, packed block).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void hex_prepare_kernel(int count_pad, const int* __restrict__ order_pad,
+                                                          const int* __restrict__ conn,
+                                                          const double* __restrict__ x0, const double4* __restrict__ mat,
+                                                          double dt, float* __restrict__ A32) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)count_pad * 36) return;
+    const int t = (int)(tid / 36), kb = (int)(tid % 36);
+    const int e = order_pad[t];
+    if (e < 0) return;
+    const int a = blk_a(kb), b = blk_b(kb);
     double X[24];
     for (int i = 0; i < 8; ++i)
         for (int c = 0; c < 3; ++c) X[3 * i + c] = x0[3 * (size_t)conn[8 * e + i] + c];
@@ -99,8 +119,11 @@ __global__ __launch_bounds__(256) void hex_prepare_kernel(int E, const int* __re
                                                   (i == j ? m.y * gab : 0.0)) +
                                         (i == j ? ms : 0.0);
             }
-    float* o = A32 + ((size_t)e * 36 + kb) * 9;
-    for (int i = 0; i < 9; ++i) o[i] = (float)A[i];
+    float* base = A32 + ((size_t)(t >> 6) * kHexChunks * 64) * 4 + (size_t)(t & 63) * 4;
+    for (int i = 0; i < 9; ++i) {
+        const int f = 9 * kb + i;
+        base[(size_t)(f >> 2) * 256 + (f & 3)] = (float)A[i];
+    }
 }
 
 // FP64 element operator y = alpha_m M p + alpha_k K p for one element (matrix-free).
@@ -139,9 +162,10 @@ __global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* 
                                                           const double4* __restrict__ mat, const double* __restrict__ p,
                                                           const double* __restrict__ xcur, const double* __restrict__ v,
                                                           double dt, const uint8_t* __restrict__ mask, double am,
-                                                          double ak, double sign, double* __restrict__ y) {
+                                                          double ak, double sign, double* __restrict__ y,
+                                                          const CgScalars* __restrict__ sc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    if (t >= count || (sc && sc->done)) return;
     const int e = order[t];
     int ids[8];
     for (int i = 0; i < 8; ++i) ids[i] = conn[8 * e + i];
@@ -158,50 +182,59 @@ __global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* 
         for (int c = 0; c < 3; ++c) y[3 * (size_t)ids[i] + c] += sign * out[3 * i + c];
 }
 
-// FP32 cached operator: q[nodes] += A_e p_e (p masked on BC DOFs), 8 lanes per element.
-__global__ __launch_bounds__(256) void hex_apply32_kernel(int count, const int* __restrict__ order,
-                                                          const int* __restrict__ conn, const float* __restrict__ A32,
-                                                          const float* __restrict__ p, const uint8_t* __restrict__ mask,
-                                                          float* __restrict__ q) {
+// FP32 cached operator over one colour: q[nodes] += A_e p_e (p masked on BC DOFs), one element per
+// lane; the 36 blocks stream in 9 groups of 4 (36 floats = 9 float4 chunks per group).
+__global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const int* __restrict__ order_pad,
+                                                          const int* __restrict__ conn,
+                                                          const float4* __restrict__ A4,
+                                                          const float* __restrict__ p,
+                                                          const uint8_t* __restrict__ mask, float* __restrict__ q,
+                                                          const CgScalars* __restrict__ sc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int el = t >> 3, a = t & 7;
-    const bool active = el < count;
-    const int e = active ? order[el] : 0;
-    const int node = active ? conn[8 * e + a] : 0;
-    float pa[3] = {0.f, 0.f, 0.f};
-    if (active)
-        for (int c = 0; c < 3; ++c) pa[c] = mask[3 * node + c] ? 0.f : p[3 * node + c];
-    float y[3] = {0.f, 0.f, 0.f};
-    const float* Ae = A32 + (size_t)e * 36 * 9;
+    if (t >= count_pad || sc->done) return;
+    const int e = order_pad[t];
+    if (e < 0) return;
+    const int4* c4 = reinterpret_cast<const int4*>(conn) + 2 * (size_t)e;
+    const int4 n0 = c4[0], n1 = c4[1];
+    const int ids[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+    float pv[24], y[24];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        float pb[3];
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pb[c] = __shfl(pa[c], b, 8);
-        float tb[3] = {0.f, 0.f, 0.f};
-        if (active && b >= a) {
-            const float* B = Ae + 9 * blk_index(a, b);
-            float Bv[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Bv[i] = B[i];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) y[i] += Bv[3 * i] * pb[0] + Bv[3 * i + 1] * pb[1] + Bv[3 * i + 2] * pb[2];
-            if (b > a) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) tb[j] = Bv[j] * pa[0] + Bv[3 + j] * pa[1] + Bv[6 + j] * pa[2];
-            }
+        for (int c = 0; c < 3; ++c) {
+            const int d = 3 * ids[a] + c;
+            pv[3 * a + c] = mask[d] ? 0.f : p[d];
+            y[3 * a + c] = 0.f;
         }
-        // sum over the lanes a < b of B_ab^T p_a; lane b keeps it
+    const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
 #pragma unroll
-        for (int o = 4; o > 0; o >>= 1)
+    for (int g = 0; g < 9; ++g) {
+        float w[36];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) tb[j] += __shfl_xor(tb[j], o, 8);
-        if (a == b)
+        for (int k = 0; k < 9; ++k) {
+            const float4 v = At[(size_t)(9 * g + k) * 64];
+            w[4 * k] = v.x;
+            w[4 * k + 1] = v.y;
+            w[4 * k + 2] = v.z;
+            w[4 * k + 3] = v.w;
+        }
 #pragma unroll
-            for (int j = 0; j < 3; ++j) y[j] += tb[j];
+        for (int m = 0; m < 4; ++m) {
+            const int a = blk_a(4 * g + m), b = blk_b(4 * g + m);
+            const float* B = w + 9 * m;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                y[3 * a + i] += B[3 * i] * pv[3 * b] + B[3 * i + 1] * pv[3 * b + 1] + B[3 * i + 2] * pv[3 * b + 2];
+            if (b != a)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    y[3 * b + j] += B[j] * pv[3 * a] + B[3 + j] * pv[3 * a + 1] + B[6 + j] * pv[3 * a + 2];
+        }
     }
-    if (active)
-        for (int c = 0; c < 3; ++c) q[3 * node + c] += y[c];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[3 * ids[a] + c] += y[3 * a + c];
 }
 
 // FP64 diagonal of A over one colour.
@@ -456,9 +489,9 @@ double host_hex_volume(const double* X, int64_t e) {
 }  // namespace
 
 void hex_release(HexData& h) {
-    for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.A32, (void*)h.diag64, (void*)h.diag32})
+    for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.order_pad, (void*)h.A32, (void*)h.diag64, (void*)h.diag32})
         if (p) (void)hipFree(p);
-    h.conn = h.order = nullptr;
+    h.conn = h.order = h.order_pad = nullptr;
     h.A32 = nullptr;
     h.diag64 = nullptr;
     h.diag32 = nullptr;
@@ -525,9 +558,17 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
         std::vector<int> p(h.color_ptr.begin(), h.color_ptr.end() - 1);
         for (int64_t e = 0; e < num_elements; ++e) order[p[color[e]]++] = (int)e;
     }
+    // padded order: each colour starts on a 64-element tile (-1 marks padding)
+    h.pad_ptr.assign(ncol + 1, 0);
+    for (int c = 0; c < ncol; ++c) h.pad_ptr[c + 1] = h.pad_ptr[c] + (h.color_ptr[c + 1] - h.color_ptr[c] + 63) / 64 * 64;
+    std::vector<int> order_pad(h.pad_ptr[ncol], -1);
+    for (int c = 0; c < ncol; ++c)
+        std::copy(order.begin() + h.color_ptr[c], order.begin() + h.color_ptr[c + 1], order_pad.begin() + h.pad_ptr[c]);
     HCHECK(hipMalloc(&h.conn, (size_t)num_elements * 8 * sizeof(int)));
     HCHECK(hipMalloc(&h.order, (size_t)num_elements * sizeof(int)));
-    HCHECK(hipMalloc(&h.A32, (size_t)num_elements * 36 * 9 * sizeof(float)));
+    HCHECK(hipMalloc(&h.order_pad, order_pad.size() * sizeof(int)));
+    HCHECK(hipMalloc(&h.A32, order_pad.size() * 36 * 9 * sizeof(float)));
+    HCHECK(hipMemcpyAsync(h.order_pad, order_pad.data(), order_pad.size() * sizeof(int), hipMemcpyHostToDevice, s));
     HCHECK(hipMalloc(&h.diag64, (size_t)num_nodes * 3 * sizeof(double)));
     HCHECK(hipMalloc(&h.diag32, (size_t)num_nodes * 3 * sizeof(float)));
     HCHECK(hipMemcpyAsync(h.conn, conn, (size_t)num_elements * 8 * sizeof(int), hipMemcpyHostToDevice, s));
@@ -540,9 +581,9 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
 namespace {
 
 void hex_prepare(HexData& h, const double* x0, const double4* mat, double dt, hipStream_t s) {
-    const int64_t E = h.num_elements;
-    hipLaunchKernelGGL(hex_prepare_kernel, blocks_for(E * 36, 256), dim3(256), 0, s, (int)E, h.conn, x0, mat, dt,
-                       h.A32);
+    const int pad = h.pad_ptr.back();
+    hipLaunchKernelGGL(hex_prepare_kernel, blocks_for((int64_t)pad * 36, 256), dim3(256), 0, s, pad, h.order_pad,
+                       h.conn, x0, mat, dt, h.A32);
     const int n = (int)(3 * h.num_nodes);
     HCHECK(hipMemsetAsync(h.diag64, 0, (size_t)n * sizeof(double), s));
     for (int c = 0; c < h.num_colors; ++c) {
@@ -560,7 +601,7 @@ void apply64(HexData& h, const double* x0, const double4* mat, const double* p, 
     for (int c = 0; c < h.num_colors; ++c) {
         const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
         hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
-                           h.conn, x0, mat, p, xcur, v, dt, mask, am, ak, sign, y);
+                           h.conn, x0, mat, p, xcur, v, dt, mask, am, ak, sign, y, (const CgScalars*)nullptr);
     }
 }
 
@@ -576,13 +617,15 @@ int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgSca
         if (k >= a.max_iter) return;
         for (int c = 0; c < h.num_colors; ++c) {
             const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
-            if (fp32)
-                hipLaunchKernelGGL(hex_apply32_kernel, blocks_for((int64_t)cnt * 8, 256), dim3(256), 0, s, cnt,
-                                   h.order + h.color_ptr[c], h.conn, h.A32, (const float*)p, a.bcmask, (float*)q);
-            else
+            if (fp32) {
+                const int c0 = h.pad_ptr[c], pc = h.pad_ptr[c + 1] - c0;
+                hipLaunchKernelGGL(hex_apply32_kernel, blocks_for(pc, 256), dim3(256), 0, s, pc, h.order_pad + c0,
+                                   h.conn, reinterpret_cast<const float4*>(h.A32) + (size_t)c0 * kHexChunks,
+                                   (const float*)p, a.bcmask, (float*)q, (const CgScalars*)sc);
+            } else
                 hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt,
                                    h.order + h.color_ptr[c], h.conn, a.x0, a.mat, (const double*)p, nullptr, nullptr,
-                                   dt, a.bcmask, 1.0 / dt, dt, 1.0, (double*)q);
+                                   dt, a.bcmask, 1.0 / dt, dt, 1.0, (double*)q, (const CgScalars*)sc);
         }
         hipLaunchKernelGGL((hex_cg_bc_pq_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, p, q,
                            a.partial, sc);

@@ -24,7 +24,10 @@ struct HexData {
     std::vector<int> color_ptr;   // elements of color c: order[color_ptr[c] .. color_ptr[c + 1])
     int* conn = nullptr;          // [E][8] node ids
     int* order = nullptr;         // elements sorted by color
-    float* A32 = nullptr;         // [E][36 blocks][9] packed upper block triangle of M/dt + dt K (FP32)
+    std::vector<int> pad_ptr;     // colour c: order_pad[pad_ptr[c] .. pad_ptr[c + 1]), multiples of 64
+    int* order_pad = nullptr;     // colour order padded to 64-element tiles (-1 = padding)
+    float* A32 = nullptr;         // packed upper block triangle of M/dt + dt K (FP32), 324 floats per padded
+                                  // position, interleaved by 64-position tiles in float4 chunks
     double* diag64 = nullptr;     // diagonal of A (per DOF)
     float* diag32 = nullptr;
     double a32_dt = 0.0;          // dt the cached A32 / diagonal were built for
