@@ -71,14 +71,28 @@ def run_timed(cfg, precision, warmup, steps, timer_every):
     st = g.stats()
     phases = g.phase_times()
     x, v = g.state()
+    op_ms = g.time_operator(100)
     g.close()
-    return dict(seconds=t1 - t0, cg_iters=st["total_cg_iterations"] - it0, stats=st, phases=phases, x=x, v=v)
+    return dict(seconds=t1 - t0, cg_iters=st["total_cg_iterations"] - it0, stats=st, phases=phases, x=x, v=v,
+                op_ms=op_ms)
 
 
 def spmv_bytes(cfg_sizes, w):
+    """Algorithmic bytes of one cg_apply_kernel launch: BSR values and indices once, z and p_old read
+    (p = z + beta p_old is formed on the fly), p_new and q written."""
     N, D, Bk = cfg_sizes
     Z = 9 * Bk
-    return Z * w + Bk * 4 + (N + 1) * 4 + 2 * D * w
+    return Z * w + Bk * 4 + (N + 1) * 4 + 4 * D * w
+
+
+def copy_probe(mib, reps=3):
+    """A plain device copy of known size (probe_copy_kernel), for scaling the FETCH_SIZE / WRITE_SIZE
+    counters and for the achievable copy bandwidth."""
+    nbytes = mib * (1 << 20)
+    ms = femgpu.probe_copy(0, nbytes, reps)
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    log("copy probe: %d MiB, %.3f ms per copy, %.0f GB/s (read + write)" % (mib, ms, gbs))
+    return gbs
 
 
 def cpu_baseline(cfg, sample_steps):
@@ -176,6 +190,8 @@ def main():
                     help="bracket every kernel of one step in N with HIP events (0: no events)")
     ap.add_argument("--workload", default="corotational", choices=["corotational", "hex"])
     ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB three times (counter calibration)")
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -184,6 +200,8 @@ def main():
         sys.exit(2)
     if args.workload == "hex":
         bench_hex(args)
+        if args.copy_probe:
+            copy_probe(args.copy_probe)
         return
 
     n = (args.n, args.n, args.n)
@@ -210,18 +228,23 @@ def main():
     value = D * args.steps / head["seconds"]
 
     w = 4 if use32 else 8
+    # average launch of the dominant kernel: 100 back-to-back launches on the final state between one
+    # event pair (per-launch event pairs inside the steps add their own overhead; those averages are
+    # reported under "phases")
     sp_ms, sp_n = head["phases"]["spmv"]
-    sp_avg_s = sp_ms / max(sp_n, 1) / 1e3
+    sp_avg_s = head["op_ms"] / 1e3
     bytes_per = spmv_bytes((N, D, Bk), w)
-    achieved = bytes_per / sp_avg_s / 1e9 if sp_n else 0.0
+    achieved = bytes_per / sp_avg_s / 1e9
+    # HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE counter runs, scaled by a copy kernel of
+    # known size (profiles/hbm_traffic_corotational_40x40x40.md); only valid for the 40^3 block
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "spmv_traffic.json")
-    if os.path.exists(pmc):
+    pmc = os.path.join(ROOT, "profiles", "cg_apply_traffic.json")
+    if os.path.exists(pmc) and n == (40, 40, 40):
         try:
             with open(pmc) as fh:
                 t = json.load(fh)
-            traffic = t.get("fp32" if use32 else "fp64")
-        except (OSError, ValueError):
+            traffic = t["bytes_per_dispatch"].get("cg_apply_kernel<%s>" % ("float" if use32 else "double"))
+        except (OSError, ValueError, KeyError):
             traffic = None
 
     cpu = None
@@ -235,6 +258,7 @@ def main():
                "gpu_fp64_vs_oracle_rel_err": err_or}
 
     phases = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in head["phases"].items()}
+    copy_gbs = copy_probe(args.copy_probe or 512)
     out = {
         "metric": METRIC,
         "value": value,
@@ -252,10 +276,11 @@ def main():
                                "Jacobi-CG 1e-8" % n,
                    "elements": int(cfg.conn.shape[0]), "nodes": int(N), "dof": int(D), "nnz": int(9 * Bk),
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": "cg_spmv_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "cg_apply_kernel<%s>" % ("float" if use32 else "double"), "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_us": sp_avg_s * 1e6,
-                     "launches": sp_n},
+                     "avg_launch_us_in_step_events": 1e3 * sp_ms / max(sp_n, 1), "in_step_launches": sp_n,
+                     "measured_copy_GBps": copy_gbs},
         "cpu_baseline": cpu,
         "fp64": {"value": D * args.steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / args.steps,
                  "cg_iters_per_step": r64["cg_iters"] / args.steps},

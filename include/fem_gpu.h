@@ -151,10 +151,12 @@ int fem_get_rhs(fem_ctx* ctx, double* rhs);
 int fem_evaluate_elements(fem_ctx* ctx, const double* x, int64_t first, int64_t count, double* rotation,
                           double* stiffness, double* force);
 
-/* Per-phase HIP-event timers on the context stream. No counterpart in the reference's FEM path; they
+/* Per-phase kernel timers on the context stream. No counterpart in the reference's FEM path; they
  * stand in for its Framework::Timer instrumentation around update(dt) (Framework/Timer.h:31-93,
- * Physics/PhysicsManager.cpp:130-152). spmv_* count only the CG operator applications that did work
- * (launches skipped after convergence are reported separately). */
+ * Physics/PhysicsManager.cpp:130-152). Each timed launch carries a start/stop event pair in the
+ * dispatch packet of the first kernel its phase issues, so a phase time is that kernel's execution
+ * time. spmv counts only the CG operator applications that did work (launches after convergence
+ * are reported as spmv_skipped). */
 enum {
     FEM_PHASE_ELEMENT = 0, FEM_PHASE_ASSEMBLE, FEM_PHASE_RHS, FEM_PHASE_CG_INIT, FEM_PHASE_SPMV,
     FEM_PHASE_CG_VECTOR, FEM_PHASE_REFINE, FEM_PHASE_UPDATE, FEM_PHASE_SPMV_SKIPPED, FEM_NUM_PHASES
@@ -165,6 +167,17 @@ typedef struct fem_phase_times {
 } fem_phase_times;
 int fem_enable_phase_timers(fem_ctx* ctx, int every);   /* instrument one step in `every` (0: off); resets totals */
 int fem_get_phase_times(fem_ctx* ctx, fem_phase_times* out);
+
+/* Measurement helper with no counterpart in the reference: `reps` back-to-back launches of the CG
+ * operator kernel (fused p update + block SpMV + p.q partials) of a tetrahedral mesh, on the CG scratch
+ * buffers of the current precision, between one event pair; average milliseconds per launch. The
+ * simulation state is untouched; call after at least one step. */
+int fem_time_operator(fem_ctx* ctx, int reps, double* avg_ms);
+
+/* Measurement helper with no counterpart in the reference: `reps` timed copies of `bytes` bytes
+ * (rounded down to 16) between two scratch buffers on `device`; average milliseconds per copy.
+ * Used to calibrate HBM byte counters and to report achievable copy bandwidth. */
+int fem_probe_copy(int device, int64_t bytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,7 @@ struct PhaseTimer {
     };
     std::vector<hipEvent_t> pool;
     std::vector<Pair> pending;
-    hipEvent_t open_ev = nullptr;
+    hipEvent_t open_ev = nullptr, close_ev = nullptr;
     int open_phase = 0, open_tag = 0;
     double ms[FEM_NUM_PHASES] = {};
     int64_t launches[FEM_NUM_PHASES] = {};
@@ -84,16 +84,23 @@ struct PhaseTimer {
         pool.pop_back();
         return e;
     }
-    void begin(hipStream_t s, int phase, int tag = 0) {
+    // Arms the first kernel launched by the phase's launcher with a start/stop event pair.
+    void begin(hipStream_t, int phase, int tag = 0) {
         open_ev = get();
+        close_ev = get();
         open_phase = phase;
         open_tag = tag;
-        HIPCHECK(hipEventRecord(open_ev, s));
+        g_launch_timing.start = open_ev;
+        g_launch_timing.stop = close_ev;
     }
-    void end(hipStream_t s) {
-        hipEvent_t b = get();
-        HIPCHECK(hipEventRecord(b, s));
-        pending.push_back({open_ev, b, open_phase, open_tag});
+    void end(hipStream_t) {
+        if (g_launch_timing.start) {  // nothing was launched: return the pair
+            g_launch_timing = LaunchTiming{};
+            pool.push_back(open_ev);
+            pool.push_back(close_ev);
+            return;
+        }
+        pending.push_back({open_ev, close_ev, open_phase, open_tag});
     }
     // Call after the stream has been synchronised. SpMV pairs with tag > spmv_last were launched after
     // convergence (no work) and are booked separately.
@@ -166,6 +173,7 @@ struct fem_ctx {
     DevBuf<unsigned char> cg_x, cg_r, cg_z, cg_p0, cg_p1, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
     DevBuf<double> partial;    // [4 * kCgBlocks] block sums of the grid reductions
     DevBuf<CgScalars> scal;    // [0] inner/FP64 solve, [1] outer refinement
+    DevBuf<CgScalars> time_sc; // operator replay timing (fem_time_operator)
     DevBuf<int> counters;      // [0..2] element flags, [3] invalid state, [4] Newton max-norm bits (int pair)
     DevBuf<unsigned long long> nrm;
     int num_lower = 0;
@@ -935,6 +943,41 @@ int fem_enable_phase_timers(fem_ctx* c, int enable) {
         c->timer_every = enable > 0 ? enable : 0;
         c->timer_count = 0;
         c->timer.enabled = false;
+        return FEM_OK;
+    });
+}
+
+int fem_time_operator(fem_ctx* c, int reps, double* avg_ms) {
+    return guarded(c, [&] {
+        if (!avg_ms || reps < 1) throw InvalidArg("reps must be positive and avg_ms non-null");
+        if (c->kind != FEM_ELEMENT_TET_LINEAR && c->kind != FEM_ELEMENT_TET_COROTATIONAL)
+            throw InvalidArg("operator replay timing covers the assembled (tetrahedral) operator");
+        if (c->stats.steps < 1) return fail(c, FEM_ERR_ORDER, "step at least once before timing the operator");
+        HIPCHECK(hipSetDevice(c->device));
+        const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
+        if (!c->time_sc.p) c->time_sc.alloc(1);
+        // never converges (threshold < 0), fixed rz so every replay does identical work on CG scratch
+        CgScalars init{};
+        init.rz[0] = init.rz[1] = 1.0;
+        init.bb = 1.0;
+        init.threshold = -1.0;
+        HIPCHECK(hipMemcpyAsync(c->time_sc.p, &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+        CgArgs a = cg_args(c, fp32, 0.0);
+        a.sc = c->time_sc.p;
+        const int big = std::numeric_limits<int>::max() - 1;
+        launch_cg_apply(a, 1, big, c->stream);  // warm-up
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) launch_cg_apply(a, 1, big, c->stream);
+        HIPCHECK(hipEventRecord(e1, c->stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = t / reps;
         return FEM_OK;
     });
 }

@@ -19,6 +19,9 @@
 
 namespace femgpu {
 
+thread_local LaunchTiming g_launch_timing;
+
+
 // ------------------------------------------------------------------------------------------
 // Tetrahedron element kernel (FP64). One thread per element: rotation, force, the 16 stiffness
 // blocks, the right-hand-side contribution f + K (xn - xs - vn dt) - M (vn - vs) / dt and the
@@ -134,11 +137,11 @@ void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     if (a.num_elements == 0) return;
     dim3 grid((a.num_elements + 255) / 256), block(256);
     if (kind == 1) {
-        if (a.general) hipLaunchKernelGGL((tet_element_kernel<true, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((tet_element_kernel<true, false>), grid, block, 0, s, a);
+        if (a.general) launch_k((tet_element_kernel<true, true>), grid, block, 0, s, a);
+        else launch_k((tet_element_kernel<true, false>), grid, block, 0, s, a);
     } else {
-        if (a.general) hipLaunchKernelGGL((tet_element_kernel<false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((tet_element_kernel<false, false>), grid, block, 0, s, a);
+        if (a.general) launch_k((tet_element_kernel<false, true>), grid, block, 0, s, a);
+        else launch_k((tet_element_kernel<false, false>), grid, block, 0, s, a);
     }
 }
 
@@ -214,7 +217,7 @@ void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk,
                      const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
                      const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s) {
     if (num_lower == 0) return;
-    hipLaunchKernelGGL(assemble_kernel, dim3((num_lower + 255) / 256), dim3(256), 0, s, num_lower, lower_blk,
+    launch_k(assemble_kernel, dim3((num_lower + 255) / 256), dim3(256), 0, s, num_lower, lower_blk,
                        mirror_blk, blk_row, blk_col, ctr_ptr, ctr_list, rec, bcmask, plane, S64, S32);
 }
 
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __re
 void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const double* rhs_rec, const double* fg,
                 const double* fext, const uint8_t* bcmask, double* b, hipStream_t s) {
     if (num_nodes == 0) return;
-    hipLaunchKernelGGL(rhs_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, node_ptr, node_list,
+    launch_k(rhs_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, node_ptr, node_list,
                        rhs_rec, fg, fext, bcmask, b);
 }
 
@@ -456,12 +459,12 @@ __global__ __launch_bounds__(kCgThreads) void cg_vector_kernel(int n, int k, TV*
 void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {
     dim3 grid(kCgBlocks), block(kCgThreads);
     if (a.fp32)
-        hipLaunchKernelGGL((cg_init_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, b, (float*)a.x,
+        launch_k((cg_init_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, b, (float*)a.x,
                            (float*)a.r, (float*)a.z, (float*)a.dinv, a.partial, skip);
     else
-        hipLaunchKernelGGL((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
+        launch_k((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
                            (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial, skip);
-    hipLaunchKernelGGL(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
+    launch_k(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
 }
 
 void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
@@ -469,25 +472,25 @@ void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
     void* pn = a.p[k & 1];
     const void* po = a.p[(k + 1) & 1];
     if (a.fp32)
-        hipLaunchKernelGGL((cg_apply_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, k, max_iter,
+        launch_k((cg_apply_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, k, max_iter,
                            (const float*)a.z, (const float*)po, (float*)pn, (float*)a.q, a.partial, a.sc);
     else
-        hipLaunchKernelGGL((cg_apply_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
+        launch_k((cg_apply_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
                            (const double*)a.z, (const double*)po, (double*)pn, (double*)a.q, a.partial, a.sc);
 }
 
 void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s) {
-    hipLaunchKernelGGL(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sc, skip, tol);
+    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sc, skip, tol);
 }
 
 void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, const void* p, const void* q,
                           const void* dinv, double* partial, const CgScalars* sc, hipStream_t s) {
     dim3 grid(kCgBlocks), block(kCgThreads);
     if (fp32)
-        hipLaunchKernelGGL((cg_vector_kernel<float>), grid, block, 0, s, n, k, (float*)x, (float*)r, (float*)z,
+        launch_k((cg_vector_kernel<float>), grid, block, 0, s, n, k, (float*)x, (float*)r, (float*)z,
                            (const float*)p, (const float*)q, (const float*)dinv, partial, sc);
     else
-        hipLaunchKernelGGL((cg_vector_kernel<double>), grid, block, 0, s, n, k, (double*)x, (double*)r, (double*)z,
+        launch_k((cg_vector_kernel<double>), grid, block, 0, s, n, k, (double*)x, (double*)r, (double*)z,
                            (const double*)p, (const double*)q, (const double*)dinv, partial, sc);
 }
 
@@ -495,10 +498,10 @@ void launch_cg_vector(const CgArgs& a, int k, hipStream_t s) {
     dim3 grid(kCgBlocks), block(kCgThreads);
     const void* p = a.p[k & 1];
     if (a.fp32)
-        hipLaunchKernelGGL((cg_vector_kernel<float>), grid, block, 0, s, a.n, k, (float*)a.x, (float*)a.r,
+        launch_k((cg_vector_kernel<float>), grid, block, 0, s, a.n, k, (float*)a.x, (float*)a.r,
                            (float*)a.z, (const float*)p, (const float*)a.q, (const float*)a.dinv, a.partial, a.sc);
     else
-        hipLaunchKernelGGL((cg_vector_kernel<double>), grid, block, 0, s, a.n, k, (double*)a.x, (double*)a.r,
+        launch_k((cg_vector_kernel<double>), grid, block, 0, s, a.n, k, (double*)a.x, (double*)a.r,
                            (double*)a.z, (const double*)p, (const double*)a.q, (const double*)a.dinv, a.partial,
                            a.sc);
 }
@@ -574,13 +577,13 @@ __global__ __launch_bounds__(kCgThreads) void accumulate_kernel(int n, const flo
 void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
                        hipStream_t s) {
-    hipLaunchKernelGGL(residual64_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, op, S64, b, dv, first, r64,
+    launch_k(residual64_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, op, S64, b, dv, first, r64,
                        partial);
-    hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, first, tol, outer, inner);
+    launch_k(refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, first, tol, outer, inner);
 }
 
 void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s) {
-    hipLaunchKernelGGL(accumulate_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, xf, dv, outer);
+    launch_k(accumulate_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, xf, dv, outer);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -600,7 +603,7 @@ __global__ __launch_bounds__(256) void update_kernel(int n, const double* __rest
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
                    hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid);
+    launch_k(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -653,7 +656,7 @@ __global__ void eval_tet_kernel(int kind, int first, int count, const int4* conn
 void launch_eval_elements(int kind, int first, int count, const int4* conn, const double* x0, const double* x,
                           const double4* mat, double* R, double* K, double* f, hipStream_t s) {
     if (count == 0) return;
-    hipLaunchKernelGGL(eval_tet_kernel, dim3((count + 127) / 128), dim3(128), 0, s, kind, first, count, conn, x0, x,
+    launch_k(eval_tet_kernel, dim3((count + 127) / 128), dim3(128), 0, s, kind, first, count, conn, x0, x,
                        mat, R, K, f);
 }
 

@@ -2,9 +2,29 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 
 namespace femgpu {
+
+// Kernel timing: when a start/stop pair is armed, the next launch made through launch_k carries the
+// two events in its dispatch packet (hipExtLaunchKernelGGL), so their interval is the kernel's own
+// execution, without the queueing gaps that events recorded around a launch would include.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
+
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
+    if (g_launch_timing.start) {
+        const LaunchTiming t = g_launch_timing;
+        g_launch_timing = LaunchTiming{};
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, t.start, t.stop, 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+}
 
 // Per-element record written by the tet element kernel and read by the assembly kernel (FP64):
 // slots [0, 54): the 6 node-pair blocks (pair order (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) of
