@@ -51,9 +51,9 @@ enum {
 /* Arithmetic of the element kernels, assembly and CG. */
 enum {
     FEM_PRECISION_FP64 = 0,         /* everything in FP64 */
-    FEM_PRECISION_FP32_REFINED = 1  /* FP32 elements/SpMV/CG, FP64 iterative refinement of the
-                                       implicit-step residual; FP64 polar decomposition for
-                                       near-degenerate or inverted elements */
+    FEM_PRECISION_FP32_REFINED = 1  /* FP32 operator copy, SpMV and CG inside FP64 iterative
+                                       refinement of the implicit-step residual; element kernels,
+                                       polar decomposition and assembly stay FP64 */
 };
 
 typedef struct fem_options {
