@@ -130,6 +130,11 @@ int fem_step(fem_ctx* ctx, int32_t nsteps);
 int fem_synchronize(fem_ctx* ctx);
 int fem_query(fem_ctx* ctx, fem_stats* stats);
 
+/* Lumped node masses (num_nodes values): each element adds rho V / (4 | 8) to its nodes
+ * (Physics/FemRepresentation.cpp:102-117); their sum is FemRepresentation::getTotalMass
+ * (Physics/FemRepresentation.cpp:192-200). */
+int fem_get_node_masses(fem_ctx* ctx, double* mass);
+
 /* Sparsity pattern as scalar CSR (int32). The pattern is structurally symmetric, so these arrays
  * equal the reference's column-major (outerIndex, innerIndex) of Eigen::SparseMatrix<double>
  * built at Physics/FemRepresentation.cpp:121-132. */

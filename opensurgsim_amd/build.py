@@ -1,4 +1,5 @@
-"""Builds libfem_gpu.so for gfx950 in-tree with hipcc (no JIT, no torch extension)."""
+"""Builds libfem_gpu.so for gfx950 in-tree with hipcc (no JIT, no torch extension), and the C++ adapter
+library libsurgsim_gpu.so (host code only, g++) on top of it."""
 import os
 import subprocess
 import sys
@@ -6,6 +7,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "_lib", "libfem_gpu.so")
+HOST_LIB = os.path.join(HERE, "_lib", "libsurgsim_gpu.so")
 SOURCES = ["fem_kernels.hip", "fem_hex.hip", "fem_probe.hip", "fem_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -41,7 +43,21 @@ def build(verbose=False):
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose=False):
+    """C++ adapters (SurgSimGpu.h) linked against libfem_gpu.so, found next to it at run time."""
+    tmp = HOST_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-shared",
+           os.path.join(HERE, "host", "SurgSimGpu.cpp"), "-o", tmp,
+           "-L" + os.path.dirname(LIB), "-lfem_gpu", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

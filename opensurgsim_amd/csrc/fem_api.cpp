@@ -997,6 +997,14 @@ int fem_get_phase_times(fem_ctx* c, fem_phase_times* out) {
 
 int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
 
+int fem_get_node_masses(fem_ctx* c, double* mass) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready || !mass) throw InvalidArg("mesh not uploaded or null output");
+        std::copy(c->h_mass_per_node.begin(), c->h_mass_per_node.end(), mass);
+        return FEM_OK;
+    });
+}
+
 int fem_get_pattern(fem_ctx* c, int32_t* rowptr, int32_t* colidx) {
     return guarded(c, [&] {
         if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");

@@ -24,7 +24,7 @@ EXPORTS = [
     "fem_set_options", "fem_set_external_force", "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
-    "fem_get_phase_times", "fem_probe_copy", "fem_time_operator",
+    "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
 ]
 
 PHASES = ["element", "assemble", "rhs", "cg_init", "spmv", "cg_vector", "refine", "update", "spmv_skipped"]
@@ -111,6 +111,7 @@ def load():
     L.fem_evaluate_elements.argtypes = [_vp, _dp, _i64, _i64, _dp, _dp, _dp]
     L.fem_enable_phase_timers.argtypes = [_vp, ctypes.c_int]
     L.fem_get_phase_times.argtypes = [_vp, ctypes.POINTER(FemPhaseTimes)]
+    L.fem_get_node_masses.argtypes = [_vp, _dp]
     L.fem_time_operator.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.fem_probe_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -252,6 +253,11 @@ class FemGpu:
         t = FemPhaseTimes()
         self._check(self._L.fem_get_phase_times(self.h, ctypes.byref(t)))
         return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(PHASES)}
+
+    def node_masses(self):
+        m = np.zeros(self.ndof // 3)
+        self._check(self._L.fem_get_node_masses(self.h, m.ctypes.data_as(_dp)))
+        return m
 
     def time_operator(self, reps=50):
         """Average milliseconds of one CG operator launch (back-to-back replay on CG scratch buffers)."""
