@@ -130,6 +130,23 @@ int fem_step(fem_ctx* ctx, int32_t nsteps);
 int fem_synchronize(fem_ctx* ctx);
 int fem_query(fem_ctx* ctx, fem_stats* stats);
 
+/* Replace the per-element Young's moduli (num_elements values, host pointer), keeping Poisson ratios
+ * and densities. Replaces FemElement::setYoungModulus on every element of an initialized
+ * representation (Physics/FemElement.h:91-99); operators are rebuilt on the next step. */
+int fem_set_young_modulus(fem_ctx* ctx, const double* young);
+
+/* Reverse of one implicit-Euler step (one Newton iteration; the linear scheme for linear tets) taken
+ * from the state (x_n, v_n) to velocity v_next, for a loss L of the later states. No counterpart in the
+ * reference (it has no gradients); it differentiates Math/OdeSolverEulerImplicit.cpp:55-161 with the
+ * elements of Physics/Fem3DElementTetrahedron.cpp and Physics/Fem3DElementCorotationalTetrahedron.cpp.
+ * Inputs lam_x, lam_v = dL/dx_{n+1}, dL/dv_{n+1}; outputs lam_x_out, lam_v_out = dL/dx_n, dL/dv_n
+ * (may alias the inputs); grad_young (num_elements) is accumulated with dL/dE through this step.
+ * All pointers are device pointers of ndof (or num_elements) doubles, ordered on the context stream.
+ * The transpose solve reuses the step's operator (symmetric) and the context's CG settings. */
+int fem_adjoint_step(fem_ctx* ctx, const double* x_n, const double* v_n, const double* v_next,
+                     const double* lam_x, const double* lam_v, double* lam_x_out, double* lam_v_out,
+                     double* grad_young);
+
 /* Lumped node masses (num_nodes values): each element adds rho V / (4 | 8) to its nodes
  * (Physics/FemRepresentation.cpp:102-117); their sum is FemRepresentation::getTotalMass
  * (Physics/FemRepresentation.cpp:192-200). */

@@ -10,6 +10,7 @@
 //                                                   Math/OdeSolverLinearEulerImplicit.cpp:43-64
 #include "../../include/fem_gpu.h"
 #include "fem_kernels.hpp"
+#include "fem_adjoint.hpp"
 #include "fem_hex.hpp"
 
 #include <hip/hip_runtime.h>
@@ -155,6 +156,8 @@ struct fem_ctx {
     // host copies needed after set-up
     std::vector<int32_t> h_rowptr_b, h_colidx_b;   // block CSR
     std::vector<double> h_mass_per_node;
+    std::vector<double> h_nu, h_rho;  // per element, for fem_set_young_modulus
+    DevBuf<double> adj_gx, adj_gv;     // adjoint per-element records [E][12]
     std::vector<uint8_t> h_bcmask;
     std::vector<int64_t> h_bc_dofs;
     bool bc_dirty = true;
@@ -724,6 +727,8 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
             check_material(young[e], poisson[e], density[e], e);
         }
         c->h_mass_per_node.assign(num_nodes, 0.0);
+        c->h_nu.assign(poisson, poisson + num_elements);
+        c->h_rho.assign(density, density + num_elements);
         std::vector<double> mat(4 * num_elements);
         for (int64_t e = 0; e < num_elements; ++e) {
             double E = young[e], nu = poisson[e], rho = density[e];
@@ -996,6 +1001,95 @@ int fem_get_phase_times(fem_ctx* c, fem_phase_times* out) {
 }
 
 int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
+
+int fem_set_young_modulus(fem_ctx* c, const double* young) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready || !young) throw InvalidArg("mesh not uploaded or null input");
+        HIPCHECK(hipSetDevice(c->device));
+        std::vector<double> mat(4 * c->num_elements);
+        for (int64_t e = 0; e < c->num_elements; ++e) {
+            const double E = young[e], nu = c->h_nu[e];
+            check_material(E, nu, c->h_rho[e], e);
+            mat[4 * e] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+            mat[4 * e + 1] = E / (2.0 * (1 + nu));
+            mat[4 * e + 2] = c->h_rho[e];
+            mat[4 * e + 3] = E;
+        }
+        c->mat.upload(reinterpret_cast<const double4*>(mat.data()), c->num_elements, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->linear_matrix_ready = false;  // operators depend on E
+        c->hex.a32_dt = 0.0;
+        return FEM_OK;
+    });
+}
+
+int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const double* v_next, const double* lam_x,
+                     const double* lam_v, double* lam_x_out, double* lam_v_out, double* grad_young) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "mesh not uploaded");
+        if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("the adjoint covers tetrahedral meshes");
+        if (c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
+            (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1))
+            throw InvalidArg("the corotational adjoint reverses one-Newton-iteration implicit Euler steps");
+        if (!x_n || !v_n || !v_next || !lam_x || !lam_v || !lam_x_out || !lam_v_out || !grad_young)
+            throw InvalidArg("null device pointer");
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_bc(c);
+        alloc_precision_buffers(c);
+        const size_t n = (size_t)c->ndof;
+        hipStream_t s = c->stream;
+        // operator S at the step's starting state (element kernel + assembly; the rhs is not needed)
+        HIPCHECK(hipMemcpyAsync(c->xs.p, x_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(c->vs.p, v_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        {
+            // element matrices evaluated at (x_n, v_n): the kernel's xn/vn are the evaluation point
+            // (the reference's newState), xs/vs the step's start state; both are x_n, v_n here
+            TetElementArgs ea;
+            ea.num_elements = (int)c->num_elements;
+            ea.conn = c->conn4.p;
+            ea.x0 = c->x0.p;
+            ea.xn = c->xs.p;
+            ea.vn = c->vs.p;
+            ea.xs = c->xs.p;
+            ea.vs = c->vs.p;
+            ea.mat = c->mat.p;
+            ea.dt = c->opt.dt;
+            ea.write_matrix = 1;
+            ea.general = 0;
+            ea.rec = c->rec.p;
+            ea.rhs_rec = c->rhs_rec.p;
+            ea.flag_counts = c->counters.p;
+            launch_tet_elements(c->kind, ea, s);
+        }
+        assemble_tet(c);
+        // S mu = lv + dt lx (transpose solve on the symmetric operator)
+        launch_adjoint_rhs((int)n, lam_x, lam_v, c->opt.dt, c->b.p, s);
+        linear_solve(c);
+        if (!c->adj_gx.p) {
+            c->adj_gx.alloc(12 * (size_t)c->num_elements);
+            c->adj_gv.alloc(12 * (size_t)c->num_elements);
+        }
+        AdjointArgs a;
+        a.num_elements = (int)c->num_elements;
+        a.conn = c->conn4.p;
+        a.x0 = c->x0.p;
+        a.x = c->xs.p;
+        a.v = c->vs.p;
+        a.v_next = v_next;
+        a.mu = c->dv.p;
+        a.bcmask = c->bcmask.p;
+        a.mat = c->mat.p;
+        a.dt = c->opt.dt;
+        a.gx_rec = c->adj_gx.p;
+        a.gv_rec = c->adj_gv.p;
+        a.grad_E = grad_young;
+        launch_adjoint_elements(c->kind, a, s);
+        launch_adjoint_gather((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->adj_gx.p, c->adj_gv.p, lam_x,
+                              c->b.p, lam_x_out, lam_v_out, s);
+        HIPCHECK(hipGetLastError());
+        return FEM_OK;
+    });
+}
 
 int fem_get_node_masses(fem_ctx* c, double* mass) {
     return guarded(c, [&] {

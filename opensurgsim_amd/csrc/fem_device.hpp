@@ -21,6 +21,38 @@
 
 namespace femgpu {
 
+// Forward-mode dual number (value, one tangent) for the adjoint's element derivatives.
+struct Dual {
+    double v, d;
+    __host__ __device__ Dual() : v(0.0), d(0.0) {}
+    __host__ __device__ explicit Dual(double value, double tangent = 0.0) : v(value), d(tangent) {}
+    __host__ __device__ Dual& operator+=(const Dual& b) { v += b.v; d += b.d; return *this; }
+    __host__ __device__ Dual& operator-=(const Dual& b) { v -= b.v; d -= b.d; return *this; }
+    __host__ __device__ Dual& operator*=(const Dual& b) { d = d * b.v + v * b.d; v *= b.v; return *this; }
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__host__ __device__ inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ inline Dual operator/(Dual a, Dual b) {
+    const double iv = 1.0 / b.v;
+    return Dual(a.v * iv, (a.d - a.v * iv * b.d) * iv);
+}
+__host__ __device__ inline Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
+__host__ __device__ inline Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+__host__ __device__ inline Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
+__host__ __device__ inline Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
+__host__ __device__ inline Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+__host__ __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+__host__ __device__ inline Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
+__host__ __device__ inline bool operator==(Dual a, Dual b) { return a.v == b.v; }
+__host__ __device__ inline bool operator<(Dual a, double b) { return a.v < b; }
+__host__ __device__ inline bool operator>(Dual a, double b) { return a.v > b; }
+__host__ __device__ inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+using ::fabs;
+__host__ __device__ inline double value_of(double a) { return a; }
+__host__ __device__ inline double value_of(Dual a) { return a.v; }
+
 template <typename T>
 __host__ __device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
     c[0] = a[1] * b[2] - a[2] * b[1];
@@ -214,6 +246,37 @@ __device__ inline int polar_rotation(const double* F, double* R) {
     return 1;
 }
 
+// Polar rotation of a dual-number F: the value as above; the tangent from differentiating F = R S
+// with S symmetric. dR = R [w]x and R^T dF = [w]x S + dS give R^T dF - dF^T R = [w]x S + S [w]x
+// = [(tr(S) I - S) w]x, so w = (tr(S) I - S)^-1 axial(R^T dF - dF^T R). Exact wherever tr(S) I - S is
+// invertible (it is singular only when two principal stretches sum to zero); the tangent is set to zero
+// otherwise, where the element's stiffness correction is dropped as well.
+__device__ inline int polar_rotation(const Dual* F, Dual* R) {
+    double Fv[9], Rv[9], dF[9];
+    for (int i = 0; i < 9; ++i) {
+        Fv[i] = F[i].v;
+        dF[i] = F[i].d;
+    }
+    const int path = polar_rotation(Fv, Rv);
+    double RtF[9], S[9], G[9], Gi[9], M[9];
+    mtm3(Rv, Fv, RtF);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[3 * i + j] = 0.5 * (RtF[3 * i + j] + RtF[3 * j + i]);
+    const double trS = S[0] + S[4] + S[8];
+    for (int i = 0; i < 9; ++i) G[i] = ((i % 4 == 0) ? trS : 0.0) - S[i];
+    double w[3] = {0.0, 0.0, 0.0};
+    if (fabs(inv3(G, Gi)) > 1e-300) {
+        mtm3(Rv, dF, M);  // M = R^T dF; A = M - M^T, axial(A) = (A_21, A_02, A_10)
+        const double a[3] = {M[7] - M[5], M[2] - M[6], M[3] - M[1]};
+        mv3(Gi, a, w);
+    }
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double dR[9];
+    mm3(Rv, W, dR);
+    for (int i = 0; i < 9; ++i) R[i] = Dual(Rv[i], dR[i]);
+    return path;
+}
+
 // Per-element geometry of a tetrahedron at rest: gradients g[4][3] and signed volume V
 // (Physics/Fem3DElementTetrahedron.cpp:218-359; g_a are the first three entries of row a of V^-1,
 // Physics/Fem3DElementCorotationalTetrahedron.cpp:56-72, 254).
@@ -254,60 +317,62 @@ struct CoroFactors {
 };
 
 // Rotation, force and stiffness factors of one corotational tet at positions x (12) with rest x0 (12),
-// rest gradients g and signed rest volume V.
-// Force f (12, FP64) follows doUpdateFMDK :101-113. Returns element flags.
-template <typename TK>
-__device__ inline int coro_factors(const double* x0, const double* x, const double g[4][3], double V, double lam,
-                                   double mu, double* f, CoroFactors<TK>& cf) {
+// rest gradients g and signed rest volume V. T is the arithmetic (double; Dual for the adjoint's
+// derivative with respect to x), TK the factor storage.
+// Force f (12) follows doUpdateFMDK :101-113. Returns element flags.
+template <typename TK, typename T = double>
+__device__ inline int coro_factors(const double* x0, const T* x, const double g[4][3], double V, double lam,
+                                   double mu, T* f, CoroFactors<TK>& cf) {
     int flags = 0;
     // F = I + sum_b (x_b - x0_b) g_b^T  (sum_b x0_b g_b^T = I; the displacement form keeps FP64 accuracy)
-    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    T F[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
     for (int b = 0; b < 4; ++b) {
-        double u[3] = {x[3 * b] - x0[3 * b], x[3 * b + 1] - x0[3 * b + 1], x[3 * b + 2] - x0[3 * b + 2]};
+        T u[3] = {x[3 * b] - x0[3 * b], x[3 * b + 1] - x0[3 * b + 1], x[3 * b + 2] - x0[3 * b + 2]};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) F[3 * i + j] += u[i] * g[b][j];
     }
-    double R[9];
+    T R[9];
     if (polar_rotation(F, R)) flags |= ELEM_FLAG_POLAR_EIG;
-    double detR = det3(R);
+    T detR = det3(R);
     // S = sym(R^T F);  E = S - I computed as sym(R^T (F - I) + (R^T - I))
-    double RtF[9];
+    T RtF[9];
     mtm3(R, F, RtF);
-    double S[9];
+    T S[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) S[3 * i + j] = 0.5 * (RtF[3 * i + j] + RtF[3 * j + i]);
-    double Fm[9];
+    T Fm[9];
     for (int i = 0; i < 9; ++i) Fm[i] = F[i] - ((i % 4 == 0) ? 1.0 : 0.0);
-    double RtD[9];
+    T RtD[9];
     mtm3(R, Fm, RtD);
-    double Es[9];
+    T Es[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
-            double rt_ij = R[3 * j + i] - (i == j ? 1.0 : 0.0);
-            double rt_ji = R[3 * i + j] - (i == j ? 1.0 : 0.0);
+            T rt_ij = R[3 * j + i] - (i == j ? 1.0 : 0.0);
+            T rt_ji = R[3 * i + j] - (i == j ? 1.0 : 0.0);
             Es[3 * i + j] = 0.5 * (RtD[3 * i + j] + RtD[3 * j + i] + rt_ij + rt_ji);
         }
-    double trE = Es[0] + Es[4] + Es[8];
+    T trE = Es[0] + Es[4] + Es[8];
     // f_a = -V R sigma g_a, sigma = lambda tr(E) I + 2 mu E
-    double sig[9];
-    for (int i = 0; i < 9; ++i) sig[i] = 2.0 * mu * Es[i] + ((i % 4 == 0) ? lam * trE : 0.0);
-    double Rsig[9];
+    T sig[9];
+    for (int i = 0; i < 9; ++i) sig[i] = 2.0 * mu * Es[i] + ((i % 4 == 0) ? lam * trE : T(0));
+    T Rsig[9];
     mm3(R, sig, Rsig);
     for (int a = 0; a < 4; ++a) {
-        double t[3];
-        mv3(Rsig, g[a], t);
+        T t[3];
+        const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
+        mv3(Rsig, ga, t);
         f[3 * a] = -V * t[0];
         f[3 * a + 1] = -V * t[1];
         f[3 * a + 2] = -V * t[2];
     }
     // G = (tr S I - S) R^T; the reference's invertibility test uses |det G| > 1e-12 (Eigen default)
-    double trS = S[0] + S[4] + S[8];
-    double G0[9];
-    for (int i = 0; i < 9; ++i) G0[i] = ((i % 4 == 0) ? trS : 0.0) - S[i];
-    double G0i[9];
-    double detG = inv3(G0, G0i) * detR;
+    T trS = S[0] + S[4] + S[8];
+    T G0[9];
+    for (int i = 0; i < 9; ++i) G0[i] = ((i % 4 == 0) ? trS : T(0)) - S[i];
+    T G0i[9];
+    T detG = inv3(G0, G0i) * detR;
     cf.correction = true;
     if (fabs(detR - 1.0) > 1e-8) {
         flags |= ELEM_FLAG_DET_R;
@@ -321,8 +386,9 @@ __device__ inline int coro_factors(const double* x0, const double* x, const doub
     cf.lam = (TK)lam;
     cf.mu = (TK)mu;
     for (int a = 0; a < 4; ++a) {
-        double h[3];
-        mv3(R, g[a], h);
+        T h[3];
+        const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
+        mv3(R, ga, h);
         for (int c = 0; c < 3; ++c) {
             cf.h[a][c] = (TK)h[c];
             cf.g[a][c] = (TK)g[a][c];
@@ -334,31 +400,32 @@ __device__ inline int coro_factors(const double* x0, const double* x, const doub
         return flags;
     }
     // Quirk term: H' = R^T (S - I) = R^T E; p_a = V [lambda tr(H') g_a + mu (H' + H'^T) g_a]; q_a = R p_a
-    double Hp[9];
+    T Hp[9];
     mtm3(R, Es, Hp);
-    double trHp = Hp[0] + Hp[4] + Hp[8];
+    T trHp = Hp[0] + Hp[4] + Hp[8];
     for (int a = 0; a < 4; ++a) {
-        double p[3];
+        T p[3];
         for (int i = 0; i < 3; ++i) {
-            double s = lam * trHp * g[a][i];
+            T s = lam * trHp * g[a][i];
             for (int k = 0; k < 3; ++k) s += mu * (Hp[3 * i + k] + Hp[3 * k + i]) * g[a][k];
             p[i] = V * s;
         }
-        double q[3];
+        T q[3];
         mv3(R, p, q);
         // N_a = -[q]x + V mu ([S g]x - S [g]x)
-        double Sg[3];
-        mv3(S, g[a], Sg);
-        double N[9];
+        T Sg[3];
+        const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
+        mv3(S, ga, Sg);
+        T N[9];
         // [w]x = [[0,-w2,w1],[w2,0,-w0],[-w1,w0,0]]
         double vm = V * mu;
-        double sx[9] = {0, -Sg[2], Sg[1], Sg[2], 0, -Sg[0], -Sg[1], Sg[0], 0};
-        double qx[9] = {0, -q[2], q[1], q[2], 0, -q[0], -q[1], q[0], 0};
-        double gx[9] = {0, -g[a][2], g[a][1], g[a][2], 0, -g[a][0], -g[a][1], g[a][0], 0};
-        double Sgx[9];
+        T sx[9] = {T(0), -Sg[2], Sg[1], Sg[2], T(0), -Sg[0], -Sg[1], Sg[0], T(0)};
+        T qx[9] = {T(0), -q[2], q[1], q[2], T(0), -q[0], -q[1], q[0], T(0)};
+        T gx[9] = {T(0), T(-g[a][2]), T(g[a][1]), T(g[a][2]), T(0), T(-g[a][0]), T(-g[a][1]), T(g[a][0]), T(0)};
+        T Sgx[9];
         mm3(S, gx, Sgx);
         for (int i = 0; i < 9; ++i) N[i] = -qx[i] + vm * (sx[i] - Sgx[i]);
-        double RN[9], W[9];
+        T RN[9], W[9];
         mm3(R, N, RN);
         mm3(RN, G0i, W);
         for (int i = 0; i < 9; ++i) cf.W[a][i] = (TK)W[i];

@@ -25,6 +25,7 @@ EXPORTS = [
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
     "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
+    "fem_set_young_modulus", "fem_adjoint_step",
 ]
 
 PHASES = ["element", "assemble", "rhs", "cg_init", "spmv", "cg_vector", "refine", "update", "spmv_skipped"]
@@ -112,6 +113,8 @@ def load():
     L.fem_enable_phase_timers.argtypes = [_vp, ctypes.c_int]
     L.fem_get_phase_times.argtypes = [_vp, ctypes.POINTER(FemPhaseTimes)]
     L.fem_get_node_masses.argtypes = [_vp, _dp]
+    L.fem_set_young_modulus.argtypes = [_vp, _dp]
+    L.fem_adjoint_step.argtypes = [_vp] + [_vp] * 8
     L.fem_time_operator.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.fem_probe_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -253,6 +256,19 @@ class FemGpu:
         t = FemPhaseTimes()
         self._check(self._L.fem_get_phase_times(self.h, ctypes.byref(t)))
         return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(PHASES)}
+
+    def set_young_modulus(self, young):
+        E = _f64(np.broadcast_to(young, (self.num_elements,)))
+        self._check(self._L.fem_set_young_modulus(self.h, _ptr(E)))
+
+    def set_stream(self, stream_handle):
+        """Run on an existing hipStream_t (an int handle, e.g. torch.cuda.current_stream().cuda_stream)."""
+        self._check(self._L.fem_set_stream(self.h, _vp(int(stream_handle)) if stream_handle else None))
+
+    def adjoint_step(self, x_n, v_n, v_next, lam_x, lam_v, lam_x_out, lam_v_out, grad_young):
+        """Device pointers (ints): reverse of one step; grad_young accumulated (fem_adjoint_step)."""
+        args = [_vp(int(p)) for p in (x_n, v_n, v_next, lam_x, lam_v, lam_x_out, lam_v_out, grad_young)]
+        self._check(self._L.fem_adjoint_step(self.h, *args))
 
     def node_masses(self):
         m = np.zeros(self.ndof // 3)

@@ -231,7 +231,27 @@ def config_adjoint(n=(20, 20, 20), seed=77):
     cfg["E_cube"] = Ecube
     tris = top_face_triangles_tet(nx, ny, nz)
     cfg["fext"] = pressure_forces(cfg.x0, tris, lambda c: np.full(c.shape[0], 200.0))
+    cfg["obs_nodes"] = observation_nodes(nx, ny, nz, 500, seed)
     return cfg
+
+
+def surface_nodes(nx, ny, nz, exclude_bottom=True):
+    """Node ids on the block's boundary (optionally without the y = 0 face)."""
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    on = (i == 0) | (i == nx) | (j == ny) | (k == 0) | (k == nz)
+    if not exclude_bottom:
+        on |= j == 0
+    else:
+        on &= j > 0
+    return node_id(i[on], j[on], k[on], ny, nz).astype(np.int64)
+
+
+def observation_nodes(nx, ny, nz, count, seed):
+    """`count` distinct seeded surface nodes (the observed nodes of the identification problem)."""
+    surf = surface_nodes(nx, ny, nz)
+    key = uniform(surf.size, seed, stream=9)
+    return np.sort(surf[np.argsort(key, kind="stable")[:min(count, surf.size)]])
 
 
 def config_hex(n=(128, 64, 128), seed=5):

@@ -50,6 +50,8 @@ def lib():
         L.orc_sim_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_sim_get_state.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_sim_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.orc_sim_adjoint_step.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_sim_set_young.argtypes = [ctypes.c_void_p, _dp]
         L.orc_sim_nnz.restype = ctypes.c_long
         L.orc_sim_nnz.argtypes = [ctypes.c_void_p]
         L.orc_sim_pattern.argtypes = [ctypes.c_void_p, _ip, _ip]
@@ -200,6 +202,22 @@ class Sim:
         if rc < 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return rc
+
+    def set_young(self, E):
+        self._E = _f64(np.broadcast_to(E, (self._keep[1].shape[0],)))
+        if lib().orc_sim_set_young(self.h, _d(self._E)) < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def adjoint_step(self, dt, xn, vn, vnext, lx, lv, gradE):
+        """Reverse of one implicit step from (xn, vn) to vnext: returns (dL/dxn, dL/dvn) and adds dL/dE
+        through the step to gradE (in place)."""
+        arrs = [_f64(a).reshape(-1) for a in (xn, vn, vnext, lx, lv)]
+        lxo = np.zeros(self.ndof)
+        lvo = np.zeros(self.ndof)
+        assert gradE.dtype == np.float64 and gradE.flags.c_contiguous
+        if lib().orc_sim_adjoint_step(self.h, dt, *[_d(a) for a in arrs], _d(lxo), _d(lvo), _d(gradE)) < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return lxo, lvo
 
     def pattern(self):
         nnz = lib().orc_sim_nnz(self.h)

@@ -1133,6 +1133,156 @@ struct Sim {
         for (size_t i = 0; i < n; ++i) x[i] = x[i] + dt * v[i];
     }
 
+    // ---- adjoint (reference gradient for the GPU adjoint; no counterpart in the reference) ----
+    // Element K, f at local positions xe through the literal element code (a copy of the element, so
+    // the simulation's own element state is untouched).
+    void element_kf(const Element& e0, const double* xe, std::vector<double>& Ke, std::vector<double>& fe) const {
+        Element e = e0;
+        Counters tmp;
+        const int n = 3 * e.nn;
+        if (e.kind == TET_COROTATIONAL) {
+            coro_rotation_mass_stiffness(e, xe, true, false, true, &tmp);
+            double Rx0[12];
+            for (int a = 0; a < 4; ++a)
+                for (int i = 0; i < 3; ++i)
+                    Rx0[3 * a + i] = e.R.m[i][0] * e.x0[3 * a] + e.R.m[i][1] * e.x0[3 * a + 1] + e.R.m[i][2] * e.x0[3 * a + 2];
+            Ke = e.K;
+            fe.assign(n, 0.0);
+            for (int i = 0; i < n; ++i) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += e.K[i * n + k] * (xe[k] - Rx0[k]);
+                fe[i] = -acc;
+            }
+        } else {
+            Ke = e.K;
+            fe.assign(n, 0.0);
+            for (int i = 0; i < n; ++i) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += e.K[i * n + k] * (xe[k] - e.x0[k]);
+                fe[i] = -acc;
+            }
+        }
+    }
+
+    // phi_e(xe) = mu.(f - dt K v) - dt mu^T S_K dv, S_K the lower-triangle self-adjoint view by global DOF
+    // index of the element's K (entry (r, c) = K(r, c) if r >= c else K(c, r)).
+    double element_phi(const Element& e, const double* xe, const double* ve, const double* me, const double* dve,
+                       double dt, std::vector<double>* Kout = nullptr) const {
+        std::vector<double> Ke, fe;
+        element_kf(e, xe, Ke, fe);
+        const int n = 3 * e.nn;
+        double phi = 0.0;
+        for (int i = 0; i < n; ++i) phi += me[i] * fe[i];
+        for (int i = 0; i < n; ++i) {
+            const int gi = 3 * e.ids[i / 3] + i % 3;
+            for (int j = 0; j < n; ++j) {
+                const int gj = 3 * e.ids[j / 3] + j % 3;
+                phi -= dt * me[i] * Ke[i * n + j] * ve[j];
+                const double sij = gi >= gj ? Ke[i * n + j] : Ke[j * n + i];
+                phi -= dt * me[i] * sij * dve[j];
+            }
+        }
+        if (Kout) *Kout = Ke;
+        return phi;
+    }
+
+    // Reverse of one implicit-Euler step (one Newton iteration; CG on the lower-triangle view) from
+    // (xn, vn) to vnext: lx, lv = dL/dx', dL/dv' -> lx_out, lv_out = dL/dx, dL/dv; gradE += dL/dE.
+    //   gv = lv + dt lx;  S mu = gv (dense solve);  mu~ = mu with BC entries zero;
+    //   lx_out = lx + d phi/dx;  lv_out = gv - dt K^T mu~;  dL/dE_e += phi_e / E_e
+    // with d phi_e / d x_e by Richardson-extrapolated central differences of element_phi.
+    void adjoint_step(double dt, const double* xn, const double* vn, const double* vnext, const double* lx,
+                      const double* lv, double* lx_out, double* lv_out, double* gradE) {
+        const size_t n = x0.size();
+        std::vector<double> xs(xn, xn + n), vs(vn, vn + n);
+        assemble(dt, xs, vs, xs, vs);
+        std::vector<double> gv(n), mu(n, 0.0);
+        for (size_t i = 0; i < n; ++i) gv[i] = lv[i] + dt * lx[i];
+        // dense lower-triangle self-adjoint view of A and Gaussian elimination with partial pivoting
+        std::vector<double> S(n * n, 0.0);
+        for (size_t j = 0; j < n; ++j)
+            for (int p = A.outer[j]; p < A.outer[j + 1]; ++p) {
+                const size_t i = (size_t)A.inner[p];
+                if (i >= j) {
+                    S[i * n + j] = A.val[p];
+                    S[j * n + i] = A.val[p];
+                }
+            }
+        std::vector<double> rhsv = gv;
+        std::vector<size_t> perm(n);
+        for (size_t k = 0; k < n; ++k) {
+            size_t piv = k;
+            for (size_t i = k + 1; i < n; ++i)
+                if (std::fabs(S[i * n + k]) > std::fabs(S[piv * n + k])) piv = i;
+            if (piv != k) {
+                for (size_t j = 0; j < n; ++j) std::swap(S[k * n + j], S[piv * n + j]);
+                std::swap(rhsv[k], rhsv[piv]);
+            }
+            for (size_t i = k + 1; i < n; ++i) {
+                const double fct = S[i * n + k] / S[k * n + k];
+                if (fct == 0.0) continue;
+                for (size_t j = k; j < n; ++j) S[i * n + j] -= fct * S[k * n + j];
+                rhsv[i] -= fct * rhsv[k];
+            }
+        }
+        for (size_t k = n; k-- > 0;) {
+            double acc = rhsv[k];
+            for (size_t j = k + 1; j < n; ++j) acc -= S[k * n + j] * mu[j];
+            mu[k] = acc / S[k * n + k];
+        }
+        for (int d : bc_dofs) mu[d] = 0.0;
+        std::vector<double> gx_n(n, 0.0), gv_n(n, 0.0);
+        for (size_t ei = 0; ei < elems.size(); ++ei) {
+            const Element& e = elems[ei];
+            const int ne = 3 * e.nn;
+            double xe[24], ve[24], me[24], dve[24];
+            double ext = 0.0;
+            for (int a = 0; a < e.nn; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    const size_t d = 3 * (size_t)e.ids[a] + c;
+                    xe[3 * a + c] = xn[d];
+                    ve[3 * a + c] = vn[d];
+                    me[3 * a + c] = mu[d];
+                    dve[3 * a + c] = vnext[d] - vn[d];
+                    ext = std::max(ext, std::fabs(e.x0[3 * a + c] - e.x0[c]));
+                }
+            std::vector<double> Ke;
+            const double phi = element_phi(e, xe, ve, me, dve, dt, &Ke);
+            gradE[ei] += phi / e.E;
+            const double h = 1e-4 * ext;
+            for (int k = 0; k < ne; ++k) {
+                auto central = [&](double step) {
+                    double xp[24], xm[24];
+                    std::memcpy(xp, xe, sizeof(double) * ne);
+                    std::memcpy(xm, xe, sizeof(double) * ne);
+                    xp[k] += step;
+                    xm[k] -= step;
+                    return (element_phi(e, xp, ve, me, dve, dt) - element_phi(e, xm, ve, me, dve, dt)) / (2 * step);
+                };
+                const double d1 = central(h), d2 = central(0.5 * h);
+                gx_n[3 * (size_t)e.ids[k / 3] + k % 3] += (4.0 * d2 - d1) / 3.0;
+            }
+            for (int j = 0; j < ne; ++j) {
+                double acc = 0.0;
+                for (int i = 0; i < ne; ++i) acc += Ke[i * ne + j] * me[i];
+                gv_n[3 * (size_t)e.ids[j / 3] + j % 3] -= dt * acc;
+            }
+        }
+        for (size_t i = 0; i < n; ++i) {
+            lx_out[i] = lx[i] + gx_n[i];
+            lv_out[i] = gv[i] + gv_n[i];
+        }
+    }
+
+    // New Young's moduli with the same Poisson ratios and densities (element re-initialisation).
+    void set_young(const double* E) {
+        for (size_t i = 0; i < elems.size(); ++i) {
+            elems[i].E = E[i];
+            element_init(elems[i], x0.data(), &cnt);
+        }
+        linear_initialized = false;
+    }
+
     // FemRepresentation::update (Physics/FemRepresentation.cpp:232-294) minus compliance.
     int step(double dt) {
         if (!valid) return 1;
@@ -1357,6 +1507,27 @@ void orc_sim_element(void* h, int i, double* R9, double* K, double* f) {
         for (int k = 0; k < 9; ++k) R9[k] = e.R.m[k / 3][k % 3];
     if (K) std::memcpy(K, e.K.data(), e.K.size() * sizeof(double));
     if (f) std::memcpy(f, e.f.data(), e.f.size() * sizeof(double));
+}
+
+int orc_sim_adjoint_step(void* h, double dt, const double* xn, const double* vn, const double* vnext,
+                         const double* lx, const double* lv, double* lx_out, double* lv_out, double* gradE) {
+    try {
+        static_cast<Sim*>(h)->adjoint_step(dt, xn, vn, vnext, lx, lv, lx_out, lv_out, gradE);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+int orc_sim_set_young(void* h, const double* E) {
+    try {
+        static_cast<Sim*>(h)->set_young(E);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
 }
 
 // Standalone Eigen-CG (lower self-adjoint view) on a CSC matrix.
