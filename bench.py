@@ -178,6 +178,90 @@ def bench_hex(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """Adjoint material identification workload: 20x20x20 corotational tet block (40,000 tets), per-cube
+    log-normal ground-truth E, top pressure 200 Pa, 50 implicit steps at dt = 2 ms; loss = squared
+    displacement mismatch at 500 seeded surface nodes; full dL/dE (all element moduli) by the adjoint with
+    checkpoints every 10 steps, at a uniform guess E = 3e4 Pa; checked against central differences along
+    32 seeded unit directions."""
+    import torch
+    from opensurgsim_amd import adjoint
+    n = (args.adj_n, args.adj_n, args.adj_n)
+    cfg = mg.config_adjoint(n)
+    steps = cfg.steps
+    log("workload: adjoint, corotational tets %s, %d elements, %d DOF, %d steps" % (n, cfg.conn.shape[0],
+                                                                                  cfg.x0.size, steps))
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    g.add_bc_nodes(cfg.bc_nodes)
+    g.set_options(dt=cfg.dt, cg_tolerance=args.adj_tol)
+    g.set_external_force(cfg.fext)
+    r = adjoint.Rollout(g, cfg.x0, np.zeros(cfg.x0.size), steps, checkpoint_every=10)
+    xr = torch.as_tensor(cfg.x0.reshape(-1), device="cuda")
+    obs = torch.as_tensor((3 * cfg.obs_nodes[:, None] + np.arange(3)).reshape(-1), device="cuda")
+    with torch.no_grad():
+        xt, _ = r.forward(cfg.E)
+    u_obs = (xt - xr)[obs].clone()
+    E0 = np.full(cfg.E.size, 3e4)
+
+    def loss_and_grad(E):
+        x, _ = r.forward(E)
+        d = (x - xr)[obs] - u_obs
+        lx = torch.zeros_like(x)
+        lx[obs] = 2 * d
+        gE, _, _ = r.backward(lx, None)
+        return float(torch.sum(d * d)), gE
+
+    def loss(E):
+        x, _ = r.forward(E)
+        d = (x - xr)[obs] - u_obs
+        return float(torch.sum(d * d))
+
+    L0, gE = loss_and_grad(E0)  # warm-up
+    device_sync(g)
+    reps = args.steps if args.steps_given else 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L0, gE = loss_and_grad(E0)
+    device_sync(g)
+    t = time.perf_counter() - t0
+    gE = gE.cpu().numpy()
+    log("adjoint: %d gradients in %.3f s (%.1f ms each), loss %.6e, |dL/dE| %.3e" % (reps, t, 1e3 * t / reps, L0,
+                                                                                   np.linalg.norm(gE)))
+    # central differences along 32 seeded unit directions (FP64)
+    dirs = mg.normal(32 * E0.size, 4242, stream=11).reshape(32, E0.size)
+    h = args.adj_fd_h * E0.mean()
+    errs = []
+    for k in range(args.adj_fd_dirs):
+        d = dirs[k] / np.linalg.norm(dirs[k])
+        fd = (loss(E0 + h * d) - loss(E0 - h * d)) / (2 * h)
+        ad = float(gE @ d)
+        errs.append(abs(ad - fd) / abs(fd))
+    log("fd check: %d directions, max rel err %.3e, median %.3e" % (len(errs), max(errs), float(np.median(errs))))
+    ident = None
+    if args.adj_ident:
+        Ehat, losses = adjoint.recover_young_modulus(r, xr, obs, u_obs, E0, iterations=args.adj_ident)
+        ident = {"lbfgs_iterations": args.adj_ident, "evaluations": len(losses), "loss_first": losses[0],
+                 "loss_best": min(losses),
+                 "rel_err_logE_first": float(np.linalg.norm(np.log(E0) - np.log(cfg.E)) / np.linalg.norm(np.log(cfg.E))),
+                 "rel_err_logE_last": float(np.linalg.norm(np.log(Ehat.cpu().numpy()) - np.log(cfg.E)) /
+                                            np.linalg.norm(np.log(cfg.E)))}
+        log("identification:", ident)
+    st = g.stats()
+    out = {"metric": METRIC, "value": reps / t, "unit": "gradients/s", "n_gpus": 1, "steps": reps,
+           "warmup": 1, "ms_per_step": 1e3 * t / reps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "adjoint dL/dE, corotational tet block %dx%dx%d, %d implicit steps dt=2ms, "
+                                  "checkpoints every 10, CG %.0e" % (n + (steps, args.adj_tol)),
+                      "elements": int(cfg.conn.shape[0]), "dof": int(cfg.x0.size), "observed_nodes":
+                      int(cfg.obs_nodes.size), "parallelism": "single-gpu"},
+           "fd_check": {"directions": len(errs), "step_rel": args.adj_fd_h, "max_rel_err": max(errs),
+                        "median_rel_err": float(np.median(errs))},
+           "identification": ident, "cg_iterations_total": st["total_cg_iterations"]}
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,7 +272,12 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--timer-every", type=int, default=10,
                     help="bracket every kernel of one step in N with HIP events (0: no events)")
-    ap.add_argument("--workload", default="corotational", choices=["corotational", "hex"])
+    ap.add_argument("--workload", default="corotational", choices=["corotational", "hex", "adjoint"])
+    ap.add_argument("--adj-n", type=int, default=20, help="adjoint block: n^3 cubes")
+    ap.add_argument("--adj-tol", type=float, default=1e-12, help="adjoint workload CG tolerance")
+    ap.add_argument("--adj-fd-dirs", type=int, default=32)
+    ap.add_argument("--adj-fd-h", type=float, default=1e-4, help="FD step relative to mean E")
+    ap.add_argument("--adj-ident", type=int, default=0, help="identification iterations to run (0: none)")
     ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB three times (counter calibration)")
@@ -198,6 +287,9 @@ def main():
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design (no --gpus > 1).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "adjoint":
+        bench_adjoint(args)
+        return
     if args.workload == "hex":
         bench_hex(args)
         if args.copy_probe:

@@ -121,24 +121,31 @@ def surface_loss(x_final, x_rest, obs_dofs, u_obs):
     return torch.sum((u[obs_dofs] - u_obs) ** 2)
 
 
-def recover_young_modulus(rollout, x_rest, obs_dofs, u_obs, young_init, iterations=10, lr=0.05,
-                          callback=None):
-    """Gradient-based identification of per-element moduli from observed displacements: Adam on
-    log E. Returns (E, losses)."""
+def recover_young_modulus(rollout, x_rest, obs_dofs, u_obs, young_init, iterations=10, callback=None):
+    """Gradient-based identification of per-element moduli from observed displacements: L-BFGS with a
+    strong-Wolfe line search on log E (every function evaluation is one forward + adjoint rollout).
+    Returns (E, losses of the evaluations)."""
     dev = rollout.device
     logE = torch.log(torch.as_tensor(young_init, dtype=torch.float64, device=dev)).clone().requires_grad_(True)
     x_rest = torch.as_tensor(x_rest, dtype=torch.float64, device=dev).reshape(-1)
     obs = torch.as_tensor(obs_dofs, dtype=torch.long, device=dev)
     u_obs = torch.as_tensor(u_obs, dtype=torch.float64, device=dev).reshape(-1)
-    opt = torch.optim.Adam([logE], lr=lr)
+    opt = torch.optim.LBFGS([logE], lr=1.0, max_iter=iterations, history_size=10, tolerance_grad=1e-12,
+                            tolerance_change=1e-14, line_search_fn="strong_wolfe")
     losses = []
-    for it in range(iterations):
+    scale = []
+
+    def closure():
         opt.zero_grad()
         x, _ = ImplicitEulerRollout.apply(torch.exp(logE), rollout.x_init, rollout.v_init, rollout)
         L = surface_loss(x, x_rest, obs, u_obs)
-        L.backward()
+        if not scale:  # the optimizer works on L / L(first evaluation), an O(1) objective
+            scale.append(max(float(L.detach()), 1e-300))
+        (L / scale[0]).backward()
         losses.append(float(L.detach()))
         if callback:
-            callback(it, losses[-1])
-        opt.step()
+            callback(len(losses) - 1, losses[-1])
+        return L / scale[0]
+
+    opt.step(closure)
     return torch.exp(logE.detach()), losses
