@@ -130,6 +130,26 @@ int fem_step(fem_ctx* ctx, int32_t nsteps);
 int fem_synchronize(fem_ctx* ctx);
 int fem_query(fem_ctx* ctx, fem_stats* stats);
 
+/* Batched independent instances. The uploaded tetrahedral mesh is declared to consist of num_instances
+ * disconnected meshes (no element joins two of them): instance i owns nodes
+ * [node_offsets[i], node_offsets[i + 1]). Each instance is one representation of the reference advanced
+ * by its own OdeSolverEulerImplicit step (one Newton iteration) with its own time step dt[i] (NULL: the
+ * options' dt for all) and its own Eigen CG (own tolerance test, iteration count and cap), as if
+ * FemRepresentation::update ran once per instance (Physics/FreeMotion.cpp:52-63 runs one task per
+ * representation). All instances advance in the same launches: per-instance offset tables, per-instance
+ * CG scalars with convergence masks, iteration chunks replayed as a captured hipGraph, and row-chunk and
+ * lane shapes picked by an autotuner on the first step. Call after fem_upload_mesh. */
+int fem_set_instances(fem_ctx* ctx, int32_t num_instances, const int64_t* node_offsets, const double* dt);
+/* Last step, per instance: CG iterations (FP32 mode: inner iterations summed over sweeps), relative residual
+ * and convergence flag. Any output may be NULL. */
+int fem_query_instances(fem_ctx* ctx, int64_t* iterations, double* errors, int32_t* converged);
+/* Fix the chunk shape (chunk_rows > 0 block rows per workgroup, lanes_per_row 4, 8 or 16) or return to
+ * autotuning (0, 0); use_graphs 0 launches the CG iterations one by one instead of replaying graphs.
+ * No counterpart in the reference. */
+int fem_set_batch_tuning(fem_ctx* ctx, int32_t chunk_rows, int32_t lanes_per_row, int32_t use_graphs);
+int fem_get_batch_tuning(fem_ctx* ctx, int32_t* chunk_rows, int32_t* lanes_per_row, int32_t* num_chunks,
+                         double* ms_per_iteration);
+
 /* Replace the per-element Young's moduli (num_elements values, host pointer), keeping Poisson ratios
  * and densities. Replaces FemElement::setYoungModulus on every element of an initialized
  * representation (Physics/FemElement.h:91-99); operators are rebuilt on the next step. */

@@ -11,6 +11,7 @@
 #include "../../include/fem_gpu.h"
 #include "fem_kernels.hpp"
 #include "fem_adjoint.hpp"
+#include "fem_batch.hpp"
 #include "fem_hex.hpp"
 
 #include <hip/hip_runtime.h>
@@ -186,6 +187,31 @@ struct fem_ctx {
     // hexahedra (element-by-element path)
     HexData hex;
 
+    // batched independent instances (fem_set_instances): instance tables, per-instance CG scalars,
+    // row-chunk tables chosen by the autotuner, captured iteration graphs
+    int num_instances = 0;
+    std::vector<int64_t> h_node_off;
+    std::vector<double> h_inst_dt;      // empty: every instance uses opt.dt
+    DevBuf<int> elem_inst, node_inst, b_chunk_inst, b_chunk_row0, b_chunk_row1, b_inst_chunk, b_inst_maxit;
+    DevBuf<double> b_inst_dt, b_partial;
+    DevBuf<CgScalars> b_sc, b_outer;
+    DevBuf<int> b_kA, b_kV;
+    CgScalars* h_bsc = nullptr;         // pinned [M]
+    int b_num_chunks = 0;
+    int b_chunk_rows = 0, b_lanes = 0;  // current table shape
+    int b_fixed_rows = 0, b_fixed_lanes = 0;  // 0: autotune
+    bool b_tuned = false;
+    double b_tune_ms = 0.0;             // tuned time per CG iteration (ms)
+    double b_maxit_for = -2.0;          // cg_max_iterations the iteration caps were built for
+    double b_dt_for = -1.0;             // opt.dt the per-instance dt table was built for
+    int b_predicted[17];
+    std::vector<int64_t> h_inst_iters;  // last step, per instance (inner iterations summed over sweeps)
+    std::vector<double> h_inst_err;
+    std::vector<int32_t> h_inst_conv;
+    hipGraph_t b_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t b_exec[2] = {nullptr, nullptr};  // 8-iteration chunk per precision
+    bool b_use_graphs = true;
+
     // host-pinned readback
     CgScalars* h_sc = nullptr;
     int* h_counters = nullptr;
@@ -260,6 +286,15 @@ void ensure_bc(fem_ctx* c) {
     }
 }
 
+void reset_batch_graphs(fem_ctx* c) {
+    for (int i = 0; i < 2; ++i) {
+        if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
+        if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
+        c->b_exec[i] = nullptr;
+        c->b_graph[i] = nullptr;
+    }
+}
+
 void alloc_precision_buffers(fem_ctx* c) {
     int prec = c->opt.precision;
     if (prec == c->last_precision) return;
@@ -276,6 +311,7 @@ void alloc_precision_buffers(fem_ctx* c) {
         b->alloc(n * tv);
     c->last_precision = prec;
     c->linear_matrix_ready = false;
+    reset_batch_graphs(c);
 }
 
 Operator op_of(fem_ctx* c) {
@@ -425,9 +461,289 @@ void state_update(fem_ctx* c, const double* xs) {
           launch_update((int)c->ndof, c->dv.p, xs, c->x.p, c->v.p, c->opt.dt, c->counters.p + 3, c->stream));
 }
 
+// ------------------------------------------------------------------------------------------
+// Batched instances
+// ------------------------------------------------------------------------------------------
+constexpr int kGraphIters = 8;  // CG iterations per captured graph
+
+// Row-chunk tables: chunks of at most `rows` block rows that never straddle an instance.
+void build_batch_tables(fem_ctx* c, int rows, int lanes) {
+    const int M = c->num_instances;
+    std::vector<int> ci, r0, r1, ic(M + 1, 0);
+    for (int i = 0; i < M; ++i) {
+        ic[i] = (int)ci.size();
+        for (int64_t a = c->h_node_off[i]; a < c->h_node_off[i + 1]; a += rows) {
+            ci.push_back(i);
+            r0.push_back((int)a);
+            r1.push_back((int)std::min<int64_t>(a + rows, c->h_node_off[i + 1]));
+        }
+    }
+    ic[M] = (int)ci.size();
+    const int K = (int)ci.size();
+    hipStream_t s = c->stream;
+    c->b_chunk_inst.alloc(K);
+    c->b_chunk_inst.upload(ci.data(), K, s);
+    c->b_chunk_row0.alloc(K);
+    c->b_chunk_row0.upload(r0.data(), K, s);
+    c->b_chunk_row1.alloc(K);
+    c->b_chunk_row1.upload(r1.data(), K, s);
+    c->b_inst_chunk.alloc(M + 1);
+    c->b_inst_chunk.upload(ic.data(), M + 1, s);
+    c->b_partial.alloc(4 * (size_t)K);
+    HIPCHECK(hipStreamSynchronize(s));
+    c->b_num_chunks = K;
+    c->b_chunk_rows = rows;
+    c->b_lanes = lanes;
+    reset_batch_graphs(c);
+}
+
+// Per-instance iteration caps (Eigen default 2 x instance DOF) and time steps, rebuilt when options change.
+void refresh_batch_params(fem_ctx* c) {
+    const int M = c->num_instances;
+    if (c->b_maxit_for != (double)c->opt.cg_max_iterations) {
+        std::vector<int> mx(M);
+        for (int i = 0; i < M; ++i) {
+            const int64_t d = 3 * (c->h_node_off[i + 1] - c->h_node_off[i]);
+            const int64_t m = c->opt.cg_max_iterations < 0 ? 2 * d : c->opt.cg_max_iterations;
+            mx[i] = (int)std::min<int64_t>(m, std::numeric_limits<int>::max() - 1);
+        }
+        c->b_inst_maxit.upload(mx.data(), M, c->stream);
+        c->b_maxit_for = (double)c->opt.cg_max_iterations;
+    }
+    if (c->h_inst_dt.empty() && c->b_dt_for != c->opt.dt) {
+        std::vector<double> dt(M, c->opt.dt);
+        c->b_inst_dt.upload(dt.data(), M, c->stream);
+        c->b_dt_for = c->opt.dt;
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+BatchCg batch_args(fem_ctx* c, bool fp32) {
+    BatchCg a;
+    a.t.num_instances = c->num_instances;
+    a.t.num_chunks = c->b_num_chunks;
+    a.t.chunk_inst = c->b_chunk_inst.p;
+    a.t.chunk_row0 = c->b_chunk_row0.p;
+    a.t.chunk_row1 = c->b_chunk_row1.p;
+    a.t.inst_chunk = c->b_inst_chunk.p;
+    a.t.inst_maxit = c->b_inst_maxit.p;
+    a.op = op_of(c);
+    a.fp32 = fp32;
+    a.S = fp32 ? (const void*)c->S32.p : (const void*)c->S64.p;
+    a.x = fp32 ? (void*)c->cg_x.p : (void*)c->dv.p;
+    a.r = c->cg_r.p;
+    a.z = c->cg_z.p;
+    a.p[0] = c->cg_p0.p;
+    a.p[1] = c->cg_p1.p;
+    a.q = c->cg_q.p;
+    a.dinv = c->cg_dinv.p;
+    a.partial = c->b_partial.p;
+    a.sc = c->b_sc.p;
+    a.kA = c->b_kA.p;
+    a.kV = c->b_kV.p;
+    return a;
+}
+
+// Launches kGraphIters CG iterations: a captured graph replayed (the iteration counters live on the
+// device), or plain launches on streams that cannot be captured.
+void launch_batch_chunk(fem_ctx* c, const BatchCg& a) {
+    const int g = a.fp32 ? 1 : 0;
+    hipStream_t s = c->stream;
+    if (c->b_use_graphs && s != nullptr) {
+        if (!c->b_exec[g]) {
+            HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < kGraphIters; ++j) launch_batch_cg_iteration(a, c->b_lanes, s);
+            HIPCHECK(hipStreamEndCapture(s, &c->b_graph[g]));
+            HIPCHECK(hipGraphInstantiate(&c->b_exec[g], c->b_graph[g], nullptr, nullptr, 0));
+        }
+        HIPCHECK(hipGraphLaunch(c->b_exec[g], s));
+        return;
+    }
+    for (int j = 0; j < kGraphIters; ++j) launch_batch_cg_iteration(a, c->b_lanes, s);
+}
+
+// Runs CG iterations until every instance is done; returns the largest iteration count.
+int run_batch_iterations(fem_ctx* c, const BatchCg& a, int& predicted) {
+    const int M = c->num_instances;
+    int launched = 0;
+    const int first = std::max(1, predicted + 1);
+    while (launched < first) {
+        launch_batch_chunk(c, a);
+        launched += kGraphIters;
+    }
+    int cap = 0;
+    for (int i = 0; i < M; ++i)
+        cap = std::max<int64_t>(cap, c->opt.cg_max_iterations < 0 ? 6 * (c->h_node_off[i + 1] - c->h_node_off[i])
+                                                                   : c->opt.cg_max_iterations);
+    for (;;) {
+        HIPCHECK(hipMemcpyAsync(c->h_bsc, a.sc, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        bool all = true;
+        for (int i = 0; i < M && all; ++i) all = c->h_bsc[i].done != 0;
+        if (all || launched > cap + 1) break;
+        const int chunk = std::max(1, std::max(2, predicted / 8) / kGraphIters);
+        for (int j = 0; j < chunk; ++j, launched += kGraphIters) launch_batch_chunk(c, a);
+    }
+    int mx = 0;
+    for (int i = 0; i < M; ++i) mx = std::max(mx, c->h_bsc[i].iters);
+    predicted = mx + 1;
+    return mx;
+}
+
+// Autotuner: time CG iterations for each (chunk rows, lanes per row) on the current operator and rhs and
+// keep the fastest. Runs once per context (or after fem_set_batch_tuning(0, 0)).
+void tune_batch(fem_ctx* c, bool fp32) {
+    if (c->b_fixed_rows > 0) {
+        if (c->b_chunk_rows != c->b_fixed_rows || c->b_lanes != c->b_fixed_lanes)
+            build_batch_tables(c, c->b_fixed_rows, c->b_fixed_lanes);
+        c->b_tuned = true;
+        return;
+    }
+    const int rows_c[] = {64, 128, 256, 512, 1024};
+    const int lanes_c[] = {4, 8, 16};
+    double best = 1e30;
+    int best_r = 256, best_l = 8;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    const bool save = c->b_use_graphs;
+    c->b_use_graphs = false;
+    for (int rows : rows_c)
+        for (int lanes : lanes_c) {
+            build_batch_tables(c, rows, lanes);
+            BatchCg a = batch_args(c, fp32);
+            if (fp32) {
+                // FP32 operator on the rhs cast to the CG precision through the init kernel
+                launch_batch_cg_init(a, c->b.p, nullptr, c->opt.inner_tolerance, lanes, c->stream);
+            } else {
+                launch_batch_cg_init(a, c->b.p, nullptr, c->opt.cg_tolerance, lanes, c->stream);
+            }
+            for (int j = 0; j < 2; ++j) launch_batch_cg_iteration(a, lanes, c->stream);
+            HIPCHECK(hipEventRecord(e0, c->stream));
+            for (int j = 0; j < 8; ++j) launch_batch_cg_iteration(a, lanes, c->stream);
+            HIPCHECK(hipEventRecord(e1, c->stream));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_r = rows;
+                best_l = lanes;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->b_use_graphs = save;
+    build_batch_tables(c, best_r, best_l);
+    c->b_tune_ms = best / 8.0;
+    c->b_tuned = true;
+}
+
+// Per-instance linear solves S_i dv_i = b_i (FP64 CG, or FP32 CG inside FP64 refinement).
+void batch_linear_solve(fem_ctx* c) {
+    const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
+    const int M = c->num_instances;
+    refresh_batch_params(c);
+    if (!c->b_tuned) tune_batch(c, fp32);
+    std::fill(c->h_inst_iters.begin(), c->h_inst_iters.end(), 0);
+    if (!fp32) {
+        BatchCg a = batch_args(c, false);
+        launch_batch_cg_init(a, c->b.p, nullptr, c->opt.cg_tolerance, c->b_lanes, c->stream);
+        const int it = run_batch_iterations(c, a, c->b_predicted[0]);
+        int64_t total = 0;
+        bool conv = true;
+        double err = 0.0;
+        for (int i = 0; i < M; ++i) {
+            c->h_inst_iters[i] = c->h_bsc[i].iters;
+            c->h_inst_err[i] = c->h_bsc[i].err;
+            c->h_inst_conv[i] = c->h_bsc[i].converged;
+            total += c->h_bsc[i].iters;
+            conv = conv && c->h_bsc[i].converged;
+            err = std::max(err, c->h_bsc[i].err);
+        }
+        c->stats.last_cg_iterations = it;
+        c->stats.total_cg_iterations += total;
+        c->stats.last_cg_error = err;
+        c->stats.converged = conv;
+        c->stats.last_refinements = 0;
+        return;
+    }
+    const size_t n = (size_t)c->ndof;
+    HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+    BatchCg a = batch_args(c, true);
+    int sweeps = 0, longest = 0;
+    for (int sweep = 0; sweep <= c->opt.max_refinements; ++sweep) {
+        launch_batch_residual(a, c->S64.p, c->b.p, c->dv.p, sweep == 0, c->r64.p, c->b_outer.p, c->opt.cg_tolerance,
+                              c->b_lanes, c->stream);
+        if (sweep == c->opt.max_refinements) break;
+        launch_batch_cg_init(a, c->r64.p, c->b_outer.p, c->opt.inner_tolerance, c->b_lanes, c->stream);
+        int& pred = c->b_predicted[1 + std::min(sweep, 15)];
+        longest += run_batch_iterations(c, a, pred) + 1;
+        for (int i = 0; i < M; ++i)
+            if (!c->h_bsc[i].converged || c->h_bsc[i].iters > 0) c->h_inst_iters[i] += c->h_bsc[i].iters;
+        HIPCHECK(hipMemcpyAsync(c->h_bsc, c->b_outer.p, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        bool all = true;
+        for (int i = 0; i < M && all; ++i) all = c->h_bsc[i].done != 0;
+        if (all) break;
+        launch_batch_accumulate(a, (const float*)c->cg_x.p, c->dv.p, c->b_outer.p, c->stream);
+        sweeps++;
+    }
+    HIPCHECK(hipMemcpyAsync(c->h_bsc, c->b_outer.p, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    int64_t total = 0;
+    bool conv = true;
+    double err = 0.0;
+    for (int i = 0; i < M; ++i) {
+        c->h_inst_err[i] = c->h_bsc[i].err;
+        c->h_inst_conv[i] = c->h_bsc[i].converged;
+        total += c->h_inst_iters[i];
+        conv = conv && c->h_bsc[i].converged;
+        err = std::max(err, c->h_bsc[i].err);
+    }
+    c->stats.last_cg_iterations = longest;
+    c->stats.total_cg_iterations += total;
+    c->stats.last_cg_error = err;
+    c->stats.converged = conv;
+    c->stats.last_refinements = sweeps;
+}
+
+int batch_step(fem_ctx* c) {
+    if (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1)
+        throw InvalidArg("batched instances step with implicit Euler, one Newton iteration");
+    refresh_batch_params(c);
+    TetElementArgs a;
+    a.num_elements = (int)c->num_elements;
+    a.conn = c->conn4.p;
+    a.x0 = c->x0.p;
+    a.xn = c->x.p;
+    a.vn = c->v.p;
+    a.xs = c->x.p;
+    a.vs = c->v.p;
+    a.mat = c->mat.p;
+    a.dt = c->opt.dt;
+    a.write_matrix = 1;
+    a.general = 0;
+    a.rec = c->rec.p;
+    a.rhs_rec = c->rhs_rec.p;
+    a.flag_counts = c->counters.p;
+    a.elem_inst = c->elem_inst.p;
+    a.inst_dt = c->b_inst_dt.p;
+    TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
+    assemble_tet(c);
+    rhs_tet(c);
+    batch_linear_solve(c);
+    TIMED(c, FEM_PHASE_UPDATE, 0,
+          launch_update((int)c->ndof, c->dv.p, c->x.p, c->x.p, c->v.p, c->opt.dt, c->counters.p + 3, c->stream,
+                        c->node_inst.p, c->b_inst_dt.p));
+    c->stats.steps++;
+    return FEM_OK;
+}
+
 // One implicit step (OdeSolverEulerImplicit::solve with computeCompliance = false, or
 // OdeSolverLinearEulerImplicit::solve). Returns FEM_OK or FEM_INACTIVE.
 int step_once(fem_ctx* c) {
+    if (c->num_instances > 0) return batch_step(c);
     const double dt = c->opt.dt;
     const size_t n = (size_t)c->ndof;
     const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
@@ -677,6 +993,11 @@ void fem_destroy(fem_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_bsc) (void)hipHostFree(c->h_bsc);
+    for (int i = 0; i < 2; ++i) {
+        if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
+        if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
+    }
     hex_release(c->hex);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1002,6 +1323,109 @@ int fem_get_phase_times(fem_ctx* c, fem_phase_times* out) {
 
 int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
 
+int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_offsets, const double* dt) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "upload the mesh before declaring instances");
+        if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("batched instances cover tetrahedral meshes");
+        if (num_instances < 1 || !node_offsets) throw InvalidArg("need at least one instance and node offsets");
+        const int M = num_instances;
+        if (node_offsets[0] != 0 || node_offsets[M] != c->num_nodes)
+            throw InvalidArg("node offsets must start at 0 and end at the node count");
+        for (int i = 0; i < M; ++i)
+            if (node_offsets[i + 1] <= node_offsets[i]) throw InvalidArg("every instance needs at least one node");
+        if (dt)
+            for (int i = 0; i < M; ++i)
+                if (!(dt[i] > 0.0)) throw InvalidArg("instance time steps must be positive");
+        HIPCHECK(hipSetDevice(c->device));
+        c->h_node_off.assign(node_offsets, node_offsets + M + 1);
+        auto inst_of = [&](int64_t node) {
+            return (int)(std::upper_bound(c->h_node_off.begin(), c->h_node_off.end(), node) - c->h_node_off.begin()) - 1;
+        };
+        std::vector<int> ni(c->num_nodes), ei(c->num_elements);
+        for (int i = 0; i < M; ++i)
+            for (int64_t a = node_offsets[i]; a < node_offsets[i + 1]; ++a) ni[a] = i;
+        std::vector<int4> conn(c->num_elements);
+        HIPCHECK(hipMemcpy(conn.data(), c->conn4.p, conn.size() * sizeof(int4), hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < c->num_elements; ++e) {
+            const int i = ni[conn[e].x];
+            if (ni[conn[e].y] != i || ni[conn[e].z] != i || ni[conn[e].w] != i)
+                throw InvalidArg("element " + std::to_string(e) + " joins two instances");
+            ei[e] = i;
+        }
+        (void)inst_of;
+        c->num_instances = M;
+        c->elem_inst.alloc(c->num_elements);
+        c->elem_inst.upload(ei.data(), c->num_elements, c->stream);
+        c->node_inst.alloc(c->num_nodes);
+        c->node_inst.upload(ni.data(), c->num_nodes, c->stream);
+        c->b_inst_dt.alloc(M);
+        if (dt) {
+            c->h_inst_dt.assign(dt, dt + M);
+            c->b_inst_dt.upload(dt, M, c->stream);
+        } else {
+            c->h_inst_dt.clear();
+            c->b_dt_for = -1.0;
+        }
+        c->b_inst_maxit.alloc(M);
+        c->b_maxit_for = -2.0;
+        c->b_sc.alloc(M);
+        c->b_outer.alloc(M);
+        c->b_kA.alloc(M);
+        c->b_kV.alloc(M);
+        if (c->h_bsc) HIPCHECK(hipHostFree(c->h_bsc));
+        c->h_bsc = nullptr;
+        HIPCHECK(hipHostMalloc((void**)&c->h_bsc, M * sizeof(CgScalars), hipHostMallocDefault));
+        std::vector<CgScalars> zero(M);
+        std::memset(zero.data(), 0, M * sizeof(CgScalars));
+        c->b_sc.upload(zero.data(), M, c->stream);
+        c->b_outer.upload(zero.data(), M, c->stream);
+        c->h_inst_iters.assign(M, 0);
+        c->h_inst_err.assign(M, 0.0);
+        c->h_inst_conv.assign(M, 0);
+        for (int i = 0; i < 17; ++i) c->b_predicted[i] = 8;
+        build_batch_tables(c, c->b_fixed_rows > 0 ? c->b_fixed_rows : 256, c->b_fixed_lanes > 0 ? c->b_fixed_lanes : 8);
+        c->b_tuned = false;
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return FEM_OK;
+    });
+}
+
+int fem_query_instances(fem_ctx* c, int64_t* iterations, double* errors, int32_t* converged) {
+    return guarded(c, [&] {
+        if (c->num_instances == 0) return fail(c, FEM_ERR_ORDER, "no instances declared");
+        for (int i = 0; i < c->num_instances; ++i) {
+            if (iterations) iterations[i] = c->h_inst_iters[i];
+            if (errors) errors[i] = c->h_inst_err[i];
+            if (converged) converged[i] = c->h_inst_conv[i];
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_set_batch_tuning(fem_ctx* c, int32_t chunk_rows, int32_t lanes_per_row, int32_t use_graphs) {
+    return guarded(c, [&] {
+        if (chunk_rows < 0 || (chunk_rows > 0 && lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16))
+            throw InvalidArg("chunk_rows >= 0 and lanes_per_row in {4, 8, 16} (0, 0: autotune)");
+        c->b_fixed_rows = chunk_rows;
+        c->b_fixed_lanes = chunk_rows > 0 ? lanes_per_row : 0;
+        c->b_tuned = false;
+        c->b_use_graphs = use_graphs != 0;
+        reset_batch_graphs(c);
+        return FEM_OK;
+    });
+}
+
+int fem_get_batch_tuning(fem_ctx* c, int32_t* chunk_rows, int32_t* lanes_per_row, int32_t* num_chunks,
+                         double* ms_per_iteration) {
+    return guarded(c, [&] {
+        if (chunk_rows) *chunk_rows = c->b_chunk_rows;
+        if (lanes_per_row) *lanes_per_row = c->b_lanes;
+        if (num_chunks) *num_chunks = c->b_num_chunks;
+        if (ms_per_iteration) *ms_per_iteration = c->b_tune_ms;
+        return FEM_OK;
+    });
+}
+
 int fem_set_young_modulus(fem_ctx* c, const double* young) {
     return guarded(c, [&] {
         if (!c->mesh_ready || !young) throw InvalidArg("mesh not uploaded or null input");
@@ -1028,6 +1452,7 @@ int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const dou
     return guarded(c, [&] {
         if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "mesh not uploaded");
         if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("the adjoint covers tetrahedral meshes");
+        if (c->num_instances > 0) throw InvalidArg("the adjoint covers single (non-batched) meshes");
         if (c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
             (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1))
             throw InvalidArg("the corotational adjoint reverses one-Newton-iteration implicit Euler steps");

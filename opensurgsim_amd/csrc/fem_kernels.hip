@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
     if (e >= A.num_elements) return;
     int4 id4 = A.conn[e];
     const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-    const double dt = A.dt;
+    const double dt = A.elem_inst ? A.inst_dt[A.elem_inst[e]] : A.dt;
     double x0[12], xn[12], d1[12], d2[12];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -590,9 +590,12 @@ void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalar
 // State update.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void update_kernel(int n, const double* __restrict__ dv, const double* xs, double* x,
-                                                     double* v, double dt, int* invalid) {
+                                                     double* v, double dt_all, int* invalid,
+                                                     const int* __restrict__ node_inst,
+                                                     const double* __restrict__ inst_dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double dt = node_inst ? inst_dt[node_inst[i / 3]] : dt_all;
     double vi = v[i] + dv[i];
     double xi = xs[i] + dt * vi;
     v[i] = vi;
@@ -601,9 +604,14 @@ __global__ __launch_bounds__(256) void update_kernel(int n, const double* __rest
 }
 
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
-                   hipStream_t s) {
+                   hipStream_t s, const int* node_inst, const double* inst_dt) {
     if (n == 0) return;
-    launch_k(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid);
+    launch_k(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid, node_inst, inst_dt);
+}
+
+void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
+                   hipStream_t s) {
+    launch_update(n, dv, xs, x, v, dt, invalid, s, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------

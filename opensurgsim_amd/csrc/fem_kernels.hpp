@@ -67,6 +67,8 @@ struct TetElementArgs {
     double* rec;          // [num_elements][kTetRec]
     double* rhs_rec;      // [num_elements][12]
     int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
+    const int* elem_inst = nullptr;     // batched instances: instance of each element (null: one dt)
+    const double* inst_dt = nullptr;    // time step per instance
 };
 
 void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s);
@@ -125,6 +127,9 @@ void launch_residual64(const Operator& op, const double* S64, const double* b, c
 void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s);
 
 // State update: v += dv; x = xs + dt v (Math/OdeSolverEulerImplicit.cpp:98-99). Sets *invalid on non-finite.
+// v += dv, x = xs + dt v with dt per node's instance when node_inst is given; flags non-finite values.
+void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
+                   hipStream_t s, const int* node_inst, const double* inst_dt);
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
                    hipStream_t s);
 

@@ -25,7 +25,8 @@ EXPORTS = [
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
     "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
-    "fem_set_young_modulus", "fem_adjoint_step",
+    "fem_set_young_modulus", "fem_adjoint_step", "fem_set_instances", "fem_query_instances",
+    "fem_set_batch_tuning", "fem_get_batch_tuning",
 ]
 
 PHASES = ["element", "assemble", "rhs", "cg_init", "spmv", "cg_vector", "refine", "update", "spmv_skipped"]
@@ -114,6 +115,11 @@ def load():
     L.fem_get_phase_times.argtypes = [_vp, ctypes.POINTER(FemPhaseTimes)]
     L.fem_get_node_masses.argtypes = [_vp, _dp]
     L.fem_set_young_modulus.argtypes = [_vp, _dp]
+    L.fem_set_instances.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _dp]
+    L.fem_query_instances.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int32)]
+    L.fem_set_batch_tuning.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.fem_get_batch_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), _dp]
     L.fem_adjoint_step.argtypes = [_vp] + [_vp] * 8
     L.fem_time_operator.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.fem_probe_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -256,6 +262,33 @@ class FemGpu:
         t = FemPhaseTimes()
         self._check(self._L.fem_get_phase_times(self.h, ctypes.byref(t)))
         return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(PHASES)}
+
+    def set_instances(self, node_offsets, dt=None):
+        """Declare the uploaded mesh as independent instances (fem_set_instances)."""
+        off = np.ascontiguousarray(node_offsets, dtype=np.int64)
+        self._inst_dt = None if dt is None else _f64(dt)
+        self._check(self._L.fem_set_instances(self.h, off.size - 1, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              _ptr(self._inst_dt) if self._inst_dt is not None else None))
+        self.num_instances = off.size - 1
+
+    def instance_stats(self):
+        m = self.num_instances
+        it = np.zeros(m, dtype=np.int64)
+        err = np.zeros(m)
+        conv = np.zeros(m, dtype=np.int32)
+        self._check(self._L.fem_query_instances(self.h, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _ptr(err),
+                                                conv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return it, err, conv
+
+    def set_batch_tuning(self, chunk_rows=0, lanes_per_row=0, use_graphs=True):
+        self._check(self._L.fem_set_batch_tuning(self.h, int(chunk_rows), int(lanes_per_row), int(bool(use_graphs))))
+
+    def batch_tuning(self):
+        r, l, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_double()
+        self._check(self._L.fem_get_batch_tuning(self.h, ctypes.byref(r), ctypes.byref(l), ctypes.byref(k),
+                                                 ctypes.byref(ms)))
+        return dict(chunk_rows=r.value, lanes_per_row=l.value, chunks=k.value, ms_per_iteration=ms.value)
 
     def set_young_modulus(self, young):
         E = _f64(np.broadcast_to(young, (self.num_elements,)))

@@ -274,6 +274,25 @@ def config_hex(n=(128, 64, 128), seed=5):
                   gravity=(0.0, -9.81, 0.0), dt=1e-3, steps=50, cg_tol=1e-6, seed=seed)
 
 
+def concat_instances(cfgs):
+    """One block-diagonal mesh from independent instance configs: node offsets, global connectivity,
+    per-element materials, global boundary nodes, per-instance time steps."""
+    off = [0]
+    for c in cfgs:
+        off.append(off[-1] + c.x0.shape[0])
+    conn = np.concatenate([c.conn + off[i] for i, c in enumerate(cfgs)]).astype(np.int32)
+    fext = None
+    if any(c.fext is not None for c in cfgs):
+        fext = np.concatenate([c.fext if c.fext is not None else np.zeros(c.x0.size) for c in cfgs])
+    return Config(name="batch-%d" % len(cfgs), kind=cfgs[0].kind, x0=np.concatenate([c.x0 for c in cfgs]),
+                  conn=conn, E=np.concatenate([c.E for c in cfgs]), nu=np.concatenate([c.nu for c in cfgs]),
+                  rho=np.concatenate([c.rho for c in cfgs]),
+                  bc_nodes=np.concatenate([c.bc_nodes + off[i] for i, c in enumerate(cfgs)]),
+                  v0=np.concatenate([c.v0 for c in cfgs]), fext=fext, node_offsets=np.asarray(off, dtype=np.int64),
+                  dt=np.asarray([c.dt for c in cfgs]), gravity=cfgs[0].gravity, steps=cfgs[0].steps,
+                  cg_tol=cfgs[0].cg_tol, instances=cfgs)
+
+
 def config_batched(count=64, seed=11, lo=6, hi=48):
     """Independent corotational block meshes with per-axis cube counts ~ U{lo..hi}, E ~ logU[1e3, 1e6],
     nu ~ U[0.30, 0.49], rho 1000, dt in {0.5, 1, 2} ms, one random fixed face each."""

@@ -1,0 +1,132 @@
+"""Batched independent instances (fem_set_instances): every instance must step exactly as its own
+representation would, against the oracle run per instance."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from opensurgsim_amd import femgpu
+from opensurgsim_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(count=5, seed=21):
+    cfgs = mg.config_batched(count, seed=seed, lo=2, hi=5)
+    for i, c in enumerate(cfgs):  # give every instance some motion besides gravity
+        top = mg.face_nodes(c.nx, c.ny, c.nz, 1, 1)
+        v0 = np.zeros((c.x0.shape[0], 3))
+        v0[top, 0] = 0.01 * (i + 1)
+        c["v0"] = v0.reshape(-1)
+    return cfgs, mg.concat_instances(cfgs)
+
+
+def run_batch(b, steps, precision=femgpu.FP64, tol=1e-12, tuning=None):
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    g.add_bc_nodes(b.bc_nodes)
+    g.set_options(dt=1e-3, cg_tolerance=tol, precision=precision)
+    g.set_state(v=b.v0)
+    g.set_instances(b.node_offsets, b.dt)
+    if tuning:
+        g.set_batch_tuning(*tuning)
+    assert g.step(steps) == femgpu.FEM_OK
+    x, v = g.state()
+    it, err, conv = g.instance_stats()
+    tun = g.batch_tuning()
+    g.close()
+    return x, v, it, conv, tun
+
+
+def run_oracle(c, steps, tol=1e-13):
+    s = oracle.Sim(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
+    for n in c.bc_nodes:
+        s.add_bc_node(n)
+    s.set_solver(tol=tol)
+    s.set_state(v=c.v0)
+    s.step(c.dt, steps)
+    return s.state(), s.last_cg()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_batch_matches_oracle_per_instance():
+    cfgs, b = small_batch()
+    steps = 8
+    say("batch of %d instances, %d steps, dts %s" % (len(cfgs), steps, list(b.dt)))
+    x, v, it, conv, tun = run_batch(b, steps, tol=1e-12)
+    say("  tuning", tun, "iterations", it)
+    assert conv.all()
+    off = b.node_offsets
+    for i, c in enumerate(cfgs):
+        (xo, vo), _ = run_oracle(c, steps, tol=1e-12)
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        x0 = c.x0.reshape(-1)
+        e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo))
+        say("  instance", i, c.conn.shape[0], "tets, dt", c.dt, e)
+        assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_batch_matches_single_instance_runs():
+    """Each instance of the batch against the same mesh stepped alone through the single-mesh path."""
+    cfgs, b = small_batch(4, seed=5)
+    steps = 6
+    x, v, it, conv, _ = run_batch(b, steps, tol=1e-10)
+    off = b.node_offsets
+    for i, c in enumerate(cfgs):
+        g = femgpu.FemGpu(0)
+        g.upload_mesh(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
+        g.add_bc_nodes(c.bc_nodes)
+        g.set_options(dt=c.dt, cg_tolerance=1e-10)
+        g.set_state(v=c.v0)
+        g.step(steps)
+        xs, vs = g.state()
+        its = g.stats()["last_cg_iterations"]
+        g.close()
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        say("  instance", i, "iterations batch", it[i], "single", its, "v rel", rel(v[sl], vs))
+        assert rel(v[sl], vs) < 1e-9
+        assert abs(int(it[i]) - int(its)) <= 1
+
+
+def test_batch_fp32_refined_close_to_fp64():
+    cfgs, b = small_batch(6, seed=8)
+    steps = 6
+    x64, v64, _, _, _ = run_batch(b, steps, femgpu.FP64, tol=1e-10)
+    x32, v32, it, conv, _ = run_batch(b, steps, femgpu.FP32_REFINED, tol=1e-10)
+    x0 = b.x0.reshape(-1)
+    e = dict(u=rel(x32 - x0, x64 - x0), v=rel(v32, v64))
+    say("  fp32+refine vs fp64", e, "inner iterations", it)
+    assert conv.all()
+    assert e["u"] < 1e-6 and e["v"] < 1e-6
+
+
+def test_batch_graphs_and_shapes_agree():
+    cfgs, b = small_batch(5, seed=13)
+    steps = 4
+    ref = run_batch(b, steps, tol=1e-11, tuning=(128, 8, True))
+    plain = run_batch(b, steps, tol=1e-11, tuning=(128, 8, False))
+    assert np.array_equal(ref[0], plain[0]) and np.array_equal(ref[1], plain[1])  # graph replay = launches
+    for shape in [(64, 4, True), (512, 16, True)]:
+        other = run_batch(b, steps, tol=1e-11, tuning=shape)
+        say("  shape", shape, "v rel", rel(other[1], ref[1]))
+        assert rel(other[1], ref[1]) < 1e-10
+
+
+def test_batch_rejects_elements_across_instances():
+    cfgs, b = small_batch(3)
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    bad = b.node_offsets.copy()
+    bad[1] += 1  # one node of instance 1 moved into instance 0: its elements straddle
+    with pytest.raises(femgpu.FemError):
+        g.set_instances(bad, b.dt)
+    g.close()
