@@ -140,9 +140,17 @@ int fem_query(fem_ctx* ctx, fem_stats* stats);
  * CG scalars with convergence masks, iteration chunks replayed as a captured hipGraph, and row-chunk and
  * lane shapes picked by an autotuner on the first step. Call after fem_upload_mesh. */
 int fem_set_instances(fem_ctx* ctx, int32_t num_instances, const int64_t* node_offsets, const double* dt);
-/* Last step, per instance: CG iterations (FP32 mode: inner iterations summed over sweeps), relative residual
- * and convergence flag. Any output may be NULL. */
-int fem_query_instances(fem_ctx* ctx, int64_t* iterations, double* errors, int32_t* converged);
+/* Per-instance statistics. An instance whose state became non-finite is deactivated and no longer stepped,
+ * as Physics/FemRepresentation.cpp:285-293 deactivates one representation; fem_step returns FEM_INACTIVE
+ * only once every instance is. */
+typedef struct fem_instance_stats {
+    int64_t last_iterations;     /* CG iterations of the last step (FP32 mode: inner, summed over sweeps) */
+    double last_error;           /* relative residual at the end of the last step's solve */
+    int64_t unconverged_steps;   /* steps whose solve stopped at its iteration cap */
+    int32_t converged;           /* last step's solve met its tolerance */
+    int32_t active;
+} fem_instance_stats;
+int fem_query_instances(fem_ctx* ctx, fem_instance_stats* stats);   /* num_instances entries */
 /* Fix the chunk shape (chunk_rows > 0 block rows per workgroup, lanes_per_row 4, 8 or 16) or return to
  * autotuning (0, 0); use_graphs 0 launches the CG iterations one by one instead of replaying graphs.
  * No counterpart in the reference. */

@@ -196,6 +196,11 @@ struct fem_ctx {
     DevBuf<double> b_inst_dt, b_partial;
     DevBuf<CgScalars> b_sc, b_outer;
     DevBuf<int> b_kA, b_kV;
+    DevBuf<int> b_active, b_invalid;   // [M] per-instance activity / non-finite flag of the last update
+    DevBuf<int> b_live;                // compacted chunk list of instances still iterating
+    int* h_live = nullptr;             // pinned staging for b_live
+    std::vector<int32_t> h_inst_active;
+    std::vector<int64_t> h_inst_unconv;   // steps whose solve stopped at the cap, per instance
     CgScalars* h_bsc = nullptr;         // pinned [M]
     int b_num_chunks = 0;
     int b_chunk_rows = 0, b_lanes = 0;  // current table shape
@@ -490,6 +495,10 @@ void build_batch_tables(fem_ctx* c, int rows, int lanes) {
     c->b_inst_chunk.alloc(M + 1);
     c->b_inst_chunk.upload(ic.data(), M + 1, s);
     c->b_partial.alloc(4 * (size_t)K);
+    c->b_live.alloc(K);
+    if (c->h_live) HIPCHECK(hipHostFree(c->h_live));
+    c->h_live = nullptr;
+    HIPCHECK(hipHostMalloc((void**)&c->h_live, K * sizeof(int), hipHostMallocDefault));
     HIPCHECK(hipStreamSynchronize(s));
     c->b_num_chunks = K;
     c->b_chunk_rows = rows;
@@ -527,6 +536,7 @@ BatchCg batch_args(fem_ctx* c, bool fp32) {
     a.t.chunk_row1 = c->b_chunk_row1.p;
     a.t.inst_chunk = c->b_inst_chunk.p;
     a.t.inst_maxit = c->b_inst_maxit.p;
+    a.t.inst_active = c->b_active.p;
     a.op = op_of(c);
     a.fp32 = fp32;
     a.S = fp32 ? (const void*)c->S32.p : (const void*)c->S64.p;
@@ -562,9 +572,13 @@ void launch_batch_chunk(fem_ctx* c, const BatchCg& a) {
     for (int j = 0; j < kGraphIters; ++j) launch_batch_cg_iteration(a, c->b_lanes, s);
 }
 
-// Runs CG iterations until every instance is done; returns the largest iteration count.
+// Runs CG iterations until every instance is done; returns the largest iteration count. While most chunks
+// belong to instances still iterating, iterations go out as captured 8-iteration graphs over all chunks;
+// once the live instances cover less than half of the chunks, iterations are launched over a compacted list
+// of their chunks only (the converged instances' workgroups would only exit).
 int run_batch_iterations(fem_ctx* c, const BatchCg& a, int& predicted) {
     const int M = c->num_instances;
+    const int K = c->b_num_chunks;
     int launched = 0;
     const int first = std::max(1, predicted + 1);
     while (launched < first) {
@@ -575,14 +589,36 @@ int run_batch_iterations(fem_ctx* c, const BatchCg& a, int& predicted) {
     for (int i = 0; i < M; ++i)
         cap = std::max<int64_t>(cap, c->opt.cg_max_iterations < 0 ? 6 * (c->h_node_off[i + 1] - c->h_node_off[i])
                                                                    : c->opt.cg_max_iterations);
+    std::vector<int> ic;
     for (;;) {
         HIPCHECK(hipMemcpyAsync(c->h_bsc, a.sc, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
+        int live_chunks = 0;
         bool all = true;
-        for (int i = 0; i < M && all; ++i) all = c->h_bsc[i].done != 0;
+        if (ic.empty()) {
+            ic.resize(M + 1);
+            HIPCHECK(hipMemcpy(ic.data(), c->b_inst_chunk.p, (M + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        for (int i = 0; i < M; ++i)
+            if (!c->h_bsc[i].done) {
+                all = false;
+                live_chunks += ic[i + 1] - ic[i];
+            }
         if (all || launched > cap + 1) break;
-        const int chunk = std::max(1, std::max(2, predicted / 8) / kGraphIters);
-        for (int j = 0; j < chunk; ++j, launched += kGraphIters) launch_batch_chunk(c, a);
+        if (2 * live_chunks > K) {
+            const int chunk = std::max(1, std::max(2, predicted / 8) / kGraphIters);
+            for (int j = 0; j < chunk; ++j, launched += kGraphIters) launch_batch_chunk(c, a);
+        } else {
+            int n = 0;
+            for (int i = 0; i < M; ++i)
+                if (!c->h_bsc[i].done)
+                    for (int q = ic[i]; q < ic[i + 1]; ++q) c->h_live[n++] = q;
+            HIPCHECK(hipMemcpyAsync(c->b_live.p, c->h_live, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            BatchCg t = a;
+            t.t.live = c->b_live.p;
+            t.t.grid = n;
+            for (int j = 0; j < 2 * kGraphIters; ++j, ++launched) launch_batch_cg_iteration(t, c->b_lanes, c->stream);
+        }
     }
     int mx = 0;
     for (int i = 0; i < M; ++i) mx = std::max(mx, c->h_bsc[i].iters);
@@ -733,9 +769,26 @@ int batch_step(fem_ctx* c) {
     assemble_tet(c);
     rhs_tet(c);
     batch_linear_solve(c);
+    for (int i = 0; i < c->num_instances; ++i)
+        if (c->h_inst_active[i] && !c->h_inst_conv[i]) c->h_inst_unconv[i]++;
     TIMED(c, FEM_PHASE_UPDATE, 0,
           launch_update((int)c->ndof, c->dv.p, c->x.p, c->x.p, c->v.p, c->opt.dt, c->counters.p + 3, c->stream,
-                        c->node_inst.p, c->b_inst_dt.p));
+                        c->node_inst.p, c->b_inst_dt.p, c->b_active.p, c->b_invalid.p));
+    // instances whose new state is not finite are deactivated (Physics/FemRepresentation.cpp:285-293), the
+    // others go on; the batch is inactive only when every instance is
+    std::vector<int> inv(c->num_instances);
+    HIPCHECK(hipMemcpyAsync(inv.data(), c->b_invalid.p, inv.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    bool changed = false, any = false;
+    for (int i = 0; i < c->num_instances; ++i) {
+        if (inv[i] && c->h_inst_active[i]) {
+            c->h_inst_active[i] = 0;
+            changed = true;
+        }
+        any = any || c->h_inst_active[i];
+    }
+    if (changed) c->b_active.upload(c->h_inst_active.data(), c->num_instances, c->stream);
+    if (!any) HIPCHECK(hipMemsetAsync(c->counters.p + 3, 0xff, sizeof(int), c->stream));
     c->stats.steps++;
     return FEM_OK;
 }
@@ -994,6 +1047,7 @@ void fem_destroy(fem_ctx* c) {
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->h_bsc) (void)hipHostFree(c->h_bsc);
+    if (c->h_live) (void)hipHostFree(c->h_live);
     for (int i = 0; i < 2; ++i) {
         if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
         if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
@@ -1372,6 +1426,12 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
         c->b_outer.alloc(M);
         c->b_kA.alloc(M);
         c->b_kV.alloc(M);
+        c->h_inst_active.assign(M, 1);
+        c->h_inst_unconv.assign(M, 0);
+        c->b_active.alloc(M);
+        c->b_active.upload(c->h_inst_active.data(), M, c->stream);
+        c->b_invalid.alloc(M);
+        HIPCHECK(hipMemsetAsync(c->b_invalid.p, 0, M * sizeof(int), c->stream));
         if (c->h_bsc) HIPCHECK(hipHostFree(c->h_bsc));
         c->h_bsc = nullptr;
         HIPCHECK(hipHostMalloc((void**)&c->h_bsc, M * sizeof(CgScalars), hipHostMallocDefault));
@@ -1390,13 +1450,16 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
     });
 }
 
-int fem_query_instances(fem_ctx* c, int64_t* iterations, double* errors, int32_t* converged) {
+int fem_query_instances(fem_ctx* c, fem_instance_stats* out) {
     return guarded(c, [&] {
         if (c->num_instances == 0) return fail(c, FEM_ERR_ORDER, "no instances declared");
+        if (!out) throw InvalidArg("null output");
         for (int i = 0; i < c->num_instances; ++i) {
-            if (iterations) iterations[i] = c->h_inst_iters[i];
-            if (errors) errors[i] = c->h_inst_err[i];
-            if (converged) converged[i] = c->h_inst_conv[i];
+            out[i].last_iterations = c->h_inst_iters[i];
+            out[i].last_error = c->h_inst_err[i];
+            out[i].unconverged_steps = c->h_inst_unconv[i];
+            out[i].converged = c->h_inst_conv[i];
+            out[i].active = c->h_inst_active[i];
         }
         return FEM_OK;
     });

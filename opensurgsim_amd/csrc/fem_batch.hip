@@ -114,6 +114,13 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init2_kernel(BatchTables t,
     double bb, rz;
     range_sum2(partial, partial + t.num_chunks, t.inst_chunk[inst], t.inst_chunk[inst + 1], bb, rz, lds);
     if (threadIdx.x == 0) {
+        if (!t.inst_active[inst]) {  // deactivated instance: no solve (dv is zeroed by the init kernel)
+            CgScalars s = {};
+            s.done = 1;
+            s.converged = 1;
+            sc[inst] = s;
+            return;
+        }
         CgScalars s;
         s.bb = bb;
         s.rz[0] = rz;
@@ -146,7 +153,7 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_apply_kernel(BatchTables t,
                                                                   CgScalars* sc, const int* __restrict__ kA,
                                                                   int* __restrict__ kV) {
     __shared__ double lds[64];
-    const int c = blockIdx.x, inst = t.chunk_inst[c];
+    const int c = t.live ? t.live[blockIdx.x] : blockIdx.x, inst = t.chunk_inst[c];
     if (sc[inst].done) return;
     const int k = kA[inst];
     const int K = t.num_chunks;
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_vector_kernel(BatchTables t
                                                                    const CgScalars* sc, const int* __restrict__ kV,
                                                                    int* __restrict__ kA) {
     __shared__ double lds[64];
-    const int c = blockIdx.x, inst = t.chunk_inst[c];
+    const int c = t.live ? t.live[blockIdx.x] : blockIdx.x, inst = t.chunk_inst[c];
     if (sc[inst].done) return;
     const int k = kV[inst];
     const int K = t.num_chunks;
@@ -302,9 +309,14 @@ __global__ __launch_bounds__(kBatchThreads) void brefine_check_kernel(BatchTable
         }
         o.rr = rr;
         o.err = o.bb > 0 ? sqrt(rr / o.bb) : 0.0;
-        if (o.bb == 0.0 || rr < o.threshold) {
+        if (!t.inst_active[inst] || o.bb == 0.0 || rr < o.threshold) {
             o.done = 1;
             o.converged = 1;
+            inner[inst].done = 1;
+        } else if (!first && inner[inst].done && !inner[inst].converged) {
+            // the last inner solve stopped at its iteration cap: further sweeps would repeat it
+            o.done = 1;
+            o.converged = 0;
             inner[inst].done = 1;
         }
     }
@@ -320,7 +332,7 @@ __global__ __launch_bounds__(kBatchThreads) void bacc_kernel(BatchTables t, cons
 
 template <typename TV, int G>
 void iteration_g(const BatchCg& a, hipStream_t s) {
-    const dim3 grid(a.t.num_chunks), block(kBatchThreads);
+    const dim3 grid(a.t.grid > 0 ? a.t.grid : a.t.num_chunks), block(kBatchThreads);
     launch_k((bcg_apply_kernel<TV, G>), grid, block, 0, s, a.t, a.op, (const TV*)a.S, (const TV*)a.z, (TV*)a.p[0],
              (TV*)a.p[1], (TV*)a.q, a.partial, a.sc, (const int*)a.kA, a.kV);
     launch_k((bcg_vector_kernel<TV>), grid, block, 0, s, a.t, (TV*)a.x, (TV*)a.r, (TV*)a.z, (const TV*)a.p[0],

@@ -22,6 +22,9 @@ struct BatchTables {
     const int* chunk_row1;
     const int* inst_chunk;   // [M + 1]
     const int* inst_maxit;   // [M] iteration cap per instance
+    const int* inst_active;  // [M] 0 once the instance's state became non-finite (deactivated)
+    const int* live = nullptr;  // iteration kernels: workgroup b works on chunk live[b] (null: chunk b)
+    int grid = 0;               // iteration kernels: workgroups launched (0: num_chunks)
 };
 
 // Device-side iteration counters, one pair per instance: the operator kernel reads kA and writes kV,

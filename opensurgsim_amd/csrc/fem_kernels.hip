@@ -592,21 +592,26 @@ void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalar
 __global__ __launch_bounds__(256) void update_kernel(int n, const double* __restrict__ dv, const double* xs, double* x,
                                                      double* v, double dt_all, int* invalid,
                                                      const int* __restrict__ node_inst,
-                                                     const double* __restrict__ inst_dt) {
+                                                     const double* __restrict__ inst_dt,
+                                                     const int* __restrict__ inst_active, int* inst_invalid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double dt = node_inst ? inst_dt[node_inst[i / 3]] : dt_all;
+    const int inst = node_inst ? node_inst[i / 3] : 0;
+    if (inst_active && !inst_active[inst]) return;
+    const double dt = node_inst ? inst_dt[inst] : dt_all;
     double vi = v[i] + dv[i];
     double xi = xs[i] + dt * vi;
     v[i] = vi;
     x[i] = xi;
-    if (!isfinite(vi) || !isfinite(xi)) atomicOr(invalid, 1);
+    if (!isfinite(vi) || !isfinite(xi)) atomicOr(inst_invalid ? &inst_invalid[inst] : invalid, 1);
 }
 
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
-                   hipStream_t s, const int* node_inst, const double* inst_dt) {
+                   hipStream_t s, const int* node_inst, const double* inst_dt, const int* inst_active,
+                   int* inst_invalid) {
     if (n == 0) return;
-    launch_k(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid, node_inst, inst_dt);
+    launch_k(update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dv, xs, x, v, dt, invalid, node_inst, inst_dt,
+             inst_active, inst_invalid);
 }
 
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,

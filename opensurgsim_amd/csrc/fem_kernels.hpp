@@ -128,8 +128,10 @@ void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalar
 
 // State update: v += dv; x = xs + dt v (Math/OdeSolverEulerImplicit.cpp:98-99). Sets *invalid on non-finite.
 // v += dv, x = xs + dt v with dt per node's instance when node_inst is given; flags non-finite values.
+// Batched: instances with inst_active[i] == 0 are left unchanged; a non-finite value sets inst_invalid[i].
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
-                   hipStream_t s, const int* node_inst, const double* inst_dt);
+                   hipStream_t s, const int* node_inst, const double* inst_dt, const int* inst_active = nullptr,
+                   int* inst_invalid = nullptr);
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
                    hipStream_t s);
 

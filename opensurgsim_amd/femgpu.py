@@ -66,6 +66,11 @@ class FemStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FemInstanceStats(ctypes.Structure):
+    _fields_ = [("last_iterations", ctypes.c_int64), ("last_error", ctypes.c_double),
+                ("unconverged_steps", ctypes.c_int64), ("converged", ctypes.c_int32), ("active", ctypes.c_int32)]
+
+
 class FemPhaseTimes(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 9), ("launches", ctypes.c_int64 * 9)]
 
@@ -116,7 +121,7 @@ def load():
     L.fem_get_node_masses.argtypes = [_vp, _dp]
     L.fem_set_young_modulus.argtypes = [_vp, _dp]
     L.fem_set_instances.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _dp]
-    L.fem_query_instances.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int32)]
+    L.fem_query_instances.argtypes = [_vp, ctypes.POINTER(FemInstanceStats)]
     L.fem_set_batch_tuning.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.fem_get_batch_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_int32), _dp]
@@ -272,12 +277,16 @@ class FemGpu:
         self.num_instances = off.size - 1
 
     def instance_stats(self):
+        """Per-instance (last iterations, last error, converged) arrays; also sets last_instance_active and
+        last_instance_unconverged_steps."""
         m = self.num_instances
-        it = np.zeros(m, dtype=np.int64)
-        err = np.zeros(m)
-        conv = np.zeros(m, dtype=np.int32)
-        self._check(self._L.fem_query_instances(self.h, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _ptr(err),
-                                                conv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        st = (FemInstanceStats * m)()
+        self._check(self._L.fem_query_instances(self.h, st))
+        it = np.array([s.last_iterations for s in st], dtype=np.int64)
+        err = np.array([s.last_error for s in st])
+        conv = np.array([s.converged for s in st], dtype=np.int32)
+        self.last_instance_active = np.array([s.active for s in st], dtype=np.int32)
+        self.last_instance_unconverged_steps = np.array([s.unconverged_steps for s in st], dtype=np.int64)
         return it, err, conv
 
     def set_batch_tuning(self, chunk_rows=0, lanes_per_row=0, use_graphs=True):

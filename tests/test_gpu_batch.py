@@ -130,3 +130,31 @@ def test_batch_rejects_elements_across_instances():
     with pytest.raises(femgpu.FemError):
         g.set_instances(bad, b.dt)
     g.close()
+
+
+def test_non_finite_instance_is_deactivated_alone():
+    """An instance whose state becomes non-finite is deactivated by itself (as one representation would
+    be); the others keep stepping and still match the oracle."""
+    cfgs, b = small_batch(3, seed=4)
+    bad = 1
+    v0 = b.v0.copy()
+    v0[3 * (b.node_offsets[bad] + 5) + 1] = np.nan
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    g.add_bc_nodes(b.bc_nodes)
+    g.set_options(dt=1e-3, cg_tolerance=1e-12)
+    g.set_state(v=v0)
+    g.set_instances(b.node_offsets, b.dt)
+    steps = 4
+    assert g.step(steps) == femgpu.FEM_OK
+    g.instance_stats()
+    active = g.last_instance_active.tolist()
+    say("  active", active)
+    assert active == [1, 0, 1]
+    x, v = g.state()
+    g.close()
+    off = b.node_offsets
+    for i in (0, 2):
+        (xo, vo), _ = run_oracle(cfgs[i], steps, tol=1e-12)
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        assert rel(v[sl], vo) < 1e-9
