@@ -51,7 +51,7 @@ def build_host(verbose=False):
     """C++ adapters (SurgSimGpu.h) linked against libfem_gpu.so, found next to it at run time."""
     tmp = HOST_LIB + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-shared",
-           os.path.join(HERE, "host", "SurgSimGpu.cpp"), "-o", tmp,
+           os.path.join(HERE, "host", "SurgSimGpu.cpp"), os.path.join(HERE, "host", "Fem3DPly.cpp"), "-o", tmp,
            "-L" + os.path.dirname(LIB), "-lfem_gpu", "-Wl,-rpath,$ORIGIN"]
     if verbose:
         print(" ".join(cmd))

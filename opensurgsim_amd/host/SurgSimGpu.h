@@ -199,6 +199,38 @@ public:
     const std::string& getClassName() const override;
 };
 
+/// Fem3D mesh asset (Physics/Fem3D.h, Physics/Fem-inl.h:90-111): vertices, elements with node ids and
+/// material, boundary-condition vertices. load() reads the Fem3D PLY format (Physics/FemPlyFormat.dox,
+/// Physics/FemPlyReaderDelegate.cpp:31-145, Physics/Fem3DPlyReaderDelegate.cpp:52-96), ASCII only.
+class Fem3D {
+public:
+    struct Element {
+        std::vector<size_t> nodeIds;
+        double massDensity = 0.0, poissonRatio = 0.0, youngModulus = 0.0;
+    };
+    /// Throws AssertionFailure for a file that is not an acceptable Fem3D PLY file.
+    void load(const std::string& fileName);
+    size_t getNumVertices() const { return m_vertices.size() / 3; }
+    Math::Vector3d getVertexPosition(size_t i) const {
+        return {{m_vertices.at(3 * i), m_vertices.at(3 * i + 1), m_vertices.at(3 * i + 2)}};
+    }
+    const std::vector<double>& getVertexPositions() const { return m_vertices; }
+    size_t getNumElements() const { return m_elements.size(); }
+    const Element& getElement(size_t i) const { return m_elements.at(i); }
+    const std::vector<Element>& getElements() const { return m_elements; }
+    const std::vector<size_t>& getBoundaryConditions() const { return m_boundaryConditions; }
+    size_t getBoundaryCondition(size_t i) const { return m_boundaryConditions.at(i); }
+
+    void addVertex(const Math::Vector3d& p) { m_vertices.insert(m_vertices.end(), p.begin(), p.end()); }
+    void addElement(const Element& e) { m_elements.push_back(e); }
+    void addBoundaryCondition(size_t vertex) { m_boundaryConditions.push_back(vertex); }
+
+private:
+    std::vector<double> m_vertices;
+    std::vector<Element> m_elements;
+    std::vector<size_t> m_boundaryConditions;
+};
+
 /// Arithmetic of the GPU step (no reference counterpart: the reference is FP64 throughout).
 enum class Precision { FP64 = FEM_PRECISION_FP64, FP32_REFINED = FEM_PRECISION_FP32_REFINED };
 
@@ -211,6 +243,13 @@ public:
     Fem3DRepresentation& operator=(const Fem3DRepresentation&) = delete;
 
     const std::string& getName() const { return m_name; }
+
+    /// Physics/Fem3DRepresentation.cpp:70-126: loads a Fem3D PLY file, builds the initial state (positions and
+    /// boundary conditions) and one element per mesh element of the representation's element type (default:
+    /// Fem3DElementTetrahedron for 4 nodes, Fem3DElementCube for 8) with the mesh's material.
+    void loadFem(const std::string& fileName);
+    void setFem(std::shared_ptr<Fem3D> mesh);
+    std::shared_ptr<Fem3D> getFem() const { return m_fem; }
 
     /// Physics/FemRepresentation.cpp:73-77: one of the three factory names; locked after initialize().
     /// Left empty, initialize() takes the class of the elements.
@@ -284,6 +323,7 @@ private:
     std::string m_name;
     int m_device;
     fem_ctx* m_ctx = nullptr;
+    std::shared_ptr<Fem3D> m_fem;
     std::string m_elementType;
     std::vector<std::shared_ptr<FemElement>> m_elements;
     std::shared_ptr<Math::OdeState> m_initialState, m_currentState, m_previousState, m_finalState;

@@ -3,7 +3,7 @@
 // Physics/UnitTests/FemRepresentationTests.cpp). GPU cases compare the GPU step with the FP64 oracle
 // (oracle/fem_oracle.cpp, C API) on the same mesh.
 //
-// usage: fem3d_representation_test --cpu | --gpu
+// usage: fem3d_representation_test --cpu | --gpu [ply data directory]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -199,8 +199,91 @@ const std::string kTet = "SurgSim::Physics::Fem3DElementTetrahedron";
 const std::string kCoro = "SurgSim::Physics::Fem3DElementCorotationalTetrahedron";
 const std::string kCube = "SurgSim::Physics::Fem3DElementCube";
 
+#ifndef FEM_TEST_DATA_DIR
+#define FEM_TEST_DATA_DIR "tests/golden/ply"
+#endif
+std::string g_data_dir = FEM_TEST_DATA_DIR;  // the reference's PLY fixtures (second argument overrides)
+
+std::string dataPath(const std::string& name) { return g_data_dir + "/" + name; }
+
+bool near(double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(1.0, std::fabs(b)); }
+
 // ---- CPU-only tests (no GPU context is created) ----------------------------------------------------
+void plyTests() {
+    // Physics/UnitTests/Fem3DPlyReaderDelegateTests.cpp, on the reference's own data files
+    run("Fem3DRepresentationReaderTests.TetrahedronMeshDelegateTest", [] {
+        Physics::Fem3D fem;
+        fem.load(dataPath("Tetrahedron.ply"));
+        const auto v0 = fem.getVertexPosition(0), v25 = fem.getVertexPosition(25);
+        EXPECT_TRUE(near(v0[0], -1.0) && near(v0[1], 0.0) && near(v0[2], 0.0));
+        EXPECT_TRUE(near(v25[0], 1.0) && near(v25[1], 1.0) && near(v25[2], 1.0));
+        EXPECT_TRUE(fem.getNumElements() == 44u);
+        const std::vector<size_t> t0 = {16, 0, 7, 5}, t11 = {13, 14, 16, 2};
+        EXPECT_TRUE(fem.getElement(0).nodeIds == t0);
+        EXPECT_TRUE(fem.getElement(11).nodeIds == t11);
+        EXPECT_TRUE(fem.getBoundaryConditions().size() == 4u);
+        EXPECT_TRUE(fem.getBoundaryCondition(0) == 19u && fem.getBoundaryCondition(1) == 21u &&
+                    fem.getBoundaryCondition(2) == 23u && fem.getBoundaryCondition(3) == 25u);
+        for (size_t e : {2u, 8u}) {
+            EXPECT_TRUE(fem.getElement(e).massDensity == 900.0);
+            EXPECT_TRUE(fem.getElement(e).poissonRatio == 0.45);
+            EXPECT_TRUE(fem.getElement(e).youngModulus == 1750000000.0);
+        }
+    });
+    run("Fem3DRepresentationReaderTests.CubeMeshDelegateTest", [] {
+        Physics::Fem3D fem;
+        fem.load(dataPath("Fem3DCube.ply"));
+        const auto v0 = fem.getVertexPosition(0), v5 = fem.getVertexPosition(5);
+        EXPECT_TRUE(near(v0[0], 1.0) && near(v0[1], 1.0) && near(v0[2], 1.0));
+        EXPECT_TRUE(near(v5[0], 2.0) && near(v5[1], 2.0) && near(v5[2], 2.0));
+        EXPECT_TRUE(fem.getNumElements() == 3u);
+        const std::vector<size_t> c0 = {0, 1, 2, 3, 4, 5, 6, 7}, c2 = {3, 4, 5, 0, 2, 6, 8, 7};
+        EXPECT_TRUE(fem.getElement(0).nodeIds == c0);
+        EXPECT_TRUE(fem.getElement(2).nodeIds == c2);
+        EXPECT_TRUE(fem.getBoundaryConditions().size() == 2u);
+        EXPECT_TRUE(fem.getBoundaryCondition(0) == 9u && fem.getBoundaryCondition(1) == 5u);
+        for (size_t e : {1u, 2u}) {
+            EXPECT_TRUE(fem.getElement(e).massDensity == 0.2);
+            EXPECT_TRUE(fem.getElement(e).poissonRatio == 0.3);
+            EXPECT_TRUE(fem.getElement(e).youngModulus == 0.4);
+        }
+    });
+    run("Fem3DRepresentationReaderTests.WrongPlyWithRotationDof", [] {
+        Physics::Fem3D fem;
+        fem.load(dataPath("Wrong3DFileWithRotationData.ply"));  // must not throw
+        EXPECT_TRUE(fem.getNumElements() == 1u);
+    });
+    run("Fem3DRepresentationReaderTests.PerElementMaterial", [] {
+        Physics::Fem3D fem;
+        fem.load(dataPath("Fem3DCubeMaterial.ply"));
+        double value = 1.0;
+        for (size_t i = 0; i < fem.getNumElements(); ++i) {
+            EXPECT_TRUE(fem.getElement(i).massDensity == value++);
+            EXPECT_TRUE(fem.getElement(i).poissonRatio == value++);
+            EXPECT_TRUE(fem.getElement(i).youngModulus == value++);
+        }
+    });
+    run("Fem3DRepresentationReaderTests.NoMaterials", [] {
+        Physics::Fem3D fem;
+        fem.load(dataPath("Fem3DCubeNoMaterial.ply"));
+        for (const auto& e : fem.getElements())
+            EXPECT_TRUE(e.massDensity == 0.0 && e.poissonRatio == 0.0 && e.youngModulus == 0.0);
+    });
+    run("Fem3DRepresentationReaderTests.MissingFileAndLoadFem", [] {
+        Physics::Fem3D fem;
+        EXPECT_THROW_ASSERT(fem.load(dataPath("does_not_exist.ply")));
+        Physics::Fem3DRepresentation rep("wound");
+        rep.loadFem(dataPath("wound_deformable.ply"));
+        EXPECT_TRUE(rep.getNumFemElements() == 1476u);
+        EXPECT_TRUE(rep.getNumDof() == 3u * 391u);
+        EXPECT_TRUE(rep.getInitialState()->getNumBoundaryConditions() == 3u * 79u);
+        EXPECT_TRUE(rep.getFemElementType() == "SurgSim::Physics::Fem3DElementTetrahedron");
+        EXPECT_TRUE(rep.getFemElement(0)->getYoungModulus() == 7.5e4 && rep.getFemElement(0)->getMassDensity() == 1000.0);
+    });
+}
+
 void cpuTests() {
+    plyTests();
     run("OdeStateTest.SetNumDof", [] {
         Math::OdeState s;
         s.setNumDof(3, 4);
@@ -421,6 +504,58 @@ void gpuTests() {
         fem->update(1e-3);  // inactive: nothing happens
         EXPECT_TRUE(fem->getStatistics().steps == 1);
     });
+    run("Fem3DRepresentationGpuTest.WoundMeshFromPlyMatchesOracle", [] {
+        // the reference's wound_deformable.ply (391 nodes, 1476 tets, 79 fixed nodes, E 7.5e4, nu 0.45,
+        // rho 1000) as corotational tets, 30 implicit steps with a downward push on one free node
+        Physics::Fem3DRepresentation fem("wound");
+        fem.setFemElementType(kCoro);
+        fem.loadFem(dataPath("wound_deformable.ply"));
+        fem.setIntegrationScheme(Math::INTEGRATIONSCHEME_EULER_IMPLICIT);
+        fem.setLinearSolver(Math::LINEARSOLVER_CONJUGATEGRADIENT);
+        fem.initialize();
+        std::dynamic_pointer_cast<Math::LinearSparseSolveAndInverseCG>(fem.getOdeSolver()->getLinearSolver())
+            ->setTolerance(1e-12);
+        const auto mesh = fem.getFem();
+        std::vector<int> conn;
+        for (const auto& e : mesh->getElements())
+            for (size_t id : e.nodeIds) conn.push_back((int)id);
+        const int ne = (int)mesh->getNumElements(), nn = (int)mesh->getNumVertices();
+        std::vector<double> E(ne, 7.5e4), nu(ne, 0.45), rho(ne, 1000.0);
+        void* o = orc_sim_create(1, nn, mesh->getVertexPositions().data(), ne, conn.data(), E.data(), nu.data(),
+                                 rho.data());
+        for (size_t b : mesh->getBoundaryConditions())
+            for (int c = 0; c < 3; ++c) orc_sim_add_bc_dof(o, 3 * (int)b + c);
+        orc_sim_set_solver(o, 0, 1, 1e-12, -1, 1, 0);
+        size_t push = 0;
+        while (fem.getInitialState()->isBoundaryCondition(3 * push)) ++push;
+        std::vector<double> f(3 * nn, 0.0);
+        f[3 * push + 1] = -0.5;
+        const double dt = 1e-3;
+        for (int s = 0; s < 30; ++s) {
+            fem.addExternalGeneralizedForce(push, {{0.0, -0.5, 0.0}});
+            fem.update(dt);
+            fem.afterUpdate(dt);
+        }
+        orc_sim_set_external_force(o, f.data());
+        EXPECT_TRUE(orc_sim_step(o, dt, 30) == 0);
+        std::vector<double> xo(3 * nn), vo(3 * nn), u(3 * nn), uo(3 * nn);
+        orc_sim_get_state(o, xo.data(), vo.data());
+        const auto& x = fem.getCurrentState()->getPositions();
+        for (int i = 0; i < 3 * nn; ++i) {
+            u[i] = x[i] - mesh->getVertexPositions()[i];
+            uo[i] = xo[i] - mesh->getVertexPositions()[i];
+        }
+        const double eu = relErr(u, uo), ev = relErr(fem.getCurrentState()->getVelocities(), vo);
+        std::printf("  wound mesh: rel err u %.3e v %.3e\n", eu, ev);
+        EXPECT_TRUE(eu < 1e-9 && ev < 1e-9);
+        std::vector<int32_t> rowptr, colidx;
+        fem.getPattern(&rowptr, &colidx);
+        std::vector<int> outer(rowptr.size()), inner((size_t)orc_sim_nnz(o));
+        orc_sim_pattern(o, outer.data(), inner.data());
+        EXPECT_TRUE(std::equal(outer.begin(), outer.end(), rowptr.begin()) && inner.size() == colidx.size() &&
+                    std::equal(inner.begin(), inner.end(), colidx.begin()));
+        orc_sim_destroy(o);
+    });
     run("Fem3DRepresentationGpuTest.DegenerateElementRejected", [] {
         Block b = makeBlock(1, 1, 1);
         b.x0[3 * b.id(1, 1, 1)] = b.x0[3 * b.id(0, 1, 1)];  // collapse the cube: tets lose volume
@@ -441,9 +576,10 @@ int main(int argc, char** argv) {
     const bool cpu = argc > 1 && std::strcmp(argv[1], "--cpu") == 0;
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
     if (!cpu && !gpu) {
-        std::printf("usage: %s --cpu | --gpu\n", argv[0]);
+        std::printf("usage: %s --cpu | --gpu [ply data directory]\n", argv[0]);
         return 2;
     }
+    if (argc > 2) g_data_dir = argv[2];
     if (cpu) cpuTests();
     if (gpu) gpuTests();
     std::printf("%s: %d failure(s)\n", cpu ? "cpu" : "gpu", g_failures);

@@ -19,7 +19,8 @@ def _binary():
 
 def _run(mode):
     print("C++ adapter tests", mode, flush=True)
-    p = subprocess.run([_binary(), mode], capture_output=True, text=True, timeout=300)
+    p = subprocess.run([_binary(), mode, os.path.join(ROOT, "tests", "golden", "ply")], capture_output=True,
+                       text=True, timeout=300)
     sys.stdout.write(p.stdout)
     sys.stdout.write(p.stderr)
     assert p.returncode == 0, "C++ adapter tests failed (%s)" % mode
