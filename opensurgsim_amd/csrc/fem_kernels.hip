@@ -29,9 +29,22 @@ thread_local LaunchTiming g_launch_timing;
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ constexpr int pair_slot(int a, int b) { return a * (7 - a) / 2 + (b - a - 1); }
 
+// XCD-aware logical workgroup index: workgroups are dispatched round-robin over the 8 XCDs, so with a grid
+// that is a multiple of 8, L = (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous slab of the index
+// space (neighbouring elements, nodes or blocks share their gathers in that XCD's L2).
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x;
+    return (nb % kXcds == 0) ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+}
+inline dim3 xcd_grid(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    b = (b + kXcds - 1) / kXcds * kXcds;
+    return dim3((unsigned)b);
+}
+
 template <bool COROT, bool GENERAL>
 __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= A.num_elements) return;
     int4 id4 = A.conn[e];
     const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
@@ -135,7 +148,7 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
 
 void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     if (a.num_elements == 0) return;
-    dim3 grid((a.num_elements + 255) / 256), block(256);
+    const dim3 grid = xcd_grid(a.num_elements, 256), block(256);
     if (kind == 1) {
         if (a.general) launch_k((tet_element_kernel<true, true>), grid, block, 0, s, a);
         else launch_k((tet_element_kernel<true, false>), grid, block, 0, s, a);
@@ -160,7 +173,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(int num_lower, const int*
                                                        const double* __restrict__ rec,
                                                        const uint8_t* __restrict__ bcmask, int64_t plane,
                                                        double* __restrict__ S64, float* __restrict__ S32) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= num_lower) return;
     const int blk = lower_blk[t];
     const int I = blk_row[blk];
@@ -217,7 +230,7 @@ void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk,
                      const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
                      const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s) {
     if (num_lower == 0) return;
-    launch_k(assemble_kernel, dim3((num_lower + 255) / 256), dim3(256), 0, s, num_lower, lower_blk,
+    launch_k(assemble_kernel, xcd_grid(num_lower, 256), dim3(256), 0, s, num_lower, lower_blk,
                        mirror_blk, blk_row, blk_col, ctr_ptr, ctr_list, rec, bcmask, plane, S64, S32);
 }
 
@@ -228,7 +241,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __re
                                                   const double* __restrict__ rhs_rec, const double* __restrict__ fg,
                                                   const double* __restrict__ fext, const uint8_t* __restrict__ bcmask,
                                                   double* __restrict__ b) {
-    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    int I = xcd_block() * blockDim.x + threadIdx.x;
     if (I >= num_nodes) return;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int c = node_ptr[I]; c < node_ptr[I + 1]; ++c) {
@@ -250,7 +263,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __re
 void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const double* rhs_rec, const double* fg,
                 const double* fext, const uint8_t* bcmask, double* b, hipStream_t s) {
     if (num_nodes == 0) return;
-    launch_k(rhs_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, node_ptr, node_list,
+    launch_k(rhs_kernel, xcd_grid(num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, node_list,
                        rhs_rec, fg, fext, bcmask, b);
 }
 
