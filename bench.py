@@ -72,12 +72,29 @@ def run_timed(cfg, precision, warmup, steps, timer_every):
                 op_ms=op_ms)
 
 
+FUSED_SUMS, CG_BLOCKS = 7, 512
+
+
+def cg_kernel_name(w):
+    """The dominant kernel of the CG iteration: the library runs one fused kernel per iteration unless
+    FEM_CG_KERNELS=split selects the operator + vector kernel pair (its operator kernel is then timed)."""
+    t = "float" if w == 4 else "double"
+    split = os.environ.get("FEM_CG_KERNELS", "") == "split"
+    return ("cg_apply_kernel<%s>" if split else "cg_fused_kernel<%s>") % t
+
+
 def spmv_bytes(cfg_sizes, w):
-    """Algorithmic bytes of one cg_apply_kernel launch: BSR values and indices once, z and p_old read
-    (p = z + beta p_old is formed on the fly), p_new and q written."""
+    """Algorithmic bytes of one launch of the dominant CG kernel.
+    cg_fused_kernel: BSR values and indices once; for its own rows it reads r, q, z, w = D q, p, D^-1 and
+    x (7 vectors) and writes r, z, p, q, w and x (6 vectors); the gathers of z, w, p at the columns read
+    the same vectors; 7 block sums read and written.
+    cg_apply_kernel: BSR values and indices once, z and p_old read (p = z + beta p_old is formed on the
+    fly), p_new and q written."""
     N, D, Bk = cfg_sizes
     Z = 9 * Bk
-    return Z * w + Bk * 4 + (N + 1) * 4 + 4 * D * w
+    if cg_kernel_name(w).startswith("cg_apply"):
+        return Z * w + Bk * 4 + (N + 1) * 4 + 4 * D * w
+    return Z * w + Bk * 4 + (N + 1) * 4 + 13 * D * w + 2 * FUSED_SUMS * CG_BLOCKS * 8
 
 
 def copy_probe(mib, reps=3):
@@ -456,7 +473,7 @@ def main():
         try:
             with open(pmc) as fh:
                 t = json.load(fh)
-            traffic = t["bytes_per_dispatch"].get("cg_apply_kernel<%s>" % ("float" if use32 else "double"))
+            traffic = t["bytes_per_dispatch"].get(cg_kernel_name(w))
         except (OSError, ValueError, KeyError):
             traffic = None
 
@@ -489,7 +506,7 @@ def main():
                                "Jacobi-CG 1e-8" % n,
                    "elements": int(cfg.conn.shape[0]), "nodes": int(N), "dof": int(D), "nnz": int(9 * Bk),
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": "cg_apply_kernel<%s>" % ("float" if use32 else "double"), "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": cg_kernel_name(w), "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_us": sp_avg_s * 1e6,
                      "avg_launch_us_in_step_events": 1e3 * sp_ms / max(sp_n, 1), "in_step_launches": sp_n,

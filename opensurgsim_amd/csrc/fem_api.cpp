@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -175,6 +176,8 @@ struct fem_ctx {
     DevBuf<float> S32;          // FP32 copy (FP32 mode)
     DevBuf<uint8_t> bcmask;
     DevBuf<unsigned char> cg_x, cg_r, cg_z, cg_p0, cg_p1, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
+    DevBuf<unsigned char> cg_r1, cg_z1, cg_q1, cg_w0, cg_w1;                    // fused CG: second buffers
+    DevBuf<double> part7;      // [2][kFusedSums * kCgBlocks] block sums of the fused CG kernel, by parity
     DevBuf<double> partial;    // [4 * kCgBlocks] block sums of the grid reductions
     DevBuf<CgScalars> scal;    // [0] inner/FP64 solve, [1] outer refinement
     DevBuf<CgScalars> time_sc; // operator replay timing (fem_time_operator)
@@ -226,6 +229,7 @@ struct fem_ctx {
     double matrix_dt = 0.0;
     int last_precision = -1;
     int predicted_iters = 16;     // FP64 solve / current inner solve
+    int cg_kernels = 0;           // 0, 1: fused single kernel, 2: apply + vector kernels (FEM_CG_KERNELS)
     int predicted_sweep[16];       // FP32 inner solves, per refinement sweep
     int predicted_hex[17];         // hexahedral path: FP64 solve, then per refinement sweep
     fem_stats stats;
@@ -314,6 +318,9 @@ void alloc_precision_buffers(fem_ctx* c) {
     }
     for (auto* b : {&c->cg_x, &c->cg_r, &c->cg_z, &c->cg_p0, &c->cg_p1, &c->cg_q, &c->cg_dinv, &c->cg_b})
         b->alloc(n * tv);
+    if (c->kind != FEM_ELEMENT_HEX)
+        for (auto* b : {&c->cg_r1, &c->cg_z1, &c->cg_q1, &c->cg_w0, &c->cg_w1}) b->alloc(n * tv);
+    if (!c->part7.p) c->part7.alloc(2 * (size_t)kFusedSums * kCgBlocks);
     c->last_precision = prec;
     c->linear_matrix_ready = false;
     reset_batch_graphs(c);
@@ -345,7 +352,33 @@ CgArgs cg_args(fem_ctx* c, bool fp32, double tol) {
     a.partial = c->partial.p;
     a.sc = c->scal.p;
     a.tol = tol;
+    a.r1 = c->cg_r1.p;
+    a.z1 = c->cg_z1.p;
+    a.q1 = c->cg_q1.p;
+    a.w[0] = c->cg_w0.p;
+    a.w[1] = c->cg_w1.p;
+    a.part7 = c->part7.p;
     return a;
+}
+
+// One kernel per CG iteration (cg_fused_kernel) unless FEM_CG_KERNELS=split selects the operator kernel +
+// vector kernel pair (same recurrence, two launches and one more pass over the vectors per iteration).
+bool use_fused_cg(const fem_ctx* c, bool fp32) {
+    (void)fp32;
+    return c->cg_kernels != 2;
+}
+
+// Fused path: the init kernel writes r_0, z_0 where the fused kernel of iteration 0 reads them (odd
+// buffers).
+void launch_solver_init(fem_ctx* c, const CgArgs& a, const double* b, const CgScalars* skip) {
+    if (!use_fused_cg(c, a.fp32)) {
+        launch_cg_init(a, b, skip, c->stream);
+        return;
+    }
+    CgArgs t = a;
+    t.r = a.r1;
+    t.z = a.z1;
+    launch_cg_init(t, b, skip, c->stream);
 }
 
 int64_t max_iters(fem_ctx* c) {
@@ -359,7 +392,12 @@ int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
     int k = 0;
     // iteration k = convergence test of k-1 + p update + operator application, then vector updates;
     // the last launch (k == max_iter) only tests.
+    const bool fused = use_fused_cg(c, a.fp32);
     auto iteration = [&](int kk) {
+        if (fused) {
+            TIMED(c, FEM_PHASE_SPMV, kk, launch_cg_fused(a, kk, max_iter, c->stream));
+            return;
+        }
         TIMED(c, FEM_PHASE_SPMV, kk, launch_cg_apply(a, kk, max_iter, c->stream));
         if (kk < max_iter) TIMED(c, FEM_PHASE_CG_VECTOR, kk, launch_cg_vector(a, kk, c->stream));
     };
@@ -388,7 +426,7 @@ void linear_solve(fem_ctx* c) {
     if (!fp32) {
         CgArgs a = cg_args(c, false, c->opt.cg_tolerance);
         a.x = c->dv.p;
-        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_cg_init(a, c->b.p, nullptr, c->stream));
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->b.p, nullptr));
         int it = run_cg_iterations(c, a, max_iter);
         c->stats.last_cg_iterations = it;
         c->stats.total_cg_iterations += it;
@@ -407,7 +445,7 @@ void linear_solve(fem_ctx* c) {
                                 c->opt.cg_tolerance, c->stream));
         if (sweep == c->opt.max_refinements) break;  // final residual test only
         // inner FP32 solve on the FP64 residual (skipped on the device once the outer test passed)
-        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_cg_init(a, c->r64.p, outer, c->stream));
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->r64.p, outer));
         const int ps = std::min(sweep, 15);
         c->predicted_iters = c->predicted_sweep[ps];
         int it = run_cg_iterations(c, a, max_iter);
@@ -1027,6 +1065,8 @@ int fem_create(int device, fem_ctx** out) {
         HIPCHECK(hipMemset(c->scal.p, 0, 2 * sizeof(CgScalars)));
         fem_default_options(&c->opt);
         for (int i = 0; i < 16; ++i) c->predicted_sweep[i] = 8;
+        if (const char* e = std::getenv("FEM_CG_KERNELS"))
+            c->cg_kernels = std::strcmp(e, "fused") == 0 ? 1 : std::strcmp(e, "split") == 0 ? 2 : 0;
         for (int i = 0; i < 17; ++i) c->predicted_hex[i] = 8;
         std::memset(&c->stats, 0, sizeof(c->stats));
         c->stats.active = 1;
@@ -1345,12 +1385,18 @@ int fem_time_operator(fem_ctx* c, int reps, double* avg_ms) {
         CgArgs a = cg_args(c, fp32, 0.0);
         a.sc = c->time_sc.p;
         const int big = std::numeric_limits<int>::max() - 1;
-        launch_cg_apply(a, 1, big, c->stream);  // warm-up
+        // the dominant kernel of an iteration: the fused kernel, or the operator kernel of the split pair
+        const bool fused = use_fused_cg(c, fp32);
+        auto launch = [&] {
+            if (fused) launch_cg_fused(a, 1, big, c->stream);
+            else launch_cg_apply(a, 1, big, c->stream);
+        };
+        launch();  // warm-up
         hipEvent_t e0, e1;
         HIPCHECK(hipEventCreate(&e0));
         HIPCHECK(hipEventCreate(&e1));
         HIPCHECK(hipEventRecord(e0, c->stream));
-        for (int r = 0; r < reps; ++r) launch_cg_apply(a, 1, big, c->stream);
+        for (int r = 0; r < reps; ++r) launch();
         HIPCHECK(hipEventRecord(e1, c->stream));
         HIPCHECK(hipEventSynchronize(e1));
         float t = 0.f;

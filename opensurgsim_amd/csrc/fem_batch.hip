@@ -22,8 +22,8 @@ __device__ __forceinline__ void range_sum2(const double* pa, const double* pb, i
             va += pa[i];
             if (pb) vb += pb[i];
         }
-        va = wave_sum(va);
-        vb = wave_sum(vb);
+        va = wave_total(va);
+        vb = wave_total(vb);
         if (threadIdx.x == 0) {
             lds[62] = va;
             lds[63] = vb;
@@ -58,12 +58,9 @@ __device__ __forceinline__ void chunk_spmv(const Operator& op, const TM* __restr
                 y2 += (TX)B[6] * p0 + (TX)B[7] * p1 + (TX)B[8] * p2;
             }
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y0 += __shfl_xor(y0, o, G);
-            y1 += __shfl_xor(y1, o, G);
-            y2 += __shfl_xor(y2, o, G);
-        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
         if (live) epi(I, lane, y0, y1, y2);
     }
 }

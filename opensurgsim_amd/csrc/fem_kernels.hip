@@ -299,12 +299,9 @@ __device__ __forceinline__ void group_spmv(const Operator& op, const TM* __restr
             y1 += (TX)B[3] * p0 + (TX)B[4] * p1 + (TX)B[5] * p2;
             y2 += (TX)B[6] * p0 + (TX)B[7] * p1 + (TX)B[8] * p2;
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y0 += __shfl_xor(y0, o, G);
-            y1 += __shfl_xor(y1, o, G);
-            y2 += __shfl_xor(y2, o, G);
-        }
+        y0 = group_sum<G>(y0);  // the whole group is active here (I is uniform over the group)
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
         epi(I, lane, y0, y1, y2);
     }
 }
@@ -467,6 +464,168 @@ __global__ __launch_bounds__(kCgThreads) void cg_vector_kernel(int n, int k, TV*
         partial[blockIdx.x] = rr;
         partial[kCgBlocks + blockIdx.x] = rz;
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused CG iteration: one kernel per iteration of the same recurrence (Eigen's ConjugateGradient with
+// the Jacobi preconditioner). Kernel k:
+//   1. reduces the 7 block sums of kernel k - 1 over p.q and the products of r, z, q, w = D q
+//      (D = dinv): alpha = (r.z) / (p.q) with r.z computed directly, as Eigen's absNew;
+//      |r_k|^2 = r.r - 2 alpha r.q + alpha^2 q.q and r_k.z_k = r.z - 2 alpha q.z + alpha^2 q.w
+//      (exact expansions of r_k = r - alpha q, z_k = z - alpha w);
+//   2. applies the stopping test of iteration k - 1 (iters = k - 1 when converged, k at the cap), and
+//      then only the x update of that iteration;
+//   3. otherwise beta = (r_k.z_k) / (r.z); for its own rows r_k, z_k, x += alpha p, p_k = z_k + beta p;
+//      p_k at every column the SpMV gathers is formed on the fly from (z, w, p); q_k = S p_k, w_k = D q_k;
+//      block sums of p_k.q_k, r_k.r_k, r_k.q_k, q_k.q_k, r_k.z_k, q_k.z_k, q_k.w_k.
+// r, z, w, p, q and the block sums are double-buffered: kernel k reads the buffers of parity (k + 1) & 1
+// and writes those of parity k & 1 (a workgroup that finishes early must not overwrite sums that others
+// have yet to read).
+// The init kernel leaves r_0, z_0 in the odd buffers; kernel 0 copies them and forms p_0 = z_0.
+// ------------------------------------------------------------------------------------------
+static_assert(kCgThreads / 64 <= 16 && kFusedSums <= 64, "block_sums7 keeps 16 wave slots per sum");
+
+// Block sums of the kFusedSums values: DPP wave totals into LDS, then lane j of wave 0 adds row j in wave
+// order and writes it to out[j * kCgBlocks + blockIdx.x]. Every thread of the block calls it.
+__device__ __forceinline__ void block_sums7(const double* v, double* lds, double* __restrict__ out) {
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kFusedSums; ++j) {
+        const double t = wave_total(v[j]);
+        if ((threadIdx.x & 63) == 0) lds[16 * j + w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < kFusedSums) {
+        double s = 0.0;
+        for (int q = 0; q < nw; ++q) s += lds[16 * threadIdx.x + q];
+        out[threadIdx.x * kCgBlocks + blockIdx.x] = s;
+    }
+}
+
+// The 7 sums of kCgBlocks partials: wave j reduces sum j in a fixed order; broadcast through LDS
+// (lds[112 ..], clear of the rows block_sums7 writes).
+__device__ __forceinline__ void reduce7(const double* __restrict__ part, double* s, double* lds) {
+    const int w = threadIdx.x >> 6;
+    if (w < kFusedSums) {
+        double v = 0.0;
+        for (int i = threadIdx.x & 63; i < kCgBlocks; i += 64) v += part[w * kCgBlocks + i];
+        v = wave_total(v);
+        if ((threadIdx.x & 63) == 0) lds[112 + w] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFusedSums; ++j) s[j] = lds[112 + j];
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8))) void cg_fused_kernel(Operator op, const TV* __restrict__ S, int k,
+                                                              int max_iter, TV* __restrict__ x, TV* r0, TV* r1,
+                                                              TV* z0, TV* z1, TV* w0, TV* w1, TV* p0, TV* p1, TV* q0,
+                                                              TV* q1, const TV* __restrict__ dinv,
+                                                              double* __restrict__ part, CgScalars* __restrict__ sc) {
+    __shared__ double lds[128];
+    if (sc->done) return;
+    const bool odd = k & 1;
+    const TV* ro = odd ? r0 : r1;
+    const TV* zo = odd ? z0 : z1;
+    const TV* wo = odd ? w0 : w1;
+    const TV* po = odd ? p0 : p1;
+    const TV* qo = odd ? q0 : q1;
+    TV* rn = odd ? r1 : r0;
+    TV* zn = odd ? z1 : z0;
+    TV* wn = odd ? w1 : w0;
+    TV* pn = odd ? p1 : p0;
+    TV* qn = odd ? q1 : q0;
+    double alpha = 0.0, beta = 0.0;
+    const double* part_old = part + (odd ? 0 : kFusedSums * kCgBlocks);
+    double* part_new = part + (odd ? kFusedSums * kCgBlocks : 0);
+    if (k > 0) {
+        double sm[kFusedSums];
+        reduce7(part_old, sm, lds);
+        alpha = sm[4] / sm[0];
+        const double rr = sm[1] - 2.0 * alpha * sm[2] + alpha * alpha * sm[3];
+        const double rz = sm[4] - 2.0 * alpha * sm[5] + alpha * alpha * sm[6];
+        const bool conv = rr < sc->threshold;
+        if (conv || k >= max_iter) {
+            const TV a = (TV)alpha;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * op.num_nodes; i += gridDim.x * blockDim.x)
+                x[i] += a * po[i];
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                sc->rr = rr;
+                sc->err = sqrt(fmax(rr, 0.0) / sc->bb);
+                sc->iters = conv ? k - 1 : k;
+                sc->converged = conv ? 1 : 0;
+                sc->done = 1;
+            }
+            return;
+        }
+        beta = rz / sm[4];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rz[k & 1] = rz;
+            sc->rr = rr;
+        }
+    }
+    const TV a = (TV)alpha, bt = (TV)beta;
+    const bool k0 = k == 0;
+    double acc[kFusedSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    group_spmv<TV, TV, kRowGroup>(
+        op, S,
+        [&](int J, TV& a0, TV& a1, TV& a2) {
+            if (k0) {
+                a0 = zo[3 * J];
+                a1 = zo[3 * J + 1];
+                a2 = zo[3 * J + 2];
+            } else {
+                a0 = (zo[3 * J] - a * wo[3 * J]) + bt * po[3 * J];
+                a1 = (zo[3 * J + 1] - a * wo[3 * J + 1]) + bt * po[3 * J + 1];
+                a2 = (zo[3 * J + 2] - a * wo[3 * J + 2]) + bt * po[3 * J + 2];
+            }
+        },
+        [&](int I, int lane, TV y0, TV y1, TV y2) {
+            if (lane < 3) {
+                const int i = 3 * I + lane;
+                const TV y = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+                TV rk, zk, pk;
+                if (k0) {
+                    rk = ro[i];
+                    zk = zo[i];
+                    pk = zk;
+                } else {
+                    rk = ro[i] - a * qo[i];
+                    zk = zo[i] - a * wo[i];
+                    pk = zk + bt * po[i];
+                    x[i] += a * po[i];
+                }
+                const TV wk = dinv[i] * y;
+                rn[i] = rk;
+                zn[i] = zk;
+                pn[i] = pk;
+                qn[i] = y;
+                wn[i] = wk;
+                const double dr = (double)rk, dy = (double)y, dz = (double)zk;
+                acc[0] += (double)pk * dy;
+                acc[1] += dr * dr;
+                acc[2] += dr * dy;
+                acc[3] += dy * dy;
+                acc[4] += dr * dz;
+                acc[5] += dy * dz;
+                acc[6] += dy * (double)wk;
+            }
+        });
+    block_sums7(acc, lds, part_new);
+}
+
+void launch_cg_fused(const CgArgs& a, int k, int max_iter, hipStream_t s) {
+    dim3 grid(kCgBlocks), block(kCgThreads);
+    if (a.fp32)
+        launch_k((cg_fused_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, k, max_iter, (float*)a.x,
+                 (float*)a.r, (float*)a.r1, (float*)a.z, (float*)a.z1, (float*)a.w[0], (float*)a.w[1],
+                 (float*)a.p[0], (float*)a.p[1], (float*)a.q, (float*)a.q1, (const float*)a.dinv, a.part7, a.sc);
+    else
+        launch_k((cg_fused_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
+                 (double*)a.x, (double*)a.r, (double*)a.r1, (double*)a.z, (double*)a.z1, (double*)a.w[0],
+                 (double*)a.w[1], (double*)a.p[0], (double*)a.p[1], (double*)a.q, (double*)a.q1,
+                 (const double*)a.dinv, a.part7, a.sc);
 }
 
 void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {

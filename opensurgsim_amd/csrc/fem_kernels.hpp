@@ -106,7 +106,18 @@ struct CgArgs {
     double* partial;    // [4 * kCgBlocks]
     CgScalars* sc;
     double tol;
+    // fused iteration (one kernel per iteration): second buffers of r, z, q and both of w = dinv q,
+    // selected by iteration parity; 7 partial sums per block
+    void* r1 = nullptr;
+    void* z1 = nullptr;
+    void* q1 = nullptr;
+    void* w[2] = {nullptr, nullptr};
+    double* part7 = nullptr;  // [2][kFusedSums * kCgBlocks], by iteration parity
 };
+constexpr int kFusedSums = 7;
+
+// One fused CG iteration k (operator, vector update of k - 1, all reductions); see fem_kernels.hip.
+void launch_cg_fused(const CgArgs& a, int k, int max_iter, hipStream_t s);
 // x = 0, r = (TV) b, z = dinv r and the solve's scalars; skipped when skip && skip->done.
 void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);
 // Iteration k: convergence test of k - 1, p = z + beta p, q = S p, p . q.

@@ -10,17 +10,68 @@ namespace femgpu {
 // ------------------------------------------------------------------------------------------
 // Reductions (deterministic: fixed lane/wave order).
 // ------------------------------------------------------------------------------------------
+// Butterfly over the wavefront by shuffles (ds_bpermute); correct with any set of active lanes that is
+// closed under the butterfly, result in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-// Sums of two values over the block with a single barrier; results valid in thread 0.
-// lds must hold 2 * blockDim/64 doubles.
+// DPP lane exchange (v_mov_b32 with a data-parallel-primitive modifier): every lane reads a fixed other
+// lane of its wavefront without going through LDS. Lanes of rows masked off by ROW get 0.
+//   0xB1 quad_perm[1,0,3,2] (lane ^ 1)       0x4E quad_perm[2,3,0,1] (lane ^ 2)
+//   0x141 row_half_mirror (i <- 7 - i in 8)  0x140 row_mirror (i <- 15 - i in 16)
+//   0x142 row_bcast:15 (rows 1, 3 <- last lane of the row before)   0x143 row_bcast:31 (rows 2, 3 <- lane 31)
+template <int CTRL, int ROW = 0xF>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW, 0xF, false));
+}
+template <int CTRL, int ROW = 0xF>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over aligned groups of G lanes, in every lane of the group (G = 4, 8, 16 by DPP, others by
+// shuffles). Every lane of the group must be active.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    if constexpr (G == 4 || G == 8 || G == 16) {
+        v += dpp_mov<0xB1>(v);
+        v += dpp_mov<0x4E>(v);
+        if constexpr (G >= 8) v += dpp_mov<0x141>(v);
+        if constexpr (G == 16) v += dpp_mov<0x140>(v);
+    } else {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    }
+    return v;
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wavefront by DPP (rows of 16, then row broadcasts; lane 63 ends with the
+// total), returned uniformly. All 64 lanes must be active.
+__device__ __forceinline__ double wave_total(double v) {
+    v = group_sum<16>(v);
+    v += dpp_mov<0x142, 0xA>(v);
+    v += dpp_mov<0x143, 0xC>(v);
+    return read_lane(v, 63);
+}
+
+// Sums of two values over the block with a single barrier; results valid in thread 0. Every thread of the
+// block calls it. lds must hold 2 * blockDim/64 doubles.
 __device__ __forceinline__ void block_sum2(double& v0, double& v1, double* lds) {
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
+    v0 = wave_total(v0);
+    v1 = wave_total(v1);
     const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         lds[w] = v0;
@@ -28,14 +79,15 @@ __device__ __forceinline__ void block_sum2(double& v0, double& v1, double* lds) 
     }
     __syncthreads();
     if (threadIdx.x < 64) {
-        v0 = wave_sum(threadIdx.x < (unsigned)nw ? lds[threadIdx.x] : 0.0);
-        v1 = wave_sum(threadIdx.x < (unsigned)nw ? lds[nw + threadIdx.x] : 0.0);
+        v0 = wave_total(threadIdx.x < (unsigned)nw ? lds[threadIdx.x] : 0.0);
+        v1 = wave_total(threadIdx.x < (unsigned)nw ? lds[nw + threadIdx.x] : 0.0);
     }
 }
 
-// Sum over the block; result valid in thread 0. lds must hold blockDim/64 doubles.
+// Sum over the block; result valid in thread 0. Every thread of the block calls it. lds must hold
+// blockDim/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double* lds) {
-    v = wave_sum(v);
+    v = wave_total(v);
     int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) lds[w] = v;
     __syncthreads();
@@ -43,7 +95,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     double t = 0.0;
     if (threadIdx.x < 64) {
         t = (threadIdx.x < (unsigned)nw) ? lds[threadIdx.x] : 0.0;
-        t = wave_sum(t);
+        t = wave_total(t);
     }
     __syncthreads();
     return t;
@@ -58,8 +110,8 @@ __device__ __forceinline__ void reduce_partials2(const double* pa, const double*
             va += pa[i];
             if (pb) vb += pb[i];
         }
-        va = wave_sum(va);
-        vb = wave_sum(vb);
+        va = wave_total(va);
+        vb = wave_total(vb);
         if (threadIdx.x == 0) {
             lds[62] = va;
             lds[63] = vb;

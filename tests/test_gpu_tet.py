@@ -125,3 +125,22 @@ def test_trajectory_fp32_refined():
     e = trajectory_errors(cfg, 20, precision=femgpu.FP32_REFINED)
     say("fp32 refined errors", e)
     assert e["u"] < 1e-6 and e["v"] < 1e-6
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+def test_trajectory_medium_mesh(precision):
+    """16^3 cubes (20,480 tets): every CG workgroup has rows, so cross-workgroup ordering is exercised."""
+    cfg = mg.config_corotational((16, 16, 16))
+    steps = 8
+    say("medium mesh, precision", precision)
+    o = make_oracle(cfg, tol=1e-10)
+    g = make_gpu(cfg, precision=precision, tol=1e-10)
+    o.step(cfg.dt, steps)
+    assert g.step(steps) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - x0, xr - x0), v=rel(v, vr))
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
