@@ -97,6 +97,11 @@ def spmv_bytes(cfg_sizes, w):
     return Z * w + Bk * 4 + (N + 1) * 4 + 13 * D * w + 2 * FUSED_SUMS * CG_BLOCKS * 8
 
 
+def phase_summary(run):
+    """Average microseconds per launch and launch count of every phase of the event-timed steps."""
+    return {k: [round(1e3 * v[0] / v[1], 2), v[1]] for k, v in run["phases"].items() if v[1]}
+
+
 def copy_probe(mib, reps=3):
     """A plain device copy of known size (probe_copy_kernel), for scaling the FETCH_SIZE / WRITE_SIZE
     counters and for the achievable copy bandwidth."""
@@ -513,10 +518,11 @@ def main():
                      "measured_copy_GBps": copy_gbs},
         "cpu_baseline": cpu,
         "fp64": {"value": D * args.steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / args.steps,
-                 "cg_iters_per_step": r64["cg_iters"] / args.steps},
+                 "cg_iters_per_step": r64["cg_iters"] / args.steps, "phases_us_per_launch": phase_summary(r64)},
         "fp32_refined": {"value": D * args.steps / r32["seconds"], "ms_per_step": 1e3 * r32["seconds"] / args.steps,
                          "cg_iters_per_step": r32["cg_iters"] / args.steps, "rel_err_vs_fp64": err32,
-                         "refinements_last_step": r32["stats"]["last_refinements"]},
+                         "refinements_last_step": r32["stats"]["last_refinements"],
+                         "phases_us_per_launch": phase_summary(r32)},
         "phases": phases,
     }
     print(json.dumps(out), flush=True)

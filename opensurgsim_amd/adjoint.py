@@ -17,6 +17,11 @@ import torch
 
 from . import femgpu
 
+# PyTorch's HIP runtime has to start before the library's (system ROCm) one in a process: start it now,
+# before a context of this module's user touches the GPU.
+if torch.cuda.is_available():
+    torch.cuda.init()
+
 
 def _dev_zeros(n, device):
     return torch.zeros(n, dtype=torch.float64, device=device)

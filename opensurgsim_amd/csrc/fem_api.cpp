@@ -120,6 +120,21 @@ struct PhaseTimer {
         }
         pending.clear();
     }
+    // As resolve, with the skipped SpMV launches named by a predicate on the tag.
+    template <typename Skipped>
+    void resolve_by(Skipped skipped) {
+        for (auto& p : pending) {
+            float t = 0.f;
+            HIPCHECK(hipEventElapsedTime(&t, p.a, p.b));
+            int ph = p.phase;
+            if (ph == FEM_PHASE_SPMV && skipped(p.tag)) ph = FEM_PHASE_SPMV_SKIPPED;
+            ms[ph] += t;
+            launches[ph]++;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
     void reset() {
         for (int i = 0; i < FEM_NUM_PHASES; ++i) {
             ms[i] = 0.0;
@@ -171,7 +186,11 @@ struct fem_ctx {
     DevBuf<double4> mat;
     DevBuf<double> rec;         // element records
     DevBuf<double> rhs_rec;
-    DevBuf<int> lower_blk, mirror_blk, blk_row, colidx_b, rowptr_b, ctr_ptr, ctr_list, node_ptr, node_list, diagblk;
+    DevBuf<int> lower_blk, mirror_blk, blk_row, colidx_b, rowptr_b, ctr_ptr, node_ptr, node_list, diagblk;
+    DevBuf<int> slot_off, rhs_off;  // staging positions of each element's block contributions / rhs rows
+    DevBuf<int4> lb_info;           // per lower block: block, mirror block, boundary bits (row DOFs 0-2, column 3-5)
+    std::vector<int4> h_lb_info;
+    std::vector<int32_t> h_lb_row, h_lb_col;
     DevBuf<double> S64;         // operator values (FP64, 9 per block)
     DevBuf<float> S32;          // FP32 copy (FP32 mode)
     DevBuf<uint8_t> bcmask;
@@ -184,6 +203,9 @@ struct fem_ctx {
     DevBuf<int> counters;      // [0..2] element flags, [3] invalid state, [4] Newton max-norm bits (int pair)
     DevBuf<unsigned long long> nrm;
     int num_lower = 0;
+    int num_lower_off = 0;  // off-diagonal lower blocks (they precede the diagonal ones)
+    int64_t nctr_off = 0;   // contributions to off-diagonal blocks
+    int64_t stage_len = 0;  // doubles of the block contribution staging
     int64_t num_blocks = 0;
     bool have_fext = false;
 
@@ -231,6 +253,10 @@ struct fem_ctx {
     int predicted_iters = 16;     // FP64 solve / current inner solve
     int cg_kernels = 0;           // 0, 1: fused single kernel, 2: apply + vector kernels (FEM_CG_KERNELS)
     int predicted_sweep[16];       // FP32 inner solves, per refinement sweep
+    int predicted_sweeps = 3;      // FP32 refinement sweeps queued without a host round trip
+    DevBuf<int> refine_iters;      // inner iteration count per sweep (-1: no correction applied)
+    int* h_refine_iters = nullptr; // pinned copy
+    int refine_iters_cap = 0;
     int predicted_hex[17];         // hexahedral path: FP64 solve, then per refinement sweep
     fem_stats stats;
     PhaseTimer timer;
@@ -281,6 +307,20 @@ void check_material(double E, double nu, double rho, int64_t e) {
 void ensure_bc(fem_ctx* c) {
     if (c->bc_dirty) {
         c->bcmask.upload(c->h_bcmask.data(), c->h_bcmask.size(), c->stream);
+        if (!c->h_lb_info.empty()) {
+            // boundary bits of every lower block for the assembly
+            for (size_t t = 0; t < c->h_lb_info.size(); ++t) {
+                const int32_t I = c->h_lb_row[t], J = c->h_lb_col[t];
+                int bits = 0;
+                for (int k = 0; k < 3; ++k) {
+                    if (c->h_bcmask[3 * (size_t)I + k]) bits |= 1 << k;
+                    if (c->h_bcmask[3 * (size_t)J + k]) bits |= 8 << k;
+                }
+                c->h_lb_info[t].z = bits;
+            }
+            c->lb_info.upload(c->h_lb_info.data(), c->h_lb_info.size(), c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));  // the host tables may change before the copy ran
+        }
         c->bc_dirty = false;
     }
     if (c->fg_dirty) {
@@ -311,7 +351,7 @@ void alloc_precision_buffers(fem_ctx* c) {
     size_t tv = fp32 ? 4 : 8;
     size_t n = (size_t)c->ndof;
     if (c->kind != FEM_ELEMENT_HEX) {
-        if (!c->rec.p) c->rec.alloc((size_t)c->num_elements * kTetRec);
+        if (!c->rec.p) c->rec.alloc((size_t)c->stage_len);
         if (!c->S64.p) c->S64.alloc((size_t)c->num_blocks * 9);
         if (fp32) c->S32.alloc((size_t)c->num_blocks * 9);
         else c->S32.release();
@@ -381,6 +421,16 @@ void launch_solver_init(fem_ctx* c, const CgArgs& a, const double* b, const CgSc
     launch_cg_init(t, b, skip, c->stream);
 }
 
+// Iteration k of an initialised solve (timer tag `tag`): the fused kernel, or the operator + vector pair.
+void launch_iteration(fem_ctx* c, const CgArgs& a, int k, int max_iter, int tag) {
+    if (use_fused_cg(c, a.fp32)) {
+        TIMED(c, FEM_PHASE_SPMV, tag, launch_cg_fused(a, k, max_iter, c->stream));
+        return;
+    }
+    TIMED(c, FEM_PHASE_SPMV, tag, launch_cg_apply(a, k, max_iter, c->stream));
+    if (k < max_iter) TIMED(c, FEM_PHASE_CG_VECTOR, tag, launch_cg_vector(a, k, c->stream));
+}
+
 int64_t max_iters(fem_ctx* c) {
     int64_t m = c->opt.cg_max_iterations < 0 ? 2 * c->ndof : c->opt.cg_max_iterations;
     return std::min<int64_t>(m, std::numeric_limits<int>::max() - 1);
@@ -392,15 +442,7 @@ int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
     int k = 0;
     // iteration k = convergence test of k-1 + p update + operator application, then vector updates;
     // the last launch (k == max_iter) only tests.
-    const bool fused = use_fused_cg(c, a.fp32);
-    auto iteration = [&](int kk) {
-        if (fused) {
-            TIMED(c, FEM_PHASE_SPMV, kk, launch_cg_fused(a, kk, max_iter, c->stream));
-            return;
-        }
-        TIMED(c, FEM_PHASE_SPMV, kk, launch_cg_apply(a, kk, max_iter, c->stream));
-        if (kk < max_iter) TIMED(c, FEM_PHASE_CG_VECTOR, kk, launch_cg_vector(a, kk, c->stream));
-    };
+    auto iteration = [&](int kk) { launch_iteration(c, a, kk, max_iter, kk); };
     // iteration k tests iteration k - 1: a converged solve with iters = i needs launches 0 .. i + 1
     int first = std::max(1, std::min(c->predicted_iters + 1, max_iter + 1));
     for (; k < first; ++k) iteration(k);
@@ -435,31 +477,78 @@ void linear_solve(fem_ctx* c) {
         c->stats.last_refinements = 0;
         return;
     }
+    // FP32 CG inside FP64 refinement. Sweep s: FP64 residual of dv and the outer test; unless that passed,
+    // an FP32 CG on the residual whose iterations are capped a little above the count sweep s took in the
+    // last step (launches past convergence cost a dispatch each; a capped inner solve still leaves its x
+    // complete and the outer test decides whether another sweep follows), then dv += x. The
+    // predicted number of sweeps goes out back to back with no host round trip; the host reads the outcome
+    // once and continues sweep by sweep only when the prediction fell short.
+    const int max_ref = std::max(0, c->opt.max_refinements);
+    if (c->refine_iters_cap < max_ref + 1) {
+        c->refine_iters.alloc((size_t)max_ref + 1);
+        if (c->h_refine_iters) (void)hipHostFree(c->h_refine_iters);
+        c->h_refine_iters = nullptr;
+        HIPCHECK(hipHostMalloc((void**)&c->h_refine_iters, ((size_t)max_ref + 1) * sizeof(int), hipHostMallocDefault));
+        c->refine_iters_cap = max_ref + 1;
+    }
     HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+    HIPCHECK(hipMemsetAsync(c->refine_iters.p, 0xff, (size_t)(max_ref + 1) * sizeof(int), c->stream));
     CgArgs a = cg_args(c, true, c->opt.inner_tolerance);
     CgScalars* outer = c->scal.p + 1;
-    int total = 0, sweeps = 0;
-    for (int sweep = 0; sweep <= c->opt.max_refinements; ++sweep) {
+    auto check = [&](int sweep) {
         TIMED(c, FEM_PHASE_REFINE, 0,
               launch_residual64(a.op, c->S64.p, c->b.p, c->dv.p, sweep == 0, c->r64.p, c->partial.p, outer, a.sc,
                                 c->opt.cg_tolerance, c->stream));
-        if (sweep == c->opt.max_refinements) break;  // final residual test only
-        // inner FP32 solve on the FP64 residual (skipped on the device once the outer test passed)
-        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->r64.p, outer));
+    };
+    auto inner = [&](int sweep) {
         const int ps = std::min(sweep, 15);
-        c->predicted_iters = c->predicted_sweep[ps];
-        int it = run_cg_iterations(c, a, max_iter);
+        const int pred = c->predicted_sweep[ps];
+        const int cap = (int)std::min<int64_t>(max_iter, std::max(4, pred + std::max(2, pred / 8)));
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->r64.p, outer));
+        for (int k = 0; k <= cap; ++k) launch_iteration(c, a, k, cap, (sweep << 20) | k);
+        TIMED(c, FEM_PHASE_REFINE, 0,
+              launch_refine_accumulate((int)n, (const float*)c->cg_x.p, c->dv.p, outer, a.sc,
+                                       c->refine_iters.p + sweep, c->stream));
+    };
+    auto read_back = [&](int upto) {
         HIPCHECK(hipMemcpyAsync(c->h_sc + 1, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(c->h_refine_iters, c->refine_iters.p, (size_t)upto * sizeof(int),
+                                hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
-        if (c->h_sc[1].done) break;
-        c->predicted_sweep[ps] = c->predicted_iters;
-        total += it + 1;
-        TIMED(c, FEM_PHASE_REFINE, 0, launch_refine_accumulate((int)n, (const float*)c->cg_x.p, c->dv.p, outer, c->stream));
-        sweeps++;
+    };
+    int sweep = 0;
+    const int ahead = std::min(max_ref, std::max(1, c->predicted_sweeps));
+    for (; sweep < ahead; ++sweep) {
+        check(sweep);
+        inner(sweep);
     }
-    CgScalars o;
-    HIPCHECK(hipMemcpyAsync(&o, outer, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    check(sweep);  // the outer test after the last queued correction
+    read_back(std::max(1, sweep));
+    while (!c->h_sc[1].done && sweep < max_ref) {
+        inner(sweep);
+        ++sweep;
+        check(sweep);
+        read_back(sweep);
+    }
+    // sweeps that applied a correction recorded their inner iteration counts
+    int total = 0, sweeps = 0;
+    for (int i = 0; i < sweep; ++i) {
+        const int it = c->h_refine_iters[i];
+        if (it < 0) break;
+        total += it + 1;
+        sweeps++;
+        c->predicted_sweep[std::min(i, 15)] = it + 1;
+    }
+    c->predicted_sweeps = std::max(1, sweeps);
+    if (c->timer.enabled) {
+        std::vector<int> last(sweep + 1, -1);
+        for (int i = 0; i < sweeps; ++i) last[i] = c->h_refine_iters[i];
+        c->timer.resolve_by([&](int tag) {
+            const int sw = tag >> 20, k = tag & ((1 << 20) - 1);
+            return sw >= (int)last.size() || k > last[sw];
+        });
+    }
+    const CgScalars& o = c->h_sc[1];
     c->stats.last_cg_iterations = total;
     c->stats.total_cg_iterations += total;
     c->stats.last_cg_error = o.err;
@@ -482,20 +571,22 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.general = (xs != c->x.p || vs != c->v.p) ? 1 : 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
+    a.slot_off = c->slot_off.p;
+    a.rhs_off = c->rhs_off.p;
     a.flag_counts = c->counters.p;
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
 }
 
 void assemble_tet(fem_ctx* c) {
     if (c->timer.enabled) c->timer.begin(c->stream, FEM_PHASE_ASSEMBLE);
-    launch_assemble(c->num_lower, c->lower_blk.p, c->mirror_blk.p, c->blk_row.p, c->colidx_b.p, c->ctr_ptr.p,
-                    c->ctr_list.p, c->rec.p, c->bcmask.p, c->num_blocks, c->S64.p,
+    launch_assemble(c->num_lower_off, c->num_lower - c->num_lower_off, c->lb_info.p, c->ctr_ptr.p,
+                    (int)c->nctr_off, c->rec.p, c->S64.p,
                     c->opt.precision == FEM_PRECISION_FP32_REFINED ? c->S32.p : nullptr, c->stream);
     if (c->timer.enabled) c->timer.end(c->stream);
 }
 
 void rhs_tet(fem_ctx* c) {
-    TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->rhs_rec.p, c->fg.p,
+    TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->node_ptr.p, c->rhs_rec.p, c->fg.p,
                                           c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream));
 }
 
@@ -800,6 +891,8 @@ int batch_step(fem_ctx* c) {
     a.general = 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
+    a.slot_off = c->slot_off.p;
+    a.rhs_off = c->rhs_off.p;
     a.flag_counts = c->counters.p;
     a.elem_inst = c->elem_inst.p;
     a.inst_dt = c->b_inst_dt.p;
@@ -942,61 +1035,78 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
         auto it = std::lower_bound(b, en, J);
         return (int32_t)(it - ci.begin());
     };
-    std::vector<int32_t> blk_row(c->num_blocks), diag(N);
+    std::vector<int32_t> blk_row(c->num_blocks), diag(N, -1);
     std::vector<int32_t> lower_of_blk(c->num_blocks, -1);
     std::vector<int32_t> lower_blk, mirror;
+    // Lower blocks: the off-diagonal ones in row-major order, then the diagonal ones in row order (waves of
+    // the assembly then see one kind; a diagonal block has about four times the contributions).
     for (int32_t I = 0; I < (int32_t)N; ++I)
         for (int32_t k = rp[I]; k < rp[I + 1]; ++k) {
             blk_row[k] = I;
             int32_t J = ci[k];
             if (J == I) diag[I] = k;
-            if (J <= I) {
+            if (J < I) {
                 lower_of_blk[k] = (int32_t)lower_blk.size();
                 lower_blk.push_back(k);
-                mirror.push_back(J == I ? k : find_blk(J, I));
+                mirror.push_back(find_blk(J, I));
             }
         }
+    c->num_lower_off = (int)lower_blk.size();
+    for (int32_t I = 0; I < (int32_t)N; ++I)
+        if (diag[I] >= 0) {
+            lower_of_blk[diag[I]] = (int32_t)lower_blk.size();
+            lower_blk.push_back(diag[I]);
+            mirror.push_back(diag[I]);
+        }
     c->num_lower = (int)lower_blk.size();
-    // Contributions per lower block, in element order (counting sort).
+    if (100 * E > (int64_t)std::numeric_limits<int32_t>::max())
+        throw InvalidArg("mesh too large for int32 staging offsets");
+    // Contributions per lower block, in element order within a block (counting sort); each element slot
+    // (pairs 0..5, diagonal 6..9) gets its staging offset: records of 10 doubles (9 values + pad, 16-byte
+    // aligned) for the off-diagonal blocks, then records of 6 for the diagonal blocks.
     std::vector<int32_t> cptr(c->num_lower + 1, 0);
-    std::vector<int32_t> codes_t(10 * E);
-    std::vector<int32_t> codes(10 * E);
+    std::vector<int32_t> slot_t(10 * E);
     for (int64_t e = 0; e < E; ++e) {
         const int32_t* id = conn + 4 * e;
-        int k = 0;
         for (int a = 0; a < 4; ++a)
             for (int b = a + 1; b < 4; ++b) {
                 int32_t I = std::max(id[a], id[b]), J = std::min(id[a], id[b]);
-                int ps = a * (7 - a) / 2 + (b - a - 1);
-                int32_t t = lower_of_blk[find_blk(I, J)];
-                codes_t[10 * e + k] = t;
-                codes[10 * e + k] = (int32_t)(e << 4) | ps;
+                const int ps = a * (7 - a) / 2 + (b - a - 1);
+                const int32_t t = lower_of_blk[find_blk(I, J)];
+                slot_t[10 * e + ps] = t;
                 cptr[t + 1]++;
-                k++;
             }
         for (int a = 0; a < 4; ++a) {
-            int32_t t = lower_of_blk[diag[id[a]]];
-            codes_t[10 * e + k] = t;
-            codes[10 * e + k] = (int32_t)(e << 4) | (6 + a);
+            const int32_t t = lower_of_blk[diag[id[a]]];
+            slot_t[10 * e + 6 + a] = t;
             cptr[t + 1]++;
-            k++;
         }
     }
     for (int t = 0; t < c->num_lower; ++t) cptr[t + 1] += cptr[t];
-    std::vector<int32_t> clist(10 * E);
+    c->nctr_off = cptr[c->num_lower_off];
+    c->stage_len = kPairRec * c->nctr_off + kDiagRec * ((int64_t)cptr[c->num_lower] - c->nctr_off);
+    std::vector<int32_t> slot_off(10 * E);
     {
-        std::vector<int32_t> p(cptr.begin(), cptr.end() - 1);
-        for (int64_t i = 0; i < 10 * E; ++i) clist[p[codes_t[i]]++] = codes[i];
+        std::vector<int64_t> p(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < 10 * E; ++i) {
+            const int64_t pos = p[slot_t[i]]++;
+            slot_off[i] = (int32_t)(pos < c->nctr_off ? kPairRec * pos
+                                                      : kPairRec * c->nctr_off + kDiagRec * (pos - c->nctr_off));
+        }
     }
-    // Node rhs contributions, element order.
-    std::vector<int32_t> nptr(N + 1, 0), nlist(4 * E);
+    // Node rhs contributions, element order; the staging offset of each element's node rows.
+    std::vector<int32_t> nptr(N + 1, 0), nlist(4 * E), rhs_off(4 * E);
     for (int64_t e = 0; e < E; ++e)
         for (int a = 0; a < 4; ++a) nptr[conn[4 * e + a] + 1]++;
     for (int64_t i = 0; i < N; ++i) nptr[i + 1] += nptr[i];
     {
         std::vector<int32_t> p(nptr.begin(), nptr.end() - 1);
         for (int64_t e = 0; e < E; ++e)
-            for (int a = 0; a < 4; ++a) nlist[p[conn[4 * e + a]]++] = (int32_t)(e << 2) | a;
+            for (int a = 0; a < 4; ++a) {
+                const int32_t pos = p[conn[4 * e + a]]++;
+                nlist[pos] = (int32_t)(e << 2) | a;
+                rhs_off[4 * e + a] = kRhsRec * pos;
+            }
     }
     hipStream_t s = c->stream;
     c->rowptr_b.alloc(N + 1);
@@ -1007,14 +1117,26 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
     c->blk_row.upload(blk_row.data(), c->num_blocks, s);
     c->diagblk.alloc(N);
     c->diagblk.upload(diag.data(), N, s);
+    c->h_lb_info.resize(c->num_lower);
+    c->h_lb_row.resize(c->num_lower);
+    c->h_lb_col.resize(c->num_lower);
+    for (int t = 0; t < c->num_lower; ++t) {
+        c->h_lb_info[t] = make_int4(lower_blk[t], mirror[t], 0, 0);
+        c->h_lb_row[t] = blk_row[lower_blk[t]];
+        c->h_lb_col[t] = ci[lower_blk[t]];
+    }
+    c->lb_info.alloc(c->num_lower);
+    c->bc_dirty = true;  // ensure_bc fills the boundary bits and uploads lb_info
     c->lower_blk.alloc(c->num_lower);
     c->lower_blk.upload(lower_blk.data(), c->num_lower, s);
     c->mirror_blk.alloc(c->num_lower);
     c->mirror_blk.upload(mirror.data(), c->num_lower, s);
     c->ctr_ptr.alloc(c->num_lower + 1);
     c->ctr_ptr.upload(cptr.data(), c->num_lower + 1, s);
-    c->ctr_list.alloc(10 * E);
-    c->ctr_list.upload(clist.data(), 10 * E, s);
+    c->slot_off.alloc(10 * E);
+    c->slot_off.upload(slot_off.data(), 10 * E, s);
+    c->rhs_off.alloc(4 * E);
+    c->rhs_off.upload(rhs_off.data(), 4 * E, s);
     c->node_ptr.alloc(N + 1);
     c->node_ptr.upload(nptr.data(), N + 1, s);
     c->node_list.alloc(4 * E);
@@ -1064,7 +1186,7 @@ int fem_create(int device, fem_ctx** out) {
         HIPCHECK(hipMemset(c->counters.p, 0, 8 * sizeof(int)));
         HIPCHECK(hipMemset(c->scal.p, 0, 2 * sizeof(CgScalars)));
         fem_default_options(&c->opt);
-        for (int i = 0; i < 16; ++i) c->predicted_sweep[i] = 8;
+        for (int i = 0; i < 16; ++i) c->predicted_sweep[i] = 32;
         if (const char* e = std::getenv("FEM_CG_KERNELS"))
             c->cg_kernels = std::strcmp(e, "fused") == 0 ? 1 : std::strcmp(e, "split") == 0 ? 2 : 0;
         for (int i = 0; i < 17; ++i) c->predicted_hex[i] = 8;
@@ -1086,6 +1208,7 @@ void fem_destroy(fem_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_refine_iters) (void)hipHostFree(c->h_refine_iters);
     if (c->h_bsc) (void)hipHostFree(c->h_bsc);
     if (c->h_live) (void)hipHostFree(c->h_live);
     for (int i = 0; i < 2; ++i) {
@@ -1140,6 +1263,14 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
                                      std::to_string(num_nodes - 1) + "]");
             }
             check_material(young[e], poisson[e], density[e], e);
+            if (kind != FEM_ELEMENT_HEX) {
+                // a tet that repeats a node has no volume (Fem3DElementCorotationalTetrahedron.cpp:48-75 rejects
+                // it; the linear tet's stiffness would not be finite); the block staging assumes distinct nodes
+                const int32_t* id = conn + 4 * e;
+                if (id[0] == id[1] || id[0] == id[2] || id[0] == id[3] || id[1] == id[2] || id[1] == id[3] ||
+                    id[2] == id[3])
+                    throw InvalidArg("element " + std::to_string(e) + " repeats a node");
+            }
         }
         c->h_mass_per_node.assign(num_nodes, 0.0);
         c->h_nu.assign(poisson, poisson + num_elements);
@@ -1168,7 +1299,7 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
                 c4[e] = make_int4(conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]);
             c->conn4.alloc(num_elements);
             c->conn4.upload(c4.data(), num_elements, c->stream);
-            c->rhs_rec.alloc((size_t)num_elements * 12);
+            c->rhs_rec.alloc((size_t)num_elements * 4 * kRhsRec);
         } else {
             hex_setup(c->hex, num_nodes, x0, num_elements, conn, mat.data(), c->h_mass_per_node, c->h_rowptr_b,
                       c->h_colidx_b, c->stream);
@@ -1592,6 +1723,8 @@ int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const dou
             ea.general = 0;
             ea.rec = c->rec.p;
             ea.rhs_rec = c->rhs_rec.p;
+            ea.slot_off = c->slot_off.p;
+            ea.rhs_off = c->rhs_off.p;
             ea.flag_counts = c->counters.p;
             launch_tet_elements(c->kind, ea, s);
         }

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init_kernel(BatchTables t, 
         const int d0 = 3 * t.chunk_row0[c], d1 = 3 * t.chunk_row1[c];
         for (int i = d0 + (int)threadIdx.x; i < d1; i += blockDim.x) {
             const int I = i / 3, cc = i - 3 * I;
-            const TV s = S[9 * (int64_t)op.diagblk[I] + 4 * cc];
+            const int db = op.diagblk[I];  // -1: a node in no element (empty row)
+            const TV s = db >= 0 ? S[9 * (int64_t)db + 4 * cc] : TV(0);
             const TV d = (s != TV(0)) ? TV(1) / s : TV(1);
             const TV bi = (TV)b[i];
             x[i] = TV(0);

@@ -182,12 +182,13 @@ __device__ inline int polar_rotation(const double* F, double* R) {
             }
             double delta = 0.0;
             double Y[9];
+            const double ig = 1.0 / g;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     // X^-T (i, j) = Xi (j, i)
-                    double y = 0.5 * (g * X[3 * i + j] + Xi[3 * j + i] / g);
+                    double y = 0.5 * (g * X[3 * i + j] + Xi[3 * j + i] * ig);
                     double e = y - X[3 * i + j];
                     delta += e * e;
                     Y[3 * i + j] = y;
@@ -316,14 +317,19 @@ struct CoroFactors {
     bool correction;  // false when G is singular (or det R invalid): K = R K0 R^T only
 };
 
-// Rotation, force and stiffness factors of one corotational tet at positions x (12) with rest x0 (12),
-// rest gradients g and signed rest volume V. T is the arithmetic (double; Dual for the adjoint's
-// derivative with respect to x), TK the factor storage.
-// Force f (12) follows doUpdateFMDK :101-113. Returns element flags.
-template <typename TK, typename T = double>
-__device__ inline int coro_factors(const double* x0, const T* x, const double g[4][3], double V, double lam,
-                                   double mu, T* f, CoroFactors<TK>& cf) {
-    int flags = 0;
+// The element-wide part of the corotational factors: F, R, S, E = S - I, R sigma, (tr S I - S)^-1 and the
+// element flags. T is the arithmetic (double; Dual for the adjoint's derivative with respect to x).
+template <typename T>
+struct CoroCommon {
+    T R[9], S[9], Es[9], Rsig[9], G0i[9];
+    bool correction;
+    int flags;
+};
+
+template <typename T>
+__device__ inline void coro_common(const double* x0, const T* x, const double g[4][3], double lam, double mu,
+                                   CoroCommon<T>& c) {
+    c.flags = 0;
     // F = I + sum_b (x_b - x0_b) g_b^T  (sum_b x0_b g_b^T = I; the displacement form keeps FP64 accuracy)
     T F[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
     for (int b = 0; b < 4; ++b) {
@@ -333,104 +339,159 @@ __device__ inline int coro_factors(const double* x0, const T* x, const double g[
 #pragma unroll
             for (int j = 0; j < 3; ++j) F[3 * i + j] += u[i] * g[b][j];
     }
-    T R[9];
-    if (polar_rotation(F, R)) flags |= ELEM_FLAG_POLAR_EIG;
+    T* R = c.R;
+    if (polar_rotation(F, R)) c.flags |= ELEM_FLAG_POLAR_EIG;
     T detR = det3(R);
     // S = sym(R^T F);  E = S - I computed as sym(R^T (F - I) + (R^T - I))
     T RtF[9];
     mtm3(R, F, RtF);
-    T S[9];
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) S[3 * i + j] = 0.5 * (RtF[3 * i + j] + RtF[3 * j + i]);
+        for (int j = 0; j < 3; ++j) c.S[3 * i + j] = 0.5 * (RtF[3 * i + j] + RtF[3 * j + i]);
     T Fm[9];
     for (int i = 0; i < 9; ++i) Fm[i] = F[i] - ((i % 4 == 0) ? 1.0 : 0.0);
     T RtD[9];
     mtm3(R, Fm, RtD);
-    T Es[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
             T rt_ij = R[3 * j + i] - (i == j ? 1.0 : 0.0);
             T rt_ji = R[3 * i + j] - (i == j ? 1.0 : 0.0);
-            Es[3 * i + j] = 0.5 * (RtD[3 * i + j] + RtD[3 * j + i] + rt_ij + rt_ji);
+            c.Es[3 * i + j] = 0.5 * (RtD[3 * i + j] + RtD[3 * j + i] + rt_ij + rt_ji);
         }
-    T trE = Es[0] + Es[4] + Es[8];
-    // f_a = -V R sigma g_a, sigma = lambda tr(E) I + 2 mu E
+    T trE = c.Es[0] + c.Es[4] + c.Es[8];
+    // sigma = lambda tr(E) I + 2 mu E
     T sig[9];
-    for (int i = 0; i < 9; ++i) sig[i] = 2.0 * mu * Es[i] + ((i % 4 == 0) ? lam * trE : T(0));
-    T Rsig[9];
-    mm3(R, sig, Rsig);
-    for (int a = 0; a < 4; ++a) {
-        T t[3];
-        const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
-        mv3(Rsig, ga, t);
-        f[3 * a] = -V * t[0];
-        f[3 * a + 1] = -V * t[1];
-        f[3 * a + 2] = -V * t[2];
-    }
+    for (int i = 0; i < 9; ++i) sig[i] = 2.0 * mu * c.Es[i] + ((i % 4 == 0) ? lam * trE : T(0));
+    mm3(R, sig, c.Rsig);
     // G = (tr S I - S) R^T; the reference's invertibility test uses |det G| > 1e-12 (Eigen default)
-    T trS = S[0] + S[4] + S[8];
+    T trS = c.S[0] + c.S[4] + c.S[8];
     T G0[9];
-    for (int i = 0; i < 9; ++i) G0[i] = ((i % 4 == 0) ? trS : T(0)) - S[i];
-    T G0i[9];
-    T detG = inv3(G0, G0i) * detR;
-    cf.correction = true;
+    for (int i = 0; i < 9; ++i) G0[i] = ((i % 4 == 0) ? trS : T(0)) - c.S[i];
+    T detG = inv3(G0, c.G0i) * detR;
+    c.correction = true;
     if (fabs(detR - 1.0) > 1e-8) {
-        flags |= ELEM_FLAG_DET_R;
-        cf.correction = false;
+        c.flags |= ELEM_FLAG_DET_R;
+        c.correction = false;
     } else if (!(fabs(detG) > 1e-12)) {
-        flags |= ELEM_FLAG_G_SINGULAR;
-        cf.correction = false;
+        c.flags |= ELEM_FLAG_G_SINGULAR;
+        c.correction = false;
     }
-    for (int i = 0; i < 9; ++i) cf.R[i] = (TK)R[i];
+}
+
+// f_a = -V R sigma g_a (doUpdateFMDK :101-113), ga = g_a
+template <typename T>
+__device__ __forceinline__ void coro_force(const CoroCommon<T>& c, const double* gav, double V, T* fa) {
+    T t[3];
+    const T ga[3] = {T(gav[0]), T(gav[1]), T(gav[2])};
+    mv3(c.Rsig, ga, t);
+    fa[0] = -V * t[0];
+    fa[1] = -V * t[1];
+    fa[2] = -V * t[2];
+}
+
+// Correction factor W_a of node a (zero when the correction is dropped).
+// Quirk term: H' = R^T (S - I) = R^T E; p_a = V [lambda tr(H') g_a + mu (H' + H'^T) g_a]; q_a = R p_a;
+// N_a = -[q]x + V mu ([S g]x - S [g]x); W_a = R N_a G^-1.
+template <typename T>
+__device__ __forceinline__ void coro_weight(const CoroCommon<T>& c, const double* ga_, double V, double lam,
+                                            double mu, T* W) {
+    if (!c.correction) {
+        for (int i = 0; i < 9; ++i) W[i] = T(0);
+        return;
+    }
+    const T* R = c.R;
+    T Hp[9];
+    mtm3(R, c.Es, Hp);
+    T trHp = Hp[0] + Hp[4] + Hp[8];
+    T p[3];
+    for (int i = 0; i < 3; ++i) {
+        T s = lam * trHp * ga_[i];
+        for (int k = 0; k < 3; ++k) s += mu * (Hp[3 * i + k] + Hp[3 * k + i]) * ga_[k];
+        p[i] = V * s;
+    }
+    T q[3];
+    mv3(R, p, q);
+    T Sg[3];
+    const T ga[3] = {T(ga_[0]), T(ga_[1]), T(ga_[2])};
+    mv3(c.S, ga, Sg);
+    T N[9];
+    // [w]x = [[0,-w2,w1],[w2,0,-w0],[-w1,w0,0]]
+    double vm = V * mu;
+    T sx[9] = {T(0), -Sg[2], Sg[1], Sg[2], T(0), -Sg[0], -Sg[1], Sg[0], T(0)};
+    T qx[9] = {T(0), -q[2], q[1], q[2], T(0), -q[0], -q[1], q[0], T(0)};
+    T gx[9] = {T(0), T(-ga_[2]), T(ga_[1]), T(ga_[2]), T(0), T(-ga_[0]), T(-ga_[1]), T(ga_[0]), T(0)};
+    T Sgx[9];
+    mm3(c.S, gx, Sgx);
+    for (int i = 0; i < 9; ++i) N[i] = -qx[i] + vm * (sx[i] - Sgx[i]);
+    T RN[9];
+    mm3(R, N, RN);
+    mm3(RN, c.G0i, W);
+}
+
+// Rotation, force and stiffness factors of one corotational tet at positions x (12) with rest x0 (12),
+// rest gradients g and signed rest volume V. T is the arithmetic, TK the factor storage. Force f (12).
+// Returns element flags.
+template <typename TK, typename T = double>
+__device__ inline int coro_factors(const double* x0, const T* x, const double g[4][3], double V, double lam,
+                                   double mu, T* f, CoroFactors<TK>& cf) {
+    CoroCommon<T> c;
+    coro_common<T>(x0, x, g, lam, mu, c);
+    for (int a = 0; a < 4; ++a) coro_force<T>(c, g[a], V, f + 3 * a);
+    cf.correction = c.correction;
+    for (int i = 0; i < 9; ++i) cf.R[i] = (TK)c.R[i];
     cf.V = (TK)V;
     cf.lam = (TK)lam;
     cf.mu = (TK)mu;
     for (int a = 0; a < 4; ++a) {
         T h[3];
         const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
-        mv3(R, ga, h);
-        for (int c = 0; c < 3; ++c) {
-            cf.h[a][c] = (TK)h[c];
-            cf.g[a][c] = (TK)g[a][c];
+        mv3(c.R, ga, h);
+        for (int k = 0; k < 3; ++k) {
+            cf.h[a][k] = (TK)h[k];
+            cf.g[a][k] = (TK)g[a][k];
         }
     }
-    if (!cf.correction) {
-        for (int a = 0; a < 4; ++a)
-            for (int i = 0; i < 9; ++i) cf.W[a][i] = TK(0);
-        return flags;
-    }
-    // Quirk term: H' = R^T (S - I) = R^T E; p_a = V [lambda tr(H') g_a + mu (H' + H'^T) g_a]; q_a = R p_a
-    T Hp[9];
-    mtm3(R, Es, Hp);
-    T trHp = Hp[0] + Hp[4] + Hp[8];
     for (int a = 0; a < 4; ++a) {
-        T p[3];
-        for (int i = 0; i < 3; ++i) {
-            T s = lam * trHp * g[a][i];
-            for (int k = 0; k < 3; ++k) s += mu * (Hp[3 * i + k] + Hp[3 * k + i]) * g[a][k];
-            p[i] = V * s;
-        }
-        T q[3];
-        mv3(R, p, q);
-        // N_a = -[q]x + V mu ([S g]x - S [g]x)
-        T Sg[3];
-        const T ga[3] = {T(g[a][0]), T(g[a][1]), T(g[a][2])};
-        mv3(S, ga, Sg);
-        T N[9];
-        // [w]x = [[0,-w2,w1],[w2,0,-w0],[-w1,w0,0]]
-        double vm = V * mu;
-        T sx[9] = {T(0), -Sg[2], Sg[1], Sg[2], T(0), -Sg[0], -Sg[1], Sg[0], T(0)};
-        T qx[9] = {T(0), -q[2], q[1], q[2], T(0), -q[0], -q[1], q[0], T(0)};
-        T gx[9] = {T(0), T(-g[a][2]), T(g[a][1]), T(g[a][2]), T(0), T(-g[a][0]), T(-g[a][1]), T(g[a][0]), T(0)};
-        T Sgx[9];
-        mm3(S, gx, Sgx);
-        for (int i = 0; i < 9; ++i) N[i] = -qx[i] + vm * (sx[i] - Sgx[i]);
-        T RN[9], W[9];
-        mm3(R, N, RN);
-        mm3(RN, G0i, W);
+        T W[9];
+        coro_weight<T>(c, g[a], V, lam, mu, W);
         for (int i = 0; i < 9; ++i) cf.W[a][i] = (TK)W[i];
     }
-    return flags;
+    return c.flags;
+}
+
+// K_ab from one row's factors: ha = R g_a, hb = R g_b, ga, gb, W = W_a, R; the same arithmetic as
+// coro_block.
+__device__ __forceinline__ void coro_row_block(const double* ha, const double* hb, const double* ga, const double* gb,
+                                               const double* W, const double* R, bool correction, double V,
+                                               double lam, double mu, double* K) {
+    double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            K[3 * i + j] = V * (lam * ha[i] * hb[j] + mu * hb[i] * ha[j] + ((i == j) ? mu * gab : 0.0));
+    if (correction) {
+        double WG[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            WG[3 * i + 0] = W[3 * i + 1] * gb[2] - W[3 * i + 2] * gb[1];
+            WG[3 * i + 1] = -W[3 * i + 0] * gb[2] + W[3 * i + 2] * gb[0];
+            WG[3 * i + 2] = W[3 * i + 0] * gb[1] - W[3 * i + 1] * gb[0];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                K[3 * i + j] += WG[3 * i] * R[3 * j] + WG[3 * i + 1] * R[3 * j + 1] + WG[3 * i + 2] * R[3 * j + 2];
+    }
+}
+
+// Linear tet stiffness block from the two gradients: K0_ab = V [lambda g_a g_b^T + mu g_b g_a^T +
+// mu (g_a . g_b) I]; the same arithmetic as linear_block.
+__device__ __forceinline__ void linear_row_block(const double* ga, const double* gb, double V, double lam, double mu,
+                                                 double* K) {
+    double gab = dot3(ga, gb);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) K[3 * i + j] = V * (lam * ga[i] * gb[j] + mu * gb[i] * ga[j] + ((i == j) ? mu * gab : 0.0));
 }
 
 // Stiffness block K_ab (row-major 3x3) from the factors.

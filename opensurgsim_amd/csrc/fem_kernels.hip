@@ -23,9 +23,16 @@ thread_local LaunchTiming g_launch_timing;
 
 
 // ------------------------------------------------------------------------------------------
-// Tetrahedron element kernel (FP64). One thread per element: rotation, force, the 16 stiffness
-// blocks, the right-hand-side contribution f + K (xn - xs - vn dt) - M (vn - vs) / dt and the
-// lower-triangle record of A_e = M_e / dt + dt K_e.
+// Tetrahedron element kernel (FP64). Four lanes per element, lane a for node a: every lane forms the
+// element's rotation (the element-wide part is cheap next to the stiffness blocks), then its own row:
+// f_a, the blocks K_ab (b = 0..3), the right-hand-side row f_a + sum_b K_ab (xn - xs - vn dt)_b
+// - sum_b M_ab (vn - vs)_b / dt, and its lower-triangle blocks of A_e = M_e / dt + dt K_e (the diagonal
+// block a, 6 values, and the pair blocks (a, b) whose node id of a is the larger, 9 values). Holding one
+// row instead of the whole element keeps the registers to a few waves per SIMD.
+// Every contribution is written where the consumer reads it: slot_off gives the position of each of the
+// element's ten lower-block contributions in the staging list ordered by block (element order within a
+// block), rhs_off that of each node row in the list ordered by node, so the assembly and the right-hand
+// side stream their inputs.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ constexpr int pair_slot(int a, int b) { return a * (7 - a) / 2 + (b - a - 1); }
 
@@ -42,113 +49,163 @@ inline dim3 xcd_grid(int64_t n, int threads) {
     return dim3((unsigned)b);
 }
 
-template <bool COROT, bool GENERAL>
-__global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
-    int e = xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= A.num_elements) return;
-    int4 id4 = A.conn[e];
-    const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-    const double dt = A.elem_inst ? A.inst_dt[A.elem_inst[e]] : A.dt;
-    double x0[12], xn[12], d1[12], d2[12];
+// v[a] for a lane-dependent a without dynamic indexing of a register array (which would go to scratch).
+template <int N>
+__device__ __forceinline__ void pick(const double (&v)[4][N], int a, double* out) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int i = 0; i < N; ++i) out[i] = a == 0 ? v[0][i] : a == 1 ? v[1][i] : a == 2 ? v[2][i] : v[3][i];
+}
+
+// out = v of lane b of this lane's quad (DPP quad_perm [b, b, b, b]); the whole quad must be active.
+__device__ __forceinline__ void quad_bcast3(int b, const double* v, double* out) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            size_t i = 3 * (size_t)ids[a] + c;
-            x0[3 * a + c] = A.x0[i];
-            double xv = A.xn[i];
-            double vv = A.vn[i];
-            xn[3 * a + c] = xv;
-            if (GENERAL) {
-                d1[3 * a + c] = (xv - A.xs[i]) - vv * dt;
-                d2[3 * a + c] = vv - A.vs[i];
-            } else {
-                d1[3 * a + c] = -(vv * dt);
-                d2[3 * a + c] = 0.0;
-            }
+    for (int i = 0; i < 3; ++i) {
+        switch (b) {
+            case 0: out[i] = dpp_mov<0x00>(v[i]); break;
+            case 1: out[i] = dpp_mov<0x55>(v[i]); break;
+            case 2: out[i] = dpp_mov<0xAA>(v[i]); break;
+            default: out[i] = dpp_mov<0xFF>(v[i]); break;
         }
     }
-    double4 m = A.mat[e];
-    const double lam = m.x, mu = m.y, rho = m.z;
-    double g[4][3];
-    const double V = tet_rest_gradients(x0, g);
-    double f[12];
-    CoroFactors<double> cf;
-    int flags = 0;
-    if (COROT) {
-        flags = coro_factors<double>(x0, xn, g, V, lam, mu, f, cf);
-    } else {
-        // f = -K0 (x - x0)  (Physics/Fem3DElementTetrahedron.cpp:77-90)
+}
+
+template <bool COROT, bool GENERAL>
+__global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
+    const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const int e = (int)(t >> 2), a = (int)(t & 3);
+    if (e >= A.num_elements) return;  // the four lanes of an element leave together
+    const int4 id4 = A.conn[e];
+    const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+    const int ida = a == 0 ? id4.x : a == 1 ? id4.y : a == 2 ? id4.z : id4.w;
+    const double dt = A.elem_inst ? A.inst_dt[A.elem_inst[e]] : A.dt;
+    // staging offsets, loaded up front so the stores at the end do not wait for them
+    int slot[10];
+    if (A.write_matrix) {
+        const int2* sp = reinterpret_cast<const int2*>(A.slot_off + 10 * (size_t)e);  // 8-byte aligned
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            double t[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < 5; ++q) {
+            const int2 v = sp[q];
+            slot[2 * q] = v.x;
+            slot[2 * q + 1] = v.y;
+        }
+    }
+    const int rhs_at = A.rhs_off[4 * (size_t)e + a];
+    const double4 m = A.mat[e];
+    const double lam = m.x, mu = m.y, rho = m.z;
+    double V, ga[3], ra[3];
+    double R[9], W[9], ha[3];
+    bool correction = false;
+    {
+        double x0[12], xn[12];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const size_t i = 3 * (size_t)ids[b] + c;
+                x0[3 * b + c] = A.x0[i];
+                xn[3 * b + c] = A.xn[i];
+            }
+        double g[4][3];
+        V = tet_rest_gradients(x0, g);
+        pick<3>(g, a, ga);
+        if (COROT) {
+            CoroCommon<double> cc;
+            coro_common<double>(x0, xn, g, lam, mu, cc);
+            coro_force<double>(cc, ga, V, ra);
+            if (cc.flags && a == 0) {
+                if (cc.flags & ELEM_FLAG_DET_R) atomicAdd(&A.flag_counts[0], 1);
+                if (cc.flags & ELEM_FLAG_G_SINGULAR) atomicAdd(&A.flag_counts[1], 1);
+                if (cc.flags & ELEM_FLAG_POLAR_EIG) atomicAdd(&A.flag_counts[2], 1);
+            }
+            correction = cc.correction;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) R[i] = cc.R[i];
+            mv3(cc.R, ga, ha);
+            coro_weight<double>(cc, ga, V, lam, mu, W);
+        } else {
+            // f_a = -sum_b K0_ab (x_b - x0_b)  (Physics/Fem3DElementTetrahedron.cpp:77-90)
+            double tt[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 double Kb[9];
-                linear_block<double>(g, V, lam, mu, a, b, Kb);
+                linear_row_block(ga, g[b], V, lam, mu, Kb);
                 double u[3] = {xn[3 * b] - x0[3 * b], xn[3 * b + 1] - x0[3 * b + 1], xn[3 * b + 2] - x0[3 * b + 2]};
 #pragma unroll
-                for (int i = 0; i < 3; ++i) t[i] += Kb[3 * i] * u[0] + Kb[3 * i + 1] * u[1] + Kb[3 * i + 2] * u[2];
+                for (int i = 0; i < 3; ++i) tt[i] += Kb[3 * i] * u[0] + Kb[3 * i + 1] * u[1] + Kb[3 * i + 2] * u[2];
             }
-            f[3 * a] = -t[0];
-            f[3 * a + 1] = -t[1];
-            f[3 * a + 2] = -t[2];
+            ra[0] = -tt[0];
+            ra[1] = -tt[1];
+            ra[2] = -tt[2];
         }
-    }
-    if (flags) {
-        if (flags & ELEM_FLAG_DET_R) atomicAdd(&A.flag_counts[0], 1);
-        if (flags & ELEM_FLAG_G_SINGULAR) atomicAdd(&A.flag_counts[1], 1);
-        if (flags & ELEM_FLAG_POLAR_EIG) atomicAdd(&A.flag_counts[2], 1);
     }
     // Mass: M_e = R M0 R^T = rho V / 20 (2 on the diagonal blocks, 1 off) x I3.
     const double mc = rho * V / 20.0;
     const double idt = 1.0 / dt;
-    double* rec = A.rec + (size_t)e * kTetRec;
-    double* rr = A.rhs_rec + (size_t)e * 12;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double ra[3] = {f[3 * a], f[3 * a + 1], f[3 * a + 2]};
+    for (int b = 0; b < 4; ++b) {
+        double gb[3];
+        quad_bcast3(b, ga, gb);  // g_b and R g_b from lane b of the quad
+        double K[9];
+        if (COROT) {
+            double hb[3];
+            quad_bcast3(b, ha, hb);
+            coro_row_block(ha, hb, ga, gb, W, R, correction, V, lam, mu, K);
+        } else {
+            linear_row_block(ga, gb, V, lam, mu, K);
+        }
+        // rhs = f + K (xn - xs - vn dt) - M (vn - vs) / dt   (Math/OdeSolverEulerImplicit.cpp:157-158)
+        double d1[3], d2[3];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            double K[9];
-            if (COROT) coro_block<double>(cf, a, b, K);
-            else linear_block<double>(g, V, lam, mu, a, b, K);
-            // rhs = f + K (xn - xs - vn dt) - M (vn - vs) / dt   (Math/OdeSolverEulerImplicit.cpp:157-158)
-            const double mab = (a == b) ? 2.0 * mc : mc;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                ra[i] += K[3 * i] * d1[3 * b] + K[3 * i + 1] * d1[3 * b + 1] + K[3 * i + 2] * d1[3 * b + 2];
-                if (GENERAL) ra[i] -= (mab * d2[3 * b + i]) / dt;
-            }
-            if (!A.write_matrix) continue;
-            // A_ab = M_ab * (1/dt) + K_ab * dt   (D = 0 for 3D elements, Physics/FemElement.cpp:131-145)
-            const double mdt = mab * idt;
-            if (a == b) {
-                double* o = rec + kTetPairSlots + 6 * a;
-                o[0] = mdt + K[0] * dt;
-                o[1] = K[3] * dt;
-                o[2] = mdt + K[4] * dt;
-                o[3] = K[6] * dt;
-                o[4] = K[7] * dt;
-                o[5] = mdt + K[8] * dt;
-            } else if (ids[a] > ids[b]) {
-                const int ps = (a < b) ? pair_slot(a, b) : pair_slot(b, a);
-                double* o = rec + 9 * ps;
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) o[3 * i + j] = ((i == j) ? mdt : 0.0) + K[3 * i + j] * dt;
+        for (int c = 0; c < 3; ++c) {
+            const size_t i = 3 * (size_t)ids[b] + c;
+            const double vv = A.vn[i];
+            if (GENERAL) {
+                d1[c] = (A.xn[i] - A.xs[i]) - vv * dt;
+                d2[c] = vv - A.vs[i];
+            } else {
+                d1[c] = -(vv * dt);
+                d2[c] = 0.0;
             }
         }
-        rr[3 * a] = ra[0];
-        rr[3 * a + 1] = ra[1];
-        rr[3 * a + 2] = ra[2];
+        const double mab = (a == b) ? 2.0 * mc : mc;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ra[i] += K[3 * i] * d1[0] + K[3 * i + 1] * d1[1] + K[3 * i + 2] * d1[2];
+            if (GENERAL) ra[i] -= (mab * d2[i]) / dt;
+        }
+        if (!A.write_matrix) continue;
+        // A_ab = M_ab * (1/dt) + K_ab * dt   (D = 0 for 3D elements, Physics/FemElement.cpp:131-145)
+        const double mdt = mab * idt;
+        if (a == b) {
+            const int sd = a == 0 ? slot[6] : a == 1 ? slot[7] : a == 2 ? slot[8] : slot[9];
+            double2* o = reinterpret_cast<double2*>(A.rec + sd);  // records start on 16 bytes
+            o[0] = make_double2(mdt + K[0] * dt, K[3] * dt);
+            o[1] = make_double2(mdt + K[4] * dt, K[6] * dt);
+            o[2] = make_double2(K[7] * dt, mdt + K[8] * dt);
+        } else if (ida > ids[b]) {
+            const int ps = (a < b) ? pair_slot(a, b) : pair_slot(b, a);
+            int sp = slot[0];
+#pragma unroll
+            for (int q = 1; q < 6; ++q) sp = ps == q ? slot[q] : sp;
+            double v[kPairRec];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) v[3 * i + j] = ((i == j) ? mdt : 0.0) + K[3 * i + j] * dt;
+            v[9] = 0.0;
+            double2* o = reinterpret_cast<double2*>(A.rec + sp);
+#pragma unroll
+            for (int q = 0; q < kPairRec / 2; ++q) o[q] = make_double2(v[2 * q], v[2 * q + 1]);
+        }
     }
+    double2* rr = reinterpret_cast<double2*>(A.rhs_rec + rhs_at);
+    rr[0] = make_double2(ra[0], ra[1]);
+    rr[1] = make_double2(ra[2], 0.0);
 }
 
 void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     if (a.num_elements == 0) return;
-    const dim3 grid = xcd_grid(a.num_elements, 256), block(256);
+    const dim3 grid = xcd_grid(4 * (int64_t)a.num_elements, 256), block(256);
     if (kind == 1) {
         if (a.general) launch_k((tet_element_kernel<true, true>), grid, block, 0, s, a);
         else launch_k((tet_element_kernel<true, false>), grid, block, 0, s, a);
@@ -158,113 +215,101 @@ void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     }
 }
 
+// sum_{c = c0}^{c1 - 1} p[stride * c], added in order; loads go out four at a time so one lane keeps
+// several in flight (the adds stay sequential, so the rounding is that of the plain loop).
+__device__ __forceinline__ double sum_strided(const double* __restrict__ p, int stride, int c0, int c1) {
+    double acc = 0.0;
+    for (int c = c0; c < c1; c += 4) {
+        const double v0 = p[stride * (int64_t)c];
+        const double v1 = c + 1 < c1 ? p[stride * (int64_t)(c + 1)] : 0.0;
+        const double v2 = c + 2 < c1 ? p[stride * (int64_t)(c + 2)] : 0.0;
+        const double v3 = c + 3 < c1 ? p[stride * (int64_t)(c + 3)] : 0.0;
+        acc += v0;
+        if (c + 1 < c1) acc += v1;
+        if (c + 2 < c1) acc += v2;
+        if (c + 3 < c1) acc += v3;
+    }
+    return acc;
+}
+
 // ------------------------------------------------------------------------------------------
-// Deterministic gather assembly: one thread per lower block (I >= J) sums its contributions in a
-// fixed (element) order, applies the boundary conditions (zero BC rows and columns, 1 on the BC
-// diagonal, Math/OdeState.cpp:197-218) and writes the block and its transposed mirror, giving the
-// lower-triangle self-adjoint view that Eigen's CG applies (Math/LinearSparseSolveAndInverse.h:122).
+// Deterministic assembly from the staged contributions: one lane per value of a lower block (9 lanes per
+// off-diagonal block, 6 per diagonal block; the off-diagonal blocks come first). A lane adds its value
+// over the block's contributions in element order, applies the boundary conditions (zero BC rows and
+// columns, 1 on the BC diagonal, Math/OdeState.cpp:197-218) and writes it and its transposed mirror,
+// giving the lower-triangle self-adjoint view that Eigen's CG applies
+// (Math/LinearSparseSolveAndInverse.h:122). Contributions of off-diagonal block t are the kPairRec-double
+// records ctr_ptr[t] .. ctr_ptr[t + 1] of stage; those of the diagonal blocks follow at
+// kPairRec * nctr_off as kDiagRec-double records (lower triangle (0,0), (1,0), (1,1), (2,0), (2,1), (2,2)).
+// lb_info[t] = (block, mirror block, boundary bits: DOF k of the row in bit k, of the column in bit 3 + k).
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void assemble_kernel(int num_lower, const int* __restrict__ lower_blk,
-                                                       const int* __restrict__ mirror_blk,
-                                                       const int* __restrict__ blk_row,
-                                                       const int* __restrict__ blk_col,
-                                                       const int* __restrict__ ctr_ptr,
-                                                       const int* __restrict__ ctr_list,
-                                                       const double* __restrict__ rec,
-                                                       const uint8_t* __restrict__ bcmask, int64_t plane,
+__global__ __launch_bounds__(256) void assemble_kernel(int num_off, int num_diag, const int4* __restrict__ lb_info,
+                                                       const int* __restrict__ ctr_ptr, int nctr_off,
+                                                       const double* __restrict__ stage,
                                                        double* __restrict__ S64, float* __restrict__ S32) {
-    int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= num_lower) return;
-    const int blk = lower_blk[t];
-    const int I = blk_row[blk];
-    const int J = blk_col[blk];
-    double acc[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-    const int c0 = ctr_ptr[t], c1 = ctr_ptr[t + 1];
-    if (I != J) {
-        for (int c = c0; c < c1; ++c) {
-            int code = ctr_list[c];
-            const double* r = rec + (size_t)(code >> 4) * kTetRec + 9 * (code & 15);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k] += r[k];
-        }
+    const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t n_off_lanes = 9 * (int64_t)num_off;
+    int lt, i, j;
+    bool diag;
+    double acc;
+    if (t < n_off_lanes) {
+        lt = (int)(t / 9);
+        const int v = (int)(t - 9 * (int64_t)lt);
+        i = v / 3;
+        j = v - 3 * i;
+        diag = false;
+        acc = sum_strided(stage + v, kPairRec, ctr_ptr[lt], ctr_ptr[lt + 1]);
     } else {
-        for (int c = c0; c < c1; ++c) {
-            int code = ctr_list[c];
-            const double* r = rec + (size_t)(code >> 4) * kTetRec + kTetPairSlots + 6 * ((code & 15) - 6);
-            acc[0] += r[0];
-            acc[3] += r[1];
-            acc[4] += r[2];
-            acc[6] += r[3];
-            acc[7] += r[4];
-            acc[8] += r[5];
-        }
-        acc[1] = acc[3];
-        acc[2] = acc[6];
-        acc[5] = acc[7];
+        const int64_t u = t - n_off_lanes;
+        if (u >= 6 * (int64_t)num_diag) return;
+        const int d = (int)(u / 6);
+        const int v = (int)(u - 6 * (int64_t)d);
+        lt = num_off + d;
+        i = v == 0 ? 0 : (v < 3 ? 1 : 2);
+        j = v == 0 ? 0 : (v < 3 ? v - 1 : v - 3);
+        diag = true;
+        acc = sum_strided(stage + kPairRec * (int64_t)nctr_off + v, kDiagRec, ctr_ptr[lt] - nctr_off,
+                          ctr_ptr[lt + 1] - nctr_off);
     }
-    uint8_t br[3], bcol[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        br[k] = bcmask[3 * I + k];
-        bcol[k] = bcmask[3 * J + k];
+    const int4 info = lb_info[lt];
+    double val = acc;
+    if (((info.z >> i) | (info.z >> (3 + j))) & 1) val = (diag && i == j) ? 1.0 : 0.0;
+    S64[9 * (int64_t)info.x + 3 * i + j] = val;
+    if (S32) S32[9 * (int64_t)info.x + 3 * i + j] = (float)val;
+    if (!diag || i != j) {  // the mirror: block (J, I) value (j, i), or the upper entry of a diagonal block
+        S64[9 * (int64_t)info.y + 3 * j + i] = val;
+        if (S32) S32[9 * (int64_t)info.y + 3 * j + i] = (float)val;
     }
-    const int mb = mirror_blk[t];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            double v = acc[3 * i + j];
-            if (br[i] | bcol[j]) v = (I == J && i == j) ? 1.0 : 0.0;
-            S64[9 * (int64_t)blk + 3 * i + j] = v;
-            if (S32) S32[9 * (int64_t)blk + 3 * i + j] = (float)v;
-            if (I != J) {
-                S64[9 * (int64_t)mb + 3 * j + i] = v;
-                if (S32) S32[9 * (int64_t)mb + 3 * j + i] = (float)v;
-            }
-        }
 }
 
-void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
-                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
-                     const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s) {
-    if (num_lower == 0) return;
-    launch_k(assemble_kernel, xcd_grid(num_lower, 256), dim3(256), 0, s, num_lower, lower_blk,
-                       mirror_blk, blk_row, blk_col, ctr_ptr, ctr_list, rec, bcmask, plane, S64, S32);
+void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* ctr_ptr, int nctr_off,
+                     const double* stage, double* S64, float* S32, hipStream_t s) {
+    const int64_t lanes = 9 * (int64_t)num_off + 6 * (int64_t)num_diag;
+    if (lanes == 0) return;
+    launch_k(assemble_kernel, xcd_grid(lanes, 256), dim3(256), 0, s, num_off, num_diag, lb_info, ctr_ptr,
+             nctr_off, stage, S64, S32);
 }
 
-// Right-hand side per node: element contributions in element order, gravity m_i g
+// Right-hand side, one lane per DOF: the node's staged element rows in element order, gravity m_i g
 // (Physics/FemRepresentation.cpp:641-662), external force; zero on BC DOFs (Math/OdeState.cpp:159-172).
 __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __restrict__ node_ptr,
-                                                  const int* __restrict__ node_list,
-                                                  const double* __restrict__ rhs_rec, const double* __restrict__ fg,
+                                                  const double* __restrict__ stage, const double* __restrict__ fg,
                                                   const double* __restrict__ fext, const uint8_t* __restrict__ bcmask,
                                                   double* __restrict__ b) {
-    int I = xcd_block() * blockDim.x + threadIdx.x;
-    if (I >= num_nodes) return;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int c = node_ptr[I]; c < node_ptr[I + 1]; ++c) {
-        int code = node_list[c];
-        const double* r = rhs_rec + (size_t)(code >> 2) * 12 + 3 * (code & 3);
-        acc[0] += r[0];
-        acc[1] += r[1];
-        acc[2] += r[2];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        size_t i = 3 * (size_t)I + k;
-        double v = acc[k] + fg[i];
-        if (fext) v += fext[i];
-        b[i] = bcmask[i] ? 0.0 : v;
-    }
+    const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= 3 * (int64_t)num_nodes) return;
+    const int I = (int)(i / 3), k = (int)(i - 3 * (int64_t)I);
+    const double acc = sum_strided(stage + k, kRhsRec, node_ptr[I], node_ptr[I + 1]);
+    double v = acc + fg[i];
+    if (fext) v += fext[i];
+    b[i] = bcmask[i] ? 0.0 : v;
 }
 
-void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const double* rhs_rec, const double* fg,
-                const double* fext, const uint8_t* bcmask, double* b, hipStream_t s) {
+void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
+                const uint8_t* bcmask, double* b, hipStream_t s) {
     if (num_nodes == 0) return;
-    launch_k(rhs_kernel, xcd_grid(num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, node_list,
-                       rhs_rec, fg, fext, bcmask, b);
+    launch_k(rhs_kernel, xcd_grid(3 * (int64_t)num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, stage, fg,
+             fext, bcmask, b);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -327,7 +372,8 @@ __global__ __launch_bounds__(kCgThreads) void cg_init_kernel(Operator op, const 
     double bb = 0.0, rz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int I = i / 3, c = i - 3 * I;
-        const TV s = S[9 * (int64_t)op.diagblk[I] + 4 * c];
+        const int db = op.diagblk[I];  // -1: a node in no element (empty row)
+        const TV s = db >= 0 ? S[9 * (int64_t)db + 4 * c] : TV(0);
         const TV d = (s != TV(0)) ? TV(1) / s : TV(1);
         const TV bi = (TV)b[i];
         x[i] = TV(0);
@@ -536,6 +582,35 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
     TV* wn = odd ? w1 : w0;
     TV* pn = odd ? p1 : p0;
     TV* qn = odd ? q1 : q0;
+    // Rows of this workgroup (XCD-aware, as group_spmv) and the first row's own inputs: loaded before the
+    // reduction of the block sums below, so their latency overlaps it.
+    const int lane = threadIdx.x & (kRowGroup - 1);
+    const int gpb = blockDim.x / kRowGroup, gib = threadIdx.x / kRowGroup;
+    const int nb = gridDim.x;
+    const int Lb = (nb % kXcds == 0) ? (blockIdx.x % kXcds) * (nb / kXcds) + blockIdx.x / kXcds : blockIdx.x;
+    const int per = (op.num_nodes + nb - 1) / nb;
+    const int row0 = Lb * per;
+    const int row1 = min(op.num_nodes, row0 + per);
+    const int cl = lane < 3 ? lane : 2;  // lanes 3.. of a group load lane 2's values (same line), unused
+    int I = row0 + gib;
+    int k0 = 0, k1 = 0;
+    TV r_o = 0, q_o = 0, z_o = 0, w_o = 0, p_o = 0, x_o = 0, d_o = 0;
+    // FP32 holds the row's own inputs across the SpMV (prefetched a row ahead); FP64 would need registers
+    // beyond the 64 that keep two workgroups per CU, so it runs the plain row loop and reads them after.
+    constexpr bool kPrefetch = sizeof(TV) == 4;
+    auto fetch_row = [&](int row) {
+        k0 = op.rowptr[row];
+        k1 = op.rowptr[row + 1];
+        const int i = 3 * row + cl;
+        r_o = ro[i];
+        q_o = qo[i];
+        z_o = zo[i];
+        w_o = wo[i];
+        p_o = po[i];
+        x_o = x[i];
+        d_o = dinv[i];
+    };
+    if (kPrefetch && I < row1) fetch_row(I);
     double alpha = 0.0, beta = 0.0;
     const double* part_old = part + (odd ? 0 : kFusedSums * kCgBlocks);
     double* part_new = part + (odd ? kFusedSums * kCgBlocks : 0);
@@ -566,37 +641,96 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
     }
     const TV a = (TV)alpha, bt = (TV)beta;
-    const bool k0 = k == 0;
+    const bool k0it = k == 0;
     double acc[kFusedSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    group_spmv<TV, TV, kRowGroup>(
-        op, S,
-        [&](int J, TV& a0, TV& a1, TV& a2) {
-            if (k0) {
-                a0 = zo[3 * J];
-                a1 = zo[3 * J + 1];
-                a2 = zo[3 * J + 2];
-            } else {
-                a0 = (zo[3 * J] - a * wo[3 * J]) + bt * po[3 * J];
-                a1 = (zo[3 * J + 1] - a * wo[3 * J + 1]) + bt * po[3 * J + 1];
-                a2 = (zo[3 * J + 2] - a * wo[3 * J + 2]) + bt * po[3 * J + 2];
+    if constexpr (!kPrefetch) {
+        // FP64: the row loop of group_spmv, own-row inputs read after the SpMV
+        group_spmv<TV, TV, kRowGroup>(
+            op, S,
+            [&](int J, TV& a0, TV& a1, TV& a2) {
+                if (k0it) {
+                    a0 = zo[3 * J];
+                    a1 = zo[3 * J + 1];
+                    a2 = zo[3 * J + 2];
+                } else {
+                    a0 = (zo[3 * J] - a * wo[3 * J]) + bt * po[3 * J];
+                    a1 = (zo[3 * J + 1] - a * wo[3 * J + 1]) + bt * po[3 * J + 1];
+                    a2 = (zo[3 * J + 2] - a * wo[3 * J + 2]) + bt * po[3 * J + 2];
+                }
+            },
+            [&](int row, int ln, TV y0, TV y1, TV y2) {
+                if (ln < 3) {
+                    const int i = 3 * row + ln;
+                    const TV y = ln == 0 ? y0 : (ln == 1 ? y1 : y2);
+                    TV rk, zk, pk;
+                    if (k0it) {
+                        rk = ro[i];
+                        zk = zo[i];
+                        pk = zk;
+                    } else {
+                        rk = ro[i] - a * qo[i];
+                        zk = zo[i] - a * wo[i];
+                        pk = zk + bt * po[i];
+                        x[i] += a * po[i];
+                    }
+                    const TV wk = dinv[i] * y;
+                    rn[i] = rk;
+                    zn[i] = zk;
+                    pn[i] = pk;
+                    qn[i] = y;
+                    wn[i] = wk;
+                    const double dr = (double)rk, dy = (double)y, dz = (double)zk;
+                    acc[0] += (double)pk * dy;
+                    acc[1] += dr * dr;
+                    acc[2] += dr * dy;
+                    acc[3] += dy * dy;
+                    acc[4] += dr * dz;
+                    acc[5] += dy * dz;
+                    acc[6] += dy * (double)wk;
+                }
+            });
+    } else {
+        while (I < row1) {
+            TV y0 = 0, y1 = 0, y2 = 0;
+            for (int kk = k0 + lane; kk < k1; kk += kRowGroup) {
+                const int J = op.colidx[kk];
+                TV a0, a1, a2;
+                if (k0it) {
+                    a0 = zo[3 * J];
+                    a1 = zo[3 * J + 1];
+                    a2 = zo[3 * J + 2];
+                } else {
+                    a0 = (zo[3 * J] - a * wo[3 * J]) + bt * po[3 * J];
+                    a1 = (zo[3 * J + 1] - a * wo[3 * J + 1]) + bt * po[3 * J + 1];
+                    a2 = (zo[3 * J + 2] - a * wo[3 * J + 2]) + bt * po[3 * J + 2];
+                }
+                const TV* B = S + 9 * (int64_t)kk;
+                y0 += B[0] * a0 + B[1] * a1 + B[2] * a2;
+                y1 += B[3] * a0 + B[4] * a1 + B[5] * a2;
+                y2 += B[6] * a0 + B[7] * a1 + B[8] * a2;
             }
-        },
-        [&](int I, int lane, TV y0, TV y1, TV y2) {
+            y0 = group_sum<kRowGroup>(y0);  // the whole group is active here (I is uniform over the group)
+            y1 = group_sum<kRowGroup>(y1);
+            y2 = group_sum<kRowGroup>(y2);
+            const int row = I;
+            const TV ro_i = r_o, qo_i = q_o, zo_i = z_o, wo_i = w_o, po_i = p_o, x_i = x_o, d_i = d_o;
+            I += gpb;
+            if (I < row1) fetch_row(I);  // the next row's inputs go out before this row's stores
             if (lane < 3) {
-                const int i = 3 * I + lane;
+                const int i = 3 * row + lane;
                 const TV y = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
                 TV rk, zk, pk;
-                if (k0) {
-                    rk = ro[i];
-                    zk = zo[i];
+                if (k0it) {
+                    rk = ro_i;
+                    zk = zo_i;
                     pk = zk;
                 } else {
-                    rk = ro[i] - a * qo[i];
-                    zk = zo[i] - a * wo[i];
-                    pk = zk + bt * po[i];
-                    x[i] += a * po[i];
+                    rk = ro_i - a * qo_i;
+                    zk = zo_i - a * wo_i;
+                    pk = zk + bt * po_i;
+                    x[i] = x_i + a * po_i;
                 }
-                const TV wk = dinv[i] * y;
+                const TV wk = d_i * y;
                 rn[i] = rk;
                 zn[i] = zk;
                 pn[i] = pk;
@@ -611,7 +745,8 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
                 acc[5] += dy * dz;
                 acc[6] += dy * (double)wk;
             }
-        });
+        }
+    }
     block_sums7(acc, lds, part_new);
 }
 
@@ -681,14 +816,17 @@ void launch_cg_vector(const CgArgs& a, int k, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // FP64 iterative refinement around the FP32 CG: r64 = b - S64 dv in FP64 (the first sweep takes
 // r64 = b) with partial |r64|^2; the check kernel then applies the outer test
-// |r64|^2 < max(tol^2 |b|^2, realmin) and marks the inner solve done when it passes.
+// |r64|^2 < max(tol^2 |b|^2, realmin) and marks the inner solve done when it passes. Once the outer
+// test has passed, later sweeps' kernels return at once (sweeps can be queued ahead of the result).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kCgThreads) void residual64_kernel(Operator op, const double* __restrict__ S,
                                                                 const double* __restrict__ b,
                                                                 const double* __restrict__ dv, int first,
                                                                 double* __restrict__ r64,
-                                                                double* __restrict__ partial) {
+                                                                double* __restrict__ partial,
+                                                                const CgScalars* __restrict__ outer) {
     __shared__ double lds[64];
+    if (!first && outer->done) return;
     double acc = 0.0;
     if (first) {
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * op.num_nodes; i += gridDim.x * blockDim.x) {
@@ -720,6 +858,7 @@ __global__ __launch_bounds__(kCgThreads) void residual64_kernel(Operator op, con
 __global__ __launch_bounds__(kCgThreads) void refine_check_kernel(const double* __restrict__ partial, int first,
                                                                   double tol, CgScalars* outer, CgScalars* inner) {
     __shared__ double lds[64];
+    if (!first && outer->done) return;
     const double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
     if (threadIdx.x == 0) {
         if (first) {
@@ -739,10 +878,15 @@ __global__ __launch_bounds__(kCgThreads) void refine_check_kernel(const double* 
     }
 }
 
+// dv += x (FP32 correction) unless the outer test has passed; records the inner solve's iteration
+// count in *iters_out (left alone when skipped).
 __global__ __launch_bounds__(kCgThreads) void accumulate_kernel(int n, const float* __restrict__ xf,
                                                                 double* __restrict__ dv,
-                                                                const CgScalars* __restrict__ outer) {
+                                                                const CgScalars* __restrict__ outer,
+                                                                const CgScalars* __restrict__ inner,
+                                                                int* __restrict__ iters_out) {
     if (outer->done) return;
+    if (iters_out && blockIdx.x == 0 && threadIdx.x == 0) *iters_out = inner->iters;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dv[i] += (double)xf[i];
 }
 
@@ -750,12 +894,13 @@ void launch_residual64(const Operator& op, const double* S64, const double* b, c
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
                        hipStream_t s) {
     launch_k(residual64_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, op, S64, b, dv, first, r64,
-                       partial);
+                       partial, (const CgScalars*)outer);
     launch_k(refine_check_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, first, tol, outer, inner);
 }
 
-void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s) {
-    launch_k(accumulate_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, xf, dv, outer);
+void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, const CgScalars* inner,
+                              int* iters_out, hipStream_t s) {
+    launch_k(accumulate_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, xf, dv, outer, inner, iters_out);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -30,8 +30,6 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_
 // slots [0, 54): the 6 node-pair blocks (pair order (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) of
 // A_e = M_e / dt + dt K_e, each oriented so that its row node has the larger global id, row-major;
 // slots [54, 78): the lower triangles (00,10,11,20,21,22) of the 4 diagonal blocks.
-constexpr int kTetRec = 80;   // padded to a multiple of 16 bytes
-constexpr int kTetPairSlots = 54;
 
 // CG launch geometry: a fixed grid of kCgBlocks blocks of kCgThreads threads (8 waves per SIMD on
 // 256 CUs); grid-stride loops cover any size, and every grid reduction has kCgBlocks partials.
@@ -52,6 +50,12 @@ struct CgScalars {
     int pad;
 };
 
+// Staged record sizes (doubles): an off-diagonal block contribution (9 values + pad, so records start on
+// 16-byte boundaries), a diagonal one (lower triangle), a node row of the right-hand side (3 + pad).
+constexpr int kPairRec = 10;
+constexpr int kDiagRec = 6;
+constexpr int kRhsRec = 4;
+
 struct TetElementArgs {
     int num_elements;
     const int4* conn;
@@ -64,8 +68,12 @@ struct TetElementArgs {
     double dt;
     int write_matrix;     // 0: only the rhs record
     int general;          // 0: xn == xs and vn == vs (first Newton iteration)
-    double* rec;          // [num_elements][kTetRec]
-    double* rhs_rec;      // [num_elements][12]
+    double* rec;          // lower-block contributions staged by block: kPairRec records of the off-diagonal
+                          // blocks, then kDiagRec records of the diagonal blocks
+    double* rhs_rec;      // element rhs rows staged by node, kRhsRec doubles each
+    const int* slot_off;  // [num_elements][10] offset in rec of the contribution to the pair slots 0..5
+                          // ((0,1), (0,2), (0,3), (1,2), (1,3), (2,3)) and the diagonal blocks 6..9
+    const int* rhs_off;   // [num_elements][4] offset in rhs_rec of node a's row
     int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
     const int* elem_inst = nullptr;     // batched instances: instance of each element (null: one dt)
     const double* inst_dt = nullptr;    // time step per instance
@@ -80,18 +88,18 @@ struct Operator {
     int64_t plane;       // number of blocks
     const int* rowptr;   // block CSR
     const int* colidx;
-    const int* diagblk;
+    const int* diagblk;  // block of the diagonal of each row (-1 for a node in no element)
 };
 
-// Gather assembly from the element records into the FP64 operator (and the FP32 copy when S32 != null).
-void launch_assemble(int num_lower, const int* lower_blk, const int* mirror_blk, const int* blk_row,
-                     const int* blk_col, const int* ctr_ptr, const int* ctr_list, const double* rec,
-                     const uint8_t* bcmask, int64_t plane, double* S64, float* S32, hipStream_t s);
+// Assembly of the staged contributions into the FP64 operator (and the FP32 copy when S32 != null):
+// num_off off-diagonal lower blocks (9 lanes each), then num_diag diagonal ones (6 lanes each).
+void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* ctr_ptr, int nctr_off,
+                     const double* stage, double* S64, float* S32, hipStream_t s);
+void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
+                const uint8_t* bcmask, double* b, hipStream_t s);
 
-void launch_rhs(int num_nodes, const int* node_ptr, const int* node_list, const double* rhs_rec, const double* fg,
-                const double* fext, const uint8_t* bcmask, double* b, hipStream_t s);
-
-// Jacobi-preconditioned CG (Eigen recurrence) on the operator; two kernels per iteration.
+// Jacobi-preconditioned CG (Eigen recurrence) on the operator: one fused kernel per iteration, or the
+// operator kernel + vector kernel pair.
 struct CgArgs {
     Operator op;
     int n;              // 3 * num_nodes
@@ -135,7 +143,9 @@ void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, co
 void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
                        hipStream_t s);
-void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, hipStream_t s);
+// dv += x unless the outer test passed; records the inner iteration count in *iters_out (may be null).
+void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, const CgScalars* inner,
+                              int* iters_out, hipStream_t s);
 
 // State update: v += dv; x = xs + dt v (Math/OdeSolverEulerImplicit.cpp:98-99). Sets *invalid on non-finite.
 // v += dv, x = xs + dt v with dt per node's instance when node_inst is given; flags non-finite values.

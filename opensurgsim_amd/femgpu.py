@@ -168,10 +168,21 @@ def probe_copy(device, nbytes, reps=5):
     return ms.value
 
 
+def _torch_first():
+    """libfem_gpu.so uses the system ROCm runtime; PyTorch ships its own HIP runtime, which fails to find
+    the GPU when it starts after the other one in the same process. When torch is already imported, let it
+    initialise the GPU before the library does."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and hasattr(torch, "cuda") and torch.cuda.is_available():
+        torch.cuda.init()
+
+
 class FemGpu:
     """One GPU representation (context) of the C ABI."""
 
     def __init__(self, device=0):
+        _torch_first()
         self._L = load()
         h = _vp()
         rc = self._L.fem_create(int(device), ctypes.byref(h))
