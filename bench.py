@@ -463,9 +463,9 @@ def main():
     value = D * args.steps / head["seconds"]
 
     w = 4 if use32 else 8
-    # average launch of the dominant kernel: 100 back-to-back launches on the final state between one
-    # event pair (per-launch event pairs inside the steps add their own overhead; those averages are
-    # reported under "phases")
+    # average launch of the dominant kernel: 100 back-to-back launches on the final state, each timed by
+    # the start/stop events of its own dispatch (execution only, as a kernel trace reports it; the event
+    # pairs inside the steps bracket a launch from the host side and are reported under "phases")
     sp_ms, sp_n = head["phases"]["spmv"]
     sp_avg_s = head["op_ms"] / 1e3
     bytes_per = spmv_bytes((N, D, Bk), w)
@@ -473,7 +473,7 @@ def main():
     # HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE counter runs, scaled by a copy kernel of
     # known size (profiles/hbm_traffic_corotational_40x40x40.md); only valid for the 40^3 block
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "cg_apply_traffic.json")
+    pmc = os.path.join(ROOT, "profiles", "cg_traffic.json")
     if os.path.exists(pmc) and n == (40, 40, 40):
         try:
             with open(pmc) as fh:

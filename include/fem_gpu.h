@@ -218,10 +218,11 @@ typedef struct fem_phase_times {
 int fem_enable_phase_timers(fem_ctx* ctx, int every);   /* instrument one step in `every` (0: off); resets totals */
 int fem_get_phase_times(fem_ctx* ctx, fem_phase_times* out);
 
-/* Measurement helper with no counterpart in the reference: `reps` back-to-back launches of the CG
- * operator kernel (fused p update + block SpMV + p.q partials) of a tetrahedral mesh, on the CG scratch
- * buffers of the current precision, between one event pair; average milliseconds per launch. The
- * simulation state is untouched; call after at least one step. */
+/* Measurement helper with no counterpart in the reference: `reps` back-to-back launches of the dominant
+ * CG kernel of a tetrahedral mesh (the one-kernel iteration: direction update, block SpMV, vector updates
+ * and the 7 block sums; or the operator kernel of the split pair under FEM_CG_KERNELS=split), on the CG
+ * scratch buffers of the current precision, each with its own start/stop events; average milliseconds of
+ * execution per launch. The simulation state is untouched; call after at least one step. */
 int fem_time_operator(fem_ctx* ctx, int reps, double* avg_ms);
 
 /* Measurement helper with no counterpart in the reference: `reps` timed copies of `bytes` bytes

@@ -26,11 +26,6 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_
     }
 }
 
-// Per-element record written by the tet element kernel and read by the assembly kernel (FP64):
-// slots [0, 54): the 6 node-pair blocks (pair order (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) of
-// A_e = M_e / dt + dt K_e, each oriented so that its row node has the larger global id, row-major;
-// slots [54, 78): the lower triangles (00,10,11,20,21,22) of the 4 diagonal blocks.
-
 // CG launch geometry: a fixed grid of kCgBlocks blocks of kCgThreads threads (8 waves per SIMD on
 // 256 CUs); grid-stride loops cover any size, and every grid reduction has kCgBlocks partials.
 constexpr int kCgBlocks = 512;
