@@ -156,11 +156,46 @@ __device__ inline void jacobi_eig3(double* A, double* ev, double* V) {
 
 // Polar decomposition F = R S with Eigen's Transform::computeRotationScaling convention: R a proper
 // rotation; when det F < 0 the sign flip goes to the largest singular value (S then has one negative
-// eigenvalue). For det F > 0 this is the orthogonal polar factor, computed by the scaled Newton
-// iteration X <- (g X + X^-T / g) / 2 (Higham 1986, Frobenius scaling); otherwise through the
-// eigen-decomposition of F^T F. Returns 1 if the eigen path was taken.
+// eigenvalue). For det F > 0 this is the orthogonal polar factor. Near a rotation (|F^T F - I|_F < 1/2)
+// it comes from the inverse-free Newton-Schulz iteration X <- X (3 I - X^T X) / 2 (Higham 2008, Functions
+// of Matrices, 8.3), which converges quadratically there; otherwise from the scaled Newton iteration
+// X <- (g X + X^-T / g) / 2 (Higham 1986, Frobenius scaling); and for det F <= 0 or a degenerate F
+// through the eigen-decomposition of F^T F. Returns 1 if the eigen path was taken.
 __device__ inline int polar_rotation(const double* F, double* R) {
     double d = det3(F);
+    if (d > 0.0) {
+        double C[9];
+        mtm3(F, F, C);
+        double e0 = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            const double e = C[i] - ((i % 4 == 0) ? 1.0 : 0.0);
+            e0 += e * e;
+        }
+        if (e0 < 0.25) {
+            double X[9];
+            for (int i = 0; i < 9; ++i) X[i] = F[i];
+            for (int it = 0; it < 16; ++it) {
+                double E[9];
+                mtm3(X, X, E);
+                double en = 0.0;
+                for (int i = 0; i < 9; ++i) {
+                    E[i] -= (i % 4 == 0) ? 1.0 : 0.0;
+                    en += E[i] * E[i];
+                }
+                double Y[9];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        Y[3 * i + j] = X[3 * i + j] - 0.5 * (X[3 * i] * E[j] + X[3 * i + 1] * E[3 + j] +
+                                                             X[3 * i + 2] * E[6 + j]);
+                for (int i = 0; i < 9; ++i) X[i] = Y[i];
+                if (en < 1e-20) break;  // this step took |X^T X - I| from below 1e-10 to the rounding level
+            }
+            for (int i = 0; i < 9; ++i) R[i] = X[i];
+            return 0;
+        }
+    }
     double fn2 = 0.0;
     for (int i = 0; i < 9; ++i) fn2 += F[i] * F[i];
     if (d > 1e-6 * fn2 * sqrt(fn2)) {

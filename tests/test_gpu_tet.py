@@ -73,6 +73,31 @@ def test_element_kernels_vs_oracle(kind):
     assert worst < 1e-12
 
 
+def test_element_kernels_large_stretch_vs_oracle():
+    """Stretches far from a rotation (|F^T F - I| > 1/2): the polar decomposition takes the scaled Newton
+    iteration instead of the Newton-Schulz one used near rotations."""
+    rng = np.random.default_rng(5)
+    cfg = mg.config_corotational((3, 3, 3))
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(oracle.TET_COROTATIONAL, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    x0 = cfg.x0
+    Rg = random_rotation(rng)
+    stretch = np.diag([1.7, 0.6, 1.25])
+    x = (x0 @ stretch.T @ Rg.T + 2e-4 * rng.normal(size=x0.shape)).reshape(-1)
+    R, K, f = g.evaluate_elements(x)
+    worst = 0.0
+    for e in range(cfg.conn.shape[0]):
+        ids = cfg.conn[e]
+        xe0 = x0[ids].reshape(-1)
+        xe = x.reshape(-1, 3)[ids].reshape(-1)
+        Ro, Mo, Ko, fo, flags = oracle.coro_element(xe0, xe, cfg.E[e], cfg.nu[e], cfg.rho[e])
+        assert flags == 0
+        assert np.abs(R[e] - Ro).max() < 1e-13
+        worst = max(worst, np.abs(K[e] - Ko).max() / np.abs(Ko).max())
+    say("large stretch: worst K rel err", worst)
+    assert worst < 1e-12
+
+
 def test_system_matrix_and_rhs_one_step():
     say("system matrix 6x5x4")
     cfg = mg.config_corotational((6, 5, 4))
@@ -144,3 +169,44 @@ def test_trajectory_medium_mesh(precision):
     say("  rel err", e)
     bound = 1e-9 if precision == femgpu.FP64 else 1e-6
     assert e["u"] < bound and e["v"] < bound
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+def test_boundary_conditions_added_between_steps(precision):
+    """Nodes and single DOFs fixed after some steps (OdeState::addBoundaryCondition at any time): the
+    assembly's per-block boundary bits are rebuilt before the next step."""
+    cfg = mg.config_corotational((6, 6, 6))
+    o = make_oracle(cfg, tol=1e-10)
+    g = make_gpu(cfg, precision=precision, tol=1e-10)
+    o.step(cfg.dt, 4)
+    assert g.step(4) == femgpu.FEM_OK
+    x0 = cfg.x0.reshape(-1, 3)
+    top = np.flatnonzero(np.isclose(x0[:, 2], x0[:, 2].max()))
+    for n in top[::3]:
+        o.add_bc_node(int(n))
+        g.add_bc_node(int(n))
+    for n in top[1::3]:
+        o.add_bc_dof(3 * int(n) + 1)
+        g.add_bc_dof(3 * int(n) + 1)
+    say("added BCs on", len(top[::3]), "nodes and", len(top[1::3]), "single DOFs")
+    o.step(cfg.dt, 6)
+    assert g.step(6) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    xs = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - xs, xr - xs), v=rel(v, vr))
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
+
+
+@pytest.mark.parametrize("kind", [femgpu.TET_LINEAR, femgpu.TET_COROTATIONAL])
+def test_tet_repeating_a_node_rejected(kind):
+    cfg = mg.config_corotational((2, 2, 2))
+    conn = cfg.conn.copy()
+    conn[3, 2] = conn[3, 0]
+    g = femgpu.FemGpu(0)
+    with pytest.raises(femgpu.FemError) as err:
+        g.upload_mesh(kind, cfg.x0, conn, cfg.E, cfg.nu, cfg.rho)
+    say("rejected:", err.value)
+    assert err.value.code == femgpu.FEM_ERR_INVALID
