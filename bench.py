@@ -85,16 +85,18 @@ def cg_kernel_name(w):
 
 def spmv_bytes(cfg_sizes, w):
     """Algorithmic bytes of one launch of the dominant CG kernel.
-    cg_fused_kernel: BSR values and indices once; for its own rows it reads r, q, z, w = D q, p, D^-1 and
-    x (7 vectors) and writes r, z, p, q, w and x (6 vectors); the gathers of z, w, p at the columns read
-    the same vectors; 7 block sums read and written.
+    cg_fused_kernel: BSR values and indices once; for its own rows it reads r, q, z, w = B q, p and x
+    (6 vectors) and the preconditioner B (FP64: the diagonal inverse, 1 value per DOF; FP32: the 3x3
+    block-diagonal inverse, 9 values per node = 3 per DOF), and writes r, z, p, q, w and x (6 vectors);
+    the gathers of z, w, p at the columns read the same vectors; 7 block sums read and written.
     cg_apply_kernel: BSR values and indices once, z and p_old read (p = z + beta p_old is formed on the
     fly), p_new and q written."""
     N, D, Bk = cfg_sizes
     Z = 9 * Bk
     if cg_kernel_name(w).startswith("cg_apply"):
         return Z * w + Bk * 4 + (N + 1) * 4 + 4 * D * w
-    return Z * w + Bk * 4 + (N + 1) * 4 + 13 * D * w + 2 * FUSED_SUMS * CG_BLOCKS * 8
+    precond = 3 * D if w == 4 else D
+    return Z * w + Bk * 4 + (N + 1) * 4 + (12 * D + precond) * w + 2 * FUSED_SUMS * CG_BLOCKS * 8
 
 
 def phase_summary(run):

@@ -195,6 +195,7 @@ struct fem_ctx {
     DevBuf<float> S32;          // FP32 copy (FP32 mode)
     DevBuf<uint8_t> bcmask;
     DevBuf<unsigned char> cg_x, cg_r, cg_z, cg_p0, cg_p1, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
+    DevBuf<float> cg_binv;  // FP32 one-kernel CG: 3x3 block-diagonal preconditioner, 9 per node
     DevBuf<unsigned char> cg_r1, cg_z1, cg_q1, cg_w0, cg_w1;                    // fused CG: second buffers
     DevBuf<double> part7;      // [2][kFusedSums * kCgBlocks] block sums of the fused CG kernel, by parity
     DevBuf<double> partial;    // [4 * kCgBlocks] block sums of the grid reductions
@@ -360,6 +361,8 @@ void alloc_precision_buffers(fem_ctx* c) {
         b->alloc(n * tv);
     if (c->kind != FEM_ELEMENT_HEX)
         for (auto* b : {&c->cg_r1, &c->cg_z1, &c->cg_q1, &c->cg_w0, &c->cg_w1}) b->alloc(n * tv);
+    if (c->kind != FEM_ELEMENT_HEX && fp32) c->cg_binv.alloc(3 * n);
+    else c->cg_binv.release();
     if (!c->part7.p) c->part7.alloc(2 * (size_t)kFusedSums * kCgBlocks);
     c->last_precision = prec;
     c->linear_matrix_ready = false;
@@ -398,6 +401,7 @@ CgArgs cg_args(fem_ctx* c, bool fp32, double tol) {
     a.w[0] = c->cg_w0.p;
     a.w[1] = c->cg_w1.p;
     a.part7 = c->part7.p;
+    a.binv = c->cg_binv.p;
     return a;
 }
 
@@ -418,7 +422,10 @@ void launch_solver_init(fem_ctx* c, const CgArgs& a, const double* b, const CgSc
     CgArgs t = a;
     t.r = a.r1;
     t.z = a.z1;
-    launch_cg_init(t, b, skip, c->stream);
+    // FP32 inner solves of the refinement: block-diagonal preconditioner (the outer FP64 test sets the
+    // accuracy); FP64: Eigen's diagonal preconditioner
+    if (a.fp32) launch_cg_init_block(t, b, skip, c->stream);
+    else launch_cg_init(t, b, skip, c->stream);
 }
 
 // Iteration k of an initialised solve (timer tag `tag`): the fused kernel, or the operator + vector pair.

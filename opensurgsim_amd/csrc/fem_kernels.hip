@@ -419,6 +419,56 @@ __global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __re
     }
 }
 
+// FP32 one-kernel iteration: x = 0, r = (float) b, the 3x3 block-diagonal preconditioner
+// B = blockdiag(S)^-1 (inverted in FP64 from the node's diagonal block; a node whose block is missing or
+// numerically singular falls back to the diagonal rule, 1 / S_ii or 1 where S_ii = 0), z = B r; partial
+// |r|^2 and r.z as cg_init_kernel. One thread per node; skipped when skip && skip->done.
+__global__ __launch_bounds__(kCgThreads) void cg_init_block_kernel(Operator op, const float* __restrict__ S,
+                                                                   const double* __restrict__ b, float* __restrict__ x,
+                                                                   float* __restrict__ r, float* __restrict__ z,
+                                                                   float* __restrict__ binv,
+                                                                   double* __restrict__ partial,
+                                                                   const CgScalars* __restrict__ skip) {
+    __shared__ double lds[64];
+    if (skip && skip->done) return;
+    double bb = 0.0, rz = 0.0;
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < op.num_nodes; I += gridDim.x * blockDim.x) {
+        double A[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int db = op.diagblk[I];  // -1: a node in no element (empty row)
+        if (db >= 0)
+            for (int k = 0; k < 9; ++k) A[k] = (double)S[9 * (int64_t)db + k];
+        double Ai[9];
+        const double det = inv3(A, Ai);
+        const double dscale = fabs(A[0] * A[4] * A[8]);
+        if (!(fabs(det) > 1e-12 * dscale) || !(dscale > 0.0)) {
+            for (int k = 0; k < 9; ++k) Ai[k] = 0.0;
+            for (int c = 0; c < 3; ++c) Ai[4 * c] = A[4 * c] != 0.0 ? 1.0 / A[4 * c] : 1.0;
+        }
+        float Bf[9];
+        for (int k = 0; k < 9; ++k) {
+            Bf[k] = (float)Ai[k];
+            binv[9 * (int64_t)I + k] = Bf[k];
+        }
+        float rf[3];
+        for (int c = 0; c < 3; ++c) {
+            rf[c] = (float)b[3 * (int64_t)I + c];
+            x[3 * (int64_t)I + c] = 0.f;
+            r[3 * (int64_t)I + c] = rf[c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const float zc = (Bf[3 * c] * rf[0] + Bf[3 * c + 1] * rf[1]) + Bf[3 * c + 2] * rf[2];
+            z[3 * (int64_t)I + c] = zc;
+            bb += (double)rf[c] * (double)rf[c];
+            rz += (double)rf[c] * (double)zc;
+        }
+    }
+    block_sum2(bb, rz, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bb;
+        partial[kCgBlocks + blockIdx.x] = rz;
+    }
+}
+
 // Iteration k, part 1. k > 0: |r|^2 of iteration k-1 below the threshold ends the solve with
 // iters = k - 1 (Eigen breaks before incrementing), k == max_iter ends it unconverged; otherwise
 // beta = rz_k / rz_{k-1}. Then p_k = z + beta p_{k-1}, formed on the fly for every column the row group
@@ -597,7 +647,11 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
     TV r_o = 0, q_o = 0, z_o = 0, w_o = 0, p_o = 0, x_o = 0, d_o = 0;
     // FP32 holds the row's own inputs across the SpMV (prefetched a row ahead); FP64 would need registers
     // beyond the 64 that keep two workgroups per CU, so it runs the plain row loop and reads them after.
+    // FP32 (inner solves of the refinement) preconditions with the 3x3 block diagonal: dinv holds its
+    // inverse, 9 per node, and lane c reads row c of its node's block (w = B q needs the row's q_0..2,
+    // which every lane of the group has after the row sum). FP64 keeps Eigen's diagonal preconditioner.
     constexpr bool kPrefetch = sizeof(TV) == 4;
+    TV d_o1 = 0, d_o2 = 0;
     auto fetch_row = [&](int row) {
         k0 = op.rowptr[row];
         k1 = op.rowptr[row + 1];
@@ -608,7 +662,10 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
         w_o = wo[i];
         p_o = po[i];
         x_o = x[i];
-        d_o = dinv[i];
+        const TV* bq = dinv + 9 * (int64_t)row + 3 * cl;
+        d_o = bq[0];
+        d_o1 = bq[1];
+        d_o2 = bq[2];
     };
     if (kPrefetch && I < row1) fetch_row(I);
     double alpha = 0.0, beta = 0.0;
@@ -713,7 +770,8 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
             y1 = group_sum<kRowGroup>(y1);
             y2 = group_sum<kRowGroup>(y2);
             const int row = I;
-            const TV ro_i = r_o, qo_i = q_o, zo_i = z_o, wo_i = w_o, po_i = p_o, x_i = x_o, d_i = d_o;
+            const TV ro_i = r_o, qo_i = q_o, zo_i = z_o, wo_i = w_o, po_i = p_o, x_i = x_o;
+            const TV wk = (d_o * y0 + d_o1 * y1) + d_o2 * y2;  // (B q)_c
             I += gpb;
             if (I < row1) fetch_row(I);  // the next row's inputs go out before this row's stores
             if (lane < 3) {
@@ -730,7 +788,6 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
                     pk = zk + bt * po_i;
                     x[i] = x_i + a * po_i;
                 }
-                const TV wk = d_i * y;
                 rn[i] = rk;
                 zn[i] = zk;
                 pn[i] = pk;
@@ -755,7 +812,7 @@ void launch_cg_fused(const CgArgs& a, int k, int max_iter, hipStream_t s) {
     if (a.fp32)
         launch_k((cg_fused_kernel<float>), grid, block, 0, s, a.op, (const float*)a.S, k, max_iter, (float*)a.x,
                  (float*)a.r, (float*)a.r1, (float*)a.z, (float*)a.z1, (float*)a.w[0], (float*)a.w[1],
-                 (float*)a.p[0], (float*)a.p[1], (float*)a.q, (float*)a.q1, (const float*)a.dinv, a.part7, a.sc);
+                 (float*)a.p[0], (float*)a.p[1], (float*)a.q, (float*)a.q1, (const float*)a.binv, a.part7, a.sc);
     else
         launch_k((cg_fused_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, k, max_iter,
                  (double*)a.x, (double*)a.r, (double*)a.r1, (double*)a.z, (double*)a.z1, (double*)a.w[0],
@@ -772,6 +829,12 @@ void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hip
         launch_k((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
                            (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial, skip);
     launch_k(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
+}
+
+void launch_cg_init_block(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {
+    launch_k(cg_init_block_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.op, (const float*)a.S, b, (float*)a.x,
+             (float*)a.r, (float*)a.z, (float*)a.binv, a.partial, skip);
+    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, a.sc, skip, a.tol);
 }
 
 void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {

@@ -116,6 +116,7 @@ struct CgArgs {
     void* q1 = nullptr;
     void* w[2] = {nullptr, nullptr};
     double* part7 = nullptr;  // [2][kFusedSums * kCgBlocks], by iteration parity
+    void* binv = nullptr;     // FP32 one-kernel iteration: 3x3 block-diagonal preconditioner, 9 floats per node
 };
 constexpr int kFusedSums = 7;
 
@@ -123,6 +124,9 @@ constexpr int kFusedSums = 7;
 void launch_cg_fused(const CgArgs& a, int k, int max_iter, hipStream_t s);
 // x = 0, r = (TV) b, z = dinv r and the solve's scalars; skipped when skip && skip->done.
 void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);
+// The same for the FP32 one-kernel iteration with the 3x3 block-diagonal preconditioner: a.binv gets
+// blockdiag(S)^-1 (9 floats per node), z = binv r.
+void launch_cg_init_block(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);
 // Iteration k: convergence test of k - 1, p = z + beta p, q = S p, p . q.
 void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s);
 // Iteration k: alpha, x += alpha p, r -= alpha q, z = dinv r, r . r and r . z.
