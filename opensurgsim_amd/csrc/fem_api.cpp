@@ -510,7 +510,7 @@ void linear_solve(fem_ctx* c) {
     auto inner = [&](int sweep) {
         const int ps = std::min(sweep, 15);
         const int pred = c->predicted_sweep[ps];
-        const int cap = (int)std::min<int64_t>(max_iter, std::max(4, pred + std::max(2, pred / 8)));
+        const int cap = (int)std::min<int64_t>(max_iter, std::max(4, pred + 1));
         TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->r64.p, outer));
         for (int k = 0; k <= cap; ++k) launch_iteration(c, a, k, cap, (sweep << 20) | k);
         TIMED(c, FEM_PHASE_REFINE, 0,
