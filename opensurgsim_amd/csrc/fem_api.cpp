@@ -682,7 +682,7 @@ BatchCg batch_args(fem_ctx* c, bool fp32) {
     a.p[0] = c->cg_p0.p;
     a.p[1] = c->cg_p1.p;
     a.q = c->cg_q.p;
-    a.dinv = c->cg_dinv.p;
+    a.dinv = fp32 ? (void*)c->cg_binv.p : (void*)c->cg_dinv.p;  // FP32: 3x3 block-diagonal preconditioner
     a.partial = c->b_partial.p;
     a.sc = c->b_sc.p;
     a.kA = c->b_kA.p;

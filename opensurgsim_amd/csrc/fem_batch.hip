@@ -5,6 +5,7 @@
 // chunk publishes partial sums for its instance, and every consumer reduces the partials of its own
 // instance in chunk order (deterministic, no atomics). Converged instances skip their chunks.
 #include "fem_batch.hpp"
+#include "fem_device.hpp"
 #include "fem_reduce.hpp"
 
 #include <cfloat>
@@ -74,7 +75,28 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init_kernel(BatchTables t, 
     __shared__ double lds[64];
     const int c = blockIdx.x, inst = t.chunk_inst[c];
     double bb = 0.0, rz = 0.0;
-    if (!(skip && skip[inst].done)) {
+    if constexpr (sizeof(TV) == 4) {
+        // FP32 inner solves of the refinement: 3x3 block-diagonal preconditioner, dinv = 9 floats per node
+        if (!(skip && skip[inst].done)) {
+            for (int I = t.chunk_row0[c] + (int)threadIdx.x; I < t.chunk_row1[c]; I += blockDim.x) {
+                float Bf[9];
+                block_jacobi_inverse(S, op.diagblk[I], Bf);
+                for (int k = 0; k < 9; ++k) dinv[9 * (int64_t)I + k] = Bf[k];
+                float rf[3];
+                for (int cc = 0; cc < 3; ++cc) {
+                    rf[cc] = (float)b[3 * (int64_t)I + cc];
+                    x[3 * (int64_t)I + cc] = 0.f;
+                    r[3 * (int64_t)I + cc] = rf[cc];
+                }
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float zc = (Bf[3 * cc] * rf[0] + Bf[3 * cc + 1] * rf[1]) + Bf[3 * cc + 2] * rf[2];
+                    z[3 * (int64_t)I + cc] = zc;
+                    bb += (double)rf[cc] * (double)rf[cc];
+                    rz += (double)rf[cc] * (double)zc;
+                }
+            }
+        }
+    } else if (!(skip && skip[inst].done)) {
         const int d0 = 3 * t.chunk_row0[c], d1 = 3 * t.chunk_row1[c];
         for (int i = d0 + (int)threadIdx.x; i < d1; i += blockDim.x) {
             const int I = i / 3, cc = i - 3 * I;
@@ -232,15 +254,35 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_vector_kernel(BatchTables t
     const TV a = (TV)(sc[inst].rz[k & 1] / pq);
     const TV* p = (k & 1) ? p1 : p0;
     double rr = 0.0, rz = 0.0;
-    const int d0 = 3 * t.chunk_row0[c], d1 = 3 * t.chunk_row1[c];
-    for (int i = d0 + (int)threadIdx.x; i < d1; i += blockDim.x) {
-        x[i] += a * p[i];
-        const TV ri = r[i] - a * q[i];
-        r[i] = ri;
-        const TV zi = dinv[i] * ri;
-        z[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
+    if constexpr (sizeof(TV) == 4) {
+        // block-diagonal preconditioner: a thread takes a whole node (z needs its three r components)
+        for (int I = t.chunk_row0[c] + (int)threadIdx.x; I < t.chunk_row1[c]; I += blockDim.x) {
+            TV rI[3];
+            for (int cc = 0; cc < 3; ++cc) {
+                const int64_t i = 3 * (int64_t)I + cc;
+                x[i] += a * p[i];
+                rI[cc] = r[i] - a * q[i];
+                r[i] = rI[cc];
+            }
+            const TV* B = dinv + 9 * (int64_t)I;
+            for (int cc = 0; cc < 3; ++cc) {
+                const TV zc = (B[3 * cc] * rI[0] + B[3 * cc + 1] * rI[1]) + B[3 * cc + 2] * rI[2];
+                z[3 * (int64_t)I + cc] = zc;
+                rr += (double)rI[cc] * (double)rI[cc];
+                rz += (double)rI[cc] * (double)zc;
+            }
+        }
+    } else {
+        const int d0 = 3 * t.chunk_row0[c], d1 = 3 * t.chunk_row1[c];
+        for (int i = d0 + (int)threadIdx.x; i < d1; i += blockDim.x) {
+            x[i] += a * p[i];
+            const TV ri = r[i] - a * q[i];
+            r[i] = ri;
+            const TV zi = dinv[i] * ri;
+            z[i] = zi;
+            rr += (double)ri * (double)ri;
+            rz += (double)ri * (double)zi;
+        }
     }
     block_sum2(rr, rz, lds);
     if (threadIdx.x == 0) {

@@ -40,7 +40,7 @@ struct BatchCg {
     void* z;
     void* p[2];
     void* q;
-    void* dinv;
+    void* dinv;        // FP64: 1 / diagonal per DOF; FP32: 3x3 block-diagonal inverse, 9 per node
     double* partial;   // [4 * num_chunks]
     CgScalars* sc;     // [M] inner
     int* kA;           // [M]

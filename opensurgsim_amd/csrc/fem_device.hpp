@@ -341,6 +341,24 @@ __device__ inline double tet_rest_gradients(const double* x0, double g[4][3]) {
 // Flags reported per element.
 enum : int { ELEM_FLAG_DET_R = 1, ELEM_FLAG_G_SINGULAR = 2, ELEM_FLAG_POLAR_EIG = 4 };
 
+// Inverse of a node's 3x3 diagonal block of the FP32 operator (block db, row-major), formed in FP64 and
+// rounded to float: the block-diagonal (block-Jacobi) preconditioner of the FP32 inner solves. A missing
+// block (db < 0, a node in no element) or a numerically singular one falls back to the rule of Eigen's
+// DiagonalPreconditioner, 1 / a_ii (1 where a_ii = 0).
+__device__ inline void block_jacobi_inverse(const float* __restrict__ S, int db, float* Bf) {
+    double A[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (db >= 0)
+        for (int k = 0; k < 9; ++k) A[k] = (double)S[9 * (int64_t)db + k];
+    double Ai[9];
+    const double det = inv3(A, Ai);
+    const double dscale = fabs(A[0] * A[4] * A[8]);
+    if (!(fabs(det) > 1e-12 * dscale) || !(dscale > 0.0)) {
+        for (int k = 0; k < 9; ++k) Ai[k] = 0.0;
+        for (int c = 0; c < 3; ++c) Ai[4 * c] = A[4 * c] != 0.0 ? 1.0 / A[4 * c] : 1.0;
+    }
+    for (int k = 0; k < 9; ++k) Bf[k] = (float)Ai[k];
+}
+
 // Shared per-element state of the corotational stiffness, built once and consumed block by block.
 template <typename TK>
 struct CoroFactors {

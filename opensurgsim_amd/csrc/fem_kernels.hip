@@ -433,22 +433,9 @@ __global__ __launch_bounds__(kCgThreads) void cg_init_block_kernel(Operator op, 
     if (skip && skip->done) return;
     double bb = 0.0, rz = 0.0;
     for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < op.num_nodes; I += gridDim.x * blockDim.x) {
-        double A[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        const int db = op.diagblk[I];  // -1: a node in no element (empty row)
-        if (db >= 0)
-            for (int k = 0; k < 9; ++k) A[k] = (double)S[9 * (int64_t)db + k];
-        double Ai[9];
-        const double det = inv3(A, Ai);
-        const double dscale = fabs(A[0] * A[4] * A[8]);
-        if (!(fabs(det) > 1e-12 * dscale) || !(dscale > 0.0)) {
-            for (int k = 0; k < 9; ++k) Ai[k] = 0.0;
-            for (int c = 0; c < 3; ++c) Ai[4 * c] = A[4 * c] != 0.0 ? 1.0 / A[4 * c] : 1.0;
-        }
         float Bf[9];
-        for (int k = 0; k < 9; ++k) {
-            Bf[k] = (float)Ai[k];
-            binv[9 * (int64_t)I + k] = Bf[k];
-        }
+        block_jacobi_inverse(S, op.diagblk[I], Bf);  // diagblk -1: a node in no element (empty row)
+        for (int k = 0; k < 9; ++k) binv[9 * (int64_t)I + k] = Bf[k];
         float rf[3];
         for (int c = 0; c < 3; ++c) {
             rf[c] = (float)b[3 * (int64_t)I + c];
