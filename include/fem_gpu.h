@@ -42,10 +42,19 @@ enum {
     FEM_ELEMENT_HEX = 2               /* "SurgSim::Physics::Fem3DElementCube" */
 };
 
-/* Integration schemes (Math/OdeSolver.h:36-49). */
+/* Integration schemes (Math/OdeSolver.h:36-49). The schemes after the two implicit ones step
+ * tetrahedral meshes of one representation (not hexahedra, not batched instances). */
 enum {
-    FEM_SCHEME_EULER_IMPLICIT = 0,        /* INTEGRATIONSCHEME_EULER_IMPLICIT */
-    FEM_SCHEME_LINEAR_EULER_IMPLICIT = 1  /* INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT */
+    FEM_SCHEME_EULER_IMPLICIT = 0,                 /* INTEGRATIONSCHEME_EULER_IMPLICIT */
+    FEM_SCHEME_LINEAR_EULER_IMPLICIT = 1,          /* INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT */
+    FEM_SCHEME_STATIC = 2,                         /* INTEGRATIONSCHEME_STATIC (Math/OdeSolverStatic.cpp) */
+    FEM_SCHEME_LINEAR_STATIC = 3,                  /* INTEGRATIONSCHEME_LINEAR_STATIC */
+    FEM_SCHEME_EULER_EXPLICIT = 4,                 /* INTEGRATIONSCHEME_EULER_EXPLICIT */
+    FEM_SCHEME_LINEAR_EULER_EXPLICIT = 5,          /* INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT */
+    FEM_SCHEME_EULER_EXPLICIT_MODIFIED = 6,        /* INTEGRATIONSCHEME_EULER_EXPLICIT_MODIFIED */
+    FEM_SCHEME_LINEAR_EULER_EXPLICIT_MODIFIED = 7, /* INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT_MODIFIED */
+    FEM_SCHEME_RUNGE_KUTTA_4 = 8,                  /* INTEGRATIONSCHEME_RUNGE_KUTTA_4 */
+    FEM_SCHEME_LINEAR_RUNGE_KUTTA_4 = 9            /* INTEGRATIONSCHEME_LINEAR_RUNGE_KUTTA_4 */
 };
 
 /* Arithmetic of the element kernels, assembly and CG. */
@@ -68,6 +77,13 @@ typedef struct fem_options {
     double inner_tolerance;      /* FP32 mode: relative tolerance of each inner FP32 CG solve */
     int32_t max_refinements;     /* FP32 mode: maximum FP64 refinement sweeps per step */
     int32_t fail_on_nonconvergence; /* 0: silent like Eigen's CG (default) */
+    double rayleigh_mass;        /* Rayleigh damping D = rayleigh_mass M + rayleigh_stiffness K, default 0
+                                    (FemRepresentation::setRayleighDampingMass / setRayleighDampingStiffness,
+                                    Physics/FemRepresentation.cpp:212-220, 510-518, 565-629) */
+    double rayleigh_stiffness;
+    int32_t mass_lumping;        /* 1: row-sum lumped mass in the implicit schemes' updates, default 0
+                                    (FemRepresentation::setMassLumping, Physics/FemRepresentation.cpp:320-323,
+                                    499-508); the other schemes' mass is never lumped (:412-421) */
 } fem_options;
 
 typedef struct fem_stats {
@@ -114,6 +130,16 @@ int fem_set_options(fem_ctx* ctx, const fem_options* opt);
  * Replaces DeformableRepresentation::addExternalGeneralizedForce (Physics/Fem3DRepresentation.cpp:133-198)
  * for constant nodal loads. */
 int fem_set_external_force(fem_ctx* ctx, const double* force);
+
+/* Localized external generalized force for the next step only: force (3) spread over the nodes of
+ * `element` with weights coordinate[a] (its natural coordinates, num_nodes of the element), and optional
+ * 3x3 row-major stiffness and damping (NULL: none) added on node blocks (a, b) with weight
+ * coordinate[a] * coordinate[b]. Calls accumulate; the terms are cleared after the next step, as
+ * DeformableRepresentation::afterUpdate clears them. Replaces Fem3DRepresentation::addExternalGeneralizedForce
+ * (Physics/Fem3DRepresentation.cpp:133-198; Physics/DeformableRepresentation.cpp:236-253). Stiffness and
+ * damping need a tetrahedral mesh and a scheme that is not one of the linear (constant-operator) variants. */
+int fem_add_external_generalized_force(fem_ctx* ctx, int64_t element, const double* coordinate, const double* force,
+                                       const double* stiffness, const double* damping);
 
 /* State transfer (x, v of ndof values each; NULL leaves one unchanged). Replaces
  * DeformableRepresentation::setInitialState / getCurrentState (Physics/DeformableRepresentation.cpp:86-119). */

@@ -17,10 +17,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -209,6 +211,19 @@ struct fem_ctx {
     int64_t stage_len = 0;  // doubles of the block contribution staging
     int64_t num_blocks = 0;
     bool have_fext = false;
+
+    // localized external terms of the next step (fem_add_external_generalized_force)
+    std::vector<int32_t> h_conn;   // element connectivity, num_elements x nn
+    std::vector<double> h_loc_f;   // force, ndof
+    std::map<std::pair<int64_t, int64_t>, std::array<double, 18>> h_loc_blk;  // node block -> K (9), D (9)
+    bool loc_pending = false;
+    DevBuf<int> ext_blk, ext_row_dof, ext_row_ptr, ext_col_dof;
+    DevBuf<double> ext_add, ext_kval, ext_fval, ext_row_dt, fstep;
+    int ext_nblk = 0, ext_nrows = 0;
+    bool fstep_set = false;
+
+    // Runge-Kutta 4 stage vectors
+    DevBuf<double> rk_x, rk_k2v, rk_k3v, rk_k4v, rk_a1, rk_a2, rk_a3;
 
     // hexahedra (element-by-element path)
     HexData hex;
@@ -563,19 +578,39 @@ void linear_solve(fem_ctx* c) {
     c->stats.last_refinements = sweeps;
 }
 
-void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs) {
+// Element-kernel mode of an integration scheme: 0 implicit Euler (M / dt + D + dt K), 1 explicit schemes
+// and Runge-Kutta 4 (M / dt), 2 static (K).
+int scheme_mode(int scheme) {
+    if (scheme == FEM_SCHEME_STATIC || scheme == FEM_SCHEME_LINEAR_STATIC) return 2;
+    if (scheme >= FEM_SCHEME_EULER_EXPLICIT) return 1;
+    return 0;
+}
+// The linear variants keep the first step's system (Math/OdeSolverLinear*.cpp).
+bool scheme_is_linear(int scheme) {
+    return scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT || scheme == FEM_SCHEME_LINEAR_STATIC ||
+           scheme == FEM_SCHEME_LINEAR_EULER_EXPLICIT || scheme == FEM_SCHEME_LINEAR_EULER_EXPLICIT_MODIFIED ||
+           scheme == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4;
+}
+
+// Element kernel at state (xn, vn) (null: the current state); xs, vs: the step's start state (Newton terms).
+void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs, const double* xn = nullptr,
+                  const double* vn = nullptr) {
     TetElementArgs a;
     a.num_elements = (int)c->num_elements;
     a.conn = c->conn4.p;
     a.x0 = c->x0.p;
-    a.xn = c->x.p;
-    a.vn = c->v.p;
+    a.xn = xn ? xn : c->x.p;
+    a.vn = vn ? vn : c->v.p;
     a.xs = xs;
     a.vs = vs;
     a.mat = c->mat.p;
     a.dt = c->opt.dt;
     a.write_matrix = write_matrix ? 1 : 0;
-    a.general = (xs != c->x.p || vs != c->v.p) ? 1 : 0;
+    a.mode = scheme_mode(c->opt.scheme);
+    a.ray_mass = c->opt.rayleigh_mass;
+    a.ray_stiff = c->opt.rayleigh_stiffness;
+    a.lumped = (a.mode == 0 && c->opt.mass_lumping) ? 1 : 0;
+    a.general = (a.mode == 0 && (xs != a.xn || vs != a.vn)) ? 1 : 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
     a.slot_off = c->slot_off.p;
@@ -586,15 +621,158 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
 
 void assemble_tet(fem_ctx* c) {
     if (c->timer.enabled) c->timer.begin(c->stream, FEM_PHASE_ASSEMBLE);
+    float* s32 = c->opt.precision == FEM_PRECISION_FP32_REFINED ? c->S32.p : nullptr;
     launch_assemble(c->num_lower_off, c->num_lower - c->num_lower_off, c->lb_info.p, c->ctr_ptr.p,
-                    (int)c->nctr_off, c->rec.p, c->S64.p,
-                    c->opt.precision == FEM_PRECISION_FP32_REFINED ? c->S32.p : nullptr, c->stream);
+                    (int)c->nctr_off, c->rec.p, c->S64.p, s32, c->stream);
+    if (c->ext_nblk) launch_ext_operator(c->ext_nblk, c->ext_blk.p, c->ext_add.p, c->S64.p, s32, c->stream);
     if (c->timer.enabled) c->timer.end(c->stream);
 }
 
-void rhs_tet(fem_ctx* c) {
+// rhs = (staged element rows + gravity + external forces) / div with boundary rows zeroed, plus the step's
+// localized external rows; (xn, vn, xs, vs) as for tet_elements.
+void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const double* vn = nullptr,
+             const double* xs = nullptr, const double* vs = nullptr) {
     TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->node_ptr.p, c->rhs_rec.p, c->fg.p,
-                                          c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream));
+                                          c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream, div));
+    if (c->ext_nrows) {
+        xn = xn ? xn : c->x.p;
+        vn = vn ? vn : c->v.p;
+        xs = xs ? xs : xn;
+        vs = vs ? vs : vn;
+        const int general = (xs != xn || vs != vn) ? 1 : 0;
+        launch_ext_rhs(c->ext_nrows, c->ext_row_dof.p, c->ext_row_ptr.p, c->ext_col_dof.p, c->ext_kval.p,
+                       c->ext_fval.p, c->ext_row_dt.p, xn, xs, vn, general, div, c->b.p, c->stream);
+    }
+}
+
+// Device lists of the localized external terms for the next step (Physics/Fem3DRepresentation.cpp:133-198 and
+// their use in computeFMDK / computeF / computeK, Physics/FemRepresentation.cpp:396-410, 460-477, 527-545):
+//  - operator blocks: D + dt K (implicit Euler) or K (static) under Eigen's lower-triangle view (strictly
+//    lower node blocks and their mirrors; the diagonal block's lower part mirrored), boundary rows and
+//    columns left as assembled;
+//  - right-hand-side rows: the force, plus K (xn - xs - vn dt) with the full external K (implicit Euler).
+void prepare_external(fem_ctx* c) {
+    c->ext_nblk = c->ext_nrows = 0;
+    c->fstep_set = false;
+    if (!c->loc_pending) return;
+    const size_t n = (size_t)c->ndof;
+    if (c->h_loc_f.size() != n) c->h_loc_f.assign(n, 0.0);
+    if (c->kind == FEM_ELEMENT_HEX) {
+        c->fstep.alloc(n);
+        c->fstep.upload(c->h_loc_f.data(), n, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->fstep_set = true;
+        return;
+    }
+    const int mode = scheme_mode(c->opt.scheme);
+    if (!c->h_loc_blk.empty() && scheme_is_linear(c->opt.scheme))
+        throw InvalidArg("external stiffness and damping are not supported by the linear (constant-operator) schemes");
+    auto node_dt = [&](int64_t node) {
+        if (c->num_instances > 0 && !c->h_inst_dt.empty()) {
+            const int i = (int)(std::upper_bound(c->h_node_off.begin(), c->h_node_off.end(), node) -
+                                c->h_node_off.begin()) - 1;
+            return c->h_inst_dt[i];
+        }
+        return c->opt.dt;
+    };
+    auto block_of = [&](int64_t I, int64_t J) -> int64_t {
+        const int32_t* b = c->h_colidx_b.data() + c->h_rowptr_b[I];
+        const int32_t* e = c->h_colidx_b.data() + c->h_rowptr_b[I + 1];
+        const int32_t* it = std::lower_bound(b, e, (int32_t)J);
+        if (it == e || *it != J) throw InvalidArg("external block outside the mesh pattern");
+        return (int64_t)(it - c->h_colidx_b.data());
+    };
+    const uint8_t* bc = c->h_bcmask.data();
+    std::map<int64_t, std::array<double, 9>> add;
+    if (mode != 1) {
+        for (const auto& kv : c->h_loc_blk) {
+            const int64_t I = kv.first.first, J = kv.first.second;
+            if (I < J) continue;
+            const double dt = node_dt(I);
+            double W[9];
+            for (int k = 0; k < 9; ++k) W[k] = mode == 0 ? kv.second[9 + k] + dt * kv.second[k] : kv.second[k];
+            if (I > J) {
+                auto& lo = add[block_of(I, J)];
+                auto& up = add[block_of(J, I)];
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q)
+                        if (!bc[3 * I + r] && !bc[3 * J + q]) {
+                            lo[3 * r + q] += W[3 * r + q];
+                            up[3 * q + r] += W[3 * r + q];
+                        }
+            } else {
+                auto& d = add[block_of(I, I)];
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q)
+                        if (!bc[3 * I + r] && !bc[3 * I + q]) d[3 * r + q] += r >= q ? W[3 * r + q] : W[3 * q + r];
+            }
+        }
+    }
+    std::vector<int> blk;
+    std::vector<double> vals;
+    for (const auto& kv : add) {
+        blk.push_back((int)kv.first);
+        vals.insert(vals.end(), kv.second.begin(), kv.second.end());
+    }
+    std::map<int64_t, std::vector<std::pair<int, double>>> rows;
+    for (size_t d = 0; d < n; ++d)
+        if (c->h_loc_f[d] != 0.0) rows[(int64_t)d];
+    if (mode == 0)
+        for (const auto& kv : c->h_loc_blk) {
+            const int64_t I = kv.first.first, J = kv.first.second;
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q)
+                    if (kv.second[3 * r + q] != 0.0) rows[3 * I + r].push_back({(int)(3 * J + q), kv.second[3 * r + q]});
+        }
+    std::vector<int> row_dof, row_ptr(1, 0), col_dof;
+    std::vector<double> kval, fval, row_dt;
+    for (const auto& kv : rows) {
+        if (bc[kv.first]) continue;
+        row_dof.push_back((int)kv.first);
+        fval.push_back(c->h_loc_f[kv.first]);
+        row_dt.push_back(node_dt(kv.first / 3));
+        for (const auto& e : kv.second) {
+            col_dof.push_back(e.first);
+            kval.push_back(e.second);
+        }
+        row_ptr.push_back((int)col_dof.size());
+    }
+    if (!blk.empty()) {
+        c->ext_blk.alloc(blk.size());
+        c->ext_blk.upload(blk.data(), blk.size(), c->stream);
+        c->ext_add.alloc(vals.size());
+        c->ext_add.upload(vals.data(), vals.size(), c->stream);
+    }
+    if (!row_dof.empty()) {
+        c->ext_row_dof.alloc(row_dof.size());
+        c->ext_row_dof.upload(row_dof.data(), row_dof.size(), c->stream);
+        c->ext_row_ptr.alloc(row_ptr.size());
+        c->ext_row_ptr.upload(row_ptr.data(), row_ptr.size(), c->stream);
+        c->ext_fval.alloc(fval.size());
+        c->ext_fval.upload(fval.data(), fval.size(), c->stream);
+        c->ext_row_dt.alloc(row_dt.size());
+        c->ext_row_dt.upload(row_dt.data(), row_dt.size(), c->stream);
+        c->ext_col_dof.alloc(std::max<size_t>(1, col_dof.size()));
+        c->ext_kval.alloc(std::max<size_t>(1, kval.size()));
+        if (!col_dof.empty()) {
+            c->ext_col_dof.upload(col_dof.data(), col_dof.size(), c->stream);
+            c->ext_kval.upload(kval.data(), kval.size(), c->stream);
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));  // host tables are freed on return
+    c->ext_nblk = (int)blk.size();
+    c->ext_nrows = (int)row_dof.size();
+}
+
+// DeformableRepresentation::afterUpdate (Physics/DeformableRepresentation.cpp:236-253): the localized
+// external terms last one step.
+void clear_external(fem_ctx* c) {
+    if (!c->loc_pending) return;
+    std::fill(c->h_loc_f.begin(), c->h_loc_f.end(), 0.0);
+    c->h_loc_blk.clear();
+    c->loc_pending = false;
+    c->ext_nblk = c->ext_nrows = 0;
+    c->fstep_set = false;
 }
 
 void state_update(fem_ctx* c, const double* xs) {
@@ -903,6 +1081,9 @@ int batch_step(fem_ctx* c) {
     a.flag_counts = c->counters.p;
     a.elem_inst = c->elem_inst.p;
     a.inst_dt = c->b_inst_dt.p;
+    a.ray_mass = c->opt.rayleigh_mass;
+    a.ray_stiff = c->opt.rayleigh_stiffness;
+    a.lumped = c->opt.mass_lumping ? 1 : 0;
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
     assemble_tet(c);
     rhs_tet(c);
@@ -931,8 +1112,61 @@ int batch_step(fem_ctx* c) {
     return FEM_OK;
 }
 
-// One implicit step (OdeSolverEulerImplicit::solve with computeCompliance = false, or
-// OdeSolverLinearEulerImplicit::solve). Returns FEM_OK or FEM_INACTIVE.
+// Static, explicit Euler, modified explicit Euler and Runge-Kutta 4 steps and their linear variants on the
+// assembled tetrahedral operator (Math/OdeSolverStatic.cpp:32-70, Math/OdeSolverEulerExplicit.cpp:32-70,
+// Math/OdeSolverEulerExplicitModified.cpp:32-71, Math/OdeSolverRungeKutta4.cpp:32-112 and
+// Math/OdeSolverLinear*.cpp). The system matrix is M / dt (K for static); the linear variants keep the first
+// step's matrix and solve later steps against it (the compliance of DeformableRepresentation::applyCompliance,
+// Physics/DeformableRepresentation.cpp:175-190: boundary rows of the right-hand side and of the solution zero).
+void step_other_scheme(fem_ctx* c) {
+    const int sc = c->opt.scheme;
+    const double dt = c->opt.dt;
+    const int n = (int)c->ndof;
+    const bool linear = scheme_is_linear(sc);
+    const bool reuse = linear && c->linear_matrix_ready && c->matrix_dt == dt;
+    double* x = c->x.p;
+    double* v = c->v.p;
+    if (sc == FEM_SCHEME_RUNGE_KUTTA_4 || sc == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4) {
+        for (DevBuf<double>* b : {&c->rk_x, &c->rk_k2v, &c->rk_k3v, &c->rk_k4v, &c->rk_a1, &c->rk_a2, &c->rk_a3})
+            if (b->n != (size_t)n) b->alloc(n);
+        const size_t bytes = (size_t)n * sizeof(double);
+        // k1 = (v, M^-1 F(x, v)): the rhs is F / dt against M / dt
+        tet_elements(c, !reuse, x, v);
+        if (!reuse) assemble_tet(c);
+        rhs_tet(c, dt);
+        linear_solve(c);
+        HIPCHECK(hipMemcpyAsync(c->rk_a1.p, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        auto stage = [&](const double* kv, const double* ka, double div, double* vstage, double* aout) {
+            launch_rk4_stage(n, x, v, kv, ka, dt, div, c->rk_x.p, vstage, c->stream);
+            tet_elements(c, false, c->rk_x.p, vstage, c->rk_x.p, vstage);
+            rhs_tet(c, dt, c->rk_x.p, vstage);
+            linear_solve(c);
+            if (aout) HIPCHECK(hipMemcpyAsync(aout, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        };
+        stage(v, c->rk_a1.p, 2.0, c->rk_k2v.p, c->rk_a2.p);
+        stage(c->rk_k2v.p, c->rk_a2.p, 2.0, c->rk_k3v.p, c->rk_a3.p);
+        stage(c->rk_k3v.p, c->rk_a3.p, 1.0, c->rk_k4v.p, nullptr);  // a4 stays in dv
+        TIMED(c, FEM_PHASE_UPDATE, 0,
+              launch_rk4_final(n, x, v, c->rk_k2v.p, c->rk_k3v.p, c->rk_k4v.p, c->rk_a1.p, c->rk_a2.p, c->rk_a3.p,
+                               c->dv.p, dt, c->counters.p + 3, c->stream));
+    } else {
+        tet_elements(c, !reuse, x, v);
+        if (!reuse) assemble_tet(c);
+        rhs_tet(c);
+        linear_solve(c);
+        const int umode = (sc == FEM_SCHEME_STATIC || sc == FEM_SCHEME_LINEAR_STATIC) ? 2
+                          : (sc == FEM_SCHEME_EULER_EXPLICIT || sc == FEM_SCHEME_LINEAR_EULER_EXPLICIT) ? 1 : 0;
+        TIMED(c, FEM_PHASE_UPDATE, 0,
+              launch_update_scheme(n, umode, c->dv.p, x, x, v, dt, c->counters.p + 3, c->stream));
+    }
+    if (linear && !reuse) {
+        c->linear_matrix_ready = true;
+        c->matrix_dt = dt;
+    }
+}
+
+// One step of the configured scheme (OdeSolver::solve with computeCompliance = false). Returns FEM_OK or
+// FEM_INACTIVE.
 int step_once(fem_ctx* c) {
     if (c->num_instances > 0) return batch_step(c);
     const double dt = c->opt.dt;
@@ -947,6 +1181,7 @@ int step_once(fem_ctx* c) {
         a.v = c->v.p;
         a.fg = c->fg.p;
         a.fext = c->have_fext ? c->fext.p : nullptr;
+        a.fstep = c->fstep_set ? c->fstep.p : nullptr;
         a.bcmask = c->bcmask.p;
         a.mat = c->mat.p;
         a.b = c->b.p;
@@ -966,6 +1201,8 @@ int step_once(fem_ctx* c) {
         a.max_iter = (int)max_iters(c);
         a.predicted = c->predicted_hex;
         hex_step(c->hex, a, c->stats, c->stream);
+    } else if (c->opt.scheme >= FEM_SCHEME_STATIC) {
+        step_other_scheme(c);
     } else if (linear_scheme && c->linear_matrix_ready && c->matrix_dt == dt) {
         // Steps >= 2 of OdeSolverLinearEulerImplicit: element forces only, constant operator.
         tet_elements(c, false, c->x.p, c->v.p);
@@ -990,7 +1227,7 @@ int step_once(fem_ctx* c) {
         for (int it = 0; it < newton; ++it) {
             tet_elements(c, true, c->xs.p, c->vs.p);
             assemble_tet(c);
-            rhs_tet(c);
+            rhs_tet(c, 1.0, c->x.p, c->v.p, c->xs.p, c->vs.p);
             linear_solve(c);
             state_update(c, c->xs.p);
             // ||dv - dv_prev||_inf < epsilon (1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112)
@@ -1280,6 +1517,7 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
             }
         }
         c->h_mass_per_node.assign(num_nodes, 0.0);
+        c->h_conn.assign(conn, conn + (size_t)nn * num_elements);
         c->h_nu.assign(poisson, poisson + num_elements);
         c->h_rho.assign(density, density + num_elements);
         std::vector<double> mat(4 * num_elements);
@@ -1371,15 +1609,21 @@ int fem_set_options(fem_ctx* c, const fem_options* o) {
     return guarded(c, [&] {
         if (!o) throw InvalidArg("null options");
         if (!(o->dt > 0.0)) throw InvalidArg("dt must be positive");
-        if (o->scheme != FEM_SCHEME_EULER_IMPLICIT && o->scheme != FEM_SCHEME_LINEAR_EULER_IMPLICIT)
+        if (o->scheme < FEM_SCHEME_EULER_IMPLICIT || o->scheme > FEM_SCHEME_LINEAR_RUNGE_KUTTA_4)
             throw InvalidArg("unknown integration scheme");
+        if (!std::isfinite(o->rayleigh_mass) || !std::isfinite(o->rayleigh_stiffness))
+            throw InvalidArg("Rayleigh damping coefficients must be finite");
+        if (scheme_mode(o->scheme) != 0 && o->mass_lumping && o->rayleigh_mass != 0.0)
+            throw InvalidArg("mass lumping with Rayleigh mass damping needs an implicit Euler scheme");
         if (o->precision != FEM_PRECISION_FP64 && o->precision != FEM_PRECISION_FP32_REFINED)
             throw InvalidArg("unknown precision");
         if (o->newton_max_iterations < 1) throw InvalidArg("The maximum iteration needs to be at least 1");
         if (!(o->cg_tolerance >= 0.0)) throw InvalidArg("negative tolerance");
         bool grav = o->gravity_enabled != c->opt.gravity_enabled || o->gravity[0] != c->opt.gravity[0] ||
                     o->gravity[1] != c->opt.gravity[1] || o->gravity[2] != c->opt.gravity[2];
-        if (o->scheme != c->opt.scheme || o->dt != c->opt.dt) c->linear_matrix_ready = false;
+        if (o->scheme != c->opt.scheme || o->dt != c->opt.dt || o->rayleigh_mass != c->opt.rayleigh_mass ||
+            o->rayleigh_stiffness != c->opt.rayleigh_stiffness || o->mass_lumping != c->opt.mass_lumping)
+            c->linear_matrix_ready = false;
         c->opt = *o;
         if (grav) c->fg_dirty = true;
         if (c->mesh_ready) {
@@ -1401,6 +1645,34 @@ int fem_set_external_force(fem_ctx* c, const double* f) {
         } else {
             c->have_fext = false;
         }
+        return FEM_OK;
+    });
+}
+
+int fem_add_external_generalized_force(fem_ctx* c, int64_t element, const double* coordinate, const double* force,
+                                       const double* stiffness, const double* damping) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        if (!coordinate || !force) throw InvalidArg("Invalid localization (null coordinate or force)");
+        if (element < 0 || element >= c->num_elements) throw InvalidArg("Invalid localization (element out of range)");
+        if ((stiffness || damping) && c->kind == FEM_ELEMENT_HEX)
+            throw InvalidArg("external stiffness and damping need a tetrahedral mesh");
+        const int nn = c->nn;
+        if (c->h_loc_f.size() != (size_t)c->ndof) c->h_loc_f.assign((size_t)c->ndof, 0.0);
+        const int32_t* ids = c->h_conn.data() + (size_t)nn * element;
+        for (int a = 0; a < nn; ++a)
+            for (int k = 0; k < 3; ++k) c->h_loc_f[3 * (size_t)ids[a] + k] += force[k] * coordinate[a];
+        if (stiffness || damping)
+            for (int a = 0; a < nn; ++a)
+                for (int b = 0; b < nn; ++b) {
+                    auto& blk = c->h_loc_blk[std::make_pair((int64_t)ids[a], (int64_t)ids[b])];
+                    const double w = coordinate[a] * coordinate[b];
+                    for (int k = 0; k < 9; ++k) {
+                        if (stiffness) blk[k] += w * stiffness[k];
+                        if (damping) blk[9 + k] += w * damping[k];
+                    }
+                }
+        c->loc_pending = true;
         return FEM_OK;
     });
 }
@@ -1458,9 +1730,16 @@ int fem_step(fem_ctx* c, int32_t nsteps) {
         HIPCHECK(hipSetDevice(c->device));
         ensure_bc(c);
         alloc_precision_buffers(c);
+        if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
+            throw InvalidArg("the static, explicit and Runge-Kutta schemes step one tetrahedral representation");
+        if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
+            c->opt.rayleigh_stiffness != 0.0)
+            throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
         for (int i = 0; i < nsteps; ++i) {
             c->timer.enabled = c->timer_every > 0 && (c->timer_count++ % c->timer_every) == 0;
+            prepare_external(c);
             step_once(c);
+            clear_external(c);
             if (c->opt.fail_on_nonconvergence && !c->stats.converged)
                 return fail(c, FEM_ERR_NOT_CONVERGED, "linear solve did not converge");
         }
@@ -1696,7 +1975,7 @@ int fem_set_young_modulus(fem_ctx* c, const double* young) {
         c->mat.upload(reinterpret_cast<const double4*>(mat.data()), c->num_elements, c->stream);
         HIPCHECK(hipStreamSynchronize(c->stream));
         c->linear_matrix_ready = false;  // operators depend on E
-        c->hex.a32_dt = 0.0;
+        c->hex.a32_lumped = -1;
         return FEM_OK;
     });
 }
@@ -1707,6 +1986,9 @@ int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const dou
         if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "mesh not uploaded");
         if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("the adjoint covers tetrahedral meshes");
         if (c->num_instances > 0) throw InvalidArg("the adjoint covers single (non-batched) meshes");
+        if (c->opt.rayleigh_mass != 0.0 || c->opt.rayleigh_stiffness != 0.0 || c->opt.mass_lumping ||
+            scheme_mode(c->opt.scheme) != 0)
+            throw InvalidArg("the adjoint reverses undamped implicit Euler steps with the consistent mass");
         if (c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
             (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1))
             throw InvalidArg("the corotational adjoint reverses one-Newton-iteration implicit Euler steps");

@@ -89,12 +89,13 @@ __host__ __device__ constexpr int blk_b(int k) {
 constexpr int kHexChunks = 81;
 
 // ------------------------------------------------------------------------------------------
-// Set-up: A_e = M_e / dt + dt K_e in FP32, one thread per (element, packed block).
+// Set-up: A_e = cm M_e + ck K_e in FP32 (implicit Euler with Rayleigh damping: cm = 1/dt + alpha_M,
+// ck = dt + alpha_K; lumped: M_e's row sums on the diagonal), one thread per (element, packed block).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void hex_prepare_kernel(int count_pad, const int* __restrict__ order_pad,
                                                           const int* __restrict__ conn,
                                                           const double* __restrict__ x0, const double4* __restrict__ mat,
-                                                          double dt, float* __restrict__ A32) {
+                                                          double cm, double ck, int lumped, float* __restrict__ A32) {
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (tid >= (int64_t)count_pad * 36) return;
     const int t = (int)(tid / 36), kb = (int)(tid % 36);
@@ -112,10 +113,12 @@ __global__ __launch_bounds__(256) void hex_prepare_kernel(int count_pad, const i
                 double N[8], g[8][3];
                 const double w = hex_gp(X, gi, gj, gk, N, g);
                 const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
-                const double ms = w * m.z * N[a] * N[b] / dt;
+                double nn = N[a] * N[b];
+                if (lumped) nn = a == b ? N[a] * (((N[0] + N[1]) + (N[2] + N[3])) + ((N[4] + N[5]) + (N[6] + N[7]))) : 0.0;
+                const double ms = w * m.z * nn * cm;
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j)
-                        A[3 * i + j] += dt * w * (m.x * g[a][i] * g[b][j] + m.y * g[b][i] * g[a][j] +
+                        A[3 * i + j] += ck * w * (m.x * g[a][i] * g[b][j] + m.y * g[b][i] * g[a][j] +
                                                   (i == j ? m.y * gab : 0.0)) +
                                         (i == j ? ms : 0.0);
             }
@@ -126,9 +129,10 @@ __global__ __launch_bounds__(256) void hex_prepare_kernel(int count_pad, const i
     }
 }
 
-// FP64 element operator y = alpha_m M p + alpha_k K p for one element (matrix-free).
-__device__ __forceinline__ void hex_apply64(const double* X, const double4 m, const double* p, double am, double ak,
-                                            double* y) {
+// FP64 element operator y = alpha_m M pm + alpha_k K p for one element (matrix-free; lumped: M's row
+// sums on the diagonal).
+__device__ __forceinline__ void hex_apply64(const double* X, const double4 m, const double* p, const double* pm,
+                                            double am, double ak, int lumped, double* y) {
     for (int i = 0; i < 24; ++i) y[i] = 0.0;
     for (int gi = 0; gi < 2; ++gi)
         for (int gj = 0; gj < 2; ++gj)
@@ -139,9 +143,10 @@ __device__ __forceinline__ void hex_apply64(const double* X, const double4 m, co
                 double np[3] = {0, 0, 0};
                 for (int b = 0; b < 8; ++b)
                     for (int i = 0; i < 3; ++i) {
-                        np[i] += N[b] * p[3 * b + i];
+                        np[i] += N[b] * pm[3 * b + i];
                         for (int j = 0; j < 3; ++j) H[3 * i + j] += p[3 * b + i] * g[b][j];
                     }
+                const double sn = ((N[0] + N[1]) + (N[2] + N[3])) + ((N[4] + N[5]) + (N[6] + N[7]));
                 const double tr = H[0] + H[4] + H[8];
                 double sig[9];
                 for (int i = 0; i < 3; ++i)
@@ -151,33 +156,38 @@ __device__ __forceinline__ void hex_apply64(const double* X, const double4 m, co
                 for (int a = 0; a < 8; ++a)
                     for (int i = 0; i < 3; ++i)
                         y[3 * a + i] += wk * (sig[3 * i] * g[a][0] + sig[3 * i + 1] * g[a][1] + sig[3 * i + 2] * g[a][2]) +
-                                        wm * N[a] * np[i];
+                                        wm * N[a] * (lumped ? sn * pm[3 * a + i] : np[i]);
             }
 }
 
 // y[nodes] += sign * (alpha_m M + alpha_k K) w_e over the elements of one colour (w masked on BC DOFs
-// when mask != null; w = (x - x0) + dt v when v != null, giving the right-hand side).
+// when mask != null). With v != null, the right-hand side terms: K acts on w = (x - x0) + kv v and M on v.
 __global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* __restrict__ order,
                                                           const int* __restrict__ conn, const double* __restrict__ x0,
                                                           const double4* __restrict__ mat, const double* __restrict__ p,
                                                           const double* __restrict__ xcur, const double* __restrict__ v,
-                                                          double dt, const uint8_t* __restrict__ mask, double am,
-                                                          double ak, double sign, double* __restrict__ y,
+                                                          double kv, const uint8_t* __restrict__ mask, double am,
+                                                          double ak, int lumped, double sign, double* __restrict__ y,
                                                           const CgScalars* __restrict__ sc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count || (sc && sc->done)) return;
     const int e = order[t];
     int ids[8];
     for (int i = 0; i < 8; ++i) ids[i] = conn[8 * e + i];
-    double X[24], w[24], out[24];
+    double X[24], w[24], wm[24], out[24];
     for (int i = 0; i < 8; ++i)
         for (int c = 0; c < 3; ++c) {
             const size_t d = 3 * (size_t)ids[i] + c;
             X[3 * i + c] = x0[d];
-            if (v) w[3 * i + c] = (xcur[d] - x0[d]) + dt * v[d];
-            else w[3 * i + c] = (mask && mask[d]) ? 0.0 : p[d];
+            if (v) {
+                w[3 * i + c] = (xcur[d] - x0[d]) + kv * v[d];
+                wm[3 * i + c] = v[d];
+            } else {
+                w[3 * i + c] = (mask && mask[d]) ? 0.0 : p[d];
+                wm[3 * i + c] = w[3 * i + c];
+            }
         }
-    hex_apply64(X, mat[e], w, am, ak, out);
+    hex_apply64(X, mat[e], w, wm, am, ak, lumped, out);
     for (int i = 0; i < 8; ++i)
         for (int c = 0; c < 3; ++c) y[3 * (size_t)ids[i] + c] += sign * out[3 * i + c];
 }
@@ -240,8 +250,8 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
 // FP64 diagonal of A over one colour.
 __global__ __launch_bounds__(128) void hex_diag_kernel(int count, const int* __restrict__ order,
                                                        const int* __restrict__ conn, const double* __restrict__ x0,
-                                                       const double4* __restrict__ mat, double dt,
-                                                       double* __restrict__ diag) {
+                                                       const double4* __restrict__ mat, double cm, double ck,
+                                                       int lumped, double* __restrict__ diag) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int e = order[t];
@@ -258,11 +268,13 @@ __global__ __launch_bounds__(128) void hex_diag_kernel(int count, const int* __r
             for (int gk = 0; gk < 2; ++gk) {
                 double N[8], g[8][3];
                 const double w = hex_gp(X, gi, gj, gk, N, g);
+                const double sn = ((N[0] + N[1]) + (N[2] + N[3])) + ((N[4] + N[5]) + (N[6] + N[7]));
                 for (int a = 0; a < 8; ++a) {
                     const double gg = g[a][0] * g[a][0] + g[a][1] * g[a][1] + g[a][2] * g[a][2];
+                    const double nn = lumped ? N[a] * sn : N[a] * N[a];
                     for (int i = 0; i < 3; ++i)
-                        d[3 * a + i] += dt * w * (m.x * g[a][i] * g[a][i] + m.y * g[a][i] * g[a][i] + m.y * gg) +
-                                        w * m.z * N[a] * N[a] / dt;
+                        d[3 * a + i] += ck * w * (m.x * g[a][i] * g[a][i] + m.y * g[a][i] * g[a][i] + m.y * gg) +
+                                        w * m.z * nn * cm;
                 }
             }
     for (int a = 0; a < 8; ++a)
@@ -275,11 +287,17 @@ __global__ __launch_bounds__(256) void hex_cast_diag_kernel(int n, const double*
     if (i < n) d32[i] = (float)d64[i];
 }
 
-// b = fg + fext before the element forces are subtracted; BC DOFs zeroed afterwards.
+// b = fg + fext (+ the step's localized force) before the element forces are subtracted; BC DOFs zeroed
+// afterwards.
 __global__ __launch_bounds__(256) void hex_rhs_init_kernel(int n, const double* __restrict__ fg,
-                                                           const double* __restrict__ fext, double* __restrict__ b) {
+                                                           const double* __restrict__ fext,
+                                                           const double* __restrict__ fstep, double* __restrict__ b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = fg[i] + (fext ? fext[i] : 0.0);
+    if (i < n) {
+        double v = fg[i] + (fext ? fext[i] : 0.0);
+        if (fstep) v += fstep[i];
+        b[i] = v;
+    }
 }
 __global__ __launch_bounds__(256) void hex_rhs_bc_kernel(int n, const uint8_t* __restrict__ mask,
                                                          double* __restrict__ b) {
@@ -574,34 +592,36 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
     HCHECK(hipMemcpyAsync(h.conn, conn, (size_t)num_elements * 8 * sizeof(int), hipMemcpyHostToDevice, s));
     HCHECK(hipMemcpyAsync(h.order, order.data(), (size_t)num_elements * sizeof(int), hipMemcpyHostToDevice, s));
     HCHECK(hipStreamSynchronize(s));
-    h.a32_dt = 0.0;
+    h.a32_lumped = -1;
     h.ready = true;
 }
 
 namespace {
 
-void hex_prepare(HexData& h, const double* x0, const double4* mat, double dt, hipStream_t s) {
+void hex_prepare(HexData& h, const double* x0, const double4* mat, double cm, double ck, int lumped, hipStream_t s) {
     const int pad = h.pad_ptr.back();
     hipLaunchKernelGGL(hex_prepare_kernel, blocks_for((int64_t)pad * 36, 256), dim3(256), 0, s, pad, h.order_pad,
-                       h.conn, x0, mat, dt, h.A32);
+                       h.conn, x0, mat, cm, ck, lumped, h.A32);
     const int n = (int)(3 * h.num_nodes);
     HCHECK(hipMemsetAsync(h.diag64, 0, (size_t)n * sizeof(double), s));
     for (int c = 0; c < h.num_colors; ++c) {
         const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
         hipLaunchKernelGGL(hex_diag_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
-                           h.conn, x0, mat, dt, h.diag64);
+                           h.conn, x0, mat, cm, ck, lumped, h.diag64);
     }
     hipLaunchKernelGGL(hex_cast_diag_kernel, blocks_for(n, 256), dim3(256), 0, s, n, h.diag64, h.diag32);
-    h.a32_dt = dt;
+    h.a32_cm = cm;
+    h.a32_ck = ck;
+    h.a32_lumped = lumped;
 }
 
 // y += sign (am M + ak K) w over all colours (FP64, matrix-free).
 void apply64(HexData& h, const double* x0, const double4* mat, const double* p, const double* xcur, const double* v,
-             double dt, const uint8_t* mask, double am, double ak, double sign, double* y, hipStream_t s) {
+             double kv, const uint8_t* mask, double am, double ak, int lumped, double sign, double* y, hipStream_t s) {
     for (int c = 0; c < h.num_colors; ++c) {
         const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
         hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
-                           h.conn, x0, mat, p, xcur, v, dt, mask, am, ak, sign, y, (const CgScalars*)nullptr);
+                           h.conn, x0, mat, p, xcur, v, kv, mask, am, ak, lumped, sign, y, (const CgScalars*)nullptr);
     }
 }
 
@@ -610,7 +630,6 @@ template <typename TV>
 int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgScalars* h_sc, int& predicted,
                TV* x, TV* r, TV* z, TV* p, TV* q, TV* dinv, hipStream_t s) {
     const int n = (int)(3 * h.num_nodes);
-    const double dt = a.opt->dt;
     auto iteration = [&](int k) {
         hipLaunchKernelGGL((hex_cg_pupdate_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, k, a.max_iter, z,
                            p, q, a.partial, sc);
@@ -625,7 +644,8 @@ int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgSca
             } else
                 hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt,
                                    h.order + h.color_ptr[c], h.conn, a.x0, a.mat, (const double*)p, nullptr, nullptr,
-                                   dt, a.bcmask, 1.0 / dt, dt, 1.0, (double*)q, (const CgScalars*)sc);
+                                   0.0, a.bcmask, h.a32_cm, h.a32_ck, h.a32_lumped, 1.0, (double*)q,
+                                   (const CgScalars*)sc);
         }
         hipLaunchKernelGGL((hex_cg_bc_pq_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, p, q,
                            a.partial, sc);
@@ -652,10 +672,16 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
     const double dt = o.dt;
     const int n = (int)(3 * h.num_nodes);
     const bool fp32 = o.precision == FEM_PRECISION_FP32_REFINED;
-    if (h.a32_dt != dt) hex_prepare(h, a.x0, a.mat, dt, s);
-    // rhs = f - dt K v + gravity + fext = -K ((x - x0) + dt v) + fg + fext, BC DOFs zeroed
-    hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.b);
-    apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt, nullptr, 0.0, 1.0, -1.0, a.b, s);
+    // A = M (1/dt + alpha_M) + K (dt + alpha_K): implicit Euler with Rayleigh damping
+    // (Math/OdeSolverEulerImplicit.cpp:148, Physics/FemRepresentation.cpp:510-518)
+    const double cm = 1.0 / dt + o.rayleigh_mass, ck = dt + o.rayleigh_stiffness;
+    const int lumped = o.mass_lumping ? 1 : 0;
+    if (h.a32_cm != cm || h.a32_ck != ck || h.a32_lumped != lumped) hex_prepare(h, a.x0, a.mat, cm, ck, lumped, s);
+    // rhs = f - dt K v - alpha_M M v - alpha_K K v + gravity + fext
+    //     = -K ((x - x0) + (dt + alpha_K) v) - alpha_M M v + fg + fext, BC DOFs zeroed
+    hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.fstep, a.b);
+    apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt + o.rayleigh_stiffness, nullptr, o.rayleigh_mass, 1.0, lumped, -1.0,
+            a.b, s);
     hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, a.b);
     CgScalars* sc = a.sc;
     if (!fp32) {
@@ -683,7 +709,7 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
             } else {
                 double* q64 = (double*)a.vec[6];
                 HCHECK(hipMemsetAsync(q64, 0, (size_t)n * sizeof(double), s));
-                apply64(h, a.x0, a.mat, a.dv, nullptr, nullptr, dt, a.bcmask, 1.0 / dt, dt, 1.0, q64, s);
+                apply64(h, a.x0, a.mat, a.dv, nullptr, nullptr, 0.0, a.bcmask, cm, ck, lumped, 1.0, q64, s);
                 hipLaunchKernelGGL(hex_residual_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, a.b,
                                    a.dv, q64, a.r64, a.partial);
             }

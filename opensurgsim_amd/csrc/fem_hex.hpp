@@ -30,7 +30,8 @@ struct HexData {
                                   // position, interleaved by 64-position tiles in float4 chunks
     double* diag64 = nullptr;     // diagonal of A (per DOF)
     float* diag32 = nullptr;
-    double a32_dt = 0.0;          // dt the cached A32 / diagonal were built for
+    double a32_cm = 0.0, a32_ck = 0.0;  // coefficients of M and K the cached A32 / diagonal were built for
+    int a32_lumped = -1;                // lumped mass flag of the cache (-1: not built)
     bool ready = false;
 };
 
@@ -50,6 +51,7 @@ struct HexStepArgs {
     double* v;
     const double* fg;       // gravity m_i g
     const double* fext;     // may be null
+    const double* fstep = nullptr;  // localized force of this step (may be null)
     const uint8_t* bcmask;
     const double4* mat;
     double* b;              // rhs

@@ -138,9 +138,15 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
             ra[2] = -tt[2];
         }
     }
-    // Mass: M_e = R M0 R^T = rho V / 20 (2 on the diagonal blocks, 1 off) x I3.
+    // Mass: M_e = R M0 R^T = rho V / 20 (2 on the diagonal blocks, 1 off) x I3; lumped: the row sum
+    // 5 rho V / 20 on the diagonal blocks (Physics/FemRepresentation.cpp:499-508).
     const double mc = rho * V / 20.0;
     const double idt = 1.0 / dt;
+    // operator coefficients of M and K, and the coefficient of K vn in the rhs, by scheme
+    // (D = alpha_M M + alpha_K K, Physics/FemRepresentation.cpp:510-518, 565-629)
+    const double cm = A.mode == 0 ? idt + A.ray_mass : (A.mode == 1 ? idt : 0.0);
+    const double ck = A.mode == 0 ? dt + A.ray_stiff : (A.mode == 1 ? 0.0 : 1.0);
+    const double kv = A.mode == 0 ? dt + A.ray_stiff : A.ray_stiff;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         double gb[3];
@@ -154,34 +160,37 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
             linear_row_block(ga, gb, V, lam, mu, K);
         }
         // rhs = f + K (xn - xs - vn dt) - M (vn - vs) / dt   (Math/OdeSolverEulerImplicit.cpp:157-158)
-        double d1[3], d2[3];
+        double d1[3], d2[3], vb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const size_t i = 3 * (size_t)ids[b] + c;
             const double vv = A.vn[i];
+            vb[c] = vv;
             if (GENERAL) {
-                d1[c] = (A.xn[i] - A.xs[i]) - vv * dt;
+                d1[c] = (A.xn[i] - A.xs[i]) - vv * kv;
                 d2[c] = vv - A.vs[i];
             } else {
-                d1[c] = -(vv * dt);
+                d1[c] = -(vv * kv);
                 d2[c] = 0.0;
             }
         }
-        const double mab = (a == b) ? 2.0 * mc : mc;
+        const double mab = A.lumped ? (a == b ? 5.0 * mc : 0.0) : ((a == b) ? 2.0 * mc : mc);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             ra[i] += K[3 * i] * d1[0] + K[3 * i + 1] * d1[1] + K[3 * i + 2] * d1[2];
             if (GENERAL) ra[i] -= (mab * d2[i]) / dt;
+            if (A.ray_mass != 0.0) ra[i] -= A.ray_mass * (mab * vb[i]);
         }
         if (!A.write_matrix) continue;
-        // A_ab = M_ab * (1/dt) + K_ab * dt   (D = 0 for 3D elements, Physics/FemElement.cpp:131-145)
-        const double mdt = mab * idt;
+        // A_ab = M_ab cm + K_ab ck   (implicit: M / dt + D + dt K, Math/OdeSolverEulerImplicit.cpp:148; the
+        // elements' own damping is zero for 3D elements, Physics/FemElement.cpp:131-145)
+        const double mdt = mab * cm;
         if (a == b) {
             const int sd = a == 0 ? slot[6] : a == 1 ? slot[7] : a == 2 ? slot[8] : slot[9];
             double2* o = reinterpret_cast<double2*>(A.rec + sd);  // records start on 16 bytes
-            o[0] = make_double2(mdt + K[0] * dt, K[3] * dt);
-            o[1] = make_double2(mdt + K[4] * dt, K[6] * dt);
-            o[2] = make_double2(K[7] * dt, mdt + K[8] * dt);
+            o[0] = make_double2(mdt + K[0] * ck, K[3] * ck);
+            o[1] = make_double2(mdt + K[4] * ck, K[6] * ck);
+            o[2] = make_double2(K[7] * ck, mdt + K[8] * ck);
         } else if (ida > ids[b]) {
             const int ps = (a < b) ? pair_slot(a, b) : pair_slot(b, a);
             int sp = slot[0];
@@ -191,7 +200,7 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) v[3 * i + j] = ((i == j) ? mdt : 0.0) + K[3 * i + j] * dt;
+                for (int j = 0; j < 3; ++j) v[3 * i + j] = ((i == j) ? mdt : 0.0) + K[3 * i + j] * ck;
             v[9] = 0.0;
             double2* o = reinterpret_cast<double2*>(A.rec + sp);
 #pragma unroll
@@ -295,21 +304,64 @@ void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* 
 __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __restrict__ node_ptr,
                                                   const double* __restrict__ stage, const double* __restrict__ fg,
                                                   const double* __restrict__ fext, const uint8_t* __restrict__ bcmask,
-                                                  double* __restrict__ b) {
+                                                  double* __restrict__ b, double div) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (i >= 3 * (int64_t)num_nodes) return;
     const int I = (int)(i / 3), k = (int)(i - 3 * (int64_t)I);
     const double acc = sum_strided(stage + k, kRhsRec, node_ptr[I], node_ptr[I + 1]);
     double v = acc + fg[i];
     if (fext) v += fext[i];
-    b[i] = bcmask[i] ? 0.0 : v;
+    b[i] = bcmask[i] ? 0.0 : v / div;
+}
+
+__global__ __launch_bounds__(256) void ext_operator_kernel(int count, const int* __restrict__ blk,
+                                                           const double* __restrict__ add, double* __restrict__ S64,
+                                                           float* __restrict__ S32) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * (int64_t)count) return;
+    const size_t o = 9 * (size_t)blk[t / 9] + (size_t)(t % 9);
+    const double v = S64[o] + add[t];
+    S64[o] = v;
+    if (S32) S32[o] = (float)v;
+}
+
+__global__ __launch_bounds__(256) void ext_rhs_kernel(int rows, const int* __restrict__ row_dof,
+                                                      const int* __restrict__ row_ptr, const int* __restrict__ col_dof,
+                                                      const double* __restrict__ kval, const double* __restrict__ fval,
+                                                      const double* __restrict__ row_dt, const double* __restrict__ xn,
+                                                      const double* __restrict__ xs, const double* __restrict__ vn,
+                                                      int general, double div, double* __restrict__ b) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const double dt = row_dt[r];
+    double acc = 0.0;
+    for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+        const int j = col_dof[p];
+        const double w = general ? (xn[j] - xs[j]) - vn[j] * dt : -(vn[j] * dt);
+        acc += kval[p] * w;
+    }
+    b[row_dof[r]] += (fval[r] + acc) / div;
+}
+
+void launch_ext_operator(int count, const int* blk, const double* add, double* S64, float* S32, hipStream_t s) {
+    if (count <= 0) return;
+    launch_k(ext_operator_kernel, dim3((unsigned)((9 * (int64_t)count + 255) / 256)), dim3(256), 0, s, count, blk, add,
+             S64, S32);
+}
+
+void launch_ext_rhs(int rows, const int* row_dof, const int* row_ptr, const int* col_dof, const double* kval,
+                    const double* fval, const double* row_dt, const double* xn, const double* xs, const double* vn,
+                    int general, double div, double* b, hipStream_t s) {
+    if (rows <= 0) return;
+    launch_k(ext_rhs_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, rows, row_dof, row_ptr, col_dof, kval, fval,
+             row_dt, xn, xs, vn, general, div, b);
 }
 
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
-                const uint8_t* bcmask, double* b, hipStream_t s) {
+                const uint8_t* bcmask, double* b, hipStream_t s, double div) {
     if (num_nodes == 0) return;
     launch_k(rhs_kernel, xcd_grid(3 * (int64_t)num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, stage, fg,
-             fext, bcmask, b);
+             fext, bcmask, b, div);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -984,6 +1036,76 @@ void launch_update(int n, const double* dv, const double* xs, double* x, double*
 void launch_update(int n, const double* dv, const double* xs, double* x, double* v, double dt, int* invalid,
                    hipStream_t s) {
     launch_update(n, dv, xs, x, v, dt, invalid, s, nullptr, nullptr);
+}
+
+// Math/OdeSolverEulerExplicit.cpp:37-38 (mode 1), Math/OdeSolverStatic.cpp:37-38 (mode 2); mode 0 is the
+// implicit and modified explicit update (Math/OdeSolverEulerImplicit.cpp:98-99,
+// Math/OdeSolverEulerExplicitModified.cpp:38-39).
+__global__ __launch_bounds__(256) void update_scheme_kernel(int n, int mode, const double* __restrict__ dv,
+                                                            const double* xs, double* x, double* v, double dt,
+                                                            int* invalid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double vi, xi;
+    if (mode == 1) {
+        xi = xs[i] + dt * v[i];
+        vi = v[i] + dv[i];
+    } else if (mode == 2) {
+        xi = xs[i] + dv[i];
+        vi = 0.0;
+    } else {
+        vi = v[i] + dv[i];
+        xi = xs[i] + dt * vi;
+    }
+    v[i] = vi;
+    x[i] = xi;
+    if (!isfinite(vi) || !isfinite(xi)) atomicOr(invalid, 1);
+}
+
+void launch_update_scheme(int n, int mode, const double* dv, const double* xs, double* x, double* v, double dt,
+                          int* invalid, hipStream_t s) {
+    if (n == 0) return;
+    launch_k(update_scheme_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, mode, dv, xs, x, v, dt, invalid);
+}
+
+// Math/OdeSolverRungeKutta4.cpp:52-87: stage states y + k dt / 2 and y + k dt.
+__global__ __launch_bounds__(256) void rk4_stage_kernel(int n, const double* __restrict__ x,
+                                                        const double* __restrict__ v, const double* __restrict__ kv,
+                                                        const double* __restrict__ ka, double dt, double div,
+                                                        double* __restrict__ xo, double* __restrict__ vo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    xo[i] = x[i] + kv[i] * dt / div;
+    vo[i] = v[i] + ka[i] * dt / div;
+}
+
+// Math/OdeSolverRungeKutta4.cpp:89-93 (k1v is the start velocity v)
+__global__ __launch_bounds__(256) void rk4_final_kernel(int n, double* __restrict__ x, double* __restrict__ v,
+                                                        const double* __restrict__ k2v, const double* __restrict__ k3v,
+                                                        const double* __restrict__ k4v, const double* __restrict__ a1,
+                                                        const double* __restrict__ a2, const double* __restrict__ a3,
+                                                        const double* __restrict__ a4, double dt, int* invalid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i] + (v[i] + k4v[i] + 2.0 * (k2v[i] + k3v[i])) * dt / 6.0;
+    const double vi = v[i] + (a1[i] + a4[i] + 2.0 * (a2[i] + a3[i])) * dt / 6.0;
+    x[i] = xi;
+    v[i] = vi;
+    if (!isfinite(vi) || !isfinite(xi)) atomicOr(invalid, 1);
+}
+
+void launch_rk4_stage(int n, const double* x, const double* v, const double* kv, const double* ka, double dt,
+                      double div, double* xo, double* vo, hipStream_t s) {
+    if (n == 0) return;
+    launch_k(rk4_stage_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, v, kv, ka, dt, div, xo, vo);
+}
+
+void launch_rk4_final(int n, double* x, double* v, const double* k2v, const double* k3v, const double* k4v,
+                      const double* a1, const double* a2, const double* a3, const double* a4, double dt, int* invalid,
+                      hipStream_t s) {
+    if (n == 0) return;
+    launch_k(rk4_final_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, v, k2v, k3v, k4v, a1, a2, a3, a4, dt,
+             invalid);
 }
 
 // ------------------------------------------------------------------------------------------

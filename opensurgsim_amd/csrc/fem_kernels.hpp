@@ -72,6 +72,14 @@ struct TetElementArgs {
     int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
     const int* elem_inst = nullptr;     // batched instances: instance of each element (null: one dt)
     const double* inst_dt = nullptr;    // time step per instance
+    // Operator and right-hand side by integration scheme:
+    //   0: A = M (1/dt + alpha_M) + K (dt + alpha_K), rhs row f - K ((dt + alpha_K) vn - (xn - xs))
+    //      - alpha_M M vn - M (vn - vs) / dt (implicit Euler with Rayleigh damping D = alpha_M M + alpha_K K);
+    //   1: A = M / dt, rhs row f - alpha_K K vn - alpha_M M vn (explicit schemes, Runge-Kutta 4);
+    //   2: A = K, rhs row as in 1 (static).
+    int mode = 0;
+    double ray_mass = 0.0, ray_stiff = 0.0;
+    int lumped = 0;                     // lumped mass (row sums on the diagonal blocks)
 };
 
 void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s);
@@ -91,7 +99,26 @@ struct Operator {
 void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* ctr_ptr, int nctr_off,
                      const double* stage, double* S64, float* S32, hipStream_t s);
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
-                const uint8_t* bcmask, double* b, hipStream_t s);
+                const uint8_t* bcmask, double* b, hipStream_t s, double div = 1.0);
+// Localized external stiffness and damping folded into assembled blocks: S[9 blk[e] + k] += add[9 e + k]
+// (the FP32 copy rewritten from the FP64 sum when S32 != null).
+void launch_ext_operator(int count, const int* blk, const double* add, double* S64, float* S32, hipStream_t s);
+// Localized external rows of the right-hand side: b[row_dof[r]] += (f[r] + sum_p kval[p] w[col_dof[p]]) / div,
+// w = (xn - xs) - vn row_dt[r] when general, -vn row_dt[r] otherwise.
+void launch_ext_rhs(int rows, const int* row_dof, const int* row_ptr, const int* col_dof, const double* kval,
+                    const double* fval, const double* row_dt, const double* xn, const double* xs, const double* vn,
+                    int general, double div, double* b, hipStream_t s);
+// State update by scheme: 0 (implicit Euler, modified explicit Euler): v += dv, x = xs + dt v;
+// 1 (explicit Euler): x = xs + dt v, v += dv; 2 (static): x = xs + dv, v = 0.
+void launch_update_scheme(int n, int mode, const double* dv, const double* xs, double* x, double* v, double dt,
+                          int* invalid, hipStream_t s);
+// Runge-Kutta 4 stage state: xo = x + kv dt / div, vo = v + ka dt / div.
+void launch_rk4_stage(int n, const double* x, const double* v, const double* kv, const double* ka, double dt,
+                      double div, double* xo, double* vo, hipStream_t s);
+// Runge-Kutta 4 update: x += (v + k4v + 2 (k2v + k3v)) dt / 6, v += (a1 + a4 + 2 (a2 + a3)) dt / 6.
+void launch_rk4_final(int n, double* x, double* v, const double* k2v, const double* k3v, const double* k4v,
+                      const double* a1, const double* a2, const double* a3, const double* a4, double dt, int* invalid,
+                      hipStream_t s);
 
 // Jacobi-preconditioned CG (Eigen recurrence) on the operator: one fused kernel per iteration, or the
 // operator kernel + vector kernel pair.

@@ -24,9 +24,33 @@ int elementKind(const std::string& type) {
 
 const char* schemeName(Math::IntegrationScheme s) {
     switch (s) {
+        case Math::INTEGRATIONSCHEME_STATIC: return "Ode Solver Static";
+        case Math::INTEGRATIONSCHEME_LINEAR_STATIC: return "Ode Solver Linear Static";
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT: return "Ode Solver Euler Explicit";
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT: return "Ode Solver Linear Euler Explicit";
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT_MODIFIED: return "Ode Solver Euler Explicit Modified";
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT_MODIFIED: return "Ode Solver Linear Euler Explicit Modified";
         case Math::INTEGRATIONSCHEME_EULER_IMPLICIT: return "Ode Solver Euler Implicit";
         case Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT: return "Ode Solver Linear Euler Implicit";
+        case Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4: return "Ode Solver Runge Kutta 4";
+        case Math::INTEGRATIONSCHEME_LINEAR_RUNGE_KUTTA_4: return "Ode Solver Linear Runge Kutta 4";
         default: return "";
+    }
+}
+
+// The library's numbering of Math::IntegrationScheme (Math/OdeSolver.h:36-49).
+int32_t schemeCode(Math::IntegrationScheme s) {
+    switch (s) {
+        case Math::INTEGRATIONSCHEME_STATIC: return FEM_SCHEME_STATIC;
+        case Math::INTEGRATIONSCHEME_LINEAR_STATIC: return FEM_SCHEME_LINEAR_STATIC;
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT: return FEM_SCHEME_EULER_EXPLICIT;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT: return FEM_SCHEME_LINEAR_EULER_EXPLICIT;
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT_MODIFIED: return FEM_SCHEME_EULER_EXPLICIT_MODIFIED;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT_MODIFIED: return FEM_SCHEME_LINEAR_EULER_EXPLICIT_MODIFIED;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT: return FEM_SCHEME_LINEAR_EULER_IMPLICIT;
+        case Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4: return FEM_SCHEME_RUNGE_KUTTA_4;
+        case Math::INTEGRATIONSCHEME_LINEAR_RUNGE_KUTTA_4: return FEM_SCHEME_LINEAR_RUNGE_KUTTA_4;
+        default: return FEM_SCHEME_EULER_IMPLICIT;
     }
 }
 
@@ -114,8 +138,7 @@ void LinearSparseSolveAndInverseCG::setTolerance(double tolerance) {
 
 OdeSolver::OdeSolver(IntegrationScheme scheme) : m_scheme(scheme), m_name(schemeName(scheme)) {
     if (m_name.empty())
-        fail("Integration scheme " + std::to_string((int)scheme) +
-             " is not provided by the GPU path (implicit Euler and linear implicit Euler only)");
+        fail("Integration scheme " + std::to_string((int)scheme) + " is unknown");
 }
 
 void OdeSolver::setLinearSolver(std::shared_ptr<LinearSparseSolveAndInverse> linearSolver) {
@@ -228,6 +251,18 @@ void Fem3DRepresentation::addExternalGeneralizedForce(const Math::Vector& force)
     m_hasExternalForce = true;
 }
 
+void Fem3DRepresentation::addExternalGeneralizedForce(size_t elementId, const Math::Vector& naturalCoordinate,
+                                                      const Math::Vector3d& force, const Math::Matrix33d* K,
+                                                      const Math::Matrix33d* D) {
+    if (!m_ctx) fail("Initialize the representation before adding localized external forces");
+    if (elementId >= m_elements.size()) fail("Invalid localization (element id out of range)");
+    if (naturalCoordinate.size() != m_elements[elementId]->getNodeIds().size())
+        fail("Invalid localization (natural coordinate size)");
+    check(fem_add_external_generalized_force(m_ctx, (int64_t)elementId, naturalCoordinate.data(), force.data(),
+                                             K ? K->data() : nullptr, D ? D->data() : nullptr),
+          "external generalized force");
+}
+
 bool Fem3DRepresentation::initialize() {
     if (isInitialized()) fail("Fem3DRepresentation already initialized");
     if (!m_initialState) fail("Initial state has not been set yet. Did you call setInitialState() ?");
@@ -298,8 +333,10 @@ void Fem3DRepresentation::applyOptions(double dt) {
     o.gravity[1] = m_gravity[1];
     o.gravity[2] = m_gravity[2];
     o.gravity_enabled = m_gravityEnabled ? 1 : 0;
-    o.scheme = m_scheme == Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT ? FEM_SCHEME_LINEAR_EULER_IMPLICIT
-                                                                         : FEM_SCHEME_EULER_IMPLICIT;
+    o.scheme = schemeCode(m_scheme);
+    o.rayleigh_mass = m_rayleighMass;
+    o.rayleigh_stiffness = m_rayleighStiffness;
+    o.mass_lumping = m_massLumping ? 1 : 0;
     o.precision = (int32_t)m_precision;
     o.newton_max_iterations = (int32_t)m_odeSolver->getNewtonRaphsonMaximumIteration();
     auto solver = m_odeSolver->getLinearSolver();

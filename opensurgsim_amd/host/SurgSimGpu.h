@@ -11,9 +11,9 @@
 // Framework/Assert.h:77-95); a state that becomes non-finite deactivates the representation
 // (Physics/FemRepresentation.cpp:285-293).
 //
-// Only what the implicit-step path needs is mirrored. Not provided: compliance matrices
-// (OdeSolver::computeComplianceMatrixFromSystemMatrix), Rayleigh damping, mass lumping, compliance
-// warping, localizations other than nodes, explicit/static/Runge-Kutta schemes.
+// Only what the stepping path needs is mirrored. Not provided: compliance matrices
+// (OdeSolver::computeComplianceMatrixFromSystemMatrix), compliance warping, Localization objects (node and
+// element localizations are passed as ids and natural coordinates).
 #pragma once
 
 #include <array>
@@ -60,6 +60,7 @@ enum LinearSolver { LINEARSOLVER_LU = 0, LINEARSOLVER_CONJUGATEGRADIENT, MAX_LIN
 
 using Vector = std::vector<double>;
 using Vector3d = std::array<double, 3>;
+using Matrix33d = std::array<double, 9>;  // row-major
 
 /// Math/OdeState.h:38-207: positions, velocities (node-major interleaved DOFs) and boundary conditions.
 class OdeState {
@@ -288,6 +289,21 @@ public:
     void addExternalGeneralizedForce(size_t nodeId, const Math::Vector3d& force);
     /// A full generalized force vector (getNumDof() values), added until afterUpdate.
     void addExternalGeneralizedForce(const Math::Vector& force);
+    /// Element localization (Physics/Fem3DRepresentation.cpp:133-198): force spread over the element's nodes
+    /// with its natural coordinates, optional stiffness K and damping D on the node blocks; applied by the
+    /// next update() only. K and D need tetrahedra and a scheme other than the linear variants.
+    void addExternalGeneralizedForce(size_t elementId, const Math::Vector& naturalCoordinate,
+                                     const Math::Vector3d& force, const Math::Matrix33d* K = nullptr,
+                                     const Math::Matrix33d* D = nullptr);
+
+    /// Physics/FemRepresentation.cpp:202-220: Rayleigh damping D = massCoef M + stiffnessCoef K.
+    void setRayleighDampingMass(double massCoef) { m_rayleighMass = massCoef; }
+    double getRayleighDampingMass() const { return m_rayleighMass; }
+    void setRayleighDampingStiffness(double stiffnessCoef) { m_rayleighStiffness = stiffnessCoef; }
+    double getRayleighDampingStiffness() const { return m_rayleighStiffness; }
+    /// Physics/FemRepresentation.cpp:320-327.
+    void setMassLumping(bool useMassLumping) { m_massLumping = useMassLumping; }
+    bool getMassLumping() const { return m_massLumping; }
     const Math::Vector& getExternalGeneralizedForce() const { return m_externalForce; }
 
     /// Element initialisation, node masses and the sparsity pattern on the device
@@ -333,6 +349,8 @@ private:
     std::shared_ptr<Math::OdeSolver> m_odeSolver;
     bool m_gravityEnabled = true;
     Math::Vector3d m_gravity{{0.0, -9.81, 0.0}};
+    double m_rayleighMass = 0.0, m_rayleighStiffness = 0.0;
+    bool m_massLumping = false;
     Math::Vector m_externalForce;
     bool m_hasExternalForce = false, m_uploadedExternalForce = false;
     bool m_active = true;

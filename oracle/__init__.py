@@ -16,6 +16,10 @@ _lib = None
 
 TET_LINEAR, TET_COROTATIONAL, HEX = 0, 1, 2
 EULER_IMPLICIT, LINEAR_EULER_IMPLICIT = 0, 1
+STATIC, LINEAR_STATIC = 2, 3
+EULER_EXPLICIT, LINEAR_EULER_EXPLICIT = 4, 5
+EULER_EXPLICIT_MODIFIED, LINEAR_EULER_EXPLICIT_MODIFIED = 6, 7
+RUNGE_KUTTA_4, LINEAR_RUNGE_KUTTA_4 = 8, 9
 LU, CG = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -45,6 +49,9 @@ def lib():
         L.orc_sim_set_gravity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_int]
         L.orc_sim_set_external_force.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_sim_set_damping.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_sim_add_localized_force.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_sim_update_fmdk.argtypes = [ctypes.c_void_p]
         L.orc_sim_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_long, ctypes.c_long, ctypes.c_int]
         L.orc_sim_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -182,6 +189,25 @@ class Sim:
             self._fext = _f64(f).reshape(-1)
             lib().orc_sim_set_external_force(self.h, _d(self._fext))
 
+    def set_damping(self, rayleigh_mass=0.0, rayleigh_stiffness=0.0, lumping=False):
+        lib().orc_sim_set_damping(self.h, float(rayleigh_mass), float(rayleigh_stiffness), int(bool(lumping)))
+
+    def add_localized_force(self, element, coord, force, K=None, D=None):
+        """Force (3) at natural coordinates coord of an element, with optional 3x3 stiffness K and damping
+        D, for the next step only."""
+        arrs = [_f64(coord).reshape(-1), _f64(force).reshape(-1)]
+        k = _f64(K).reshape(-1) if K is not None else None
+        d = _f64(D).reshape(-1) if D is not None else None
+        rc = lib().orc_sim_add_localized_force(self.h, int(element), _d(arrs[0]), _d(arrs[1]),
+                                               _d(k) if k is not None else None, _d(d) if d is not None else None)
+        if rc < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def update_fmdk(self):
+        """Rebuild f, M, D, K at the current state as computeFMDK does, without stepping."""
+        if lib().orc_sim_update_fmdk(self.h) < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
     def set_solver(self, scheme=EULER_IMPLICIT, linsolver=CG, tol=2.220446049250313e-16, max_iter=-1,
                    newton_max_iter=1, faithful_bc=False):
         lib().orc_sim_set_solver(self.h, scheme, linsolver, tol, max_iter, newton_max_iter, int(faithful_bc))
@@ -227,7 +253,7 @@ class Sim:
         return outer, inner
 
     def matrix_values(self, which):
-        """which: 0 = M, 1 = K, 2 = system matrix A (after boundary conditions)."""
+        """which: 0 = M, 1 = K, 2 = system matrix A (after boundary conditions), 3 = D."""
         vals = np.zeros(lib().orc_sim_nnz(self.h))
         lib().orc_sim_matrix_values(self.h, which, _d(vals))
         return vals

@@ -13,6 +13,7 @@
 // Parity notes (what pins each function) are in the test files that exercise it.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -929,7 +930,21 @@ struct BandLU {
     }
 };
 
-enum Scheme { EULER_IMPLICIT = 0, LINEAR_EULER_IMPLICIT = 1 };
+// Math/OdeSolver.h:36-49 (numbered as the GPU library numbers them: the two implicit schemes first)
+enum Scheme {
+    EULER_IMPLICIT = 0,
+    LINEAR_EULER_IMPLICIT = 1,
+    STATIC = 2,
+    LINEAR_STATIC = 3,
+    EULER_EXPLICIT = 4,
+    LINEAR_EULER_EXPLICIT = 5,
+    EULER_EXPLICIT_MODIFIED = 6,
+    LINEAR_EULER_EXPLICIT_MODIFIED = 7,
+    RUNGE_KUTTA_4 = 8,
+    LINEAR_RUNGE_KUTTA_4 = 9
+};
+// OdeEquation update flags (Math/OdeEquation.h:34-41)
+enum { UPD_F = 1, UPD_M = 2, UPD_D = 4, UPD_K = 8, UPD_FMDK = 15 };
 enum LinSolver { LU = 0, CG = 1 };
 
 // ------------------------------------------------------------------------------------------
@@ -947,6 +962,16 @@ struct Sim {
     bool gravity_enabled = true;
     std::vector<double> ext_force;          // external generalized force (constant while set)
     bool has_ext = false;
+    // Rayleigh damping and mass lumping (Physics/FemRepresentation.h:89-142)
+    double ray_mass = 0.0, ray_stiff = 0.0;
+    bool lumping = false;
+    int init_state = 0;                     // OdeEquation::m_initState (Math/OdeEquation.cpp:71-104)
+    // Localized external generalized force, stiffness and damping of the next update
+    // (Physics/Fem3DRepresentation.cpp:133-198), reset after it (Physics/DeformableRepresentation.cpp:236-253).
+    // Stiffness and damping are kept per node block (row node, column node), 3x3 row-major.
+    std::vector<double> loc_f;
+    std::map<std::pair<int, int>, std::array<double, 9>> loc_K, loc_D;
+    bool has_loc = false;
     Csc M, D, K, A;
     std::vector<double> f, rhs, solution;
     int scheme = EULER_IMPLICIT;
@@ -979,6 +1004,108 @@ struct Sim {
         x = x0;
         v.assign(x0.size(), 0.0);
         ext_force.assign(x0.size(), 0.0);
+        loc_f.assign(x0.size(), 0.0);
+    }
+
+    // Fem3DRepresentation::addExternalGeneralizedForce (Physics/Fem3DRepresentation.cpp:133-198):
+    // force * coordinate[a] on node a; coordinate[a] * coordinate[b] * K (and D) on node block (a, b).
+    void add_localized_force(int element, const double* coord, const double* force, const double* Kl,
+                             const double* Dl) {
+        if (element < 0 || element >= (int)elems.size()) throw std::runtime_error("Invalid localization");
+        const Element& e = elems[element];
+        for (int a = 0; a < e.nn; ++a)
+            for (int c = 0; c < 3; ++c) loc_f[3 * (size_t)e.ids[a] + c] += force[c] * coord[a];
+        if (Kl || Dl) {
+            for (int a = 0; a < e.nn; ++a)
+                for (int b = 0; b < e.nn; ++b) {
+                    const auto key = std::make_pair(e.ids[a], e.ids[b]);
+                    const double w = coord[a] * coord[b];
+                    if (Kl) {
+                        auto it = loc_K.find(key);
+                        if (it == loc_K.end()) it = loc_K.emplace(key, std::array<double, 9>{}).first;
+                        for (int k = 0; k < 9; ++k) it->second[k] += w * Kl[k];
+                    }
+                    if (Dl) {
+                        auto it = loc_D.find(key);
+                        if (it == loc_D.end()) it = loc_D.emplace(key, std::array<double, 9>{}).first;
+                        for (int k = 0; k < 9; ++k) it->second[k] += w * Dl[k];
+                    }
+                }
+        }
+        has_loc = true;
+    }
+    // DeformableRepresentation::afterUpdate (Physics/DeformableRepresentation.cpp:236-253)
+    void clear_localized() {
+        std::fill(loc_f.begin(), loc_f.end(), 0.0);
+        loc_K.clear();
+        loc_D.clear();
+        has_loc = false;
+    }
+    // m_K += m_externalGeneralizedStiffness (and D): the blocks lie inside the element pattern.
+    static void add_blocks(const std::map<std::pair<int, int>, std::array<double, 9>>& m, Csc* S) {
+        for (const auto& kv : m) {
+            const int I = kv.first.first, J = kv.first.second;
+            for (int c = 0; c < 3; ++c) {
+                const int idx = S->find(3 * I, 3 * J + c);
+                if (idx < 0) throw std::runtime_error("external block outside the pattern");
+                for (int r = 0; r < 3; ++r) S->val[idx + r] += kv.second[3 * r + c];
+            }
+        }
+    }
+    void add_external_force(std::vector<double>& fv) {
+        if (has_ext)
+            for (size_t i = 0; i < fv.size(); ++i) fv[i] += ext_force[i];
+        if (has_loc)
+            for (size_t i = 0; i < fv.size(); ++i) fv[i] += loc_f[i];
+    }
+
+    // FemRepresentation::addRayleighDampingForce (Physics/FemRepresentation.cpp:565-629), scale 1:
+    // F -= alpha_M M v and F -= alpha_K K v with the global matrices, or element by element through
+    // FemElement::addMatVec (Physics/FemElement.cpp:168-210).
+    void rayleigh_force(std::vector<double>& fv, const std::vector<double>& vs, bool global_k, bool global_m) {
+        const size_t n = vs.size();
+        std::vector<double> tmp(n);
+        auto element_pass = [&](double alpha, bool mass) {
+            for (const auto& e : elems) {
+                const int ne = 3 * e.nn;
+                const std::vector<double>& Me = mass ? e.M : e.K;
+                for (int i = 0; i < ne; ++i) {
+                    double acc = 0.0;
+                    for (int k = 0; k < ne; ++k) acc += Me[i * ne + k] * vs[3 * (size_t)e.ids[k / 3] + k % 3];
+                    fv[3 * (size_t)e.ids[i / 3] + i % 3] += alpha * acc;
+                }
+            }
+        };
+        if (ray_mass != 0.0) {
+            if (global_m) {
+                if (lumping) {
+                    for (size_t i = 0; i < n; ++i) fv[i] -= ray_mass * (M.val[M.find((int)i, (int)i)] * vs[i]);
+                } else {
+                    csc_matvec(M, vs.data(), tmp.data());
+                    for (size_t i = 0; i < n; ++i) fv[i] -= ray_mass * tmp[i];
+                }
+            } else {
+                element_pass(-ray_mass, true);
+            }
+        }
+        if (ray_stiff != 0.0) {
+            if (global_k) {
+                csc_matvec(K, vs.data(), tmp.data());
+                for (size_t i = 0; i < n; ++i) fv[i] -= ray_stiff * tmp[i];
+            } else {
+                element_pass(-ray_stiff, false);
+            }
+        }
+    }
+
+    // Mass lumping of computeFMDK (Physics/FemRepresentation.cpp:499-508): each row's sum (columns in
+    // ascending order) replaces the row, on the diagonal; the pattern is kept.
+    void lump_mass() {
+        std::vector<double> rs(M.n, 0.0);
+        for (int j = 0; j < M.n; ++j)
+            for (int p = M.outer[j]; p < M.outer[j + 1]; ++p) rs[M.inner[p]] += M.val[p];
+        std::fill(M.val.begin(), M.val.end(), 0.0);
+        for (int i = 0; i < M.n; ++i) M.val[M.find(i, i)] = rs[i];
     }
 
     void add_bc_dof(int dof) {
@@ -996,9 +1123,10 @@ struct Sim {
             for (int c = 0; c < 3; ++c) fv[3 * n + c] += gravity[c] * (1.0 * mass_per_node[n]);
     }
 
-    // FemRepresentation::updateFMDK + computeFMDK (Physics/FemRepresentation.cpp:479-563)
-    void update_fmdk(const std::vector<double>& xs) {
-        for (auto& e : elems) element_update(e, xs.data(), 1 | 2 | 4 | 8, &cnt);
+    // FemRepresentation::updateFMDK + computeFMDK (Physics/FemRepresentation.cpp:479-563); vs is the
+    // velocity of the state the update is taken at (Rayleigh damping force).
+    void update_fmdk(const std::vector<double>& xs, const std::vector<double>& vs) {
+        for (auto& e : elems) element_update(e, xs.data(), UPD_FMDK, &cnt);
         f.assign(xs.size(), 0.0);
         std::fill(M.val.begin(), M.val.end(), 0.0);
         std::fill(D.val.begin(), D.val.end(), 0.0);
@@ -1010,21 +1138,59 @@ struct Sim {
             // addDamping is a no-op for 3D elements (m_useDamping false, FemElement.cpp:131-145)
             add_element_matrix(e, e.K, 1.0, &K);
         }
+        if (lumping) lump_mass();
+        if (ray_mass != 0.0)
+            for (size_t p = 0; p < D.val.size(); ++p) D.val[p] += M.val[p] * ray_mass;
+        if (ray_stiff != 0.0)
+            for (size_t p = 0; p < D.val.size(); ++p) D.val[p] += K.val[p] * ray_stiff;
         add_gravity(f);
-        if (has_ext)
-            for (size_t i = 0; i < f.size(); ++i) f[i] += ext_force[i];
+        rayleigh_force(f, vs, (init_state & UPD_K) != 0, true);
+        add_external_force(f);
+        if (has_loc) {
+            add_blocks(loc_K, &K);
+            add_blocks(loc_D, &D);
+        }
+        init_state |= UPD_FMDK;
     }
 
-    // FemRepresentation::computeF (Physics/FemRepresentation.cpp:396-410) after element F updates.
-    void update_f(const std::vector<double>& xs) {
-        for (auto& e : elems) element_update(e, xs.data(), 1, &cnt);
-        f.assign(xs.size(), 0.0);
+    // FemRepresentation::computeF (Physics/FemRepresentation.cpp:396-410): gravity, Rayleigh damping
+    // force, element forces, external force. The elements were updated by the caller.
+    void compute_f(const std::vector<double>& vs) {
+        f.assign(vs.size(), 0.0);
         add_gravity(f);
+        rayleigh_force(f, vs, (init_state & UPD_K) != 0, (init_state & UPD_F) != 0);
         for (const auto& e : elems)
             for (int a = 0; a < e.nn; ++a)
                 for (int c = 0; c < 3; ++c) f[3 * e.ids[a] + c] += e.f[3 * a + c];
-        if (has_ext)
-            for (size_t i = 0; i < f.size(); ++i) f[i] += ext_force[i];
+        add_external_force(f);
+    }
+    // FemRepresentation::computeM (:412-421): element masses, never lumped.
+    void compute_m() {
+        std::fill(M.val.begin(), M.val.end(), 0.0);
+        for (const auto& e : elems) add_element_matrix(e, e.M, 1.0, &M);
+    }
+    // FemRepresentation::computeK (:460-477)
+    void compute_k() {
+        std::fill(K.val.begin(), K.val.end(), 0.0);
+        for (const auto& e : elems) add_element_matrix(e, e.K, 1.0, &K);
+        if (has_loc) add_blocks(loc_K, &K);
+    }
+    // OdeEquation::updateFMDK with options other than FMDK (Math/OdeEquation.cpp:78-103), after
+    // FemRepresentation::updateFMDK's element loop (Physics/FemRepresentation.cpp:547-563).
+    void update_parts(const std::vector<double>& xs, const std::vector<double>& vs, int options) {
+        for (auto& e : elems) element_update(e, xs.data(), options, &cnt);
+        if (options & UPD_F) {
+            compute_f(vs);
+            init_state |= UPD_F;
+        }
+        if (options & UPD_M) {
+            compute_m();
+            init_state |= UPD_M;
+        }
+        if (options & UPD_K) {
+            compute_k();
+            init_state |= UPD_K;
+        }
     }
 
     // OdeState::applyBoundaryConditionsToMatrix(SparseMatrix*) (Math/OdeState.cpp:197-218)
@@ -1057,7 +1223,7 @@ struct Sim {
     // OdeSolverEulerImplicit::assembleLinearSystem (Math/OdeSolverEulerImplicit.cpp:124-161)
     void assemble(double dt, const std::vector<double>& xs, const std::vector<double>& vs,
                   const std::vector<double>& xn, const std::vector<double>& vn) {
-        update_fmdk(xn);
+        update_fmdk(xn, vn);
         double idt = 1.0 / dt;
         for (size_t p = 0; p < A.val.size(); ++p) A.val[p] = M.val[p] * idt + D.val[p] + K.val[p] * dt;
         if (bc_mask.empty()) bc_mask.assign(xs.size(), 0);
@@ -1118,7 +1284,7 @@ struct Sim {
             linear_initialized = true;
             return;
         }
-        update_f(x);
+        update_parts(x, v, UPD_F);
         size_t n = x.size();
         std::vector<double> vdt(n), Kv(n);
         for (size_t i = 0; i < n; ++i) vdt[i] = v[i] * dt;
@@ -1131,6 +1297,137 @@ struct Sim {
         apply_bc_vector(dv);
         for (size_t i = 0; i < n; ++i) v[i] = v[i] + dv[i];
         for (size_t i = 0; i < n; ++i) x[i] = x[i] + dt * v[i];
+    }
+
+    // ---- integrators other than implicit Euler (Math/OdeSolver*.cpp) ----
+    // System matrix M / dt (explicit schemes, RK4) or K (static), boundary conditions applied and
+    // handed to the linear solver (m_linearSolver->setMatrix).
+    void set_system(bool stiffness, double dt) {
+        if (stiffness) A.val = K.val;
+        else
+            for (size_t p = 0; p < A.val.size(); ++p) A.val[p] = M.val[p] / dt;
+        if (bc_mask.empty()) bc_mask.assign(x.size(), 0);
+        apply_bc_matrix(&A);
+        if (linsolver == LU) lu.factor(A);
+    }
+    std::vector<double> rhs_from_f(double div = 1.0) {
+        std::vector<double> b = f;
+        apply_bc_vector(b);
+        if (div != 1.0)
+            for (double& q : b) q /= div;
+        return b;
+    }
+    // DeformableRepresentation::applyCompliance (Physics/DeformableRepresentation.cpp:175-190)
+    std::vector<double> apply_compliance(std::vector<double> b) {
+        apply_bc_vector(b);
+        std::vector<double> out;
+        linear_solve(b, out);
+        apply_bc_vector(out);
+        return out;
+    }
+
+    // OdeSolverStatic::solve / assembleLinearSystem (Math/OdeSolverStatic.cpp:32-70); the linear variant
+    // (Math/OdeSolverLinearStatic.cpp:33-50) keeps the first system and applies its compliance to F.
+    void solve_static(double dt, bool linear) {
+        std::vector<double> dx;
+        if (linear && linear_initialized) {
+            update_parts(x, v, UPD_F);
+            dx = apply_compliance(f);
+        } else {
+            update_parts(x, v, UPD_F | UPD_K);
+            set_system(true, dt);
+            rhs = rhs_from_f();
+            linear_solve(rhs, dx);
+            solution = dx;
+            linear_initialized = linear;
+        }
+        for (size_t i = 0; i < x.size(); ++i) x[i] = x[i] + dx[i];
+        std::fill(v.begin(), v.end(), 0.0);
+    }
+
+    // OdeSolverEulerExplicit (Math/OdeSolverEulerExplicit.cpp:32-70) and OdeSolverEulerExplicitModified
+    // (Math/OdeSolverEulerExplicitModified.cpp:32-71): (M / dt) dv = F; the explicit scheme moves x with
+    // the old velocity, the modified one with the new. Linear variants (Math/OdeSolverLinearEulerExplicit.cpp,
+    // Math/OdeSolverLinearEulerExplicitModified.cpp): later steps apply the first step's compliance.
+    void solve_explicit(double dt, bool modified, bool linear) {
+        std::vector<double> dv;
+        if (linear && linear_initialized) {
+            update_parts(x, v, UPD_F);
+            dv = apply_compliance(f);
+        } else {
+            update_parts(x, v, UPD_F | UPD_M);
+            set_system(false, dt);
+            rhs = rhs_from_f();
+            linear_solve(rhs, dv);
+            solution = dv;
+            linear_initialized = linear;
+        }
+        const size_t n = x.size();
+        if (modified) {
+            for (size_t i = 0; i < n; ++i) v[i] = v[i] + dv[i];
+            for (size_t i = 0; i < n; ++i) x[i] = x[i] + dt * v[i];
+        } else {
+            for (size_t i = 0; i < n; ++i) x[i] = x[i] + dt * v[i];
+            for (size_t i = 0; i < n; ++i) v[i] = v[i] + dv[i];
+        }
+    }
+
+    // OdeSolverRungeKutta4::solve (Math/OdeSolverRungeKutta4.cpp:32-112); the linear variant
+    // (Math/OdeSolverLinearRungeKutta4.cpp:33-91) evaluates every stage through the first step's compliance.
+    void solve_rk4(double dt, bool linear) {
+        const size_t n = x.size();
+        const bool first = !(linear && linear_initialized);
+        std::vector<double> xn(n), vn(n), k1v = v, k2v, k3v, k4v, k1a, k2a, k3a, k4a;
+        if (first) {
+            update_parts(x, v, UPD_F | UPD_M);
+            set_system(false, dt);
+            rhs = rhs_from_f();
+            std::vector<double> b = rhs;
+            for (double& q : b) q /= dt;
+            linear_solve(b, k1a);
+        } else {
+            update_parts(x, v, UPD_F);
+            std::vector<double> b = f;
+            for (double& q : b) q /= dt;
+            k1a = apply_compliance(b);
+        }
+        auto stage = [&](const std::vector<double>& kv, const std::vector<double>& ka, double h,
+                         std::vector<double>& vout, std::vector<double>& aout) {
+            for (size_t i = 0; i < n; ++i) xn[i] = x[i] + kv[i] * h;
+            for (size_t i = 0; i < n; ++i) vn[i] = v[i] + ka[i] * h;
+            vout = vn;
+            update_parts(xn, vn, UPD_F);
+            if (first) {
+                std::vector<double> b = rhs_from_f(dt);
+                linear_solve(b, aout);
+            } else {
+                std::vector<double> b = f;
+                for (double& q : b) q /= dt;
+                aout = apply_compliance(b);
+            }
+        };
+        // y + k dt / 2: Eigen evaluates (k * dt) / 2.0 per coefficient
+        auto half = [&](const std::vector<double>& kv, const std::vector<double>& ka, std::vector<double>& vout,
+                        std::vector<double>& aout) {
+            for (size_t i = 0; i < n; ++i) xn[i] = x[i] + kv[i] * dt / 2.0;
+            for (size_t i = 0; i < n; ++i) vn[i] = v[i] + ka[i] * dt / 2.0;
+            vout = vn;
+            update_parts(xn, vn, UPD_F);
+            if (first) {
+                std::vector<double> b = rhs_from_f(dt);
+                linear_solve(b, aout);
+            } else {
+                std::vector<double> b = f;
+                for (double& q : b) q /= dt;
+                aout = apply_compliance(b);
+            }
+        };
+        half(k1v, k1a, k2v, k2a);
+        half(k2v, k2a, k3v, k3a);
+        stage(k3v, k3a, dt, k4v, k4a);
+        for (size_t i = 0; i < n; ++i) x[i] = x[i] + (k1v[i] + k4v[i] + 2.0 * (k2v[i] + k3v[i])) * dt / 6.0;
+        for (size_t i = 0; i < n; ++i) v[i] = v[i] + (k1a[i] + k4a[i] + 2.0 * (k2a[i] + k3a[i])) * dt / 6.0;
+        if (first) linear_initialized = linear;
     }
 
     // ---- adjoint (reference gradient for the GPU adjoint; no counterpart in the reference) ----
@@ -1286,10 +1583,19 @@ struct Sim {
     // FemRepresentation::update (Physics/FemRepresentation.cpp:232-294) minus compliance.
     int step(double dt) {
         if (!valid) return 1;
-        if (scheme == LINEAR_EULER_IMPLICIT)
-            solve_linear_implicit(dt);
-        else
-            solve_implicit(dt);
+        switch (scheme) {
+            case LINEAR_EULER_IMPLICIT: solve_linear_implicit(dt); break;
+            case STATIC: solve_static(dt, false); break;
+            case LINEAR_STATIC: solve_static(dt, true); break;
+            case EULER_EXPLICIT: solve_explicit(dt, false, false); break;
+            case LINEAR_EULER_EXPLICIT: solve_explicit(dt, false, true); break;
+            case EULER_EXPLICIT_MODIFIED: solve_explicit(dt, true, false); break;
+            case LINEAR_EULER_EXPLICIT_MODIFIED: solve_explicit(dt, true, true); break;
+            case RUNGE_KUTTA_4: solve_rk4(dt, false); break;
+            case LINEAR_RUNGE_KUTTA_4: solve_rk4(dt, true); break;
+            default: solve_implicit(dt); break;
+        }
+        clear_localized();
         // OdeState::isValid (Math/OdeState.cpp:268-278)
         double sx = 0.0, sv = 0.0;
         for (double q : x) sx += q;
@@ -1472,9 +1778,43 @@ void orc_sim_system_values(void* h, double* vals) {
     Sim* s = static_cast<Sim*>(h);
     std::memcpy(vals, s->A.val.data(), s->A.val.size() * sizeof(double));
 }
+// Rayleigh damping coefficients and mass lumping (FemRepresentation::setRayleighDampingMass /
+// setRayleighDampingStiffness / setMassLumping, Physics/FemRepresentation.cpp:212-220, 320-323).
+void orc_sim_set_damping(void* h, double rayleigh_mass, double rayleigh_stiffness, int lumping) {
+    Sim* s = static_cast<Sim*>(h);
+    s->ray_mass = rayleigh_mass;
+    s->ray_stiff = rayleigh_stiffness;
+    s->lumping = lumping != 0;
+}
+
+// Fem3DRepresentation::addExternalGeneralizedForce (Physics/Fem3DRepresentation.cpp:133-198) for the next
+// step; K and D (3x3 row-major) may be null.
+int orc_sim_add_localized_force(void* h, int element, const double* coord, const double* force, const double* K,
+                                const double* D) {
+    try {
+        static_cast<Sim*>(h)->add_localized_force(element, coord, force, K, D);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// Rebuild f, M, D, K at the current state as computeFMDK does (no step): for checks of the assembled terms.
+int orc_sim_update_fmdk(void* h) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        s->update_fmdk(s->x, s->v);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 void orc_sim_matrix_values(void* h, int which, double* vals) {
     Sim* s = static_cast<Sim*>(h);
-    const Csc& S = which == 0 ? s->M : (which == 1 ? s->K : s->A);
+    const Csc& S = which == 0 ? s->M : (which == 1 ? s->K : which == 3 ? s->D : s->A);
     std::memcpy(vals, S.val.data(), S.val.size() * sizeof(double));
 }
 void orc_sim_rhs(void* h, double* out) {

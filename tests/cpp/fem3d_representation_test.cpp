@@ -335,8 +335,13 @@ void cpuTests() {
         EXPECT_TRUE(implicitSolver.getName() == "Ode Solver Euler Implicit");
         Math::OdeSolver linear(Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT);
         EXPECT_TRUE(linear.getNewtonRaphsonMaximumIteration() == 1);
-        EXPECT_THROW_ASSERT(Math::OdeSolver(Math::INTEGRATIONSCHEME_EULER_EXPLICIT));
-        EXPECT_THROW_ASSERT(Math::OdeSolver(Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4));
+        // every scheme of Math/OdeSolver.h:36-49, with the reference's solver names (Math/OdeSolver*.cpp)
+        EXPECT_TRUE(Math::OdeSolver(Math::INTEGRATIONSCHEME_STATIC).getName() == "Ode Solver Static");
+        EXPECT_TRUE(Math::OdeSolver(Math::INTEGRATIONSCHEME_EULER_EXPLICIT).getName() == "Ode Solver Euler Explicit");
+        EXPECT_TRUE(Math::OdeSolver(Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT_MODIFIED).getName() ==
+                    "Ode Solver Linear Euler Explicit Modified");
+        EXPECT_TRUE(Math::OdeSolver(Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4).getName() == "Ode Solver Runge Kutta 4");
+        EXPECT_THROW_ASSERT(Math::OdeSolver(Math::MAX_INTEGRATIONSCHEMES));
         EXPECT_THROW_ASSERT(linear.setNewtonRaphsonMaximumIteration(0));
     });
     run("Fem3DRepresentationTest.ElementTypeAndSettings", [] {

@@ -11,9 +11,9 @@ BIN = os.path.join(ROOT, "tests", "cpp", "_build", "fem3d_representation_test")
 
 
 def _binary():
-    if not os.path.exists(BIN):
-        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")])
+    # make rebuilds the program when the adapter header or the libraries changed
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")])
     return BIN
 
 

@@ -1,0 +1,200 @@
+"""GPU parity, through the C ABI, of the integration schemes other than implicit Euler, Rayleigh damping,
+mass lumping and localized external forces against the FP64 oracle (Math/OdeSolver*.cpp,
+Physics/FemRepresentation.cpp:479-629, Physics/Fem3DRepresentation.cpp:133-198).
+
+Both sides solve with Eigen-semantics Jacobi CG at a 1e-12 relative residual; the bounds are relative
+errors of displacement and velocity after the steps. The oracle's schemes, damping, lumping and external
+terms are pinned by tests/test_oracle_schemes.py (modal closed forms and the reference's
+FemRepresentationTests checks).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from opensurgsim_amd import femgpu
+from opensurgsim_amd import meshgen as mg
+
+from helpers import make_gpu, make_oracle, rel
+
+pytestmark = pytest.mark.gpu
+
+AM, AK = 2.0, 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def soft_config(kind, n=(4, 4, 4), dt=2e-4):
+    """Soft block (E 5 kPa) with the corotational config's tangential top velocities and a small dt, so the
+    explicit schemes stay bounded over the steps."""
+    cfg = mg.config_corotational(n, seed=31)
+    cfg["kind"] = kind
+    cfg["E"] = np.full(cfg.conn.shape[0], 5e3)
+    cfg["dt"] = dt
+    cfg["cg_tol"] = 1e-12
+    return cfg
+
+
+def run_pair(cfg, steps, scheme, precision=femgpu.FP64, am=0.0, ak=0.0, lumping=False, before_step=None):
+    o = make_oracle(cfg, scheme=scheme, tol=1e-12)
+    o.set_damping(am, ak, lumping)
+    g = make_gpu(cfg, precision=precision, scheme=scheme, tol=1e-12, rayleigh_mass=am, rayleigh_stiffness=ak,
+                 mass_lumping=int(lumping))
+    for k in range(steps):
+        if before_step:
+            before_step(k, o, g)
+        assert o.step(cfg.dt, 1) == 0
+        assert g.step(1) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    return dict(u=rel(x - x0, xr - x0), v=rel(v, vr) if np.linalg.norm(vr) > 0 else np.linalg.norm(v))
+
+
+NONLINEAR = [femgpu.STATIC, femgpu.EULER_EXPLICIT, femgpu.EULER_EXPLICIT_MODIFIED, femgpu.RUNGE_KUTTA_4]
+LINEAR = [femgpu.LINEAR_STATIC, femgpu.LINEAR_EULER_EXPLICIT, femgpu.LINEAR_EULER_EXPLICIT_MODIFIED,
+          femgpu.LINEAR_RUNGE_KUTTA_4]
+
+
+@pytest.mark.parametrize("scheme", NONLINEAR)
+@pytest.mark.parametrize("damped", [False, True])
+def test_corotational_schemes_vs_oracle(scheme, damped):
+    cfg = soft_config(oracle.TET_COROTATIONAL)
+    steps = 2 if scheme == femgpu.STATIC else 10
+    say("corotational scheme", scheme, "damped", damped)
+    e = run_pair(cfg, steps, scheme, am=AM if damped else 0.0, ak=AK if damped else 0.0)
+    say("  rel err", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+@pytest.mark.parametrize("scheme", LINEAR)
+def test_linear_variants_vs_oracle(scheme):
+    cfg = soft_config(oracle.TET_LINEAR)
+    steps = 3 if scheme == femgpu.LINEAR_STATIC else 10
+    say("linear-tet scheme", scheme)
+    e = run_pair(cfg, steps, scheme, am=AM, ak=AK)
+    say("  rel err", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+@pytest.mark.parametrize("scheme", [femgpu.EULER_EXPLICIT, femgpu.RUNGE_KUTTA_4])
+def test_explicit_schemes_fp32_refined(scheme):
+    cfg = soft_config(oracle.TET_COROTATIONAL)
+    say("fp32 refined scheme", scheme)
+    e = run_pair(cfg, 8, scheme, precision=femgpu.FP32_REFINED, am=AM, ak=AK)
+    say("  rel err", e)
+    assert e["u"] < 1e-8 and e["v"] < 1e-8
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+@pytest.mark.parametrize("lumping", [False, True])
+def test_implicit_rayleigh_and_lumping_vs_oracle(precision, lumping):
+    cfg = mg.config_corotational((6, 6, 6), seed=5)
+    cfg["dt"] = 1e-3
+    say("implicit Rayleigh, lumping", lumping, "precision", precision)
+    e = run_pair(cfg, 10, femgpu.EULER_IMPLICIT, precision=precision, am=AM, ak=AK, lumping=lumping)
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
+
+
+def test_linear_implicit_rayleigh_lumping_vs_oracle():
+    cfg = soft_config(oracle.TET_LINEAR, dt=1e-3)
+    e = run_pair(cfg, 10, femgpu.LINEAR_EULER_IMPLICIT, am=AM, ak=AK, lumping=True)
+    say("linear implicit Rayleigh + lumping", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+@pytest.mark.parametrize("lumping", [False, True])
+def test_hex_rayleigh_and_lumping_vs_oracle(precision, lumping):
+    cfg = mg.config_hex((5, 3, 4))
+    cfg["cg_tol"] = 1e-12 if precision == femgpu.FP64 else 1e-10
+    say("hex Rayleigh, lumping", lumping, "precision", precision)
+    o = make_oracle(cfg, tol=1e-12)
+    o.set_damping(AM, AK, lumping)
+    g = make_gpu(cfg, precision=precision, rayleigh_mass=AM, rayleigh_stiffness=AK, mass_lumping=int(lumping))
+    o.step(cfg.dt, 6)
+    assert g.step(6) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - x0, xr - x0), v=rel(v, vr))
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
+
+
+def _localized(k, o, g, with_kd=True):
+    """Localized terms before the first and third steps (each lasts one step)."""
+    if k not in (0, 2):
+        return
+    rng = np.random.default_rng(100 + k)
+    for e in (3, 41, 77):
+        coord = rng.dirichlet(np.ones(4))
+        F = rng.normal(size=3) * 0.05
+        K = rng.normal(size=(3, 3)) * 20.0 + 60.0 * np.eye(3) if with_kd else None
+        D = rng.normal(size=(3, 3)) * 0.05 + 0.1 * np.eye(3) if with_kd else None
+        o.add_localized_force(e, coord, F, K, D)
+        g.add_external_generalized_force(e, coord, F, K, D)
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+def test_localized_force_stiffness_damping_implicit(precision):
+    cfg = mg.config_corotational((5, 5, 5), seed=8)
+    say("localized external terms, implicit, precision", precision)
+    e = run_pair(cfg, 4, femgpu.EULER_IMPLICIT, precision=precision, am=AM, ak=AK, before_step=_localized)
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
+
+
+@pytest.mark.parametrize("scheme", [femgpu.STATIC, femgpu.EULER_EXPLICIT, femgpu.RUNGE_KUTTA_4])
+def test_localized_terms_other_schemes(scheme):
+    cfg = soft_config(oracle.TET_COROTATIONAL, n=(5, 5, 5))
+    say("localized external terms, scheme", scheme)
+    e = run_pair(cfg, 4, scheme, before_step=_localized)
+    say("  rel err", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_hex_localized_force():
+    cfg = mg.config_hex((4, 3, 4))
+    o = make_oracle(cfg, tol=1e-12)
+    g = make_gpu(cfg, tol=1e-12)
+    for k in range(3):
+        if k == 0:
+            for e, coord in ((5, np.full(8, 0.125)), (30, np.eye(8)[6])):
+                o.add_localized_force(e, coord, [0.0, -2.0, 0.5])
+                g.add_external_generalized_force(e, coord, [0.0, -2.0, 0.5])
+        o.step(cfg.dt, 1)
+        assert g.step(1) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - x0, xr - x0), v=rel(v, vr))
+    say("hex localized force", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_unsupported_combinations_rejected():
+    cfg = mg.config_hex((2, 2, 2))
+    g = make_gpu(cfg)
+    with pytest.raises(femgpu.FemError):
+        g.add_external_generalized_force(0, np.full(8, 0.125), [0, 0, 1.0], np.eye(3), None)
+    g.set_options(dt=cfg.dt, scheme=femgpu.RUNGE_KUTTA_4)
+    with pytest.raises(femgpu.FemError):
+        g.step(1)
+    t = make_gpu(soft_config(oracle.TET_LINEAR))
+    with pytest.raises(femgpu.FemError):
+        t.set_options(dt=1e-3, scheme=femgpu.EULER_EXPLICIT, rayleigh_mass=1.0, mass_lumping=1)
+    t.set_options(dt=1e-3, scheme=femgpu.LINEAR_EULER_IMPLICIT)
+    t.add_external_generalized_force(0, np.full(4, 0.25), [0, 0, 1.0], np.eye(3), None)
+    with pytest.raises(femgpu.FemError):
+        t.step(1)
+    with pytest.raises(femgpu.FemError):
+        t.add_external_generalized_force(10 ** 6, np.full(4, 0.25), [0, 0, 1.0])
