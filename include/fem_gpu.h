@@ -144,6 +144,16 @@ int fem_add_external_generalized_force(fem_ctx* ctx, int64_t element, const doub
 /* State transfer (x, v of ndof values each; NULL leaves one unchanged). Replaces
  * DeformableRepresentation::setInitialState / getCurrentState (Physics/DeformableRepresentation.cpp:86-119). */
 int fem_set_state(fem_ctx* ctx, const double* x, const double* v);
+
+/* out = C b for num_columns column-major columns of ndof values (host pointers), C the compliance of the
+ * last step's system matrix: boundary rows of b zeroed, one linear solve per column with the context's
+ * solver settings (Eigen CG semantics per column, as Eigen solves a matrix right-hand side column by column),
+ * boundary rows of the solution zeroed. Replaces DeformableRepresentation::applyCompliance
+ * (Physics/DeformableRepresentation.cpp:175-190) and, with b = H^T, the constraints' getComplianceMatrix() * H^T
+ * (Physics/FemConstraintFixedPoint.cpp:107, FemConstraintFrictionlessContact.cpp:108, ...) without the dense
+ * inverse of OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76). FEM_ERR_ORDER before
+ * the first step. The step statistics are left as the last step set them. */
+int fem_apply_compliance(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
 int fem_get_state(fem_ctx* ctx, double* x, double* v);
 int fem_set_state_device(fem_ctx* ctx, const double* x, const double* v);
 int fem_get_state_device(fem_ctx* ctx, double* x, double* v);

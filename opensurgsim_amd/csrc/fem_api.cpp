@@ -263,6 +263,7 @@ struct fem_ctx {
     int* h_counters = nullptr;
 
     // state
+    bool have_operator = false;   // a step assembled (or cached) the system matrix
     bool linear_matrix_ready = false;
     double matrix_dt = 0.0;
     int last_precision = -1;
@@ -1112,6 +1113,36 @@ int batch_step(fem_ctx* c) {
     return FEM_OK;
 }
 
+HexStepArgs hex_args(fem_ctx* c) {
+    HexStepArgs a;
+    a.opt = &c->opt;
+    a.x0 = c->x0.p;
+    a.x = c->x.p;
+    a.v = c->v.p;
+    a.fg = c->fg.p;
+    a.fext = c->have_fext ? c->fext.p : nullptr;
+    a.fstep = c->fstep_set ? c->fstep.p : nullptr;
+    a.bcmask = c->bcmask.p;
+    a.mat = c->mat.p;
+    a.b = c->b.p;
+    a.dv = c->dv.p;
+    a.r64 = c->r64.p;
+    a.vec[0] = c->cg_x.p;
+    a.vec[1] = c->cg_r.p;
+    a.vec[2] = c->cg_z.p;
+    a.vec[3] = c->cg_p0.p;
+    a.vec[4] = c->cg_q.p;
+    a.vec[5] = c->cg_dinv.p;
+    a.vec[6] = c->dvprev.p;
+    a.partial = c->partial.p;
+    a.sc = c->scal.p;
+    a.h_sc = c->h_sc;
+    a.invalid = c->counters.p + 3;
+    a.max_iter = (int)max_iters(c);
+    a.predicted = c->predicted_hex;
+    return a;
+}
+
 // Static, explicit Euler, modified explicit Euler and Runge-Kutta 4 steps and their linear variants on the
 // assembled tetrahedral operator (Math/OdeSolverStatic.cpp:32-70, Math/OdeSolverEulerExplicit.cpp:32-70,
 // Math/OdeSolverEulerExplicitModified.cpp:32-71, Math/OdeSolverRungeKutta4.cpp:32-112 and
@@ -1174,32 +1205,7 @@ int step_once(fem_ctx* c) {
     const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
     const int newton = std::max(1, c->opt.newton_max_iterations);
     if (c->kind == FEM_ELEMENT_HEX) {
-        HexStepArgs a;
-        a.opt = &c->opt;
-        a.x0 = c->x0.p;
-        a.x = c->x.p;
-        a.v = c->v.p;
-        a.fg = c->fg.p;
-        a.fext = c->have_fext ? c->fext.p : nullptr;
-        a.fstep = c->fstep_set ? c->fstep.p : nullptr;
-        a.bcmask = c->bcmask.p;
-        a.mat = c->mat.p;
-        a.b = c->b.p;
-        a.dv = c->dv.p;
-        a.r64 = c->r64.p;
-        a.vec[0] = c->cg_x.p;
-        a.vec[1] = c->cg_r.p;
-        a.vec[2] = c->cg_z.p;
-        a.vec[3] = c->cg_p0.p;
-        a.vec[4] = c->cg_q.p;
-        a.vec[5] = c->cg_dinv.p;
-        a.vec[6] = c->dvprev.p;
-        a.partial = c->partial.p;
-        a.sc = c->scal.p;
-        a.h_sc = c->h_sc;
-        a.invalid = c->counters.p + 3;
-        a.max_iter = (int)max_iters(c);
-        a.predicted = c->predicted_hex;
+        HexStepArgs a = hex_args(c);
         hex_step(c->hex, a, c->stats, c->stream);
     } else if (c->opt.scheme >= FEM_SCHEME_STATIC) {
         step_other_scheme(c);
@@ -1242,6 +1248,7 @@ int step_once(fem_ctx* c) {
         }
     }
     c->stats.steps++;
+    c->have_operator = true;
     return FEM_OK;
 }
 
@@ -1673,6 +1680,50 @@ int fem_add_external_generalized_force(fem_ctx* c, int64_t element, const double
                     }
                 }
         c->loc_pending = true;
+        return FEM_OK;
+    });
+}
+
+int fem_apply_compliance(fem_ctx* c, const double* b, int64_t num_columns, double* out) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        if (num_columns < 0 || (num_columns > 0 && (!b || !out))) throw InvalidArg("null or negative columns");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_bc(c);
+        alloc_precision_buffers(c);
+        const size_t n = (size_t)c->ndof;
+        // the step's statistics and iteration predictions are the step's, not the compliance solves'
+        const fem_stats saved = c->stats;
+        const int saved_iters = c->predicted_iters, saved_sweeps = c->predicted_sweeps;
+        int saved_sweep[16], saved_hex[17], saved_batch[17];
+        std::memcpy(saved_sweep, c->predicted_sweep, sizeof saved_sweep);
+        std::memcpy(saved_hex, c->predicted_hex, sizeof saved_hex);
+        std::memcpy(saved_batch, c->b_predicted, sizeof saved_batch);
+        std::vector<double> col(n);
+        for (int64_t j = 0; j < num_columns; ++j) {
+            std::memcpy(col.data(), b + (size_t)j * n, n * sizeof(double));
+            for (int64_t d : c->h_bc_dofs) col[(size_t)d] = 0.0;
+            c->b.upload(col.data(), n, c->stream);
+            if (c->kind == FEM_ELEMENT_HEX) {
+                hex_solve(c->hex, hex_args(c), c->stats, c->stream);
+            } else if (c->num_instances > 0) {
+                HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+                batch_linear_solve(c);
+            } else {
+                linear_solve(c);
+            }
+            double* o = out + (size_t)j * n;
+            HIPCHECK(hipMemcpyAsync(o, c->dv.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            for (int64_t d : c->h_bc_dofs) o[d] = 0.0;
+        }
+        c->stats = saved;
+        c->predicted_iters = saved_iters;
+        c->predicted_sweeps = saved_sweeps;
+        std::memcpy(c->predicted_sweep, saved_sweep, sizeof saved_sweep);
+        std::memcpy(c->predicted_hex, saved_hex, sizeof saved_hex);
+        std::memcpy(c->b_predicted, saved_batch, sizeof saved_batch);
         return FEM_OK;
     });
 }

@@ -683,6 +683,16 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
     apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt + o.rayleigh_stiffness, nullptr, o.rayleigh_mass, 1.0, lumped, -1.0,
             a.b, s);
     hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, a.b);
+    hex_solve(h, a, stats, s);
+    launch_update(n, a.dv, a.x, a.x, a.v, dt, a.invalid, s);
+}
+
+void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s) {
+    const fem_options& o = *a.opt;
+    const int n = (int)(3 * h.num_nodes);
+    const bool fp32 = o.precision == FEM_PRECISION_FP32_REFINED;
+    const double cm = h.a32_cm, ck = h.a32_ck;
+    const int lumped = h.a32_lumped;
     CgScalars* sc = a.sc;
     if (!fp32) {
         double* x = a.dv;
@@ -741,7 +751,6 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
         stats.converged = oh.converged;
         stats.last_refinements = sweeps;
     }
-    launch_update(n, a.dv, a.x, a.x, a.v, dt, a.invalid, s);
 }
 
 void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, const double4* mat,

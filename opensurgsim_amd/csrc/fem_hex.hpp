@@ -66,6 +66,9 @@ struct HexStepArgs {
     int* predicted;         // [17]: FP64 solve, then per refinement sweep
 };
 void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
+// The linear solve of hex_step alone: dv = A^-1 b with the operator cached by the last step (b's boundary
+// rows already zero).
+void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
 
 // FP64 element outputs for tests: K (24x24 row-major) and f = -K (x - x0).
 void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, const double4* mat,

@@ -25,7 +25,8 @@ FP64, FP32_REFINED = 0, 1
 EXPORTS = [
     "fem_create", "fem_destroy", "fem_last_error", "fem_set_stream", "fem_upload_mesh",
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
-    "fem_set_options", "fem_set_external_force", "fem_add_external_generalized_force", "fem_set_state", "fem_get_state", "fem_set_state_device",
+    "fem_set_options", "fem_set_external_force", "fem_add_external_generalized_force", "fem_apply_compliance",
+    "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
     "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
@@ -111,6 +112,7 @@ def load():
     L.fem_set_options.argtypes = [_vp, ctypes.POINTER(FemOptions)]
     L.fem_set_external_force.argtypes = [_vp, _dp]
     L.fem_add_external_generalized_force.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp, _dp]
+    L.fem_apply_compliance.argtypes = [_vp, _dp, ctypes.c_int64, _dp]
     L.fem_set_state.argtypes = [_vp, _dp, _dp]
     L.fem_get_state.argtypes = [_vp, _dp, _dp]
     L.fem_set_state_device.argtypes = [_vp, _vp, _vp]
@@ -258,6 +260,16 @@ class FemGpu:
         d = _f64(D).reshape(-1) if D is not None else None
         self._check(self._L.fem_add_external_generalized_force(self.h, int(element), _ptr(arrs[0]), _ptr(arrs[1]),
                                                                _ptr(k), _ptr(d)))
+
+    def apply_compliance(self, b):
+        """C b for b of shape (ndof,) or (ndof, k): the last step's system matrix solved per column with
+        boundary rows zeroed (DeformableRepresentation::applyCompliance)."""
+        b = np.asarray(b, dtype=np.float64)
+        vec = b.ndim == 1
+        cols = np.ascontiguousarray((b[:, None] if vec else b).T)  # column-major: one row per column here
+        out = np.zeros_like(cols)
+        self._check(self._L.fem_apply_compliance(self.h, _ptr(cols), cols.shape[0], _ptr(out)))
+        return out[0] if vec else out.T
 
     def set_state(self, x=None, v=None):
         xa = _f64(x).reshape(-1) if x is not None else None

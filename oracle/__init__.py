@@ -52,6 +52,7 @@ def lib():
         L.orc_sim_set_damping.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_sim_add_localized_force.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_sim_update_fmdk.argtypes = [ctypes.c_void_p]
+        L.orc_sim_apply_compliance.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.orc_sim_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_long, ctypes.c_long, ctypes.c_int]
         L.orc_sim_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -207,6 +208,16 @@ class Sim:
         """Rebuild f, M, D, K at the current state as computeFMDK does, without stepping."""
         if lib().orc_sim_update_fmdk(self.h) < 0:
             raise RuntimeError(lib().orc_last_error().decode())
+
+    def apply_compliance(self, b):
+        """C b (b of shape (ndof,) or (ndof, k)) with the last system matrix (applyCompliance)."""
+        b = np.asarray(b, dtype=np.float64)
+        vec = b.ndim == 1
+        cols = np.ascontiguousarray((b[:, None] if vec else b).T)
+        out = np.zeros_like(cols)
+        if lib().orc_sim_apply_compliance(self.h, _d(cols), cols.shape[0], _d(out)) < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out[0] if vec else out.T
 
     def set_solver(self, scheme=EULER_IMPLICIT, linsolver=CG, tol=2.220446049250313e-16, max_iter=-1,
                    newton_max_iter=1, faithful_bc=False):

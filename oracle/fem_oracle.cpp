@@ -1812,6 +1812,24 @@ int orc_sim_update_fmdk(void* h) {
     }
 }
 
+// DeformableRepresentation::applyCompliance (Physics/DeformableRepresentation.cpp:175-190) with the last
+// system matrix, column by column (num_columns column-major columns of ndof values).
+int orc_sim_apply_compliance(void* h, const double* b, int num_columns, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        const size_t n = s->x.size();
+        for (int j = 0; j < num_columns; ++j) {
+            std::vector<double> col(b + (size_t)j * n, b + (size_t)(j + 1) * n);
+            std::vector<double> r = s->apply_compliance(col);
+            std::memcpy(out + (size_t)j * n, r.data(), n * sizeof(double));
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 void orc_sim_matrix_values(void* h, int which, double* vals) {
     Sim* s = static_cast<Sim*>(h);
     const Csc& S = which == 0 ? s->M : (which == 1 ? s->K : which == 3 ? s->D : s->A);

@@ -198,3 +198,50 @@ def test_unsupported_combinations_rejected():
         t.step(1)
     with pytest.raises(femgpu.FemError):
         t.add_external_generalized_force(10 ** 6, np.full(4, 0.25), [0, 0, 1.0])
+
+
+def _fixed_point_h(cfg, element, coord):
+    """H of a fixed-point constraint at natural coordinates of an element (three rows, one per axis), as
+    Physics/FemConstraintFixedPoint.cpp builds it; returned as the ndof x 3 matrix H^T."""
+    nn = cfg.conn.shape[1]
+    Ht = np.zeros((cfg.x0.size, 3))
+    for a in range(nn):
+        for k in range(3):
+            Ht[3 * cfg.conn[element, a] + k, k] = coord[a]
+    return Ht
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+@pytest.mark.parametrize("kind", ["corotational", "hex"])
+def test_apply_compliance_vs_oracle(precision, kind):
+    if kind == "hex":
+        cfg = mg.config_hex((4, 3, 5))
+        coord = np.full(8, 0.125)
+    else:
+        cfg = mg.config_corotational((6, 6, 6), seed=3)
+        coord = np.array([0.1, 0.2, 0.3, 0.4])
+    say("apply compliance", kind, "precision", precision)
+    o = make_oracle(cfg, tol=1e-12)
+    g = make_gpu(cfg, precision=precision, tol=1e-12)
+    with pytest.raises(femgpu.FemError):
+        g.apply_compliance(np.zeros(cfg.x0.size))  # no system matrix before the first step
+    o.step(cfg.dt, 3)
+    assert g.step(3) == femgpu.FEM_OK
+    stats = g.stats()
+    rng = np.random.default_rng(4)
+    B = np.concatenate([rng.normal(size=(cfg.x0.size, 3)), _fixed_point_h(cfg, 7, coord)], axis=1)
+    X = g.apply_compliance(B)
+    Xr = o.apply_compliance(B)
+    err = np.linalg.norm(X - Xr) / np.linalg.norm(Xr)
+    say("  rel err", err)
+    assert err < (1e-9 if precision == femgpu.FP64 else 1e-8)
+    bc = (3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)).reshape(-1)
+    assert np.all(X[bc] == 0.0)
+    assert g.stats() == stats  # the step's statistics are untouched
+    # the state moves on unchanged by the compliance solves
+    o.step(cfg.dt, 1)
+    assert g.step(1) == femgpu.FEM_OK
+    xr, _ = o.state()
+    x, _ = g.state()
+    x0 = cfg.x0.reshape(-1)
+    assert rel(x - x0, xr - x0) < (1e-9 if precision == femgpu.FP64 else 1e-6)

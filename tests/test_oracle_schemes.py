@@ -265,3 +265,26 @@ def test_localized_external_force_stiffness_damping():
     assert np.array_equal(a.force(), b.force())
     assert np.array_equal(a.matrix_values(1), b.matrix_values(1))
     assert np.array_equal(a.matrix_values(3), b.matrix_values(3))
+
+
+@pytest.mark.parametrize("scheme", [oracle.EULER_IMPLICIT, oracle.EULER_EXPLICIT, oracle.STATIC])
+def test_apply_compliance_equals_dense_compliance(scheme):
+    """applyCompliance (zero boundary rows, solve, zero boundary rows; Physics/DeformableRepresentation.cpp:175-190)
+    equals the dense compliance matrix of OdeSolver::computeComplianceMatrixFromSystemMatrix (the inverse of the
+    system matrix with boundary rows and columns zeroed, Math/OdeSolver.cpp:69-76) applied to the columns."""
+    x0, tets, bc, K, M, free, u, v = setup(6)
+    s = make_sim(x0, tets, bc, scheme, AM, AK)
+    s.set_state(x=x0.reshape(-1) + u, v=v)
+    s.step(DT, 1)
+    n = x0.size
+    outer, inner = s.pattern()
+    A = np.zeros((n, n))
+    A[inner, np.repeat(np.arange(n), np.diff(outer))] = s.matrix_values(2)
+    C = np.linalg.inv(A)
+    bcd = np.setdiff1d(np.arange(n), free)
+    C[bcd, :] = 0.0
+    C[:, bcd] = 0.0
+    B = np.random.default_rng(1).normal(size=(n, 5))
+    X = s.apply_compliance(B)
+    assert np.allclose(X, C @ B, rtol=0, atol=1e-10 * np.abs(C @ B).max())
+    assert np.array_equal(s.apply_compliance(B[:, 2]), X[:, 2])
