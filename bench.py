@@ -197,6 +197,108 @@ def bench_hex(args):
     print(json.dumps(out), flush=True)
 
 
+SCHEME_NAMES = {femgpu.EULER_EXPLICIT: "euler_explicit", femgpu.EULER_EXPLICIT_MODIFIED: "euler_explicit_modified",
+                femgpu.RUNGE_KUTTA_4: "runge_kutta_4", femgpu.STATIC: "static",
+                femgpu.LINEAR_EULER_EXPLICIT: "linear_euler_explicit",
+                femgpu.LINEAR_RUNGE_KUTTA_4: "linear_runge_kutta_4", femgpu.LINEAR_STATIC: "linear_static"}
+
+
+def bench_schemes(args):
+    """The other integration schemes on the 40^3 corotational block: explicit Euler, modified explicit Euler,
+    Runge-Kutta 4 (dt 0.1 ms, within their stability limit) and static, with their linear variants; Jacobi-CG
+    to 1e-8 on M / dt (K for static), FP64. Rayleigh damping alpha_M = 1, alpha_K = 1e-4."""
+    n = (args.n, args.n, args.n)
+    cfg = mg.config_corotational(n)
+    N = cfg.x0.shape[0]
+    D = 3 * N
+    log("workload: schemes on corotational tets %s, %d DOF" % (n, D))
+    res = {}
+    for sc, name in SCHEME_NAMES.items():
+        static = sc in (femgpu.STATIC, femgpu.LINEAR_STATIC)
+        dt = 1e-3 if static else 1e-4
+        steps = 4 if static else 20
+        g = femgpu.FemGpu(0)
+        g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+        g.add_bc_nodes(cfg.bc_nodes)
+        g.set_options(dt=dt, cg_tolerance=cfg.cg_tol, precision=femgpu.FP64, gravity=cfg.gravity, scheme=sc,
+                      rayleigh_mass=1.0, rayleigh_stiffness=1e-4)
+        g.set_state(v=cfg.v0)
+        g.step(2)
+        device_sync(g)
+        it0 = g.stats()["total_cg_iterations"]
+        t0 = time.perf_counter()
+        rc = g.step(steps)
+        device_sync(g)
+        t = time.perf_counter() - t0
+        assert rc == femgpu.FEM_OK, (name, rc)
+        it = g.stats()["total_cg_iterations"] - it0
+        g.close()
+        res[name] = {"dof_steps_per_s": D * steps / t, "ms_per_step": 1e3 * t / steps, "dt": dt,
+                     "cg_iters_per_step": it / steps}
+        log("  %-26s %8.3f ms/step, %7.1f CG iterations/step" % (name, 1e3 * t / steps, it / steps))
+    main_name = "runge_kutta_4"
+    out = {"metric": METRIC, "value": res[main_name]["dof_steps_per_s"], "unit": "DOF-steps/s", "n_gpus": 1,
+           "steps": 20, "warmup": 2, "ms_per_step": res[main_name]["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "corotational-tet-block %dx%dx%d, Runge-Kutta 4 dt=0.1ms (other schemes below), "
+                                  "Rayleigh 1 / 1e-4, Jacobi-CG 1e-8" % n, "dof": int(D),
+                      "parallelism": "single-gpu"},
+           "schemes": res}
+    print(json.dumps(out), flush=True)
+
+
+def bench_compliance(args):
+    """Compliance solves on the 40^3 corotational block after warm-up steps: C H^T for 8 fixed-point
+    constraints (24 columns, the getComplianceMatrix() * H^T of Physics/FemConstraintFixedPoint.cpp:107) through
+    fem_apply_compliance, FP64 and FP32 + refinement; checked by the residual of the system matrix."""
+    import scipy.sparse as sps
+    n = (args.n, args.n, args.n)
+    cfg = mg.config_corotational(n)
+    N = cfg.x0.shape[0]
+    D = 3 * N
+    ncon = 8
+    rng = np.random.default_rng(3)
+    elems = rng.choice(cfg.conn.shape[0], ncon, replace=False)
+    Ht = np.zeros((D, 3 * ncon))
+    for i, e in enumerate(elems):
+        coord = rng.dirichlet(np.ones(4))
+        for a in range(4):
+            for k in range(3):
+                Ht[3 * cfg.conn[e, a] + k, 3 * i + k] = coord[a]
+    log("workload: compliance C H^T, %d columns, corotational tets %s, %d DOF" % (Ht.shape[1], n, D))
+    res = {}
+    for prec, name in ((femgpu.FP64, "fp64"), (femgpu.FP32_REFINED, "fp32_refined")):
+        g = make_gpu(cfg, prec)
+        g.step(args.warmup)
+        X = g.apply_compliance(Ht)  # warm
+        device_sync(g)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            X = g.apply_compliance(Ht)
+        t = (time.perf_counter() - t0) / reps
+        rowptr, colidx = g.pattern()
+        A = sps.csr_matrix((g.system_matrix(), colidx, rowptr), shape=(D, D))
+        bc = (3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)).reshape(-1)
+        B = Ht.copy()
+        B[bc] = 0.0
+        resid = float(np.linalg.norm(A @ X - B) / np.linalg.norm(B))
+        g.close()
+        res[name] = {"ms_per_column": 1e3 * t / Ht.shape[1], "columns_per_s": Ht.shape[1] / t,
+                     "relative_residual": resid}
+        log("  %s: %.3f ms per column, residual %.2e" % (name, 1e3 * t / Ht.shape[1], resid))
+    best = min(res, key=lambda k: res[k]["ms_per_column"])
+    out = {"metric": "compliance columns/s (C H^T, applyCompliance)", "value": res[best]["columns_per_s"],
+           "unit": "columns/s", "n_gpus": 1, "steps": 3, "warmup": args.warmup,
+           "ms_per_step": res[best]["ms_per_column"] * Ht.shape[1], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64refine" if best == "fp32_refined" else "f64", "data": "synthetic",
+           "config": {"workload": "C H^T of %d fixed-point constraints (%d columns) on corotational-tet-block "
+                                  "%dx%dx%d after %d implicit steps, CG 1e-8" % ((ncon, 3 * ncon) + n + (args.warmup,)),
+                      "dof": int(D), "parallelism": "single-gpu"},
+           "precisions": res}
+    print(json.dumps(out), flush=True)
+
+
 def bench_adjoint(args):
     """Adjoint material identification workload: 20x20x20 corotational tet block (40,000 tets), per-cube
     log-normal ground-truth E, top pressure 200 Pa, 50 implicit steps at dt = 2 ms; loss = squared
@@ -411,7 +513,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--timer-every", type=int, default=10,
                     help="bracket every kernel of one step in N with HIP events (0: no events)")
-    ap.add_argument("--workload", default="corotational", choices=["corotational", "hex", "adjoint", "batch"])
+    ap.add_argument("--workload", default="corotational",
+                    choices=["corotational", "hex", "adjoint", "batch", "schemes", "compliance"])
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-maxit", type=int, default=2000,
                     help="CG iteration cap per instance (-1: Eigen's 2 x DOF)")
@@ -431,6 +534,12 @@ def main():
         sys.exit(2)
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "schemes":
+        bench_schemes(args)
+        return
+    if args.workload == "compliance":
+        bench_compliance(args)
         return
     if args.workload == "batch":
         bench_batch(args)
