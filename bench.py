@@ -252,6 +252,8 @@ def bench_compliance(args):
     constraints (24 columns, the getComplianceMatrix() * H^T of Physics/FemConstraintFixedPoint.cpp:107) through
     fem_apply_compliance, FP64 and FP32 + refinement; checked by the residual of the system matrix."""
     import scipy.sparse as sps
+    import torch  # device buffers for the device-pointer variant (started before the library's runtime)
+    torch.cuda.init()
     n = (args.n, args.n, args.n)
     cfg = mg.config_corotational(n)
     N = cfg.x0.shape[0]
@@ -276,7 +278,18 @@ def bench_compliance(args):
         t0 = time.perf_counter()
         for _ in range(reps):
             X = g.apply_compliance(Ht)
+        t_host = (time.perf_counter() - t0) / reps
+        bd = torch.from_numpy(np.ascontiguousarray(Ht.T)).cuda()
+        xd = torch.empty_like(bd)
+        torch.cuda.synchronize()
+        g.apply_compliance_device(bd.data_ptr(), Ht.shape[1], xd.data_ptr())
+        device_sync(g)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.apply_compliance_device(bd.data_ptr(), Ht.shape[1], xd.data_ptr())
+        device_sync(g)
         t = (time.perf_counter() - t0) / reps
+        dev_diff = float(np.abs(xd.cpu().numpy().T - X).max() / np.abs(X).max())
         rowptr, colidx = g.pattern()
         A = sps.csr_matrix((g.system_matrix(), colidx, rowptr), shape=(D, D))
         bc = (3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)).reshape(-1)
@@ -285,8 +298,10 @@ def bench_compliance(args):
         resid = float(np.linalg.norm(A @ X - B) / np.linalg.norm(B))
         g.close()
         res[name] = {"ms_per_column": 1e3 * t / Ht.shape[1], "columns_per_s": Ht.shape[1] / t,
-                     "relative_residual": resid}
-        log("  %s: %.3f ms per column, residual %.2e" % (name, 1e3 * t / Ht.shape[1], resid))
+                     "ms_per_column_host_pointers": 1e3 * t_host / Ht.shape[1], "relative_residual": resid,
+                     "device_vs_host_max_rel_diff": dev_diff}
+        log("  %s: %.3f ms per column (device pointers), %.3f (host pointers), residual %.2e" %
+            (name, 1e3 * t / Ht.shape[1], 1e3 * t_host / Ht.shape[1], resid))
     best = min(res, key=lambda k: res[k]["ms_per_column"])
     out = {"metric": "compliance columns/s (C H^T, applyCompliance)", "value": res[best]["columns_per_s"],
            "unit": "columns/s", "n_gpus": 1, "steps": 3, "warmup": args.warmup,

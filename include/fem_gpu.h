@@ -154,6 +154,10 @@ int fem_set_state(fem_ctx* ctx, const double* x, const double* v);
  * inverse of OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76). FEM_ERR_ORDER before
  * the first step. The step statistics are left as the last step set them. */
 int fem_apply_compliance(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
+/* The same with device pointers (column-major, ndof x num_columns), ordered on the context stream. On a
+ * tetrahedral mesh the columns are solved up to 8 at a time by a multi-column FP64 CG that reads the
+ * operator once per iteration for all of them (each column keeps its own Eigen CG recurrence and test). */
+int fem_apply_compliance_device(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
 int fem_get_state(fem_ctx* ctx, double* x, double* v);
 int fem_set_state_device(fem_ctx* ctx, const double* x, const double* v);
 int fem_get_state_device(fem_ctx* ctx, double* x, double* v);

@@ -13,6 +13,7 @@
 #include "fem_adjoint.hpp"
 #include "fem_batch.hpp"
 #include "fem_hex.hpp"
+#include "fem_mcg.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -224,6 +225,11 @@ struct fem_ctx {
 
     // Runge-Kutta 4 stage vectors
     DevBuf<double> rk_x, rk_k2v, rk_k3v, rk_k4v, rk_a1, rk_a2, rk_a3;
+
+    // multi-column compliance solves (fem_mcg.hip): [ndof][8] interleaved vectors, per-column state
+    DevBuf<double> mcg_b, mcg_x, mcg_r, mcg_z, mcg_p, mcg_q, mcg_dinv, mcg_part;
+    DevBuf<McgColumn> mcg_col;
+    McgColumn* h_mcg = nullptr;  // pinned [8]
 
     // hexahedra (element-by-element path)
     HexData hex;
@@ -1462,6 +1468,7 @@ void fem_destroy(fem_ctx* c) {
     if (c->h_refine_iters) (void)hipHostFree(c->h_refine_iters);
     if (c->h_bsc) (void)hipHostFree(c->h_bsc);
     if (c->h_live) (void)hipHostFree(c->h_live);
+    if (c->h_mcg) (void)hipHostFree(c->h_mcg);
     for (int i = 0; i < 2; ++i) {
         if (c->b_exec[i]) (void)hipGraphExecDestroy(c->b_exec[i]);
         if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
@@ -1684,46 +1691,127 @@ int fem_add_external_generalized_force(fem_ctx* c, int64_t element, const double
     });
 }
 
+namespace {
+// Compliance columns of a tetrahedral mesh, up to 8 at a time through the multi-column FP64 CG: the columns
+// (device, column-major) are interleaved [dof][kw] with their boundary rows zeroed, solved together against
+// S64, and scattered back with the boundary rows of the solutions zeroed.
+void compliance_tet(fem_ctx* c, const double* b, int64_t ncols, double* out) {
+    const size_t n = (size_t)c->ndof;
+    const int max_it = (int)std::min<int64_t>(max_iters(c), std::numeric_limits<int>::max() - 1);
+    if (c->mcg_b.n != n * 8) {
+        for (DevBuf<double>* v : {&c->mcg_b, &c->mcg_x, &c->mcg_r, &c->mcg_z, &c->mcg_p, &c->mcg_q}) v->alloc(n * 8);
+        c->mcg_dinv.alloc(n);
+        c->mcg_part.alloc((size_t)3 * kCgBlocks * 8);
+        c->mcg_col.alloc(8);
+        if (!c->h_mcg) HIPCHECK(hipHostMalloc((void**)&c->h_mcg, 8 * sizeof(McgColumn), hipHostMallocDefault));
+    }
+    for (int64_t done = 0; done < ncols;) {
+        const int64_t left = ncols - done;
+        const int kw = left >= 5 ? 8 : left >= 3 ? 4 : (int)left;
+        const int cnt = (int)std::min<int64_t>(left, kw);
+        launch_mcg_gather((int64_t)n, kw, cnt, b + (size_t)done * n, c->bcmask.p, c->mcg_b.p, c->stream);
+        McgArgs a;
+        a.kw = kw;
+        a.num_nodes = (int)c->num_nodes;
+        a.rowptr = c->rowptr_b.p;
+        a.colidx = c->colidx_b.p;
+        a.diagblk = c->diagblk.p;
+        a.S = c->S64.p;
+        a.b = c->mcg_b.p;
+        a.x = c->mcg_x.p;
+        a.r = c->mcg_r.p;
+        a.z = c->mcg_z.p;
+        a.p = c->mcg_p.p;
+        a.q = c->mcg_q.p;
+        a.dinv = c->mcg_dinv.p;
+        a.partial = c->mcg_part.p;
+        a.col = c->mcg_col.p;
+        a.tol = c->opt.cg_tolerance;
+        a.max_iter = max_it;
+        launch_mcg_init(a, c->stream);
+        const int pred = std::max(4, c->predicted_iters + 2);
+        int k = 0;
+        for (; k < std::min(pred, max_it); ++k) launch_mcg_iteration(a, k, c->stream);
+        for (;;) {
+            HIPCHECK(hipMemcpyAsync(c->h_mcg, c->mcg_col.p, kw * sizeof(McgColumn), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            bool running = false;
+            for (int j = 0; j < kw; ++j) running = running || c->h_mcg[j].fin == kMcgRunning;
+            if (!running || k >= max_it) break;
+            const int chunk = std::max(2, pred / 8);
+            for (int i = 0; i < chunk && k < max_it; ++i, ++k) launch_mcg_iteration(a, k, c->stream);
+        }
+        launch_mcg_scatter((int64_t)n, kw, cnt, c->mcg_x.p, c->bcmask.p, out + (size_t)done * n, c->stream);
+        done += cnt;
+    }
+}
+
+// C b for num_columns device columns (column-major) on the context stream.
+void compliance_device(fem_ctx* c, const double* b, int64_t num_columns, double* out) {
+    ensure_bc(c);
+    alloc_precision_buffers(c);
+    const size_t n = (size_t)c->ndof;
+    // the step's statistics and iteration predictions are the step's, not the compliance solves'
+    const fem_stats saved = c->stats;
+    const int saved_iters = c->predicted_iters, saved_sweeps = c->predicted_sweeps;
+    int saved_sweep[16], saved_hex[17], saved_batch[17];
+    std::memcpy(saved_sweep, c->predicted_sweep, sizeof saved_sweep);
+    std::memcpy(saved_hex, c->predicted_hex, sizeof saved_hex);
+    std::memcpy(saved_batch, c->b_predicted, sizeof saved_batch);
+    if (c->kind != FEM_ELEMENT_HEX && c->num_instances == 0) {
+        compliance_tet(c, b, num_columns, out);
+    } else {
+        for (int64_t j = 0; j < num_columns; ++j) {
+            launch_masked_copy((int64_t)n, b + (size_t)j * n, c->bcmask.p, c->b.p, c->stream);
+            if (c->kind == FEM_ELEMENT_HEX) {
+                hex_solve(c->hex, hex_args(c), c->stats, c->stream);
+            } else {
+                HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+                batch_linear_solve(c);
+            }
+            launch_masked_copy((int64_t)n, c->dv.p, c->bcmask.p, out + (size_t)j * n, c->stream);
+        }
+    }
+    c->stats = saved;
+    c->predicted_iters = saved_iters;
+    c->predicted_sweeps = saved_sweeps;
+    std::memcpy(c->predicted_sweep, saved_sweep, sizeof saved_sweep);
+    std::memcpy(c->predicted_hex, saved_hex, sizeof saved_hex);
+    std::memcpy(c->b_predicted, saved_batch, sizeof saved_batch);
+}
+}  // namespace
+
 int fem_apply_compliance(fem_ctx* c, const double* b, int64_t num_columns, double* out) {
     return guarded(c, [&] {
         if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
         if (num_columns < 0 || (num_columns > 0 && (!b || !out))) throw InvalidArg("null or negative columns");
         if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
         HIPCHECK(hipSetDevice(c->device));
-        ensure_bc(c);
-        alloc_precision_buffers(c);
         const size_t n = (size_t)c->ndof;
-        // the step's statistics and iteration predictions are the step's, not the compliance solves'
-        const fem_stats saved = c->stats;
-        const int saved_iters = c->predicted_iters, saved_sweeps = c->predicted_sweeps;
-        int saved_sweep[16], saved_hex[17], saved_batch[17];
-        std::memcpy(saved_sweep, c->predicted_sweep, sizeof saved_sweep);
-        std::memcpy(saved_hex, c->predicted_hex, sizeof saved_hex);
-        std::memcpy(saved_batch, c->b_predicted, sizeof saved_batch);
-        std::vector<double> col(n);
-        for (int64_t j = 0; j < num_columns; ++j) {
-            std::memcpy(col.data(), b + (size_t)j * n, n * sizeof(double));
-            for (int64_t d : c->h_bc_dofs) col[(size_t)d] = 0.0;
-            c->b.upload(col.data(), n, c->stream);
-            if (c->kind == FEM_ELEMENT_HEX) {
-                hex_solve(c->hex, hex_args(c), c->stats, c->stream);
-            } else if (c->num_instances > 0) {
-                HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
-                batch_linear_solve(c);
-            } else {
-                linear_solve(c);
-            }
-            double* o = out + (size_t)j * n;
-            HIPCHECK(hipMemcpyAsync(o, c->dv.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        // stage through device memory, up to 64 columns at a time
+        const int64_t chunk = std::min<int64_t>(std::max<int64_t>(num_columns, 1), 64);
+        DevBuf<double> db, dx;
+        db.alloc(n * chunk);
+        dx.alloc(n * chunk);
+        for (int64_t j0 = 0; j0 < num_columns; j0 += chunk) {
+            const int64_t m = std::min(chunk, num_columns - j0);
+            db.upload(b + (size_t)j0 * n, n * m, c->stream);
+            compliance_device(c, db.p, m, dx.p);
+            HIPCHECK(hipMemcpyAsync(out + (size_t)j0 * n, dx.p, n * m * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
             HIPCHECK(hipStreamSynchronize(c->stream));
-            for (int64_t d : c->h_bc_dofs) o[d] = 0.0;
         }
-        c->stats = saved;
-        c->predicted_iters = saved_iters;
-        c->predicted_sweeps = saved_sweeps;
-        std::memcpy(c->predicted_sweep, saved_sweep, sizeof saved_sweep);
-        std::memcpy(c->predicted_hex, saved_hex, sizeof saved_hex);
-        std::memcpy(c->b_predicted, saved_batch, sizeof saved_batch);
+        return FEM_OK;
+    });
+}
+
+int fem_apply_compliance_device(fem_ctx* c, const double* b, int64_t num_columns, double* out) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        if (num_columns < 0 || (num_columns > 0 && (!b || !out))) throw InvalidArg("null or negative columns");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        HIPCHECK(hipSetDevice(c->device));
+        compliance_device(c, b, num_columns, out);
         return FEM_OK;
     });
 }

@@ -26,6 +26,7 @@ EXPORTS = [
     "fem_create", "fem_destroy", "fem_last_error", "fem_set_stream", "fem_upload_mesh",
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
     "fem_set_options", "fem_set_external_force", "fem_add_external_generalized_force", "fem_apply_compliance",
+    "fem_apply_compliance_device",
     "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
@@ -113,6 +114,7 @@ def load():
     L.fem_set_external_force.argtypes = [_vp, _dp]
     L.fem_add_external_generalized_force.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp, _dp]
     L.fem_apply_compliance.argtypes = [_vp, _dp, ctypes.c_int64, _dp]
+    L.fem_apply_compliance_device.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
     L.fem_set_state.argtypes = [_vp, _dp, _dp]
     L.fem_get_state.argtypes = [_vp, _dp, _dp]
     L.fem_set_state_device.argtypes = [_vp, _vp, _vp]
@@ -270,6 +272,11 @@ class FemGpu:
         out = np.zeros_like(cols)
         self._check(self._L.fem_apply_compliance(self.h, _ptr(cols), cols.shape[0], _ptr(out)))
         return out[0] if vec else out.T
+
+    def apply_compliance_device(self, b_ptr, num_columns, out_ptr):
+        """C b with device pointers (column-major ndof x num_columns FP64), ordered on the context stream."""
+        self._check(self._L.fem_apply_compliance_device(self.h, ctypes.c_void_p(b_ptr), int(num_columns),
+                                                        ctypes.c_void_p(out_ptr)))
 
     def set_state(self, x=None, v=None):
         xa = _f64(x).reshape(-1) if x is not None else None

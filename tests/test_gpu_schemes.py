@@ -245,3 +245,41 @@ def test_apply_compliance_vs_oracle(precision, kind):
     x, _ = g.state()
     x0 = cfg.x0.reshape(-1)
     assert rel(x - x0, xr - x0) < (1e-9 if precision == femgpu.FP64 else 1e-6)
+
+
+def test_apply_compliance_many_columns():
+    """Columns go through the solver up to 8 at a time (here 8 + 3 padded to 4): a zero column returns zero,
+    equal columns of one batch return equal solutions, every column matches the oracle's solve."""
+    cfg = mg.config_corotational((5, 6, 4), seed=6)
+    o = make_oracle(cfg, tol=1e-12)
+    g = make_gpu(cfg, tol=1e-12)
+    o.step(cfg.dt, 2)
+    assert g.step(2) == femgpu.FEM_OK
+    rng = np.random.default_rng(8)
+    B = rng.normal(size=(cfg.x0.size, 11))
+    B[:, 4] = 0.0
+    B[:, 6] = B[:, 1]
+    X = g.apply_compliance(B)
+    Xr = o.apply_compliance(B)
+    err = np.linalg.norm(X - Xr) / np.linalg.norm(Xr)
+    say("many columns: rel err", err)
+    assert err < 1e-9
+    assert np.all(X[:, 4] == 0.0)
+    assert np.array_equal(X[:, 6], X[:, 1])
+    x1 = g.apply_compliance(B[:, 9])  # a single column on its own
+    assert np.linalg.norm(x1 - Xr[:, 9]) <= 1e-9 * np.linalg.norm(Xr[:, 9])
+
+
+def test_apply_compliance_device_pointers():
+    import torch
+    cfg = mg.config_corotational((4, 5, 4), seed=9)
+    g = make_gpu(cfg, tol=1e-12)
+    assert g.step(2) == femgpu.FEM_OK
+    B = np.random.default_rng(2).normal(size=(cfg.x0.size, 5))
+    X = g.apply_compliance(B)
+    bd = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+    xd = torch.zeros_like(bd)
+    torch.cuda.synchronize()
+    g.apply_compliance_device(bd.data_ptr(), B.shape[1], xd.data_ptr())
+    g.synchronize()
+    assert np.array_equal(xd.cpu().numpy().T, X)
