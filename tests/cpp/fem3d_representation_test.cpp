@@ -33,6 +33,9 @@ void orc_sim_get_state(void* h, double* x, double* v);
 int orc_sim_step(void* h, double dt, int nsteps);
 long orc_sim_nnz(void* h);
 void orc_sim_pattern(void* h, int* outer, int* inner);
+void orc_sim_set_damping(void* h, double rayleigh_mass, double rayleigh_stiffness, int lumping);
+int orc_sim_add_localized_force(void* h, int element, const double* coord, const double* force, const double* K,
+                                const double* D);
 }
 
 // ---- minimal test harness ------------------------------------------------------------------------
@@ -381,11 +384,6 @@ void cpuTests() {
             fem->addFemElement(std::make_shared<Physics::Fem3DElementCube>(std::array<size_t, 8>{0, 1, 2, 3, 4, 5, 6, 7}));
             EXPECT_THROW_ASSERT(fem->initialize());  // mixed element classes
         }
-        {
-            auto fem = makeRepresentation(b, kCoro, m, {});
-            fem->setIntegrationScheme(Math::INTEGRATIONSCHEME_STATIC);
-            EXPECT_THROW_ASSERT(fem->initialize());  // scheme not provided
-        }
     });
 }
 
@@ -429,7 +427,84 @@ void compareWithOracle(const std::string& type, int kind, Math::IntegrationSchem
     orc_sim_destroy(o);
 }
 
+// The oracle's (and the library's) number of a Math::IntegrationScheme (Math/OdeSolver.h:36-49).
+int oracleScheme(Math::IntegrationScheme s) {
+    switch (s) {
+        case Math::INTEGRATIONSCHEME_EULER_IMPLICIT: return 0;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT: return 1;
+        case Math::INTEGRATIONSCHEME_STATIC: return 2;
+        case Math::INTEGRATIONSCHEME_LINEAR_STATIC: return 3;
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT: return 4;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT: return 5;
+        case Math::INTEGRATIONSCHEME_EULER_EXPLICIT_MODIFIED: return 6;
+        case Math::INTEGRATIONSCHEME_LINEAR_EULER_EXPLICIT_MODIFIED: return 7;
+        case Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4: return 8;
+        default: return 9;
+    }
+}
+
+// A representation with Rayleigh damping (and mass lumping) under scheme, element-localized external terms
+// before the first update, against the oracle.
+void compareDampedScheme(Math::IntegrationScheme scheme, bool lumping, bool localized, double dt, int steps) {
+    const Block b = makeBlock(3, 3, 2);
+    const Material m{5e3, 0.45, 1000.0};
+    const auto v0 = topVelocities(b, 0.02);
+    auto fem = makeRepresentation(b, kCoro, m, v0);
+    fem->setIntegrationScheme(scheme);
+    fem->setLinearSolver(Math::LINEARSOLVER_CONJUGATEGRADIENT);
+    fem->setRayleighDampingMass(2.0);
+    fem->setRayleighDampingStiffness(1e-4);
+    fem->setMassLumping(lumping);
+    EXPECT_TRUE(fem->getRayleighDampingMass() == 2.0 && fem->getRayleighDampingStiffness() == 1e-4);
+    EXPECT_TRUE(fem->getMassLumping() == lumping);
+    fem->initialize();
+    std::dynamic_pointer_cast<Math::LinearSparseSolveAndInverseCG>(fem->getOdeSolver()->getLinearSolver())
+        ->setTolerance(1e-12);
+    void* o = makeOracle(b, 1, m, v0, oracleScheme(scheme), 1, 1e-12);
+    orc_sim_set_damping(o, 2.0, 1e-4, lumping ? 1 : 0);
+    const Math::Vector coord{0.1, 0.2, 0.3, 0.4};
+    const Math::Vector3d force{{0.02, -0.05, 0.01}};
+    const Math::Matrix33d K{{40.0, 2.0, 0.0, 1.0, 50.0, 3.0, 0.0, -2.0, 45.0}};
+    const Math::Matrix33d D{{0.1, 0.0, 0.01, 0.0, 0.2, 0.0, 0.02, 0.0, 0.15}};
+    for (int s = 0; s < steps; ++s) {
+        if (localized && s == 0) {
+            fem->addExternalGeneralizedForce(7, coord, force, &K, &D);
+            EXPECT_TRUE(orc_sim_add_localized_force(o, 7, coord.data(), force.data(), K.data(), D.data()) == 0);
+        }
+        fem->update(dt);
+        fem->afterUpdate(dt);
+        EXPECT_TRUE(orc_sim_step(o, dt, 1) == 0);
+    }
+    std::vector<double> xo(b.x0.size()), vo(b.x0.size());
+    orc_sim_get_state(o, xo.data(), vo.data());
+    const auto& x = fem->getCurrentState()->getPositions();
+    std::vector<double> u(x.size()), uo(x.size());
+    for (size_t i = 0; i < x.size(); ++i) {
+        u[i] = x[i] - b.x0[i];
+        uo[i] = xo[i] - b.x0[i];
+    }
+    const double eu = relErr(u, uo);
+    double vn = 0.0;
+    for (double q : vo) vn += q * q;
+    const double ev = vn > 0 ? relErr(fem->getCurrentState()->getVelocities(), vo) : 0.0;
+    std::printf("  %s: rel err u %.3e v %.3e\n", g_current.c_str(), eu, ev);
+    EXPECT_TRUE(eu < 1e-9 && ev < 1e-9);
+    orc_sim_destroy(o);
+}
+
 void gpuTests() {
+    run("Fem3DRepresentationGpuTest.RayleighLumpingLocalizedImplicit", [] {
+        compareDampedScheme(Math::INTEGRATIONSCHEME_EULER_IMPLICIT, true, true, 1e-3, 6);
+    });
+    run("Fem3DRepresentationGpuTest.RayleighRungeKutta4", [] {
+        compareDampedScheme(Math::INTEGRATIONSCHEME_RUNGE_KUTTA_4, false, true, 2e-4, 6);
+    });
+    run("Fem3DRepresentationGpuTest.RayleighModifiedExplicitEuler", [] {
+        compareDampedScheme(Math::INTEGRATIONSCHEME_EULER_EXPLICIT_MODIFIED, false, false, 2e-4, 6);
+    });
+    run("Fem3DRepresentationGpuTest.StaticWithLocalizedStiffness", [] {
+        compareDampedScheme(Math::INTEGRATIONSCHEME_STATIC, false, true, 1e-3, 2);
+    });
     run("Fem3DRepresentationGpuTest.CorotationalImplicitEulerCG", [] {
         compareWithOracle(kCoro, 1, Math::INTEGRATIONSCHEME_EULER_IMPLICIT, Math::LINEARSOLVER_CONJUGATEGRADIENT,
                           1e-12, 20, 1e-9);

@@ -27,11 +27,11 @@ def small_batch(count=5, seed=21):
     return cfgs, mg.concat_instances(cfgs)
 
 
-def run_batch(b, steps, precision=femgpu.FP64, tol=1e-12, tuning=None):
+def run_batch(b, steps, precision=femgpu.FP64, tol=1e-12, tuning=None, **opts):
     g = femgpu.FemGpu(0)
     g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
     g.add_bc_nodes(b.bc_nodes)
-    g.set_options(dt=1e-3, cg_tolerance=tol, precision=precision)
+    g.set_options(dt=1e-3, cg_tolerance=tol, precision=precision, **opts)
     g.set_state(v=b.v0)
     g.set_instances(b.node_offsets, b.dt)
     if tuning:
@@ -44,11 +44,13 @@ def run_batch(b, steps, precision=femgpu.FP64, tol=1e-12, tuning=None):
     return x, v, it, conv, tun
 
 
-def run_oracle(c, steps, tol=1e-13):
+def run_oracle(c, steps, tol=1e-13, damping=None):
     s = oracle.Sim(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
     for n in c.bc_nodes:
         s.add_bc_node(n)
     s.set_solver(tol=tol)
+    if damping:
+        s.set_damping(*damping)
     s.set_state(v=c.v0)
     s.step(c.dt, steps)
     return s.state(), s.last_cg()
@@ -72,6 +74,23 @@ def test_batch_matches_oracle_per_instance():
         x0 = c.x0.reshape(-1)
         e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo))
         say("  instance", i, c.conn.shape[0], "tets, dt", c.dt, e)
+        assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_batch_rayleigh_damping_and_lumping_per_instance():
+    """Rayleigh damping and mass lumping with a time step per instance: the element kernel forms each
+    instance's M (1/dt_i + alpha_M) + K (dt_i + alpha_K)."""
+    cfgs, b = small_batch(4, seed=17)
+    steps = 6
+    x, v, it, conv, _ = run_batch(b, steps, tol=1e-12, rayleigh_mass=2.0, rayleigh_stiffness=1e-4, mass_lumping=1)
+    assert conv.all()
+    off = b.node_offsets
+    for i, c in enumerate(cfgs):
+        (xo, vo), _ = run_oracle(c, steps, tol=1e-12, damping=(2.0, 1e-4, True))
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        x0 = c.x0.reshape(-1)
+        e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo))
+        say("  damped instance", i, "dt", c.dt, e)
         assert e["u"] < 1e-9 and e["v"] < 1e-9
 
 
