@@ -652,6 +652,8 @@ void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const dou
     }
 }
 
+void clear_external(fem_ctx* c);
+
 // Device lists of the localized external terms for the next step (Physics/Fem3DRepresentation.cpp:133-198 and
 // their use in computeFMDK / computeF / computeK, Physics/FemRepresentation.cpp:396-410, 460-477, 527-545):
 //  - operator blocks: D + dt K (implicit Euler) or K (static) under Eigen's lower-triangle view (strictly
@@ -672,8 +674,10 @@ void prepare_external(fem_ctx* c) {
         return;
     }
     const int mode = scheme_mode(c->opt.scheme);
-    if (!c->h_loc_blk.empty() && scheme_is_linear(c->opt.scheme))
+    if (!c->h_loc_blk.empty() && scheme_is_linear(c->opt.scheme)) {
+        clear_external(c);  // the rejected terms do not linger into later steps
         throw InvalidArg("external stiffness and damping are not supported by the linear (constant-operator) schemes");
+    }
     auto node_dt = [&](int64_t node) {
         if (c->num_instances > 0 && !c->h_inst_dt.empty()) {
             const int i = (int)(std::upper_bound(c->h_node_off.begin(), c->h_node_off.end(), node) -
@@ -1671,6 +1675,8 @@ int fem_add_external_generalized_force(fem_ctx* c, int64_t element, const double
         if (element < 0 || element >= c->num_elements) throw InvalidArg("Invalid localization (element out of range)");
         if ((stiffness || damping) && c->kind == FEM_ELEMENT_HEX)
             throw InvalidArg("external stiffness and damping need a tetrahedral mesh");
+        if ((stiffness || damping) && scheme_is_linear(c->opt.scheme))
+            throw InvalidArg("external stiffness and damping are not supported by the linear (constant-operator) schemes");
         const int nn = c->nn;
         if (c->h_loc_f.size() != (size_t)c->ndof) c->h_loc_f.assign((size_t)c->ndof, 0.0);
         const int32_t* ids = c->h_conn.data() + (size_t)nn * element;
