@@ -139,6 +139,23 @@ def cpu_baseline(cfg, sample_steps):
     return x0.size * sample_steps / t, t, err
 
 
+def cpu_faithful(cfg, steps):
+    """The oracle with the reference's own boundary-condition zeroing (zeroRow / zeroColumn through coeff()
+    look-ups of every row and column, O(#BC x D), Math/SparseMatrix.h:352-377), one core, after one warm-up step:
+    the faithful CPU variant, to set beside the restated-efficient one."""
+    import oracle
+    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    for n in cfg.bc_nodes:
+        s.add_bc_node(n)
+    s.set_solver(tol=cfg.cg_tol, faithful_bc=True)
+    s.set_state(v=cfg.v0)
+    s.step(cfg.dt, 1)
+    t0 = time.perf_counter()
+    s.step(cfg.dt, steps)
+    t = time.perf_counter() - t0
+    return cfg.x0.size * steps / t, t
+
+
 def bench_hex(args):
     """Hexahedral block (Fem3DElementCube), 128 x 64 x 128 cubes by default: FP32 element-by-element CG with
     cached element matrices inside FP64 refinement to 1e-6, against the FP64 matrix-free path at 1e-10."""
@@ -526,6 +543,8 @@ def main():
     ap.add_argument("--n", type=int, default=40, help="cubes per axis")
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle sample steps (after 1 warm-up)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-faithful-steps", type=int, default=0,
+                    help="also time the oracle with the reference's O(#BC x D) boundary zeroing (about 15-30 s a step)")
     ap.add_argument("--timer-every", type=int, default=10,
                     help="bracket every kernel of one step in N with HIP events (0: no events)")
     ap.add_argument("--workload", default="corotational",
@@ -617,6 +636,11 @@ def main():
                "sample": "same workload, %d implicit steps after 1 warm-up step, FP64 C++ oracle on one host core "
                          "(literal 12x12 corotational products, O(nnz) boundary conditions)" % args.cpu_steps,
                "gpu_fp64_vs_oracle_rel_err": err_or}
+        if args.cpu_faithful_steps:
+            fps, ft = cpu_faithful(cfg, args.cpu_faithful_steps)
+            log("cpu oracle with the reference's O(#BC x D) boundary zeroing: %.1f DOF-steps/s (%.1f s)" % (fps, ft))
+            cpu["faithful_bc_variant"] = {"value": fps, "unit": "DOF-steps/s", "cores": 1,
+                                          "sample": "%d steps after 1 warm-up step" % args.cpu_faithful_steps}
 
     phases = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in head["phases"].items()}
     copy_gbs = copy_probe(args.copy_probe or 512)
