@@ -387,8 +387,10 @@ __device__ __forceinline__ void group_spmv(const Operator& op, const TM* __restr
     for (int I = r0 + gib; I < r1; I += gpb) {
         const int k0 = op.rowptr[I], k1 = op.rowptr[I + 1];
         TX y0 = 0, y1 = 0, y2 = 0;
+        int jn = k0 + lane < k1 ? op.colidx[k0 + lane] : 0;
         for (int k = k0 + lane; k < k1; k += G) {
-            const int J = op.colidx[k];
+            const int J = jn;
+            if (k + G < k1) jn = op.colidx[k + G];  // the next block's column goes out before this block's gathers
             TX p0, p1, p2;
             fetch(J, p0, p1, p2);
             const TM* B = S + 9 * (int64_t)k;
@@ -691,9 +693,11 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
     // which every lane of the group has after the row sum). FP64 keeps Eigen's diagonal preconditioner.
     constexpr bool kPrefetch = sizeof(TV) == 4;
     TV d_o1 = 0, d_o2 = 0;
+    int j_pf = 0;  // FP32: the column of the lane's first block of the row, fetched with the row's inputs
     auto fetch_row = [&](int row) {
         k0 = op.rowptr[row];
         k1 = op.rowptr[row + 1];
+        if (k0 + lane < k1) j_pf = op.colidx[k0 + lane];
         const int i = 3 * row + cl;
         r_o = ro[i];
         q_o = qo[i];
@@ -788,8 +792,10 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
     } else {
         while (I < row1) {
             TV y0 = 0, y1 = 0, y2 = 0;
+            int jn = j_pf;
             for (int kk = k0 + lane; kk < k1; kk += kRowGroup) {
-                const int J = op.colidx[kk];
+                const int J = jn;
+                if (kk + kRowGroup < k1) jn = op.colidx[kk + kRowGroup];  // the next block's column goes out early
                 TV a0, a1, a2;
                 if (k0it) {
                     a0 = zo[3 * J];
