@@ -49,8 +49,10 @@ __device__ __forceinline__ void chunk_spmv(const Operator& op, const TM* __restr
         TX y0 = 0, y1 = 0, y2 = 0;
         if (live) {
             const int k0 = op.rowptr[I], k1 = op.rowptr[I + 1];
+            int jn = k0 + lane < k1 ? op.colidx[k0 + lane] : 0;
             for (int k = k0 + lane; k < k1; k += G) {
-                const int J = op.colidx[k];
+                const int J = jn;
+                if (k + G < k1) jn = op.colidx[k + G];  // the next block's column goes out before this block's gathers
                 TX p0, p1, p2;
                 fetch(J, p0, p1, p2);
                 const TM* B = S + 9 * (int64_t)k;

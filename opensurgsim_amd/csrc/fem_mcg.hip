@@ -119,9 +119,12 @@ __global__ __launch_bounds__(kCgThreads) void mcg_spmv_kernel(McgArgs a, int k) 
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         if (act) {
             const int b1 = a.rowptr[row + 1];
-            for (int blk = a.rowptr[row] + slot; blk < b1; blk += NS) {
+            int blk = a.rowptr[row] + slot;
+            int jn = blk < b1 ? a.colidx[blk] : 0;
+            for (; blk < b1; blk += NS) {
                 const double* Sb = a.S + 9 * (int64_t)blk;
-                const int64_t cb = 3 * (int64_t)a.colidx[blk] * KW + j;
+                const int64_t cb = 3 * (int64_t)jn * KW + j;
+                if (blk + NS < b1) jn = a.colidx[blk + NS];  // the next block's column goes out early
                 const double p0 = a.p[cb], p1 = a.p[cb + KW], p2 = a.p[cb + 2 * KW];
                 acc0 += Sb[0] * p0 + Sb[1] * p1 + Sb[2] * p2;
                 acc1 += Sb[3] * p0 + Sb[4] * p1 + Sb[5] * p2;
