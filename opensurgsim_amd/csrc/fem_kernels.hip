@@ -226,17 +226,16 @@ void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
 
 // sum_{c = c0}^{c1 - 1} p[stride * c], added in order; loads go out four at a time so one lane keeps
 // several in flight (the adds stay sequential, so the rounding is that of the plain loop).
+template <int U = 4>
 __device__ __forceinline__ double sum_strided(const double* __restrict__ p, int stride, int c0, int c1) {
     double acc = 0.0;
-    for (int c = c0; c < c1; c += 4) {
-        const double v0 = p[stride * (int64_t)c];
-        const double v1 = c + 1 < c1 ? p[stride * (int64_t)(c + 1)] : 0.0;
-        const double v2 = c + 2 < c1 ? p[stride * (int64_t)(c + 2)] : 0.0;
-        const double v3 = c + 3 < c1 ? p[stride * (int64_t)(c + 3)] : 0.0;
-        acc += v0;
-        if (c + 1 < c1) acc += v1;
-        if (c + 2 < c1) acc += v2;
-        if (c + 3 < c1) acc += v3;
+    for (int c = c0; c < c1; c += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = c + u < c1 ? p[stride * (int64_t)(c + u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (c + u < c1) acc += v[u];  // in contribution order
     }
     return acc;
 }
@@ -256,29 +255,31 @@ __global__ __launch_bounds__(256) void assemble_kernel(int num_off, int num_diag
                                                        const int* __restrict__ ctr_ptr, int nctr_off,
                                                        const double* __restrict__ stage,
                                                        double* __restrict__ S64, float* __restrict__ S32) {
+    // The diagonal blocks' lanes (about 4x the contributions of an off-diagonal block on a tet mesh) take the
+    // first workgroups, so that their longer chains of loads overlap the off-diagonal lanes instead of trailing.
     const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    const int64_t n_off_lanes = 9 * (int64_t)num_off;
+    const int64_t n_diag_lanes = 6 * (int64_t)num_diag;
     int lt, i, j;
     bool diag;
     double acc;
-    if (t < n_off_lanes) {
-        lt = (int)(t / 9);
-        const int v = (int)(t - 9 * (int64_t)lt);
+    if (t >= n_diag_lanes) {
+        const int64_t u = t - n_diag_lanes;
+        if (u >= 9 * (int64_t)num_off) return;
+        lt = (int)(u / 9);
+        const int v = (int)(u - 9 * (int64_t)lt);
         i = v / 3;
         j = v - 3 * i;
         diag = false;
-        acc = sum_strided(stage + v, kPairRec, ctr_ptr[lt], ctr_ptr[lt + 1]);
+        acc = sum_strided<8>(stage + v, kPairRec, ctr_ptr[lt], ctr_ptr[lt + 1]);
     } else {
-        const int64_t u = t - n_off_lanes;
-        if (u >= 6 * (int64_t)num_diag) return;
-        const int d = (int)(u / 6);
-        const int v = (int)(u - 6 * (int64_t)d);
+        const int d = (int)(t / 6);
+        const int v = (int)(t - 6 * (int64_t)d);
         lt = num_off + d;
         i = v == 0 ? 0 : (v < 3 ? 1 : 2);
         j = v == 0 ? 0 : (v < 3 ? v - 1 : v - 3);
         diag = true;
-        acc = sum_strided(stage + kPairRec * (int64_t)nctr_off + v, kDiagRec, ctr_ptr[lt] - nctr_off,
-                          ctr_ptr[lt + 1] - nctr_off);
+        acc = sum_strided<8>(stage + kPairRec * (int64_t)nctr_off + v, kDiagRec, ctr_ptr[lt] - nctr_off,
+                             ctr_ptr[lt + 1] - nctr_off);
     }
     const int4 info = lb_info[lt];
     double val = acc;
