@@ -185,6 +185,7 @@ struct fem_ctx {
 
     // device data
     DevBuf<double> x0, x, v, xs, vs, fg, fext, b, dv, dvprev, r64;
+    DevBuf<double> dv2;         // FP32 refinement: dv of the odd sweeps (dv_s in dv / dv2 by sweep parity)
     DevBuf<int4> conn4;
     DevBuf<double4> mat;
     DevBuf<double> rec;         // element records
@@ -199,6 +200,7 @@ struct fem_ctx {
     DevBuf<uint8_t> bcmask;
     DevBuf<unsigned char> cg_x, cg_r, cg_z, cg_p0, cg_p1, cg_q, cg_dinv, cg_b;  // CG vectors (TV)
     DevBuf<float> cg_binv;  // FP32 one-kernel CG: 3x3 block-diagonal preconditioner, 9 per node
+    DevBuf<unsigned char> cg_x2;  // FP32 refinement: inner solution of the odd sweeps
     DevBuf<unsigned char> cg_r1, cg_z1, cg_q1, cg_w0, cg_w1;                    // fused CG: second buffers
     DevBuf<double> part7;      // [2][kFusedSums * kCgBlocks] block sums of the fused CG kernel, by parity
     DevBuf<double> partial;    // [4 * kCgBlocks] block sums of the grid reductions
@@ -383,8 +385,14 @@ void alloc_precision_buffers(fem_ctx* c) {
         b->alloc(n * tv);
     if (c->kind != FEM_ELEMENT_HEX)
         for (auto* b : {&c->cg_r1, &c->cg_z1, &c->cg_q1, &c->cg_w0, &c->cg_w1}) b->alloc(n * tv);
-    if (c->kind != FEM_ELEMENT_HEX && fp32) c->cg_binv.alloc(3 * n);
-    else c->cg_binv.release();
+    if (c->kind != FEM_ELEMENT_HEX && fp32) {
+        c->cg_binv.alloc(3 * n);
+        c->cg_x2.alloc(n * tv);
+        if (c->dv2.n != n) c->dv2.alloc(n);
+    } else {
+        c->cg_binv.release();
+        c->cg_x2.release();
+    }
     if (!c->part7.p) c->part7.alloc(2 * (size_t)kFusedSums * kCgBlocks);
     c->last_precision = prec;
     c->linear_matrix_ready = false;
@@ -520,11 +528,28 @@ void linear_solve(fem_ctx* c) {
         HIPCHECK(hipHostMalloc((void**)&c->h_refine_iters, ((size_t)max_ref + 1) * sizeof(int), hipHostMallocDefault));
         c->refine_iters_cap = max_ref + 1;
     }
-    HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
-    HIPCHECK(hipMemsetAsync(c->refine_iters.p, 0xff, (size_t)(max_ref + 1) * sizeof(int), c->stream));
     CgArgs a = cg_args(c, true, c->opt.inner_tolerance);
     CgScalars* outer = c->scal.p + 1;
+    // One-kernel CG path: sweep s is one launch for dv_s = dv_{s-1} + x_{s-1}, the FP64 residual and the inner
+    // solve's start, and one for the outer test with the inner scalars; dv_s and x_s alternate between two
+    // buffers by parity (dv_0 = 0 in dv). The operator + vector pair path keeps separate launches.
+    const bool fused = use_fused_cg(c, true);
+    double* dvb[2] = {c->dv.p, c->dv2.p};
+    float* xb[2] = {(float*)c->cg_x.p, (float*)c->cg_x2.p};
+    if (!fused) {
+        HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
+        HIPCHECK(hipMemsetAsync(c->refine_iters.p, 0xff, (size_t)(max_ref + 1) * sizeof(int), c->stream));
+    }
     auto check = [&](int sweep) {
+        if (fused) {
+            TIMED(c, FEM_PHASE_REFINE, 0,
+                  launch_refine_sweep(a.op, c->S64.p, c->S32.p, c->b.p, dvb[(sweep + 1) & 1], xb[(sweep + 1) & 1],
+                                      dvb[sweep & 1], sweep == 0, xb[sweep & 1], (float*)a.r1, (float*)a.z1,
+                                      (float*)a.binv, c->partial.p, outer, c->stream));
+            launch_refine_check_init(c->partial.p, sweep, c->opt.cg_tolerance, c->opt.inner_tolerance, outer, a.sc,
+                                     c->refine_iters.p, max_ref + 1, c->stream);
+            return;
+        }
         TIMED(c, FEM_PHASE_REFINE, 0,
               launch_residual64(a.op, c->S64.p, c->b.p, c->dv.p, sweep == 0, c->r64.p, c->partial.p, outer, a.sc,
                                 c->opt.cg_tolerance, c->stream));
@@ -533,6 +558,12 @@ void linear_solve(fem_ctx* c) {
         const int ps = std::min(sweep, 15);
         const int pred = c->predicted_sweep[ps];
         const int cap = (int)std::min<int64_t>(max_iter, std::max(4, pred + 1));
+        if (fused) {
+            CgArgs ai = a;
+            ai.x = xb[sweep & 1];
+            for (int k = 0; k <= cap; ++k) launch_iteration(c, ai, k, cap, (sweep << 20) | k);
+            return;
+        }
         TIMED(c, FEM_PHASE_CG_INIT, 0, launch_solver_init(c, a, c->r64.p, outer));
         for (int k = 0; k <= cap; ++k) launch_iteration(c, a, k, cap, (sweep << 20) | k);
         TIMED(c, FEM_PHASE_REFINE, 0,
@@ -569,6 +600,7 @@ void linear_solve(fem_ctx* c) {
         c->predicted_sweep[std::min(i, 15)] = it + 1;
     }
     c->predicted_sweeps = std::max(1, sweeps);
+    if (fused && (sweeps & 1)) std::swap(c->dv.p, c->dv2.p);  // dv_sweeps, the corrected solution, is in dv2
     if (c->timer.enabled) {
         std::vector<int> last(sweep + 1, -1);
         for (int i = 0; i < sweeps; ++i) last[i] = c->h_refine_iters[i];

@@ -999,6 +999,136 @@ __global__ __launch_bounds__(kCgThreads) void accumulate_kernel(int n, const flo
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dv[i] += (double)xf[i];
 }
 
+// FP32 refinement on the one-kernel CG path, sweep s in one launch: dv_s = dv_{s-1} + x_{s-1} (s > 0; the
+// row's own values written to dv_out, the gathers form it on the fly, so no launch adds the correction in
+// place), the FP64 residual r64 = b - S64 dv_s, and the start of sweep s's FP32 inner solve as
+// cg_init_block_kernel has it: x = 0, r = (float) r64, z = B r with the 3x3 block-diagonal inverse B (formed
+// at s == 0 from S32 and kept for the step's later sweeps). Partial |r64|^2 (slot 3), |r|^2 (slot 0), r.z
+// (slot 1). Skipped once the outer test has passed.
+__global__ __launch_bounds__(kCgThreads) void refine_sweep_kernel(
+    Operator op, const double* __restrict__ S64, const float* __restrict__ S32, const double* __restrict__ b,
+    const double* __restrict__ dv_in, const float* __restrict__ x_in, double* __restrict__ dv_out, int first,
+    float* __restrict__ x_out, float* __restrict__ r, float* __restrict__ z, float* __restrict__ binv,
+    double* __restrict__ partial, const CgScalars* __restrict__ outer) {
+    __shared__ double lds[64];
+    if (!first && outer->done) return;
+    double rr = 0.0, bb = 0.0, rz = 0.0;
+    // lane c of node I's row starts DOF c of the inner solve from the node's residual e[0..2]
+    auto start = [&](int I, const double* e, const float* B, int c) {
+        const float r0 = (float)e[0], r1 = (float)e[1], r2 = (float)e[2];
+        const float rc = c == 0 ? r0 : (c == 1 ? r1 : r2);
+        const float zc = (B[3 * c] * r0 + B[3 * c + 1] * r1) + B[3 * c + 2] * r2;
+        const int64_t i = 3 * (int64_t)I + c;
+        x_out[i] = 0.f;
+        r[i] = rc;
+        z[i] = zc;
+        bb += (double)rc * (double)rc;
+        rz += (double)rc * (double)zc;
+    };
+    if (first) {
+        for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < op.num_nodes; I += gridDim.x * blockDim.x) {
+            float Bf[9];
+            block_jacobi_inverse(S32, op.diagblk[I], Bf);  // diagblk -1: a node in no element (empty row)
+            for (int k = 0; k < 9; ++k) binv[9 * (int64_t)I + k] = Bf[k];
+            double e[3];
+            for (int c = 0; c < 3; ++c) {
+                e[c] = b[3 * (int64_t)I + c];
+                dv_out[3 * (int64_t)I + c] = 0.0;
+                rr += e[c] * e[c];
+            }
+            for (int c = 0; c < 3; ++c) start(I, e, Bf, c);
+        }
+    } else {
+        group_spmv<double, double, kRowGroup>(
+            op, S64,
+            [&](int J, double& a0, double& a1, double& a2) {
+                a0 = dv_in[3 * J] + (double)x_in[3 * J];
+                a1 = dv_in[3 * J + 1] + (double)x_in[3 * J + 1];
+                a2 = dv_in[3 * J + 2] + (double)x_in[3 * J + 2];
+            },
+            [&](int I, int lane, double y0, double y1, double y2) {
+                if (lane < 3) {
+                    const int64_t i0 = 3 * (int64_t)I;
+                    const double e[3] = {b[i0] - y0, b[i0 + 1] - y1, b[i0 + 2] - y2};
+                    dv_out[i0 + lane] = dv_in[i0 + lane] + (double)x_in[i0 + lane];
+                    rr += e[lane] * e[lane];
+                    start(I, e, binv + 9 * (int64_t)I, lane);
+                }
+            });
+    }
+    const double srr = block_sum(rr, lds);  // ends with a barrier: lds is free for the pair below
+    block_sum2(bb, rz, lds);
+    if (threadIdx.x == 0) {
+        partial[3 * kCgBlocks + blockIdx.x] = srr;
+        partial[blockIdx.x] = bb;
+        partial[kCgBlocks + blockIdx.x] = rz;
+    }
+}
+
+// The outer test of sweep s and, unless it passes, the scalars of sweep s's inner solve (refine_check_kernel and
+// cg_init2_kernel in one launch). Records the iteration count of sweep s - 1's inner solve (s > 0) and clears the
+// records at s == 0.
+__global__ __launch_bounds__(kCgThreads) void refine_check_init_kernel(const double* __restrict__ partial, int sweep,
+                                                                       double tol, double inner_tol,
+                                                                       CgScalars* outer, CgScalars* inner,
+                                                                       int* __restrict__ refine_iters, int nrec) {
+    __shared__ double lds[64];
+    if (sweep > 0 && outer->done) return;
+    const double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
+    double bb, rz;
+    reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
+    if (threadIdx.x == 0) {
+        if (sweep == 0) {
+            for (int s = 0; s < nrec; ++s) refine_iters[s] = -1;
+            outer->bb = rr;
+            outer->threshold = fmax(tol * tol * rr, DBL_MIN);
+            outer->iters = 0;
+            outer->done = 0;
+            outer->converged = 0;
+        } else {
+            refine_iters[sweep - 1] = inner->iters;
+        }
+        outer->rr = rr;
+        outer->err = outer->bb > 0 ? sqrt(rr / outer->bb) : 0.0;
+        if (outer->bb == 0.0 || rr < outer->threshold) {
+            outer->done = 1;
+            outer->converged = 1;
+            inner->done = 1;
+        } else {
+            inner->bb = bb;
+            inner->rz[0] = rz;
+            inner->rz[1] = 0.0;
+            inner->threshold = fmax(inner_tol * inner_tol * bb, DBL_MIN);
+            inner->rr = bb;
+            inner->iters = 0;
+            inner->err = 0.0;
+            inner->converged = 0;
+            inner->done = 0;
+            if (bb == 0.0) {
+                inner->done = 1;
+                inner->converged = 1;
+            } else if (bb < inner->threshold) {
+                inner->done = 1;
+                inner->converged = 1;
+                inner->err = 1.0;
+            }
+        }
+    }
+}
+
+void launch_refine_sweep(const Operator& op, const double* S64, const float* S32, const double* b,
+                         const double* dv_in, const float* x_in, double* dv_out, int first, float* x_out, float* r,
+                         float* z, float* binv, double* partial, const CgScalars* outer, hipStream_t s) {
+    launch_k(refine_sweep_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, op, S64, S32, b, dv_in, x_in, dv_out,
+             first, x_out, r, z, binv, partial, outer);
+}
+
+void launch_refine_check_init(const double* partial, int sweep, double tol, double inner_tol, CgScalars* outer,
+                              CgScalars* inner, int* refine_iters, int nrec, hipStream_t s) {
+    launch_k(refine_check_init_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sweep, tol, inner_tol, outer, inner,
+             refine_iters, nrec);
+}
+
 void launch_residual64(const Operator& op, const double* S64, const double* b, const double* dv, int first,
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
                        hipStream_t s) {

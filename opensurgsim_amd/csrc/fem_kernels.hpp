@@ -170,6 +170,14 @@ void launch_residual64(const Operator& op, const double* S64, const double* b, c
                        double* r64, double* partial, CgScalars* outer, CgScalars* inner, double tol,
                        hipStream_t s);
 // dv += x unless the outer test passed; records the inner iteration count in *iters_out (may be null).
+// FP32 refinement on the one-kernel CG path: sweep s's residual, correction and inner-solve start in one
+// launch (dv_s = dv_{s-1} + x_{s-1} into dv_out, r64 = b - S64 dv_s, x_out = 0, r = (float) r64, z = B r;
+// B formed when first), and the outer test with the inner scalars in one more.
+void launch_refine_sweep(const Operator& op, const double* S64, const float* S32, const double* b,
+                         const double* dv_in, const float* x_in, double* dv_out, int first, float* x_out, float* r,
+                         float* z, float* binv, double* partial, const CgScalars* outer, hipStream_t s);
+void launch_refine_check_init(const double* partial, int sweep, double tol, double inner_tol, CgScalars* outer,
+                              CgScalars* inner, int* refine_iters, int nrec, hipStream_t s);
 void launch_refine_accumulate(int n, const float* xf, double* dv, const CgScalars* outer, const CgScalars* inner,
                               int* iters_out, hipStream_t s);
 
