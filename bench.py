@@ -279,8 +279,10 @@ def bench_compliance(args):
     rng = np.random.default_rng(3)
     elems = rng.choice(cfg.conn.shape[0], ncon, replace=False)
     Ht = np.zeros((D, 3 * ncon))
+    coords = []
     for i, e in enumerate(elems):
         coord = rng.dirichlet(np.ones(4))
+        coords.append(coord)
         for a in range(4):
             for k in range(3):
                 Ht[3 * cfg.conn[e, a] + k, 3 * i + k] = coord[a]
@@ -307,6 +309,21 @@ def bench_compliance(args):
         device_sync(g)
         t = (time.perf_counter() - t0) / reps
         dev_diff = float(np.abs(xd.cpu().numpy().T - X).max() / np.abs(X).max())
+        # the same constraints one by one through fem_constraint_compliance (H^T built on the device; C H^T and
+        # H C H^T returned, or H C H^T only)
+        for want in (True, False):
+            g.constraint_compliance(elems[0], coords[0], np.eye(3), 1.0, want_cht=want)
+        t_con = {}
+        hdiff = 0.0
+        for want in (True, False):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for i, e in enumerate(elems):
+                    g.constraint_compliance(e, coords[i], np.eye(3), 1.0, want_cht=want)
+            t_con[want] = (time.perf_counter() - t0) / (reps * ncon)
+        for i, e in enumerate(elems):
+            cht, _ = g.constraint_compliance(e, coords[i], np.eye(3), 1.0)
+            hdiff = max(hdiff, float(np.abs(cht - X[:, 3 * i:3 * i + 3]).max() / np.abs(X).max()))
         rowptr, colidx = g.pattern()
         A = sps.csr_matrix((g.system_matrix(), colidx, rowptr), shape=(D, D))
         bc = (3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)).reshape(-1)
@@ -316,9 +333,13 @@ def bench_compliance(args):
         g.close()
         res[name] = {"ms_per_column": 1e3 * t / Ht.shape[1], "columns_per_s": Ht.shape[1] / t,
                      "ms_per_column_host_pointers": 1e3 * t_host / Ht.shape[1], "relative_residual": resid,
-                     "device_vs_host_max_rel_diff": dev_diff}
-        log("  %s: %.3f ms per column (device pointers), %.3f (host pointers), residual %.2e" %
-            (name, 1e3 * t / Ht.shape[1], 1e3 * t_host / Ht.shape[1], resid))
+                     "device_vs_host_max_rel_diff": dev_diff,
+                     "ms_per_fixed_point_constraint": 1e3 * t_con[True],
+                     "ms_per_fixed_point_constraint_hcht_only": 1e3 * t_con[False],
+                     "constraint_vs_dense_max_rel_diff": hdiff}
+        log("  %s: %.3f ms per column (device pointers), %.3f (host pointers), residual %.2e; "
+            "fem_constraint_compliance %.3f ms per fixed point (%.3f with H C H^T only)" %
+            (name, 1e3 * t / Ht.shape[1], 1e3 * t_host / Ht.shape[1], resid, 1e3 * t_con[True], 1e3 * t_con[False]))
     best = min(res, key=lambda k: res[k]["ms_per_column"])
     out = {"metric": "compliance columns/s (C H^T, applyCompliance)", "value": res[best]["columns_per_s"],
            "unit": "columns/s", "n_gpus": 1, "steps": 3, "warmup": args.warmup,

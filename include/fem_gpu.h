@@ -158,6 +158,17 @@ int fem_apply_compliance(fem_ctx* ctx, const double* b, int64_t num_columns, dou
  * tetrahedral mesh the columns are solved up to 8 at a time by a multi-column FP64 CG that reads the
  * operator once per iteration for all of them (each column keeps its own Eigen CG recurrence and test). */
 int fem_apply_compliance_device(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
+/* The compliance terms of a constraint localized in one element, as the constraint builders form them for
+ * the MLCP (Physics/FemConstraintFixedPoint.cpp:84-106, Physics/FemConstraintFrictionlessContact.cpp:84-112,
+ * consumed by MlcpPhysicsProblem::updateConstraint, Physics/MlcpPhysicsProblem.cpp:46-76). H has
+ * num_directions (1..8) rows; row j holds coordinate[a] * directions[3 j + k] * scale at DOF 3 node_a + k for
+ * every node a of the element with a non-zero coordinate (coordinate: the element's natural coordinates, one
+ * per node; scale: the builder's dt * scale; fixed point: the three unit axes, frictionless contact: the
+ * normal). cht (host, ndof x num_directions column-major, or NULL) = C H^T with C applied as
+ * fem_apply_compliance does; hcht (host, num_directions^2 row-major, or NULL) = H C H^T. The H^T columns are
+ * built and solved on the device (multi-column CG on tetrahedral meshes). FEM_ERR_ORDER before the first step. */
+int fem_constraint_compliance(fem_ctx* ctx, int64_t element, const double* coordinate, int num_directions,
+                              const double* directions, double scale, double* cht, double* hcht);
 int fem_get_state(fem_ctx* ctx, double* x, double* v);
 int fem_set_state_device(fem_ctx* ctx, const double* x, const double* v);
 int fem_get_state_device(fem_ctx* ctx, double* x, double* v);

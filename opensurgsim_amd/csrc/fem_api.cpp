@@ -217,6 +217,9 @@ struct fem_ctx {
 
     // localized external terms of the next step (fem_add_external_generalized_force)
     std::vector<int32_t> h_conn;   // element connectivity, num_elements x nn
+    // localized constraint compliance (fem_constraint_compliance): H^T columns, C H^T, H's non-zeros
+    DevBuf<double> con_b, con_x, con_val, con_out;
+    DevBuf<int64_t> con_idx;
     std::vector<double> h_loc_f;   // force, ndof
     std::map<std::pair<int64_t, int64_t>, std::array<double, 18>> h_loc_blk;  // node block -> K (9), D (9)
     bool loc_pending = false;
@@ -1850,6 +1853,70 @@ int fem_apply_compliance_device(fem_ctx* c, const double* b, int64_t num_columns
         if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
         HIPCHECK(hipSetDevice(c->device));
         compliance_device(c, b, num_columns, out);
+        return FEM_OK;
+    });
+}
+
+int fem_constraint_compliance(fem_ctx* c, int64_t element, const double* coordinate, int num_directions,
+                              const double* directions, double scale, double* cht, double* hcht) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        if (!coordinate || !directions) throw InvalidArg("Invalid localization (null coordinate or directions)");
+        if (element < 0 || element >= c->num_elements) throw InvalidArg("Invalid localization (element out of range)");
+        if (num_directions < 1 || num_directions > 8) throw InvalidArg("num_directions must be 1 to 8");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        HIPCHECK(hipSetDevice(c->device));
+        const int nn = c->nn, m = num_directions;
+        const size_t n = (size_t)c->ndof;
+        // The non-zeros of one row of H in DOF order, as the constraint's SparseVector holds them: DOF
+        // 3 node_a + k for every node with a non-zero coordinate.
+        const int32_t* ids = c->h_conn.data() + (size_t)nn * element;
+        std::vector<std::pair<int64_t, int>> nz;  // (DOF, 3 a + k)
+        for (int a = 0; a < nn; ++a)
+            if (coordinate[a] != 0.0)
+                for (int k = 0; k < 3; ++k) nz.emplace_back(3 * (int64_t)ids[a] + k, 3 * a + k);
+        std::sort(nz.begin(), nz.end());
+        const int cnt = (int)nz.size();
+        std::vector<double> val((size_t)m * cnt);  // H_j at its non-zeros
+        std::vector<int64_t> idx((size_t)m * cnt);  // the same positions in each of the m columns
+        for (int j = 0; j < m; ++j)
+            for (int t = 0; t < cnt; ++t) {
+                const int a = nz[t].second / 3, k = nz[t].second % 3;
+                val[(size_t)j * cnt + t] = coordinate[a] * directions[3 * j + k] * scale;
+                idx[(size_t)j * cnt + t] = (int64_t)j * (int64_t)n + nz[t].first;
+            }
+        if (c->con_b.n != n * 8) {
+            c->con_b.alloc(n * 8);
+            c->con_x.alloc(n * 8);
+            c->con_idx.alloc(8 * 8 * 3);
+            c->con_val.alloc(8 * 8 * 3);
+            c->con_out.alloc(8 * 8 * 3);
+        }
+        HIPCHECK(hipMemsetAsync(c->con_b.p, 0, n * m * sizeof(double), c->stream));
+        if (cnt > 0) {
+            HIPCHECK(hipMemcpyAsync(c->con_idx.p, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                    c->stream));
+            HIPCHECK(hipMemcpyAsync(c->con_val.p, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    c->stream));
+            launch_scatter_values(m * cnt, c->con_idx.p, c->con_val.p, c->con_b.p, c->stream);
+        }
+        compliance_device(c, c->con_b.p, m, c->con_x.p);  // C H^T, boundary rows zero
+        std::vector<double> g((size_t)m * cnt);
+        if (hcht && cnt > 0) {
+            launch_gather_values(m * cnt, c->con_idx.p, c->con_x.p, c->con_out.p, c->stream);
+            HIPCHECK(hipMemcpyAsync(g.data(), c->con_out.p, g.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+        }
+        if (cht)
+            HIPCHECK(hipMemcpyAsync(cht, c->con_x.p, n * m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (hcht)
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) {
+                    double s = 0.0;  // H_i (C H_j^T) over H_i's non-zeros in DOF order
+                    for (int t = 0; t < cnt; ++t) s += val[(size_t)i * cnt + t] * g[(size_t)j * cnt + t];
+                    hcht[i * m + j] = s;
+                }
         return FEM_OK;
     });
 }

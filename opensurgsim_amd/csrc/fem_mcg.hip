@@ -271,7 +271,27 @@ __global__ __launch_bounds__(256) void masked_copy_kernel(int64_t n, const doubl
     if (i < n) dst[i] = mask[i] ? 0.0 : src[i];
 }
 
+// dst[idx[k]] = val[k] (distinct indices), and dst[k] = src[idx[k]]: the few non-zeros of a localized
+// constraint's H^T columns and the solution entries at them
+__global__ __launch_bounds__(64) void scatter_values_kernel(int cnt, const int64_t* __restrict__ idx,
+                                                            const double* __restrict__ val, double* __restrict__ dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < cnt) dst[idx[k]] = val[k];
+}
+__global__ __launch_bounds__(64) void gather_values_kernel(int cnt, const int64_t* __restrict__ idx,
+                                                           const double* __restrict__ src, double* __restrict__ dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < cnt) dst[k] = src[idx[k]];
+}
+
 }  // namespace
+
+void launch_scatter_values(int cnt, const int64_t* idx, const double* val, double* dst, hipStream_t s) {
+    if (cnt > 0) launch_k(scatter_values_kernel, dim3((unsigned)((cnt + 63) / 64)), dim3(64), 0, s, cnt, idx, val, dst);
+}
+void launch_gather_values(int cnt, const int64_t* idx, const double* src, double* dst, hipStream_t s) {
+    if (cnt > 0) launch_k(gather_values_kernel, dim3((unsigned)((cnt + 63) / 64)), dim3(64), 0, s, cnt, idx, src, dst);
+}
 
 void launch_mcg_gather(int64_t n, int kw, int cnt, const double* b, const uint8_t* mask, double* dst, hipStream_t s) {
     launch_k(mcg_gather_kernel, dim3((unsigned)((n * kw + 255) / 256)), dim3(256), 0, s, n, kw, cnt, b, mask, dst);

@@ -48,6 +48,9 @@ void launch_mcg_gather(int64_t n, int kw, int cnt, const double* b, const uint8_
 void launch_mcg_scatter(int64_t n, int kw, int cnt, const double* x, const uint8_t* mask, double* out, hipStream_t s);
 // dst = src with the boundary rows zero.
 void launch_masked_copy(int64_t n, const double* src, const uint8_t* mask, double* dst, hipStream_t s);
+// dst[idx[k]] = val[k] and dst[k] = src[idx[k]] for k < cnt (indices checked by the caller).
+void launch_scatter_values(int cnt, const int64_t* idx, const double* val, double* dst, hipStream_t s);
+void launch_gather_values(int cnt, const int64_t* idx, const double* src, double* dst, hipStream_t s);
 void launch_mcg_iteration(const McgArgs& a, int k, hipStream_t s);
 
 }  // namespace femgpu

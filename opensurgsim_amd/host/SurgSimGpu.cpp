@@ -384,6 +384,32 @@ void Fem3DRepresentation::afterUpdate(double) {
     }
 }
 
+Math::Vector Fem3DRepresentation::applyCompliance(const Math::Vector& b, size_t numColumns) const {
+    if (!m_ctx) fail("Ode solver not initialized, it should have been initialized on wake-up");
+    const size_t n = m_initialState->getNumDof();
+    if (b.size() != n * numColumns) fail("applyCompliance: b needs getNumDof() values per column");
+    Math::Vector out(b.size(), 0.0);
+    check(fem_apply_compliance(m_ctx, b.data(), (int64_t)numColumns, out.data()), "applyCompliance");
+    return out;
+}
+
+void Fem3DRepresentation::applyComplianceToConstraint(size_t elementId, const Math::Vector& naturalCoordinate,
+                                                      const std::vector<Math::Vector3d>& directions, double scale,
+                                                      Math::Vector* cht, Math::Vector* hcht) const {
+    if (!m_ctx) fail("Ode solver not initialized, it should have been initialized on wake-up");
+    if (elementId >= m_elements.size()) fail("Invalid localization (element id out of range)");
+    if (naturalCoordinate.size() != m_elements[elementId]->getNodeIds().size())
+        fail("Invalid localization (natural coordinate size)");
+    const size_t m = directions.size();
+    std::vector<double> dirs;
+    for (const auto& d : directions) dirs.insert(dirs.end(), d.begin(), d.end());
+    if (cht) cht->assign(m_initialState->getNumDof() * m, 0.0);
+    if (hcht) hcht->assign(m * m, 0.0);
+    check(fem_constraint_compliance(m_ctx, (int64_t)elementId, naturalCoordinate.data(), (int)m, dirs.data(), scale,
+                                    cht ? cht->data() : nullptr, hcht ? hcht->data() : nullptr),
+          "constraint compliance");
+}
+
 fem_stats Fem3DRepresentation::getStatistics() const {
     if (!m_ctx) fail("Not initialized");
     fem_stats s;

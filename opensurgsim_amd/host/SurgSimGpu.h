@@ -11,9 +11,10 @@
 // Framework/Assert.h:77-95); a state that becomes non-finite deactivates the representation
 // (Physics/FemRepresentation.cpp:285-293).
 //
-// Only what the stepping path needs is mirrored. Not provided: compliance matrices
-// (OdeSolver::computeComplianceMatrixFromSystemMatrix), compliance warping, Localization objects (node and
-// element localizations are passed as ids and natural coordinates).
+// Only what the stepping path and its compliance consumers need is mirrored. Not provided: dense compliance
+// matrices (OdeSolver::computeComplianceMatrixFromSystemMatrix; the compliance is applied to columns instead),
+// compliance warping, Localization objects (node and element localizations are passed as ids and natural
+// coordinates).
 #pragma once
 
 #include <array>
@@ -295,6 +296,20 @@ public:
     void addExternalGeneralizedForce(size_t elementId, const Math::Vector& naturalCoordinate,
                                      const Math::Vector3d& force, const Math::Matrix33d* K = nullptr,
                                      const Math::Matrix33d* D = nullptr);
+
+    /// DeformableRepresentation::applyCompliance (Physics/DeformableRepresentation.cpp:175-190) on numColumns
+    /// columns of b (getNumDof() values each, column-major): boundary rows zeroed, one CG solve per column
+    /// against the last update's system matrix, boundary rows of the result zeroed.
+    Math::Vector applyCompliance(const Math::Vector& b, size_t numColumns) const;
+    /// The compliance terms a constraint localized in one element hands to the MLCP
+    /// (Physics/FemConstraintFixedPoint.cpp:84-106, Physics/FemConstraintFrictionlessContact.cpp:84-112,
+    /// Physics/MlcpPhysicsProblem.cpp:46-76): row j of H is scale * naturalCoordinate[a] * directions[j] at
+    /// node a's DOFs (fixed point: the three axes; frictionless contact: the normal; scale: dt times the
+    /// builder's scale). cht (or null) receives C H^T, getNumDof() x directions.size() column-major; hcht (or
+    /// null) receives H C H^T, row-major.
+    void applyComplianceToConstraint(size_t elementId, const Math::Vector& naturalCoordinate,
+                                     const std::vector<Math::Vector3d>& directions, double scale, Math::Vector* cht,
+                                     Math::Vector* hcht) const;
 
     /// Physics/FemRepresentation.cpp:202-220: Rayleigh damping D = massCoef M + stiffnessCoef K.
     void setRayleighDampingMass(double massCoef) { m_rayleighMass = massCoef; }

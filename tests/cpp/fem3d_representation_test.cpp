@@ -36,6 +36,7 @@ void orc_sim_pattern(void* h, int* outer, int* inner);
 void orc_sim_set_damping(void* h, double rayleigh_mass, double rayleigh_stiffness, int lumping);
 int orc_sim_add_localized_force(void* h, int element, const double* coord, const double* force, const double* K,
                                 const double* D);
+int orc_sim_apply_compliance(void* h, const double* b, int num_columns, double* out);
 }
 
 // ---- minimal test harness ------------------------------------------------------------------------
@@ -533,6 +534,62 @@ void gpuTests() {
         EXPECT_TRUE(std::equal(inner.begin(), inner.end(), colidx.begin()));
         // total mass = rho V of the block (Physics/FemRepresentation.cpp:192-200)
         EXPECT_TRUE(std::fabs(fem->getTotalMass() - 1000.0 * 2 * 3 * 2 * 1e-6) < 1e-12);
+        orc_sim_destroy(o);
+    });
+    run("Fem3DRepresentationGpuTest.ConstraintComplianceMatchesOracle", [] {
+        // C H^T and H C H^T of a fixed-point constraint (FemConstraintFixedPoint) and a frictionless contact
+        // (FemConstraintFrictionlessContact) in element 57 (cube (1, 2, 1), clear of the fixed y = 0 face) after
+        // one update: C H^T equals applyCompliance of
+        // the dense H^T bit for bit and the oracle's solve to the CG tolerance; H C H^T equals H (C H^T).
+        const Block b = makeBlock(3, 3, 2);
+        const Material m{5e3, 0.45, 1000.0};
+        const auto v0 = topVelocities(b, 0.02);
+        auto fem = makeRepresentation(b, kCoro, m, v0);
+        fem->setIntegrationScheme(Math::INTEGRATIONSCHEME_EULER_IMPLICIT);
+        fem->setLinearSolver(Math::LINEARSOLVER_CONJUGATEGRADIENT);
+        EXPECT_THROW_ASSERT(fem->applyCompliance(Math::Vector(b.x0.size(), 0.0), 1));
+        fem->initialize();
+        std::dynamic_pointer_cast<Math::LinearSparseSolveAndInverseCG>(fem->getOdeSolver()->getLinearSolver())
+            ->setTolerance(1e-12);
+        EXPECT_THROW_ASSERT(fem->applyCompliance(Math::Vector(b.x0.size(), 0.0), 1));  // no system matrix yet
+        const double dt = 1e-3;
+        fem->update(dt);
+        fem->afterUpdate(dt);
+        void* o = makeOracle(b, 1, m, v0, 0, 1, 1e-12);
+        EXPECT_TRUE(orc_sim_step(o, dt, 1) == 0);
+        const size_t n = b.x0.size();
+        const Math::Vector coord{0.1, 0.0, 0.45, 0.45};
+        const auto& ids = fem->getFemElement(57)->getNodeIds();
+        const double nrm = std::sqrt(0.09 + 0.64 + 0.25);
+        for (const auto& dirs : {std::vector<Math::Vector3d>{{{1, 0, 0}}, {{0, 1, 0}}, {{0, 0, 1}}},
+                                 std::vector<Math::Vector3d>{{{0.3 / nrm, -0.8 / nrm, 0.5 / nrm}}}}) {
+            const size_t k = dirs.size();
+            const double scale = dt * 0.7;
+            Math::Vector Ht(n * k, 0.0);
+            for (size_t j = 0; j < k; ++j)
+                for (size_t a = 0; a < 4; ++a)
+                    if (coord[a] != 0.0)
+                        for (size_t c = 0; c < 3; ++c) Ht[j * n + 3 * ids[a] + c] = coord[a] * dirs[j][c] * scale;
+            Math::Vector cht, hcht;
+            fem->applyComplianceToConstraint(57, coord, dirs, scale, &cht, &hcht);
+            EXPECT_TRUE(cht == fem->applyCompliance(Ht, k));
+            Math::Vector ref(n * k);
+            EXPECT_TRUE(orc_sim_apply_compliance(o, Ht.data(), (int)k, ref.data()) == 0);
+            const double err = relErr(cht, ref);
+            double hmax = 0.0, hdiff = 0.0;
+            for (size_t i = 0; i < k; ++i)
+                for (size_t j = 0; j < k; ++j) {
+                    double s = 0.0;
+                    for (size_t q = 0; q < n; ++q) s += Ht[i * n + q] * ref[j * n + q];
+                    hmax = std::max(hmax, std::fabs(s));
+                    hdiff = std::max(hdiff, std::fabs(s - hcht[i * k + j]));
+                }
+            std::printf("  %zu direction(s): C H^T rel err %.3e, H C H^T rel err %.3e\n", k, err, hdiff / hmax);
+            EXPECT_TRUE(hmax > 0.0 && err < 1e-9 && hdiff < 1e-9 * hmax);
+        }
+        Math::Vector cht;
+        EXPECT_THROW_ASSERT(fem->applyComplianceToConstraint(57, {0.5, 0.5}, {{{1, 0, 0}}}, 1.0, &cht, nullptr));
+        EXPECT_THROW_ASSERT(fem->applyComplianceToConstraint(1000, coord, {{{1, 0, 0}}}, 1.0, &cht, nullptr));
         orc_sim_destroy(o);
     });
     run("Fem3DRepresentationGpuTest.ExternalForceLastsOneUpdate", [] {

@@ -286,3 +286,70 @@ def test_apply_compliance_device_pointers():
     g.apply_compliance_device(bd.data_ptr(), B.shape[1], xd.data_ptr())
     g.synchronize()
     assert np.array_equal(xd.cpu().numpy().T, X)
+
+
+def _localized_h(cfg, element, coord, dirs, scale):
+    """H^T (ndof x m) of a constraint localized in an element: row j of H is scale * coord[a] * dirs[j] at node
+    a's DOFs (Physics/FemConstraintFixedPoint.cpp:84-106, FemConstraintFrictionlessContact.cpp:84-112)."""
+    Ht = np.zeros((cfg.x0.size, len(dirs)))
+    for j, d in enumerate(dirs):
+        for a in range(cfg.conn.shape[1]):
+            if coord[a] != 0.0:
+                for k in range(3):
+                    Ht[3 * cfg.conn[element, a] + k, j] = coord[a] * d[k] * scale
+    return Ht
+
+
+@pytest.mark.parametrize("kind", ["corotational", "hex"])
+def test_constraint_compliance(kind):
+    """fem_constraint_compliance builds H^T on the device and returns C H^T and H C H^T: C H^T equals
+    fem_apply_compliance of the same dense columns bit for bit and the oracle's solve to the CG tolerance;
+    H C H^T equals H (C H^T); boundary rows are zero; the step's state and statistics are untouched."""
+    if kind == "hex":
+        cfg = mg.config_hex((4, 3, 5))
+        coord = np.array([0.2, 0.0, 0.1, 0.15, 0.05, 0.3, 0.1, 0.1])
+    else:
+        cfg = mg.config_corotational((6, 5, 6), seed=12)
+        coord = np.array([0.5, 0.0, 0.3, 0.2])
+    say("constraint compliance", kind)
+    o = make_oracle(cfg, tol=1e-12)
+    g = make_gpu(cfg, tol=1e-12)
+    with pytest.raises(femgpu.FemError):
+        g.constraint_compliance(0, coord, np.eye(3), 1.0)  # no system matrix before the first step
+    o.step(cfg.dt, 2)
+    assert g.step(2) == femgpu.FEM_OK
+    stats = g.stats()
+    bcset = set(int(n) for n in cfg.bc_nodes)
+    # an element with fixed nodes and at least one free node of non-zero coordinate
+    touching = [e for e in range(cfg.conn.shape[0]) if bcset & set(int(n) for n in cfg.conn[e])
+                and any(coord[a] != 0.0 and int(cfg.conn[e, a]) not in bcset for a in range(cfg.conn.shape[1]))]
+    free = [e for e in range(cfg.conn.shape[0]) if not bcset & set(int(n) for n in cfg.conn[e])]
+    bc = (3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)).reshape(-1)
+    normal = np.array([0.3, -0.8, 0.5]) / np.linalg.norm([0.3, -0.8, 0.5])
+    scale = cfg.dt * 1.3
+    for element, dirs in ((free[len(free) // 2], np.eye(3)), (touching[0], np.eye(3)),
+                          (free[len(free) // 3], normal[None, :])):
+        cht, hcht = g.constraint_compliance(element, coord, dirs, scale)
+        Ht = _localized_h(cfg, element, coord, dirs, scale)
+        assert np.array_equal(cht, g.apply_compliance(Ht))
+        Xr = o.apply_compliance(Ht)
+        err = np.linalg.norm(cht - Xr) / np.linalg.norm(Xr)
+        href = Ht.T @ Xr
+        herr = np.abs(hcht - href).max() / np.abs(href).max()
+        say("  element", element, "directions", len(dirs), "rel err", err, "H C H^T rel err", herr)
+        assert err < 1e-9 and herr < 1e-9
+        assert np.abs(hcht - Ht.T @ cht).max() <= 1e-13 * np.abs(hcht).max()
+        assert np.all(cht[bc] == 0.0)
+        assert np.all(np.linalg.eigvalsh(0.5 * (hcht + hcht.T)) > 0.0)
+        _, h_only = g.constraint_compliance(element, coord, dirs, scale, want_cht=False)
+        assert np.array_equal(h_only, hcht)
+    assert g.stats() == stats
+    with pytest.raises(femgpu.FemError):
+        g.constraint_compliance(cfg.conn.shape[0], coord, np.eye(3), scale)  # element out of range
+    with pytest.raises(femgpu.FemError):
+        g.constraint_compliance(0, coord, np.zeros((9, 3)), scale)  # more than 8 directions
+    with pytest.raises(ValueError):
+        g.constraint_compliance(0, coord[:-1], np.eye(3), scale)
+    # a zero localization gives zero terms
+    cht, hcht = g.constraint_compliance(free[0], np.zeros_like(coord), np.eye(3), scale)
+    assert not cht.any() and not hcht.any()
