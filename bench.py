@@ -187,8 +187,9 @@ def bench_hex(args):
         st = g.stats()
         x, v = g.state()
         ph = g.phase_times()
+        distinct = g.hex_distinct_matrices()
         g.close()
-        return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, stats=st, phases=ph)
+        return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, stats=st, phases=ph, distinct=distinct)
 
     r32 = run(femgpu.FP32_REFINED, 1e-6)
     log("fp32+refine: %.3f s, %d CG iterations, last sweeps %d" % (r32["seconds"], r32["cg"],
@@ -209,7 +210,8 @@ def bench_hex(args):
            "rel_err_vs_fp64": err,
            "fp64": {"value": D * steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / steps,
                     "cg_iters_per_step": r64["cg"] / steps},
-           "fp32_refined": {"cg_iters_per_step": r32["cg"] / steps},
+           "fp32_refined": {"cg_iters_per_step": r32["cg"] / steps,
+                            "distinct_element_matrices": r32["distinct"]},
            "phases_fp32": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in r32["phases"].items()}}
     print(json.dumps(out), flush=True)
 

@@ -235,6 +235,11 @@ int fem_get_node_masses(fem_ctx* ctx, double* mass);
  * equal the reference's column-major (outerIndex, innerIndex) of Eigen::SparseMatrix<double>
  * built at Physics/FemRepresentation.cpp:121-132. */
 int64_t fem_num_nonzeros(fem_ctx* ctx);
+/* Hexahedral meshes: the number of distinct FP32 element matrices the inner-solve operator reads from its
+ * table (a structured mesh repeats a few), or 0 when it reads one cached matrix per element (more than 32
+ * distinct, FEM_HEX_TABLE=0 in the environment, no FP32 operator built yet, or not a hexahedral mesh). No
+ * counterpart in the reference, which keeps a matrix per element (Physics/Fem3DElementCube.cpp:143-167). */
+int fem_hex_distinct_matrices(fem_ctx* ctx);
 int fem_get_pattern(fem_ctx* ctx, int32_t* rowptr, int32_t* colidx);
 
 /* Last assembled system matrix as the CG operator sees it: the lower-triangle self-adjoint view of

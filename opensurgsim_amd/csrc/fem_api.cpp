@@ -67,6 +67,29 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Events of a timing loop, destroyed when the scope ends (error paths included), with no event pair left
+// pending for the next launch.
+struct ScopedEvents {
+    std::vector<hipEvent_t> ev;
+    explicit ScopedEvents(size_t n) {
+        ev.reserve(n);
+        for (size_t i = 0; i < n; ++i) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) {
+                release();
+                throw HipError("hipEventCreate failed");
+            }
+            ev.push_back(e);
+        }
+    }
+    ~ScopedEvents() { release(); }
+    void release() {
+        g_launch_timing = LaunchTiming{};
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+    }
+};
+
 // HIP-event timers per phase (resolved at host synchronisation points).
 struct PhaseTimer {
     bool enabled = false;
@@ -999,9 +1022,8 @@ void tune_batch(fem_ctx* c, bool fp32) {
     const int lanes_c[] = {4, 8, 16};
     double best = 1e30;
     int best_r = 256, best_l = 8;
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    ScopedEvents events(2);
+    hipEvent_t e0 = events.ev[0], e1 = events.ev[1];
     const bool save = c->b_use_graphs;
     c->b_use_graphs = false;
     for (int rows : rows_c)
@@ -1027,8 +1049,6 @@ void tune_batch(fem_ctx* c, bool fp32) {
                 best_l = lanes;
             }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     c->b_use_graphs = save;
     build_batch_tables(c, best_r, best_l);
     c->b_tune_ms = best / 8.0;
@@ -2055,8 +2075,8 @@ int fem_time_operator(fem_ctx* c, int reps, double* avg_ms) {
         launch();  // warm-up
         // each launch carries its own start/stop events in its dispatch packet (the kernel's execution, as a
         // profiler's kernel trace reports it, without the gaps between back-to-back dispatches)
-        std::vector<hipEvent_t> ev(2 * (size_t)reps);
-        for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+        ScopedEvents events(2 * (size_t)reps);
+        const std::vector<hipEvent_t>& ev = events.ev;
         for (int r = 0; r < reps; ++r) {
             g_launch_timing.start = ev[2 * r];
             g_launch_timing.stop = ev[2 * r + 1];
@@ -2070,7 +2090,6 @@ int fem_time_operator(fem_ctx* c, int reps, double* avg_ms) {
             HIPCHECK(hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]));
             sum += t;
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
         *avg_ms = sum / reps;
         return FEM_OK;
     });
@@ -2090,6 +2109,10 @@ int fem_get_phase_times(fem_ctx* c, fem_phase_times* out) {
 }
 
 int64_t fem_num_nonzeros(fem_ctx* c) { return (c && c->mesh_ready) ? 9 * c->num_blocks : 0; }
+
+int fem_hex_distinct_matrices(fem_ctx* c) {
+    return (c && c->mesh_ready && c->kind == FEM_ELEMENT_HEX) ? c->hex.num_distinct : 0;
+}
 
 int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_offsets, const double* dt) {
     return guarded(c, [&] {

@@ -14,9 +14,12 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
+#include <vector>
 
 namespace femgpu {
 
@@ -87,6 +90,7 @@ __host__ __device__ constexpr int blk_b(int k) {
 // order (tile T = t / 64, lane l = t % 64) keeps its 324 floats (36 blocks x 9, row-major within a
 // block) as 81 float4 chunks at A4[(81 T + k) 64 + l], so each wave-wide load reads 1 KiB contiguous.
 constexpr int kHexChunks = 81;
+constexpr int kHexMaxDistinct = 32;  // distinct-matrix table: at most 41.5 KB of LDS per workgroup
 
 // ------------------------------------------------------------------------------------------
 // Set-up: A_e = cm M_e + ck K_e in FP32 (implicit Euler with Rayleigh damping: cm = 1/dt + alpha_M,
@@ -193,15 +197,26 @@ __global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* 
 }
 
 // FP32 cached operator over one colour: q[nodes] += A_e p_e (p masked on BC DOFs), one element per
-// lane; the 36 blocks stream in 9 groups of 4 (36 floats = 9 float4 chunks per group).
+// lane; the 36 blocks stream in 9 groups of 4 (36 floats = 9 float4 chunks per group). TABLE: A_e is
+// entry table_idx[t] of the distinct-matrix table, staged in LDS by every workgroup (the same floats and
+// the same arithmetic as the per-position cache).
+template <bool TABLE>
 __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const int* __restrict__ order_pad,
                                                           const int* __restrict__ conn,
                                                           const float4* __restrict__ A4,
+                                                          const float4* __restrict__ table,
+                                                          const int* __restrict__ table_idx, int num_distinct,
                                                           const float* __restrict__ p,
                                                           const uint8_t* __restrict__ mask, float* __restrict__ q,
                                                           const CgScalars* __restrict__ sc) {
+    extern __shared__ float4 lds_table[];
+    if (sc->done) return;  // uniform over the grid
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count_pad || sc->done) return;
+    if (TABLE) {
+        for (int i = threadIdx.x; i < num_distinct * kHexChunks; i += blockDim.x) lds_table[i] = table[i];
+        __syncthreads();
+    }
+    if (t >= count_pad) return;
     const int e = order_pad[t];
     if (e < 0) return;
     const int4* c4 = reinterpret_cast<const int4*>(conn) + 2 * (size_t)e;
@@ -217,12 +232,13 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
             y[3 * a + c] = 0.f;
         }
     const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
+    const float4* Tt = TABLE ? lds_table + table_idx[t] * kHexChunks : nullptr;
 #pragma unroll
     for (int g = 0; g < 9; ++g) {
         float w[36];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            const float4 v = At[(size_t)(9 * g + k) * 64];
+            const float4 v = TABLE ? Tt[9 * g + k] : At[(size_t)(9 * g + k) * 64];
             w[4 * k] = v.x;
             w[4 * k + 1] = v.y;
             w[4 * k + 2] = v.z;
@@ -245,6 +261,62 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int c = 0; c < 3; ++c) q[3 * ids[a] + c] += y[3 * a + c];
+}
+
+// 64-bit FNV-1a hash of each padded position's 324 cached floats (bit patterns, chunk order); 0 for padding.
+__global__ __launch_bounds__(256) void hex_hash_kernel(int count_pad, const int* __restrict__ order_pad,
+                                                       const float4* __restrict__ A4,
+                                                       unsigned long long* __restrict__ hash) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count_pad) return;
+    if (order_pad[t] < 0) {
+        hash[t] = 0ull;
+        return;
+    }
+    const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
+    unsigned long long h = 1469598103934665603ull;
+    for (int k = 0; k < kHexChunks; ++k) {
+        const float4 v = At[(size_t)k * 64];
+        const unsigned int w[4] = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                                   __float_as_uint(v.w)};
+        for (int i = 0; i < 4; ++i) {
+            h ^= w[i];
+            h *= 1099511628211ull;
+        }
+    }
+    hash[t] = h;
+}
+
+// bad[block] = 1 when a position of the block differs (bit patterns) from the representative of its table
+// entry; bad is cleared by the caller.
+__global__ __launch_bounds__(256) void hex_table_verify_kernel(int count_pad, const int* __restrict__ order_pad,
+                                                               const float4* __restrict__ A4,
+                                                               const int* __restrict__ table_idx,
+                                                               const int* __restrict__ rep, int* __restrict__ bad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int diff = 0;
+    if (t < count_pad && order_pad[t] >= 0) {
+        const int r = rep[table_idx[t]];
+        const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
+        const float4* Ar = A4 + (size_t)(r >> 6) * kHexChunks * 64 + (r & 63);
+        for (int k = 0; k < kHexChunks; ++k) {
+            const float4 a = At[(size_t)k * 64], b = Ar[(size_t)k * 64];
+            diff |= (__float_as_uint(a.x) != __float_as_uint(b.x)) | (__float_as_uint(a.y) != __float_as_uint(b.y)) |
+                    (__float_as_uint(a.z) != __float_as_uint(b.z)) | (__float_as_uint(a.w) != __float_as_uint(b.w));
+        }
+    }
+    if (__syncthreads_or(diff) && threadIdx.x == 0) bad[blockIdx.x] = 1;
+}
+
+// table[u] = the cached floats of position rep[u], as 81 consecutive float4 chunks.
+__global__ __launch_bounds__(256) void hex_table_fill_kernel(int num_distinct, const int* __restrict__ rep,
+                                                             const float4* __restrict__ A4,
+                                                             float4* __restrict__ table) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_distinct * kHexChunks) return;
+    const int u = i / kHexChunks, k = i - u * kHexChunks;
+    const int r = rep[u];
+    table[i] = A4[(size_t)(r >> 6) * kHexChunks * 64 + (size_t)k * 64 + (r & 63)];
 }
 
 // FP64 diagonal of A over one colour.
@@ -507,9 +579,13 @@ double host_hex_volume(const double* X, int64_t e) {
 }  // namespace
 
 void hex_release(HexData& h) {
-    for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.order_pad, (void*)h.A32, (void*)h.diag64, (void*)h.diag32})
+    for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.order_pad, (void*)h.A32, (void*)h.diag64, (void*)h.diag32,
+                    (void*)h.table, (void*)h.table_idx})
         if (p) (void)hipFree(p);
     h.conn = h.order = h.order_pad = nullptr;
+    h.table = nullptr;
+    h.table_idx = nullptr;
+    h.num_distinct = 0;
     h.A32 = nullptr;
     h.diag64 = nullptr;
     h.diag32 = nullptr;
@@ -586,6 +662,7 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
     HCHECK(hipMalloc(&h.order, (size_t)num_elements * sizeof(int)));
     HCHECK(hipMalloc(&h.order_pad, order_pad.size() * sizeof(int)));
     HCHECK(hipMalloc(&h.A32, order_pad.size() * 36 * 9 * sizeof(float)));
+    h.h_order_pad = order_pad;
     HCHECK(hipMemcpyAsync(h.order_pad, order_pad.data(), order_pad.size() * sizeof(int), hipMemcpyHostToDevice, s));
     HCHECK(hipMalloc(&h.diag64, (size_t)num_nodes * 3 * sizeof(double)));
     HCHECK(hipMalloc(&h.diag32, (size_t)num_nodes * 3 * sizeof(float)));
@@ -597,6 +674,78 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
 }
 
 namespace {
+
+// The distinct-matrix table of the cached operator (HexData::table), or none (num_distinct = 0) when the
+// positions hold more than kHexMaxDistinct distinct matrices or FEM_HEX_TABLE=0 is set. Positions are
+// grouped by a hash of their floats and every position is then compared with its group's representative
+// on the device, so the table holds exactly the floats of the per-position cache.
+void build_table(HexData& h, hipStream_t s) {
+    h.num_distinct = 0;
+    const char* env = getenv("FEM_HEX_TABLE");
+    if (env && std::strcmp(env, "0") == 0) return;
+    const int pad = h.pad_ptr.back();
+    if (pad == 0) return;
+    const float4* A4 = reinterpret_cast<const float4*>(h.A32);
+    unsigned long long* d_hash = nullptr;
+    HCHECK(hipMalloc(&d_hash, (size_t)pad * sizeof(unsigned long long)));
+    std::vector<unsigned long long> hash(pad);
+    hipLaunchKernelGGL(hex_hash_kernel, blocks_for(pad, 256), dim3(256), 0, s, pad, h.order_pad, A4, d_hash);
+    const hipError_t e1 = hipMemcpyAsync(hash.data(), d_hash, (size_t)pad * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(d_hash);
+    HCHECK(e1);
+    HCHECK(e2);
+    std::vector<int> idx(pad, 0), rep;
+    {
+        std::vector<std::pair<unsigned long long, int>> seen;  // (hash, entry), at most kHexMaxDistinct
+        for (int t = 0; t < pad; ++t) {
+            if (h.h_order_pad[t] < 0) continue;
+            int u = -1;
+            for (const auto& q : seen)
+                if (q.first == hash[t]) {
+                    u = q.second;
+                    break;
+                }
+            if (u < 0) {
+                if ((int)rep.size() == kHexMaxDistinct) return;
+                u = (int)rep.size();
+                seen.emplace_back(hash[t], u);
+                rep.push_back(t);
+            }
+            idx[t] = u;
+        }
+    }
+    if (rep.empty()) return;
+    const int nd = (int)rep.size();
+    if (!h.table) HCHECK(hipMalloc(&h.table, (size_t)kHexMaxDistinct * kHexChunks * sizeof(float4)));
+    if (!h.table_idx) HCHECK(hipMalloc(&h.table_idx, (size_t)pad * sizeof(int)));
+    const int nblk = (pad + 255) / 256;
+    int *d_rep = nullptr, *d_bad = nullptr;
+    HCHECK(hipMalloc(&d_rep, (size_t)nd * sizeof(int)));
+    if (hipMalloc(&d_bad, (size_t)nblk * sizeof(int)) != hipSuccess) {
+        (void)hipFree(d_rep);
+        throw std::runtime_error("hipMalloc failed for the table check");
+    }
+    std::vector<int> bad(nblk, 0);
+    hipError_t err = hipMemcpyAsync(h.table_idx, idx.data(), (size_t)pad * sizeof(int), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipMemcpyAsync(d_rep, rep.data(), (size_t)nd * sizeof(int), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipMemsetAsync(d_bad, 0, (size_t)nblk * sizeof(int), s);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(hex_table_verify_kernel, dim3(nblk), dim3(256), 0, s, pad, h.order_pad, A4, h.table_idx,
+                           d_rep, d_bad);
+        hipLaunchKernelGGL(hex_table_fill_kernel, blocks_for((int64_t)nd * kHexChunks, 256), dim3(256), 0, s, nd,
+                           d_rep, A4, reinterpret_cast<float4*>(h.table));
+        err = hipMemcpyAsync(bad.data(), d_bad, (size_t)nblk * sizeof(int), hipMemcpyDeviceToHost, s);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    (void)hipFree(d_rep);
+    (void)hipFree(d_bad);
+    HCHECK(err);
+    for (int b : bad)
+        if (b) return;  // a hash collision between different matrices: keep the per-position cache
+    h.num_distinct = nd;
+}
 
 void hex_prepare(HexData& h, const double* x0, const double4* mat, double cm, double ck, int lumped, hipStream_t s) {
     const int pad = h.pad_ptr.back();
@@ -610,6 +759,7 @@ void hex_prepare(HexData& h, const double* x0, const double4* mat, double cm, do
                            h.conn, x0, mat, cm, ck, lumped, h.diag64);
     }
     hipLaunchKernelGGL(hex_cast_diag_kernel, blocks_for(n, 256), dim3(256), 0, s, n, h.diag64, h.diag32);
+    build_table(h, s);
     h.a32_cm = cm;
     h.a32_ck = ck;
     h.a32_lumped = lumped;
@@ -638,9 +788,16 @@ int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgSca
             const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
             if (fp32) {
                 const int c0 = h.pad_ptr[c], pc = h.pad_ptr[c + 1] - c0;
-                hipLaunchKernelGGL(hex_apply32_kernel, blocks_for(pc, 256), dim3(256), 0, s, pc, h.order_pad + c0,
-                                   h.conn, reinterpret_cast<const float4*>(h.A32) + (size_t)c0 * kHexChunks,
-                                   (const float*)p, a.bcmask, (float*)q, (const CgScalars*)sc);
+                const float4* A4 = reinterpret_cast<const float4*>(h.A32) + (size_t)c0 * kHexChunks;
+                if (h.num_distinct > 0)
+                    hipLaunchKernelGGL(hex_apply32_kernel<true>, blocks_for(pc, 256), dim3(256),
+                                       (size_t)h.num_distinct * kHexChunks * sizeof(float4), s, pc, h.order_pad + c0,
+                                       h.conn, A4, reinterpret_cast<const float4*>(h.table), h.table_idx + c0,
+                                       h.num_distinct, (const float*)p, a.bcmask, (float*)q, (const CgScalars*)sc);
+                else
+                    hipLaunchKernelGGL(hex_apply32_kernel<false>, blocks_for(pc, 256), dim3(256), 0, s, pc,
+                                       h.order_pad + c0, h.conn, A4, (const float4*)nullptr, (const int*)nullptr, 0,
+                                       (const float*)p, a.bcmask, (float*)q, (const CgScalars*)sc);
             } else
                 hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt,
                                    h.order + h.color_ptr[c], h.conn, a.x0, a.mat, (const double*)p, nullptr, nullptr,

@@ -30,6 +30,13 @@ struct HexData {
                                   // position, interleaved by 64-position tiles in float4 chunks
     double* diag64 = nullptr;     // diagonal of A (per DOF)
     float* diag32 = nullptr;
+    // Distinct cached matrices (a structured mesh repeats a few element matrices): when every padded
+    // position's 324 floats equal one of at most kHexMaxDistinct representatives bit for bit, the FP32
+    // operator reads table[u] (81 float4 chunks each, staged in LDS) through table_idx instead of A32.
+    float* table = nullptr;             // [num_distinct][324]
+    int* table_idx = nullptr;           // [padded positions] table entry of each position
+    int num_distinct = 0;               // 0: A32 per position
+    std::vector<int> h_order_pad;       // host copy of order_pad (padding positions are -1)
     double a32_cm = 0.0, a32_ck = 0.0;  // coefficients of M and K the cached A32 / diagonal were built for
     int a32_lumped = -1;                // lumped mass flag of the cache (-1: not built)
     bool ready = false;

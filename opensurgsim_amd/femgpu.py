@@ -32,7 +32,7 @@ EXPORTS = [
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
     "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
     "fem_set_young_modulus", "fem_adjoint_step", "fem_set_instances", "fem_query_instances",
-    "fem_set_batch_tuning", "fem_get_batch_tuning",
+    "fem_set_batch_tuning", "fem_get_batch_tuning", "fem_hex_distinct_matrices",
 ]
 
 PHASES = ["element", "assemble", "rhs", "cg_init", "spmv", "cg_vector", "refine", "update", "spmv_skipped"]
@@ -125,6 +125,8 @@ def load():
     L.fem_query.argtypes = [_vp, ctypes.POINTER(FemStats)]
     L.fem_num_nonzeros.argtypes = [_vp]
     L.fem_num_nonzeros.restype = _i64
+    L.fem_hex_distinct_matrices.argtypes = [_vp]
+    L.fem_hex_distinct_matrices.restype = ctypes.c_int
     L.fem_get_pattern.argtypes = [_vp, _i32p, _i32p]
     L.fem_get_system_matrix.argtypes = [_vp, _dp]
     L.fem_get_rhs.argtypes = [_vp, _dp]
@@ -313,6 +315,10 @@ class FemGpu:
 
     def step(self, n=1):
         return self._check(self._L.fem_step(self.h, int(n)), allow_inactive=True)
+
+    def hex_distinct_matrices(self):
+        """Distinct FP32 element matrices the hexahedral inner solve reads from its table (0: per element)."""
+        return int(self._L.fem_hex_distinct_matrices(self.h))
 
     def synchronize(self):
         self._check(self._L.fem_synchronize(self.h))

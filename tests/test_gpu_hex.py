@@ -76,3 +76,35 @@ def test_hex_invalid_element_rejected():
     g = femgpu.FemGpu(0)
     with pytest.raises(femgpu.FemError):
         g.upload_mesh(femgpu.HEX, cfg.x0, conn, cfg.E, cfg.nu, cfg.rho)
+
+
+def test_hex_distinct_matrix_table(monkeypatch):
+    """The FP32 operator of a structured hex block reads a table of its few distinct element matrices (the
+    table holds exactly the floats of the per-element cache), so the trajectory equals the per-element
+    cache's bit for bit. A mesh with perturbed nodes has more than 32 distinct matrices and keeps the
+    per-element cache, and still follows the oracle."""
+    cfg = mg.config_hex((8, 5, 7))
+    runs = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("FEM_HEX_TABLE", flag)
+        say("hex table", flag)
+        g = make_gpu(cfg, precision=femgpu.FP32_REFINED, tol=1e-9)
+        assert g.hex_distinct_matrices() == 0  # built with the first step's operator
+        assert g.step(6) == femgpu.FEM_OK
+        runs[flag] = (g.state(), g.hex_distinct_matrices(), g.stats()["total_cg_iterations"])
+        g.close()
+    (x1, v1), nd1, it1 = runs["1"]
+    (x0, v0), nd0, it0 = runs["0"]
+    say("  distinct matrices", nd1, "iterations", it1, it0)
+    assert 1 <= nd1 <= 32 and nd0 == 0
+    assert np.array_equal(x1, x0) and np.array_equal(v1, v0) and it1 == it0
+    monkeypatch.delenv("FEM_HEX_TABLE")
+    irr = mg.config_hex((4, 3, 4))
+    rng = np.random.default_rng(11)
+    irr["x0"] = irr.x0 + rng.uniform(-0.1, 0.1, irr.x0.shape) * irr.h
+    e = trajectory(irr, 4, femgpu.FP32_REFINED, 1e-12, 1e-9)
+    g = make_gpu(irr, precision=femgpu.FP32_REFINED, tol=1e-9)
+    assert g.step(1) == femgpu.FEM_OK
+    say("  perturbed mesh: distinct", g.hex_distinct_matrices(), "errors", e)
+    assert g.hex_distinct_matrices() == 0
+    assert e["u"] < 1e-6 and e["v"] < 1e-6
