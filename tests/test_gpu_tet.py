@@ -210,3 +210,26 @@ def test_tet_repeating_a_node_rejected(kind):
         g.upload_mesh(kind, cfg.x0, conn, cfg.E, cfg.nu, cfg.rho)
     say("rejected:", err.value)
     assert err.value.code == femgpu.FEM_ERR_INVALID
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+def test_node_in_no_element(precision):
+    """A node that no element uses: no mass, no stiffness and no pattern entries, so its rows are empty (the
+    Jacobi rule of Eigen's preconditioner takes 1 for the missing diagonal, the block preconditioner falls
+    back to it). It stays where it is while the rest of the block follows the oracle."""
+    cfg = mg.config_corotational((4, 3, 5), seed=21)
+    n0 = cfg.x0.shape[0]
+    cfg["x0"] = np.vstack([cfg.x0, [[0.5, 0.5, 0.5]]])
+    cfg["v0"] = np.concatenate([cfg.v0, np.zeros(3)])
+    if cfg.fext is not None:
+        cfg["fext"] = np.concatenate([cfg.fext, np.zeros(3)])
+    say("node in no element, precision", precision)
+    e = trajectory_errors(cfg, 6, precision=precision)
+    say("  errors", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-6
+    assert e["u"] < bound and e["v"] < bound
+    g = make_gpu(cfg, precision=precision)
+    assert g.step(3) == femgpu.FEM_OK
+    x, v = g.state()
+    assert np.array_equal(x[3 * n0:], cfg.x0[n0]) and not v[3 * n0:].any()
+    assert g.stats()["active"] == 1
