@@ -21,7 +21,7 @@ EULER_EXPLICIT_MODIFIED, LINEAR_EULER_EXPLICIT_MODIFIED = 6, 7
 RUNGE_KUTTA_4, LINEAR_RUNGE_KUTTA_4 = 8, 9
 FP64, FP32_REFINED = 0, 1
 
-# Names of the exported entry points (the judge of the ABI test compares this list with include/fem_gpu.h).
+# Names of the exported entry points (tests/test_abi.py compares this list with include/fem_gpu.h).
 EXPORTS = [
     "fem_create", "fem_destroy", "fem_last_error", "fem_set_stream", "fem_upload_mesh",
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
