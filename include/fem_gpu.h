@@ -75,7 +75,7 @@ typedef struct fem_options {
     double cg_tolerance;         /* relative residual tolerance (Eigen default: machine epsilon) */
     int64_t cg_max_iterations;   /* < 0: 2 * ndof (Eigen default) */
     double inner_tolerance;      /* FP32 mode: relative tolerance of each inner FP32 CG solve */
-    int32_t max_refinements;     /* FP32 mode: maximum FP64 refinement sweeps per step */
+    int32_t max_refinements;     /* FP32 mode: maximum FP64 refinement sweeps per step (>= 1) */
     int32_t fail_on_nonconvergence; /* 0: silent like Eigen's CG (default) */
     double rayleigh_mass;        /* Rayleigh damping D = rayleigh_mass M + rayleigh_stiffness K, default 0
                                     (FemRepresentation::setRayleighDampingMass / setRayleighDampingStiffness,

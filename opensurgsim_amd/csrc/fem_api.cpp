@@ -1690,6 +1690,8 @@ int fem_set_options(fem_ctx* c, const fem_options* o) {
             throw InvalidArg("mass lumping with Rayleigh mass damping needs an implicit Euler scheme");
         if (o->precision != FEM_PRECISION_FP64 && o->precision != FEM_PRECISION_FP32_REFINED)
             throw InvalidArg("unknown precision");
+        if (o->precision == FEM_PRECISION_FP32_REFINED && o->max_refinements < 1)
+            throw InvalidArg("FP32 precision needs at least one refinement sweep (max_refinements >= 1)");
         if (o->newton_max_iterations < 1) throw InvalidArg("The maximum iteration needs to be at least 1");
         if (!(o->cg_tolerance >= 0.0)) throw InvalidArg("negative tolerance");
         bool grav = o->gravity_enabled != c->opt.gravity_enabled || o->gravity[0] != c->opt.gravity[0] ||

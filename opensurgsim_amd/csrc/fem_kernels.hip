@@ -662,7 +662,14 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
                                                               TV* q1, const TV* __restrict__ dinv,
                                                               double* __restrict__ part, CgScalars* __restrict__ sc) {
     __shared__ double lds[128];
-    if (sc->done) return;
+    // Invariant: every workgroup of the launch that finds the solve converged (launch k, testing iteration
+    // k - 1) must still apply that iteration's x += alpha p to its rows. Block 0 publishes completion as
+    // done = 2 + k inside that same launch, and workgroups dispatched after its write would otherwise skip
+    // the update; so only a flag from an earlier launch (or from another writer, done = 1) ends the kernel
+    // here. The converging launch writes no block sums, so a late workgroup re-reads the same sums of
+    // launch k - 1 and takes the same branch.
+    const int done = sc->done;
+    if (done != 0 && done != 2 + k) return;
     const bool odd = k & 1;
     const TV* ro = odd ? r0 : r1;
     const TV* zo = odd ? z0 : z1;
@@ -731,7 +738,7 @@ __global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8)))
                 sc->err = sqrt(fmax(rr, 0.0) / sc->bb);
                 sc->iters = conv ? k - 1 : k;
                 sc->converged = conv ? 1 : 0;
-                sc->done = 1;
+                sc->done = 2 + k;
             }
             return;
         }

@@ -39,7 +39,8 @@ struct CgScalars {
     double threshold;   // max(tol^2 |b|^2, realmin)
     double rr;          // |r|^2 at the last test
     double err;         // sqrt(|r|^2 / |b|^2) at exit
-    int done;           // solve finished (converged or capped)
+    int done;           // solve finished (converged or capped): 0 while running, 1 from every writer but the
+                        // one-kernel CG, which writes 2 + k from its launch k (see cg_fused_kernel)
     int converged;
     int iters;          // Eigen's iteration count
     int pad;
