@@ -158,6 +158,18 @@ int fem_apply_compliance(fem_ctx* ctx, const double* b, int64_t num_columns, dou
  * tetrahedral mesh the columns are solved up to 8 at a time by a multi-column FP64 CG that reads the
  * operator once per iteration for all of them (each column keeps its own Eigen CG recurrence and test). */
 int fem_apply_compliance_device(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
+/* The same as fem_apply_compliance_device, allocating nothing: no hipMalloc, hipFree, hipHostMalloc or other
+ * allocating routine. No counterpart in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation). The multi-column CG works in a fixed
+ * device workspace of FEM_NOALLOC_SCRATCH_BYTES per GPU, held from fem_create until the process ends, which
+ * bounds the mesh to about 256,000 DOFs (larger meshes: FEM_ERR_INVALID with the limit in the message). Every
+ * column goes through the multi-column FP64 CG (in FP64 mode the results equal fem_apply_compliance_device's
+ * bit for bit; in the FP32 + refinement mode they agree to the CG tolerance). Tetrahedral meshes of one
+ * instance only; FEM_ERR_ORDER before the first step and after boundary conditions were added since the last
+ * step. All contexts on one GPU share its workspace and each call leaves its contents behind: calls that
+ * overlap on one GPU (other streams or threads) race. Blocks the calling thread until the columns are done. */
+#define FEM_NOALLOC_SCRATCH_BYTES (96ull << 20)
+int fem_apply_compliance_device_noalloc(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
 /* The compliance terms of a constraint localized in one element, as the constraint builders form them for
  * the MLCP (Physics/FemConstraintFixedPoint.cpp:84-106, Physics/FemConstraintFrictionlessContact.cpp:84-112,
  * consumed by MlcpPhysicsProblem::updateConstraint, Physics/MlcpPhysicsProblem.cpp:46-76). H has

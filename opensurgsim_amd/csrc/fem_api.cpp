@@ -258,6 +258,7 @@ struct fem_ctx {
     DevBuf<double> mcg_b, mcg_x, mcg_r, mcg_z, mcg_p, mcg_q, mcg_dinv, mcg_part;
     DevBuf<McgColumn> mcg_col;
     McgColumn* h_mcg = nullptr;  // pinned [8]
+    void* scratch = nullptr;     // device address of this device's allocation-free workspace
 
     // hexahedra (element-by-element path)
     HexData hex;
@@ -1494,6 +1495,7 @@ int fem_create(int device, fem_ctx** out) {
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
+        c->scratch = noalloc_scratch_address(device);  // once per device, with the device current
         HIPCHECK(hipHostMalloc((void**)&c->h_sc, 2 * sizeof(CgScalars), hipHostMallocDefault));
         HIPCHECK(hipHostMalloc((void**)&c->h_counters, 8 * sizeof(int), hipHostMallocDefault));
         c->partial.alloc(4 * kCgBlocks);
@@ -1777,6 +1779,73 @@ void compliance_column(fem_ctx* c, const double* b, double* out) {
 // remainder of fewer than 5 columns goes column by column through the step's FP32 solver instead: on the
 // 40^3 block a fixed-point constraint's 3 columns take 2.04 ms that way against 2.19 ms in one 4-column FP64
 // pass, while a full 8-column pass costs 0.35 ms per column.
+// Largest DOF count whose multi-column CG buffers fit the allocation-free workspace.
+int64_t noalloc_max_dof() {
+    size_t n = kNoAllocScratchBytes / (49 * sizeof(double));
+    while (n > 0 && mcg_layout(n).total > kNoAllocScratchBytes) --n;
+    return (int64_t)n;
+}
+
+// Work buffers of the multi-column CG: 8 interleaved columns of n DOFs for b, x, r, z, p, q; n for the
+// Jacobi inverse; 3 x kCgBlocks x 8 block sums; 8 column states. status: host copy of the column states.
+struct McgWork {
+    double *b, *x, *r, *z, *p, *q, *dinv, *partial;
+    McgColumn* col;
+    McgColumn* status;
+};
+
+// Columns [0, ncols) of b (device, column-major) through the multi-column FP64 CG, up to 8 at a time: each
+// batch is interleaved [dof][kw] with its boundary rows zeroed, solved against S64, and scattered back with
+// the boundary rows of the solutions zeroed. The host reads the column states between chunks of launches
+// (iterations past a column's finish leave it unchanged, so the launch count does not change the result).
+void mcg_columns(fem_ctx* c, const double* b, int64_t ncols, double* out, const McgWork& w) {
+    const size_t n = (size_t)c->ndof;
+    const int max_it = (int)std::min<int64_t>(max_iters(c), std::numeric_limits<int>::max() - 1);
+    for (int64_t done = 0; done < ncols;) {
+        const int64_t left = ncols - done;
+        const int kw = left >= 5 ? 8 : left >= 3 ? 4 : (int)left;
+        const int cnt = (int)std::min<int64_t>(left, kw);
+        launch_mcg_gather((int64_t)n, kw, cnt, b + (size_t)done * n, c->bcmask.p, w.b, c->stream);
+        McgArgs a;
+        a.kw = kw;
+        a.num_nodes = (int)c->num_nodes;
+        a.rowptr = c->rowptr_b.p;
+        a.colidx = c->colidx_b.p;
+        a.diagblk = c->diagblk.p;
+        a.S = c->S64.p;
+        a.b = w.b;
+        a.x = w.x;
+        a.r = w.r;
+        a.z = w.z;
+        a.p = w.p;
+        a.q = w.q;
+        a.dinv = w.dinv;
+        a.partial = w.partial;
+        a.col = w.col;
+        a.tol = c->opt.cg_tolerance;
+        a.max_iter = max_it;
+        launch_mcg_init(a, c->stream);
+        const int pred = std::max(4, c->predicted_iters + 2);
+        int k = 0;
+        for (; k < std::min(pred, max_it); ++k) launch_mcg_iteration(a, k, c->stream);
+        for (;;) {
+            HIPCHECK(hipMemcpyAsync(w.status, w.col, kw * sizeof(McgColumn), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            bool running = false;
+            for (int j = 0; j < kw; ++j) running = running || w.status[j].fin == kMcgRunning;
+            if (!running || k >= max_it) break;
+            const int chunk = std::max(2, pred / 8);
+            for (int i = 0; i < chunk && k < max_it; ++i, ++k) launch_mcg_iteration(a, k, c->stream);
+        }
+        launch_mcg_scatter((int64_t)n, kw, cnt, w.x, c->bcmask.p, out + (size_t)done * n, c->stream);
+        done += cnt;
+    }
+}
+
+// Compliance columns of a tetrahedral mesh through the multi-column FP64 CG (mcg_columns) on the context's
+// own work buffers. In the FP32 + refinement mode a remainder of fewer than 5 columns goes column by column
+// through the step's FP32 solver instead: on the 40^3 block a fixed-point constraint's 3 columns take 2.04 ms
+// that way against 2.19 ms in one 4-column FP64 pass, while a full 8-column pass costs 0.35 ms per column.
 void compliance_tet(fem_ctx* c, const double* b, int64_t ncols, double* out) {
     const size_t n = (size_t)c->ndof;
     if (c->opt.precision == FEM_PRECISION_FP32_REFINED) {
@@ -1786,7 +1855,6 @@ void compliance_tet(fem_ctx* c, const double* b, int64_t ncols, double* out) {
             ncols = full;
         }
     }
-    const int max_it = (int)std::min<int64_t>(max_iters(c), std::numeric_limits<int>::max() - 1);
     if (c->mcg_b.n != n * 8) {
         for (DevBuf<double>* v : {&c->mcg_b, &c->mcg_x, &c->mcg_r, &c->mcg_z, &c->mcg_p, &c->mcg_q}) v->alloc(n * 8);
         c->mcg_dinv.alloc(n);
@@ -1794,45 +1862,9 @@ void compliance_tet(fem_ctx* c, const double* b, int64_t ncols, double* out) {
         c->mcg_col.alloc(8);
         if (!c->h_mcg) HIPCHECK(hipHostMalloc((void**)&c->h_mcg, 8 * sizeof(McgColumn), hipHostMallocDefault));
     }
-    for (int64_t done = 0; done < ncols;) {
-        const int64_t left = ncols - done;
-        const int kw = left >= 5 ? 8 : left >= 3 ? 4 : (int)left;
-        const int cnt = (int)std::min<int64_t>(left, kw);
-        launch_mcg_gather((int64_t)n, kw, cnt, b + (size_t)done * n, c->bcmask.p, c->mcg_b.p, c->stream);
-        McgArgs a;
-        a.kw = kw;
-        a.num_nodes = (int)c->num_nodes;
-        a.rowptr = c->rowptr_b.p;
-        a.colidx = c->colidx_b.p;
-        a.diagblk = c->diagblk.p;
-        a.S = c->S64.p;
-        a.b = c->mcg_b.p;
-        a.x = c->mcg_x.p;
-        a.r = c->mcg_r.p;
-        a.z = c->mcg_z.p;
-        a.p = c->mcg_p.p;
-        a.q = c->mcg_q.p;
-        a.dinv = c->mcg_dinv.p;
-        a.partial = c->mcg_part.p;
-        a.col = c->mcg_col.p;
-        a.tol = c->opt.cg_tolerance;
-        a.max_iter = max_it;
-        launch_mcg_init(a, c->stream);
-        const int pred = std::max(4, c->predicted_iters + 2);
-        int k = 0;
-        for (; k < std::min(pred, max_it); ++k) launch_mcg_iteration(a, k, c->stream);
-        for (;;) {
-            HIPCHECK(hipMemcpyAsync(c->h_mcg, c->mcg_col.p, kw * sizeof(McgColumn), hipMemcpyDeviceToHost, c->stream));
-            HIPCHECK(hipStreamSynchronize(c->stream));
-            bool running = false;
-            for (int j = 0; j < kw; ++j) running = running || c->h_mcg[j].fin == kMcgRunning;
-            if (!running || k >= max_it) break;
-            const int chunk = std::max(2, pred / 8);
-            for (int i = 0; i < chunk && k < max_it; ++i, ++k) launch_mcg_iteration(a, k, c->stream);
-        }
-        launch_mcg_scatter((int64_t)n, kw, cnt, c->mcg_x.p, c->bcmask.p, out + (size_t)done * n, c->stream);
-        done += cnt;
-    }
+    const McgWork w{c->mcg_b.p, c->mcg_x.p, c->mcg_r.p, c->mcg_z.p, c->mcg_p.p, c->mcg_q.p,
+                    c->mcg_dinv.p, c->mcg_part.p, c->mcg_col.p, c->h_mcg};
+    mcg_columns(c, b, ncols, out, w);
 }
 
 // C b for num_columns device columns (column-major) on the context stream.
@@ -1892,6 +1924,33 @@ int fem_apply_compliance_device(fem_ctx* c, const double* b, int64_t num_columns
         if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
         HIPCHECK(hipSetDevice(c->device));
         compliance_device(c, b, num_columns, out);
+        return FEM_OK;
+    });
+}
+
+int fem_apply_compliance_device_noalloc(fem_ctx* c, const double* b, int64_t num_columns, double* out) {
+    return guarded(c, [&] {
+        if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+        if (num_columns < 0 || (num_columns > 0 && (!b || !out))) throw InvalidArg("null or negative columns");
+        if (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0)
+            throw InvalidArg("the allocation-free compliance needs a tetrahedral mesh of one instance");
+        const McgLayout L = mcg_layout((size_t)c->ndof);
+        if (L.total > kNoAllocScratchBytes)
+            throw InvalidArg("the allocation-free compliance supports at most " +
+                             std::to_string(noalloc_max_dof()) + " DOFs (" +
+                             std::to_string(kNoAllocScratchBytes >> 20) + " MiB workspace); this mesh has " +
+                             std::to_string(c->ndof));
+        if (!c->scratch) throw InvalidArg("no workspace on this device");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        if (c->bc_dirty)
+            return fail(c, FEM_ERR_ORDER, "boundary conditions changed since the last step: step first");
+        HIPCHECK(hipSetDevice(c->device));
+        char* base = (char*)c->scratch;
+        McgColumn status[8];
+        const McgWork w{(double*)(base + L.b), (double*)(base + L.x), (double*)(base + L.r),
+                        (double*)(base + L.z), (double*)(base + L.p), (double*)(base + L.q),
+                        (double*)(base + L.dinv), (double*)(base + L.partial), (McgColumn*)(base + L.col), status};
+        mcg_columns(c, b, num_columns, out, w);
         return FEM_OK;
     });
 }

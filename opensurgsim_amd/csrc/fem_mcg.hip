@@ -12,8 +12,50 @@
 #include "fem_reduce.hpp"
 
 #include <cfloat>
+#include <mutex>
+#include <stdexcept>
+#include <string>
 
 namespace femgpu {
+
+// The allocation-free entry point's workspace (uninitialised; every kernel that reads a part of it writes
+// that part first in the same call). Host code never passes `scratch` itself: its name on the host is a
+// stand-in, not the device address, which is taken per device below.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void* noalloc_scratch_address(int device) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[device] = ptr;
+    }
+    return addr[device];
+}
+
+McgLayout mcg_layout(size_t n) {
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    McgLayout L;
+    size_t o = 0;
+    const size_t vec = up(n * 8 * sizeof(double));
+    L.b = o, o += vec;
+    L.x = o, o += vec;
+    L.r = o, o += vec;
+    L.z = o, o += vec;
+    L.p = o, o += vec;
+    L.q = o, o += vec;
+    L.dinv = o, o += up(n * sizeof(double));
+    L.partial = o, o += up((size_t)3 * kCgBlocks * 8 * sizeof(double));
+    L.col = o, o += up(8 * sizeof(McgColumn));
+    L.total = o;
+    return L;
+}
 
 namespace {
 

@@ -41,6 +41,20 @@ struct McgArgs {
     int max_iter;
 };
 
+// Fixed device workspace of the allocation-free compliance entry point: a file-scope device array of
+// kNoAllocScratchBytes per GPU, held from the library's first use of the GPU until the process ends.
+// noalloc_scratch_address returns its address on `device` (which must be the current device), taken once per
+// device with hipGetSymbolAddress and cached; it throws std::runtime_error when the lookup fails. Every call
+// on one GPU shares the one array, so calls that overlap on that GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;
+void* noalloc_scratch_address(int device);
+
+// Byte offsets of the multi-column CG buffers in one workspace for n DOFs (8 columns); total bytes needed.
+struct McgLayout {
+    size_t b, x, r, z, p, q, dinv, partial, col, total;
+};
+McgLayout mcg_layout(size_t n);
+
 void launch_mcg_init(const McgArgs& a, hipStream_t s);
 // Column-major right-hand sides b (cnt columns of n) into the interleaved [n][kw] layout, boundary rows and
 // padding columns zero; and back: out (cnt columns of n) from x [n][kw], boundary rows zero.

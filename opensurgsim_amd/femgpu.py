@@ -26,7 +26,7 @@ EXPORTS = [
     "fem_create", "fem_destroy", "fem_last_error", "fem_set_stream", "fem_upload_mesh",
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
     "fem_set_options", "fem_set_external_force", "fem_add_external_generalized_force", "fem_apply_compliance",
-    "fem_apply_compliance_device", "fem_constraint_compliance",
+    "fem_apply_compliance_device", "fem_apply_compliance_device_noalloc", "fem_constraint_compliance",
     "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
@@ -115,6 +115,7 @@ def load():
     L.fem_add_external_generalized_force.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _dp, _dp]
     L.fem_apply_compliance.argtypes = [_vp, _dp, ctypes.c_int64, _dp]
     L.fem_apply_compliance_device.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
+    L.fem_apply_compliance_device_noalloc.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
     L.fem_constraint_compliance.argtypes = [_vp, _i64, _dp, ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
     L.fem_set_state.argtypes = [_vp, _dp, _dp]
     L.fem_get_state.argtypes = [_vp, _dp, _dp]
@@ -280,6 +281,11 @@ class FemGpu:
         """C b with device pointers (column-major ndof x num_columns FP64), ordered on the context stream."""
         self._check(self._L.fem_apply_compliance_device(self.h, ctypes.c_void_p(b_ptr), int(num_columns),
                                                         ctypes.c_void_p(out_ptr)))
+
+    def apply_compliance_device_noalloc(self, b_ptr, num_columns, out_ptr):
+        """apply_compliance_device through the entry point that allocates nothing (fixed per-GPU workspace)."""
+        self._check(self._L.fem_apply_compliance_device_noalloc(self.h, ctypes.c_void_p(b_ptr), int(num_columns),
+                                                                ctypes.c_void_p(out_ptr)))
 
     def constraint_compliance(self, element, coordinate, directions, scale, want_cht=True):
         """(C H^T of shape (ndof, m) or None, H C H^T of shape (m, m)) for a constraint localized in one element:
