@@ -1,15 +1,22 @@
-"""Benchmark: implicit-Euler DOF-steps/s of the corotational-tet block on one MI355X.
+"""Benchmark: implicit-Euler DOF-steps/s of the corotational-tet block on one MI355X, with every other
+single-GPU configuration as a sub-record of the same JSON line.
 
-Workload (one GPU): 40x40x40 cubes split into 5 corotational tetrahedra each (320,000 tets,
-68,921 nodes, 206,763 DOF), E=5e4 Pa, nu=0.45, rho=1000, bottom face fixed, seeded tangential
-top-face velocities N(0, 0.01 m/s), gravity, implicit Euler dt=1 ms, Jacobi-CG to 1e-8.
-A step is one implicit-Euler step (element kernels, assembly, right-hand side, CG, update) over
-the whole mesh; inputs are resident in HBM when the timed region starts.
+Headline (one GPU): 40x40x40 cubes split into 5 corotational tetrahedra each (320,000 tets, 68,921 nodes,
+206,763 DOF), E=5e4 Pa, nu=0.45, rho=1000, bottom face fixed, seeded tangential top-face velocities
+N(0, 0.01 m/s), gravity, implicit Euler dt=1 ms, Jacobi-CG to 1e-8. A step is one implicit-Euler step (element
+kernels, assembly, right-hand side, CG, update) over the whole mesh; inputs are resident in HBM when the timed
+region starts. Both precisions run: FP64, and FP32 operator/CG inside FP64 iterative refinement. The headline is
+the faster of the runs whose displacement and velocity agree with the FP64 run to <= 1e-6 relative (the FP64
+run always qualifies). The FP64 path is checked against the CPU oracle over the baseline's first sample.
 
-Both precisions run: FP64, and FP32 operator/CG inside FP64 iterative refinement. The headline is
-the faster of the runs whose displacement and velocity after warmup+steps agree with the FP64 run
-to <= 1e-6 relative (the FP64 run always qualifies).
-The FP64 path itself is checked here against the CPU oracle over a short sample.
+CPU baseline: the FP64 oracle built with -march=native on this host, one core, one warm-up step and 3 timed
+samples (median), the host's CPU model; plus the faithful variant with the reference's O(#BC x D) boundary
+zeroing.
+
+Sub-records ("configs"), each at its own configuration's step count: the linear 10^3 case with the CPU's LU
+and first-step dense compliance, the 64-instance batch, the adjoint gradient (gradients/s and the 32-direction
+finite-difference check) and the 128x64x128 hexahedral block; each with its dominant kernel's roofline.
+--headline-only skips them; --workload runs one of them alone.
 
 Prints one JSON line on stdout (everything else goes to stderr).
 """
@@ -114,20 +121,71 @@ def copy_probe(mib, reps=3):
     return gbs
 
 
-def cpu_baseline(cfg, sample_steps):
-    """Oracle (FP64 C++ restatement, one core) on the same workload; also checks the GPU FP64 path
-    against it over the same steps."""
+def cpu_model():
+    """The host CPU's model string and logical core count (the CPU baseline's hardware)."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "logical_cores": os.cpu_count()}
+
+
+_NATIVE = {"done": False, "flags": "-O3 -march=x86-64-v3 (prebuilt, portable)"}
+
+
+def use_native_oracle():
+    """Build the oracle with -O3 -march=native for this host (about 10 s) and load it for the CPU legs; if the
+    compiler is missing the prebuilt portable build stays in use. Returns the flags in use."""
+    import oracle
+    if not _NATIVE["done"]:
+        _NATIVE["done"] = True
+        try:
+            import tempfile
+            out = tempfile.mkdtemp(prefix="oracle_native_")
+            t0 = time.perf_counter()
+            path = oracle.build_native(out)
+            oracle.use_library(path)
+            _NATIVE["flags"] = "-O3 -march=native (built on this host in %.1f s)" % (time.perf_counter() - t0)
+        except Exception as e:  # noqa: BLE001 - the portable build is a valid fallback for a baseline
+            log("native oracle build failed (%r); using the prebuilt portable build" % (e,))
+    return _NATIVE["flags"]
+
+
+def oracle_sim(cfg, scheme=None, linsolver=None, tol=None, faithful_bc=False):
     import oracle
     s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
     for n in cfg.bc_nodes:
         s.add_bc_node(n)
-    s.set_solver(tol=cfg.cg_tol)
+    s.set_gravity(cfg.gravity, True)
+    if cfg.fext is not None:
+        s.set_external_force(cfg.fext)
+    s.set_solver(scheme=oracle.EULER_IMPLICIT if scheme is None else scheme,
+                 linsolver=oracle.CG if linsolver is None else linsolver,
+                 tol=cfg.cg_tol if tol is None else tol, faithful_bc=faithful_bc)
     s.set_state(v=cfg.v0)
+    return s
+
+
+def cpu_baseline(cfg, sample_steps, runs=3):
+    """Oracle (FP64 C++ restatement, one core, native build) on the same workload: one warm-up step, then
+    `runs` timed samples of `sample_steps` steps each; the median is the baseline. Also checks the GPU FP64
+    path against the oracle state after the first 1 + sample_steps steps."""
+    s = oracle_sim(cfg)
     s.step(cfg.dt, 1)  # warm-up step
-    t0 = time.perf_counter()
-    s.step(cfg.dt, sample_steps)
-    t = time.perf_counter() - t0
-    xr, vr = s.state()
+    times = []
+    ref = None
+    for r in range(runs):
+        t0 = time.perf_counter()
+        s.step(cfg.dt, sample_steps)
+        times.append(time.perf_counter() - t0)
+        if r == 0:
+            ref = s.state()
+    xr, vr = ref
     g = make_gpu(cfg, femgpu.FP64)
     g.step(1 + sample_steps)
     x, v = g.state()
@@ -136,19 +194,15 @@ def cpu_baseline(cfg, sample_steps):
     err = dict(x=float(np.linalg.norm(x - xr) / np.linalg.norm(xr)),
                u=float(np.linalg.norm((x - x0) - (xr - x0)) / np.linalg.norm(xr - x0)),
                v=float(np.linalg.norm(v - vr) / np.linalg.norm(vr)))
-    return x0.size * sample_steps / t, t, err
+    t = float(np.median(times))
+    return x0.size * sample_steps / t, times, err
 
 
 def cpu_faithful(cfg, steps):
     """The oracle with the reference's own boundary-condition zeroing (zeroRow / zeroColumn through coeff()
     look-ups of every row and column, O(#BC x D), Math/SparseMatrix.h:352-377), one core, after one warm-up step:
     the faithful CPU variant, to set beside the restated-efficient one."""
-    import oracle
-    s = oracle.Sim(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
-    for n in cfg.bc_nodes:
-        s.add_bc_node(n)
-    s.set_solver(tol=cfg.cg_tol, faithful_bc=True)
-    s.set_state(v=cfg.v0)
+    s = oracle_sim(cfg, faithful_bc=True)
     s.step(cfg.dt, 1)
     t0 = time.perf_counter()
     s.step(cfg.dt, steps)
@@ -157,63 +211,94 @@ def cpu_faithful(cfg, steps):
 
 
 def bench_hex(args):
-    """Hexahedral block (Fem3DElementCube), 128 x 64 x 128 cubes by default: FP32 element-by-element CG with
-    cached element matrices inside FP64 refinement to 1e-6, against the FP64 matrix-free path at 1e-10."""
+    """Hexahedral block (Fem3DElementCube), 128 x 64 x 128 cubes by default: FP32 element-by-element CG inside
+    FP64 refinement to 1e-6 with one cached 24x24 element matrix per element (the headline form), the same with
+    the table of distinct cached matrices a structured mesh allows (identical results), and the FP64 matrix-free
+    path at 1e-10 as the reference for the errors."""
     n = (args.hex_n, args.hex_n // 2, args.hex_n)
     cfg = mg.config_hex(n)
     N = cfg.x0.shape[0]
     D = 3 * N
+    E = cfg.conn.shape[0]
     steps = args.steps if args.steps_given else cfg.steps
-    log("workload: hexahedra %s, %d elements, %d nodes, %d DOF, %d steps" % (n, cfg.conn.shape[0], N, D, steps))
+    log("workload: hexahedra %s, %d elements, %d nodes, %d DOF, %d steps" % (n, E, N, D, steps))
 
-    def run(precision, tol):
-        g = femgpu.FemGpu(0)
-        t0 = time.perf_counter()
-        g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
-        g.add_bc_nodes(cfg.bc_nodes)
-        g.set_options(dt=cfg.dt, cg_tolerance=tol, precision=precision, gravity=cfg.gravity)
-        g.set_external_force(cfg.fext)
-        log("  set-up %.1f s" % (time.perf_counter() - t0))
-        if args.warmup:
-            g.step(args.warmup)
-        device_sync(g)
-        g.enable_phase_timers(args.timer_every)
-        it0 = g.stats()["total_cg_iterations"]
-        t0 = time.perf_counter()
-        rc = g.step(steps)
-        device_sync(g)
-        t = time.perf_counter() - t0
-        assert rc == femgpu.FEM_OK
-        st = g.stats()
-        x, v = g.state()
-        ph = g.phase_times()
-        distinct = g.hex_distinct_matrices()
-        g.close()
-        return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, stats=st, phases=ph, distinct=distinct)
+    def run(precision, tol, table=True):
+        saved = os.environ.get("FEM_HEX_TABLE")
+        if not table:
+            os.environ["FEM_HEX_TABLE"] = "0"
+        try:
+            g = femgpu.FemGpu(0)
+            t0 = time.perf_counter()
+            g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+            g.add_bc_nodes(cfg.bc_nodes)
+            g.set_options(dt=cfg.dt, cg_tolerance=tol, precision=precision, gravity=cfg.gravity)
+            g.set_external_force(cfg.fext)
+            log("  set-up %.1f s" % (time.perf_counter() - t0))
+            if args.warmup:
+                g.step(args.warmup)
+            device_sync(g)
+            g.enable_phase_timers(args.timer_every)
+            it0 = g.stats()["total_cg_iterations"]
+            t0 = time.perf_counter()
+            rc = g.step(steps)
+            device_sync(g)
+            t = time.perf_counter() - t0
+            assert rc == femgpu.FEM_OK
+            st = g.stats()
+            x, v = g.state()
+            ph = g.phase_times()
+            distinct = g.hex_distinct_matrices()
+            op_ms = g.time_operator(20)
+            g.close()
+        finally:
+            if saved is None:
+                os.environ.pop("FEM_HEX_TABLE", None)
+            else:
+                os.environ["FEM_HEX_TABLE"] = saved
+        return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, stats=st, phases=ph, distinct=distinct,
+                    op_ms=op_ms)
 
-    r32 = run(femgpu.FP32_REFINED, 1e-6)
-    log("fp32+refine: %.3f s, %d CG iterations, last sweeps %d" % (r32["seconds"], r32["cg"],
-                                                                    r32["stats"]["last_refinements"]))
+    r32 = run(femgpu.FP32_REFINED, 1e-6, table=False)
+    log("fp32+refine, per-element cache: %.3f s, %d CG iterations, last sweeps %d, operator %.3f ms" % (
+        r32["seconds"], r32["cg"], r32["stats"]["last_refinements"], r32["op_ms"]))
+    rt = run(femgpu.FP32_REFINED, 1e-6, table=True)
+    log("fp32+refine, distinct-matrix table (%d): %.3f s, operator %.3f ms" % (rt["distinct"], rt["seconds"],
+                                                                               rt["op_ms"]))
     r64 = run(femgpu.FP64, 1e-10)
-    log("fp64 (1e-10): %.3f s, %d CG iterations" % (r64["seconds"], r64["cg"]))
+    log("fp64 (1e-10): %.3f s, %d CG iterations, operator %.3f ms" % (r64["seconds"], r64["cg"], r64["op_ms"]))
     x0 = cfg.x0.reshape(-1)
-    err = dict(u=float(np.linalg.norm(r32["x"] - r64["x"]) / np.linalg.norm(r64["x"] - x0)),
-               v=float(np.linalg.norm(r32["v"] - r64["v"]) / np.linalg.norm(r64["v"])))
-    log("fp32+refine vs fp64 rel err", err)
-    out = {"metric": METRIC, "value": D * steps / r32["seconds"], "unit": "DOF-steps/s", "n_gpus": 1,
-           "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / steps,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",
-           "data": "synthetic",
-           "config": {"workload": "hex-block %dx%dx%d Fem3DElementCube, EbE FP32 cached Ke + FP64 refinement 1e-6, "
-                                  "dt=1ms" % n, "elements": int(cfg.conn.shape[0]), "nodes": int(N), "dof": int(D),
-                      "parallelism": "single-gpu"},
-           "rel_err_vs_fp64": err,
-           "fp64": {"value": D * steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / steps,
-                    "cg_iters_per_step": r64["cg"] / steps},
-           "fp32_refined": {"cg_iters_per_step": r32["cg"] / steps,
-                            "distinct_element_matrices": r32["distinct"]},
-           "phases_fp32": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in r32["phases"].items()}}
-    print(json.dumps(out), flush=True)
+
+    def err(r):
+        return dict(u=float(np.linalg.norm(r["x"] - r64["x"]) / np.linalg.norm(r64["x"] - x0)),
+                    v=float(np.linalg.norm(r["v"] - r64["v"]) / np.linalg.norm(r64["v"])))
+    e32, et = err(r32), err(rt)
+    log("fp32+refine vs fp64 rel err", e32, "table", et, "identical", bool(np.array_equal(r32["x"], rt["x"])))
+    # one application of the FP32 operator (8 colours on a block mesh): per element the cached blocks
+    # (1,296 B; table: a 4 B index), 32 B of node ids and its 4 B slot; per colour p read, q read and written
+    # and the boundary mask over the DOFs
+    colours = 8
+    cache_bytes = E * (1296 + 32 + 4) + colours * 13 * D
+    table_bytes = E * (4 + 32 + 4) + colours * 13 * D
+    return {"metric": METRIC, "value": D * steps / r32["seconds"], "unit": "DOF-steps/s", "n_gpus": 1,
+            "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",
+            "data": "synthetic",
+            "config": {"workload": "hex-block %dx%dx%d Fem3DElementCube, EbE FP32 with one cached 24x24 Ke per "
+                                   "element + FP64 refinement 1e-6, dt=1ms" % n, "elements": int(E), "nodes": int(N),
+                       "dof": int(D), "parallelism": "single-gpu"},
+            "rel_err_vs_fp64": e32,
+            "roofline": roofline("hex_apply32_kernel<false> (all 8 colours)", cache_bytes, r32["op_ms"] / 1e3),
+            "table_variant": {"value": D * steps / rt["seconds"], "ms_per_step": 1e3 * rt["seconds"] / steps,
+                              "distinct_element_matrices": rt["distinct"], "rel_err_vs_fp64": et,
+                              "identical_to_per_element_cache": bool(np.array_equal(r32["x"], rt["x"]) and
+                                                                     np.array_equal(r32["v"], rt["v"])),
+                              "roofline": roofline("hex_apply32_kernel<true> (all 8 colours)", table_bytes,
+                                                   rt["op_ms"] / 1e3)},
+            "fp64": {"value": D * steps / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / steps,
+                     "cg_iters_per_step": r64["cg"] / steps, "operator_ms": r64["op_ms"]},
+            "fp32_refined": {"cg_iters_per_step": r32["cg"] / steps},
+            "phases_fp32": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in r32["phases"].items()}}
 
 
 SCHEME_NAMES = {femgpu.EULER_EXPLICIT: "euler_explicit", femgpu.EULER_EXPLICIT_MODIFIED: "euler_explicit_modified",
@@ -263,7 +348,7 @@ def bench_schemes(args):
                                   "Rayleigh 1 / 1e-4, Jacobi-CG 1e-8" % n, "dof": int(D),
                       "parallelism": "single-gpu"},
            "schemes": res}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_compliance(args):
@@ -351,7 +436,7 @@ def bench_compliance(args):
                                   "%dx%dx%d after %d implicit steps, CG 1e-8" % ((ncon, 3 * ncon) + n + (args.warmup,)),
                       "dof": int(D), "parallelism": "single-gpu"},
            "precisions": res}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_adjoint(args):
@@ -427,7 +512,13 @@ def bench_adjoint(args):
                                             np.linalg.norm(np.log(cfg.E)))}
         log("identification:", ident)
     st = g.stats()
+    op_ms = g.time_operator(100)
+    N = cfg.x0.shape[0]
+    Bk = g.num_nonzeros() // 9
+    roof = roofline("cg_fused_kernel<double>", spmv_bytes((N, 3 * N, Bk), 8), op_ms / 1e3,
+                    "the forward and transpose solves' CG iteration; operator and vectors (11.5 MB) stay in the MALL")
     out = {"metric": METRIC, "value": reps / t, "unit": "gradients/s", "n_gpus": 1, "steps": reps,
+           "roofline": roof,
            "warmup": 1, "ms_per_step": 1e3 * t / reps, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "adjoint dL/dE, corotational tet block %dx%dx%d, %d implicit steps dt=2ms, "
@@ -438,7 +529,7 @@ def bench_adjoint(args):
                         "median_rel_err": float(np.median(errs))},
            "identification": ident, "cg_iterations_total": st["total_cg_iterations"]}
     g.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_batch(args):
@@ -463,6 +554,7 @@ def bench_batch(args):
         g.set_instances(b.node_offsets, b.dt)
         if tuning:
             g.set_batch_tuning(*tuning)
+        nnz = g.num_nonzeros()
         log("  set-up %.1f s" % (time.perf_counter() - t0))
         if args.warmup:
             g.step(args.warmup)
@@ -491,7 +583,7 @@ def bench_batch(args):
         g.close()
         log("  %d of %d instances active at the end" % (int(active.sum()), active.size))
         return dict(seconds=t, x=x, v=v, cg=st["total_cg_iterations"] - it0, tuning=tun, last_iters=its.tolist(),
-                    converged=bool(conv.all()), active=active, dof_steps=dof_steps, unconv=unconv)
+                    converged=bool(conv.all()), active=active, dof_steps=dof_steps, unconv=unconv, nnz=nnz)
 
     r32 = run(femgpu.FP32_REFINED)
     log("fp32+refine: %.3f s (%.2f ms/step), %d instance-iterations, tuning %s" % (
@@ -533,7 +625,14 @@ def bench_batch(args):
     log("fp32+refine vs fp64, worst instance:", worst)
     use32 = worst["u"] <= 1e-6 and worst["v"] <= 1e-6 and r32["seconds"] < r64["seconds"]
     head = r32 if use32 else r64
+    # one batched CG iteration over every instance (operator kernel: BSR values and indices, z and p_old
+    # gathered, p and q written; vector kernel: x and r read and written, p and q read, z written, the
+    # preconditioner: FP32 3x3 blocks, 3 values per DOF, FP64 diagonal, 1)
+    w = 4 if use32 else 8
+    Bk = head["nnz"] // 9
+    it_bytes = 9 * Bk * w + 4 * Bk + 4 * (N + len(cfgs)) + (11 * D + (3 * D if use32 else D)) * w
     out = {"metric": METRIC, "value": head["dof_steps"] / head["seconds"], "unit": "DOF-steps/s", "n_gpus": 1,
+           "iteration_bytes": it_bytes,
            "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * head["seconds"] / steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32+f64refine" if use32 else "f64", "data": "synthetic",
@@ -555,65 +654,19 @@ def bench_batch(args):
                             "tuning": r32["tuning"], "instance_iterations": r32["cg"]},
            "fp64": {"value": r64["dof_steps"] / r64["seconds"], "ms_per_step": 1e3 * r64["seconds"] / steps,
                     "tuning": r64["tuning"], "instance_iterations": r64["cg"]}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=40, help="cubes per axis")
-    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle sample steps (after 1 warm-up)")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-faithful-steps", type=int, default=0,
-                    help="also time the oracle with the reference's O(#BC x D) boundary zeroing (about 15-30 s a step)")
-    ap.add_argument("--timer-every", type=int, default=10,
-                    help="bracket every kernel of one step in N with HIP events (0: no events)")
-    ap.add_argument("--workload", default="corotational",
-                    choices=["corotational", "hex", "adjoint", "batch", "schemes", "compliance"])
-    ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--batch-maxit", type=int, default=2000,
-                    help="CG iteration cap per instance (-1: Eigen's 2 x DOF)")
-    ap.add_argument("--adj-n", type=int, default=20, help="adjoint block: n^3 cubes")
-    ap.add_argument("--adj-tol", type=float, default=1e-12, help="adjoint workload CG tolerance")
-    ap.add_argument("--adj-fd-dirs", type=int, default=32)
-    ap.add_argument("--adj-fd-h", type=float, default=1e-4, help="FD step relative to mean E")
-    ap.add_argument("--adj-ident", type=int, default=0, help="identification iterations to run (0: none)")
-    ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB three times (counter calibration)")
-    args = ap.parse_args()
-    args.steps_given = "--steps" in sys.argv
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design (no --gpus > 1).",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "adjoint":
-        bench_adjoint(args)
-        return
-    if args.workload == "schemes":
-        bench_schemes(args)
-        return
-    if args.workload == "compliance":
-        bench_compliance(args)
-        return
-    if args.workload == "batch":
-        bench_batch(args)
-        return
-    if args.workload == "hex":
-        bench_hex(args)
-        if args.copy_probe:
-            copy_probe(args.copy_probe)
-        return
-
+def bench_corotational(args):
+    """The headline: the 40^3 corotational block in FP64 and FP32 + refinement, the dominant kernel's roofline,
+    and the CPU baseline (median of 3 samples, native build, plus the faithful boundary-condition variant)."""
     n = (args.n, args.n, args.n)
     cfg = mg.config_corotational(n)
     N = cfg.x0.shape[0]
     D = 3 * N
     log("workload: corotational tets %s, %d elements, %d nodes, %d DOF" % (n, cfg.conn.shape[0], N, D))
     probe = make_gpu(cfg, femgpu.FP64)
-    Bk = probe.pattern()[1].size // 9
+    Bk = probe.num_nonzeros() // 9
     probe.close()
 
     r64 = run_timed(cfg, femgpu.FP64, args.warmup, args.steps, args.timer_every)
@@ -652,22 +705,27 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        cps, ct, err_or = cpu_baseline(cfg, args.cpu_steps)
-        log("cpu oracle: %.1f DOF-steps/s over %d steps (%.1f s); gpu fp64 vs oracle %s" % (cps, args.cpu_steps, ct,
-                                                                                          err_or))
+        flags = use_native_oracle()
+        cps, times, err_or = cpu_baseline(cfg, args.cpu_steps)
+        log("cpu oracle: %.1f DOF-steps/s (median of %s s over %d steps); gpu fp64 vs oracle %s" % (
+            cps, ["%.2f" % t for t in times], args.cpu_steps, err_or))
         cpu = {"value": cps, "unit": "DOF-steps/s", "cores": 1, "kind": "port",
-               "sample": "same workload, %d implicit steps after 1 warm-up step, FP64 C++ oracle on one host core "
-                         "(literal 12x12 corotational products, O(nnz) boundary conditions)" % args.cpu_steps,
+               "sample": "same workload: 1 warm-up step, then 3 timed samples of %d implicit steps; median; FP64 "
+                         "C++ oracle on one host core (literal 12x12 corotational products, O(nnz) boundary "
+                         "conditions)" % args.cpu_steps,
+               "sample_seconds": times, "build": flags, "cpu": cpu_model(),
                "gpu_fp64_vs_oracle_rel_err": err_or}
         if args.cpu_faithful_steps:
             fps, ft = cpu_faithful(cfg, args.cpu_faithful_steps)
             log("cpu oracle with the reference's O(#BC x D) boundary zeroing: %.1f DOF-steps/s (%.1f s)" % (fps, ft))
             cpu["faithful_bc_variant"] = {"value": fps, "unit": "DOF-steps/s", "cores": 1,
-                                          "sample": "%d steps after 1 warm-up step" % args.cpu_faithful_steps}
+                                          "sample": "%d step(s) after 1 warm-up step; the reference's zeroRow / "
+                                                    "zeroColumn through coeff() look-ups (O(#BC x D))"
+                                                    % args.cpu_faithful_steps}
 
     phases = {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in head["phases"].items()}
     copy_gbs = copy_probe(args.copy_probe or 512)
-    out = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": "DOF-steps/s",
@@ -698,6 +756,162 @@ def main():
                          "phases_us_per_launch": phase_summary(r32)},
         "phases": phases,
     }
+
+
+def roofline(kernel, nbytes, seconds, note=None):
+    a = nbytes / seconds / 1e9
+    out = {"bound": "hbm", "kernel": kernel, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": a / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": nbytes, "avg_launch_us": seconds * 1e6}
+    if note:
+        out["note"] = note
+    return out
+
+
+def bench_linear_case(args):
+    """The reference's CPU-runnable case: linear tets on 10 x 10 x 10 cubes (3,993 DOF), bottom fixed, gravity,
+    OdeSolverLinearEulerImplicit with the direct solver, dt 1 ms, 100 steps. GPU: the linear scheme with the
+    operator built once; CPU: the oracle's faithful linear implicit Euler with its banded LU and the dense
+    compliance of the first step (Math/OdeSolverLinearEulerImplicit.cpp:43-64, Math/OdeSolver.cpp:69-76)."""
+    import oracle
+    cfg = mg.config_linear_cpu()
+    steps = cfg.steps
+    N = cfg.x0.shape[0]
+    D = 3 * N
+    log("workload C1: linear tets 10^3, %d DOF, linear implicit Euler, %d steps" % (D, steps))
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(cfg.kind, cfg.x0, cfg.conn, cfg.E, cfg.nu, cfg.rho)
+    g.add_bc_nodes(cfg.bc_nodes)
+    g.set_options(dt=cfg.dt, cg_tolerance=1e-13, scheme=femgpu.LINEAR_EULER_IMPLICIT, gravity=cfg.gravity)
+    g.set_state(v=cfg.v0)
+    g.step(1)  # the first step builds the constant operator
+    device_sync(g)
+    t0 = time.perf_counter()
+    assert g.step(steps - 1) == femgpu.FEM_OK
+    device_sync(g)
+    t = time.perf_counter() - t0
+    x, v = g.state()
+    op_ms = g.time_operator(100)
+    Bk = g.num_nonzeros() // 9
+    g.close()
+    flags = use_native_oracle()
+    times = []
+    for r in range(3):
+        s = oracle_sim(cfg, scheme=oracle.LINEAR_EULER_IMPLICIT, linsolver=oracle.LU)
+        s.set_compliance_at_first_linear_step(True)
+        t1 = time.perf_counter()
+        s.step(cfg.dt, steps)
+        times.append(time.perf_counter() - t1)
+        if r == 0:
+            xr, vr = s.state()
+        del s
+    x0 = cfg.x0.reshape(-1)
+    err = dict(u=float(np.linalg.norm((x - x0) - (xr - x0)) / np.linalg.norm(xr - x0)),
+               v=float(np.linalg.norm(v - vr) / np.linalg.norm(vr)))
+    tc = float(np.median(times))
+    log("C1: gpu %.3f ms/step (steps 2..%d), cpu %.2f s for %d steps (median of %s), rel err %s" % (
+        1e3 * t / (steps - 1), steps, tc, steps, ["%.2f" % a for a in times], err))
+    return {"value": D * (steps - 1) / t, "unit": "DOF-steps/s", "ms_per_step": 1e3 * t / (steps - 1),
+            "steps_timed": steps - 1, "dtype": "f64",
+            "config": {"workload": "linear-tet-block 10x10x10, linear implicit Euler dt=1ms, %d steps" % steps,
+                       "dof": int(D), "solver": "Jacobi CG to 1e-13 on the constant operator"},
+            "rel_err_vs_oracle_lu": err,
+            "roofline": roofline("cg_fused_kernel<double>", spmv_bytes((N, D, Bk), 8), op_ms / 1e3,
+                                 "operator and vectors (5.6 MB) stay in the 256 MB MALL at this size"),
+            "cpu_baseline": {"value": D * steps / tc, "unit": "DOF-steps/s", "cores": 1, "kind": "port",
+                             "sample": "the whole 100-step run, 3 runs, median: OdeSolverLinearEulerImplicit with "
+                                       "the banded LU of the first system matrix and its dense compliance "
+                                       "(3993 x 3993) at step 1", "sample_seconds": times, "build": flags}}
+
+
+def bench_sub_batch(args):
+    out = bench_batch(args)
+    tun = out["fp32_refined" if out["dtype"].startswith("f32") else "fp64"]["tuning"]
+    sub = {k: out[k] for k in ("value", "unit", "ms_per_step", "steps", "dtype", "config",
+                               "rel_err_fp32_vs_fp64_worst_instance", "instances_in_gate",
+                               "instances_excluded_from_gate", "fp32_refined", "fp64")}
+    if out.get("iteration_bytes") and tun.get("ms_per_iteration"):
+        sub["roofline"] = roofline("bcg_apply_kernel + bcg_vector_kernel (one CG iteration, every instance)",
+                                   out["iteration_bytes"], tun["ms_per_iteration"] / 1e3,
+                                   "time per iteration from the autotuner's replay on the real operator")
+    return sub
+
+
+def bench_sub_adjoint(args):
+    out = bench_adjoint(args)
+    sub = {k: out[k] for k in ("value", "unit", "ms_per_step", "dtype", "config", "fd_check",
+                               "cg_iterations_total") if k in out}
+    if "roofline" in out:
+        sub["roofline"] = out["roofline"]
+    return sub
+
+
+def bench_sub_hex(args):
+    out = bench_hex(args)
+    keep = ("value", "unit", "ms_per_step", "steps", "dtype", "config", "rel_err_vs_fp64", "fp64", "fp32_refined",
+            "table_variant", "roofline")
+    return {k: out[k] for k in keep if k in out}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=40, help="cubes per axis")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps per timed sample (3 samples)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-faithful-steps", type=int, default=1,
+                    help="steps of the oracle with the reference's O(#BC x D) boundary zeroing (0: skip)")
+    ap.add_argument("--timer-every", type=int, default=10,
+                    help="event-time the kernels of one step in this many (0: off)")
+    ap.add_argument("--workload", default="corotational",
+                    help="corotational (headline and every config's sub-record), linear, batch, adjoint, hex, "
+                         "schemes, compliance")
+    ap.add_argument("--headline-only", action="store_true", help="corotational: skip the other configs")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-maxit", type=int, default=2000,
+                    help="per-instance CG iteration cap of the batch workload")
+    ap.add_argument("--adj-n", type=int, default=20, help="adjoint block: n^3 cubes")
+    ap.add_argument("--adj-tol", type=float, default=1e-12, help="adjoint workload CG tolerance")
+    ap.add_argument("--adj-fd-dirs", type=int, default=32)
+    ap.add_argument("--adj-fd-h", type=float, default=1e-4, help="FD step relative to mean E")
+    ap.add_argument("--adj-ident", type=int, default=0, help="identification iterations to run (0: none)")
+    ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also time a copy of this many MiB (probe_copy_kernel)")
+    args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design (no --gpus > 1).",
+              file=sys.stderr)
+        sys.exit(2)
+    single = {"adjoint": bench_adjoint, "schemes": bench_schemes, "compliance": bench_compliance,
+              "batch": bench_batch, "hex": bench_hex, "linear": bench_linear_case}
+    if args.workload in single:
+        out = single[args.workload](args)
+        if args.workload == "hex" and args.copy_probe:
+            copy_probe(args.copy_probe)
+        print(json.dumps(out), flush=True)
+        return
+    out = bench_corotational(args)
+    if not args.headline_only:
+        # every other configuration that fits one GPU, at its own step count (the headline's --steps and
+        # --warmup do not apply to them)
+        sub_args = argparse.Namespace(**vars(args))
+        sub_args.steps_given = False
+        sub_args.warmup = 2
+        configs = {}
+        for name, fn in (("C1_linear_tets_10x10x10", bench_linear_case), ("C3_batch_64_instances", bench_sub_batch),
+                         ("C4_adjoint_20x20x20", bench_sub_adjoint), ("C5_hex_128x64x128", bench_sub_hex)):
+            t0 = time.perf_counter()
+            log("--- sub-record %s" % name)
+            try:
+                configs[name] = fn(sub_args)
+            except Exception as e:  # noqa: BLE001 - a failed sub-record is reported, the headline still prints
+                log("sub-record %s failed: %r" % (name, e))
+                configs[name] = {"error": repr(e)}
+            configs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        out["configs"] = configs
     print(json.dumps(out), flush=True)
 
 

@@ -84,6 +84,9 @@ typedef struct fem_options {
     int32_t mass_lumping;        /* 1: row-sum lumped mass in the implicit schemes' updates, default 0
                                     (FemRepresentation::setMassLumping, Physics/FemRepresentation.cpp:320-323,
                                     499-508); the other schemes' mass is never lumped (:412-421) */
+    double newton_epsilon;       /* Newton-Raphson stops once ||dv - dv_prev||_inf < newton_epsilon, default 1e-5
+                                    (OdeSolverEulerImplicit::setNewtonRaphsonEpsilonConvergence,
+                                    Math/OdeSolverEulerImplicit.cpp:26, 43-48, 101-112) */
 } fem_options;
 
 typedef struct fem_stats {
@@ -97,6 +100,8 @@ typedef struct fem_stats {
     int64_t fp64_polar_fallbacks;  /* FP32 mode: elements whose rotation was redone in FP64 */
     int32_t active;                /* 0 once the state became non-finite */
     int32_t converged;             /* last solve reached its tolerance */
+    int64_t last_newton_iterations; /* linear solves of the last step's Newton-Raphson loop
+                                       (Math/OdeSolverEulerImplicit.cpp:84-115); 1 for the single-solve schemes */
 } fem_stats;
 
 /* Context lifetime. Replaces constructing a Fem3DRepresentation (Physics/Fem3DRepresentation.h:54-121). */
@@ -290,7 +295,9 @@ int fem_get_phase_times(fem_ctx* ctx, fem_phase_times* out);
  * CG kernel of a tetrahedral mesh (the one-kernel iteration: direction update, block SpMV, vector updates
  * and the 7 block sums; or the operator kernel of the split pair under FEM_CG_KERNELS=split), on the CG
  * scratch buffers of the current precision, each with its own start/stop events; average milliseconds of
- * execution per launch. The simulation state is untouched; call after at least one step. */
+ * execution per launch. On a hexahedral mesh: the average of `reps` applications of the element-by-element
+ * operator over all colours (the FP32 cached operator, or the FP64 matrix-free one), each between its own
+ * pair of events. The simulation state is untouched; call after at least one step (single meshes only). */
 int fem_time_operator(fem_ctx* ctx, int reps, double* avg_ms);
 
 /* Measurement helper with no counterpart in the reference: `reps` timed copies of `bytes` bytes

@@ -296,6 +296,8 @@ struct fem_ctx {
     // host-pinned readback
     CgScalars* h_sc = nullptr;
     int* h_counters = nullptr;
+    DevBuf<double> nvar;         // Newton-Raphson ||dv - dv_prev||_inf of the last iteration
+    double* h_nvar = nullptr;    // pinned copy
 
     // state
     bool have_operator = false;   // a step assembled (or cached) the system matrix
@@ -1265,6 +1267,7 @@ void step_other_scheme(fem_ctx* c) {
 // One step of the configured scheme (OdeSolver::solve with computeCompliance = false). Returns FEM_OK or
 // FEM_INACTIVE.
 int step_once(fem_ctx* c) {
+    c->stats.last_newton_iterations = 1;  // the Newton-Raphson loop below sets its own count
     if (c->num_instances > 0) return batch_step(c);
     const double dt = c->opt.dt;
     const size_t n = (size_t)c->ndof;
@@ -1296,21 +1299,20 @@ int step_once(fem_ctx* c) {
         HIPCHECK(hipMemcpyAsync(c->xs.p, c->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         HIPCHECK(hipMemcpyAsync(c->vs.p, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         HIPCHECK(hipMemsetAsync(c->dvprev.p, 0, n * sizeof(double), c->stream));
+        c->stats.last_newton_iterations = 0;
         for (int it = 0; it < newton; ++it) {
             tet_elements(c, true, c->xs.p, c->vs.p);
             assemble_tet(c);
             rhs_tet(c, 1.0, c->x.p, c->v.p, c->xs.p, c->vs.p);
             linear_solve(c);
             state_update(c, c->xs.p);
-            // ||dv - dv_prev||_inf < epsilon (1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112)
-            std::vector<double> h_dv(n), h_prev(n);
-            HIPCHECK(hipMemcpyAsync(h_dv.data(), c->dv.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HIPCHECK(hipMemcpyAsync(h_prev.data(), c->dvprev.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            // ||dv - dv_prev||_inf < epsilon (default 1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112), reduced on
+            // the device; only the scalar comes back
+            launch_newton_variation((int64_t)n, c->dv.p, c->dvprev.p, c->partial.p, c->nvar.p, c->stream);
+            HIPCHECK(hipMemcpyAsync(c->h_nvar, c->nvar.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPCHECK(hipStreamSynchronize(c->stream));
-            double var = 0.0;
-            for (size_t i = 0; i < n; ++i) var = std::max(var, std::fabs(h_dv[i] - h_prev[i]));
-            if (var < 1e-5) break;
-            HIPCHECK(hipMemcpyAsync(c->dvprev.p, c->dv.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            c->stats.last_newton_iterations = it + 1;
+            if (*c->h_nvar < c->opt.newton_epsilon) break;
         }
     }
     c->stats.steps++;
@@ -1481,6 +1483,7 @@ void fem_default_options(fem_options* o) {
     o->inner_tolerance = 1e-4;
     o->max_refinements = 8;
     o->fail_on_nonconvergence = 0;
+    o->newton_epsilon = 1e-5;
 }
 
 int fem_create(int device, fem_ctx** out) {
@@ -1498,6 +1501,8 @@ int fem_create(int device, fem_ctx** out) {
         c->scratch = noalloc_scratch_address(device);  // once per device, with the device current
         HIPCHECK(hipHostMalloc((void**)&c->h_sc, 2 * sizeof(CgScalars), hipHostMallocDefault));
         HIPCHECK(hipHostMalloc((void**)&c->h_counters, 8 * sizeof(int), hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void**)&c->h_nvar, sizeof(double), hipHostMallocDefault));
+        c->nvar.alloc(1);
         c->partial.alloc(4 * kCgBlocks);
         c->scal.alloc(2);
         c->counters.alloc(8);
@@ -1526,6 +1531,7 @@ void fem_destroy(fem_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_nvar) (void)hipHostFree(c->h_nvar);
     if (c->h_refine_iters) (void)hipHostFree(c->h_refine_iters);
     if (c->h_bsc) (void)hipHostFree(c->h_bsc);
     if (c->h_live) (void)hipHostFree(c->h_live);
@@ -1695,6 +1701,7 @@ int fem_set_options(fem_ctx* c, const fem_options* o) {
         if (o->precision == FEM_PRECISION_FP32_REFINED && o->max_refinements < 1)
             throw InvalidArg("FP32 precision needs at least one refinement sweep (max_refinements >= 1)");
         if (o->newton_max_iterations < 1) throw InvalidArg("The maximum iteration needs to be at least 1");
+        if (!(o->newton_epsilon >= 0.0)) throw InvalidArg("The epsilon convergence cannot be negative");
         if (!(o->cg_tolerance >= 0.0)) throw InvalidArg("negative tolerance");
         bool grav = o->gravity_enabled != c->opt.gravity_enabled || o->gravity[0] != c->opt.gravity[0] ||
                     o->gravity[1] != c->opt.gravity[1] || o->gravity[2] != c->opt.gravity[2];
@@ -2129,11 +2136,14 @@ int fem_enable_phase_timers(fem_ctx* c, int enable) {
 int fem_time_operator(fem_ctx* c, int reps, double* avg_ms) {
     return guarded(c, [&] {
         if (!avg_ms || reps < 1) throw InvalidArg("reps must be positive and avg_ms non-null");
-        if (c->kind != FEM_ELEMENT_TET_LINEAR && c->kind != FEM_ELEMENT_TET_COROTATIONAL)
-            throw InvalidArg("operator replay timing covers the assembled (tetrahedral) operator");
         if (c->stats.steps < 1) return fail(c, FEM_ERR_ORDER, "step at least once before timing the operator");
         HIPCHECK(hipSetDevice(c->device));
         const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
+        if (c->kind == FEM_ELEMENT_HEX) {
+            *avg_ms = hex_time_operator(c->hex, hex_args(c), fp32, reps, c->stream);
+            return FEM_OK;
+        }
+        if (c->num_instances > 0) throw InvalidArg("operator replay timing covers single meshes");
         if (!c->time_sc.p) c->time_sc.alloc(1);
         // never converges (threshold < 0), fixed rz so every replay does identical work on CG scratch
         CgScalars init{};

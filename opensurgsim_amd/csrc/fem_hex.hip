@@ -824,6 +824,59 @@ int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgSca
 
 }  // namespace
 
+double hex_time_operator(HexData& h, const HexStepArgs& a, bool fp32, int reps, hipStream_t s) {
+    if (fp32 && (h.a32_lumped < 0 || !h.A32)) throw std::runtime_error("no FP32 operator cached yet: step first");
+    void* p = a.vec[3];
+    void* q = a.vec[4];
+    auto apply = [&] {
+        for (int c = 0; c < h.num_colors; ++c) {
+            const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
+            if (fp32) {
+                const int c0 = h.pad_ptr[c], pc = h.pad_ptr[c + 1] - c0;
+                const float4* A4 = reinterpret_cast<const float4*>(h.A32) + (size_t)c0 * kHexChunks;
+                if (h.num_distinct > 0)
+                    hipLaunchKernelGGL(hex_apply32_kernel<true>, blocks_for(pc, 256), dim3(256),
+                                       (size_t)h.num_distinct * kHexChunks * sizeof(float4), s, pc, h.order_pad + c0,
+                                       h.conn, A4, reinterpret_cast<const float4*>(h.table), h.table_idx + c0,
+                                       h.num_distinct, (const float*)p, a.bcmask, (float*)q,
+                                       (const CgScalars*)nullptr);
+                else
+                    hipLaunchKernelGGL(hex_apply32_kernel<false>, blocks_for(pc, 256), dim3(256), 0, s, pc,
+                                       h.order_pad + c0, h.conn, A4, (const float4*)nullptr, (const int*)nullptr, 0,
+                                       (const float*)p, a.bcmask, (float*)q, (const CgScalars*)nullptr);
+            } else {
+                hipLaunchKernelGGL(hex_apply64_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt,
+                                   h.order + h.color_ptr[c], h.conn, a.x0, a.mat, (const double*)p, nullptr, nullptr,
+                                   0.0, a.bcmask, h.a32_cm, h.a32_ck, h.a32_lumped, 1.0, (double*)q,
+                                   (const CgScalars*)nullptr);
+            }
+        }
+    };
+    apply();  // warm-up
+    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+    double sum = 0.0;
+    hipError_t err = hipSuccess;
+    for (auto& e : ev)
+        if (err == hipSuccess) err = hipEventCreate(&e);
+    if (err == hipSuccess) {
+        for (int r = 0; r < reps && err == hipSuccess; ++r) {
+            err = hipEventRecord(ev[2 * r], s);
+            apply();
+            if (err == hipSuccess) err = hipEventRecord(ev[2 * r + 1], s);
+        }
+        if (err == hipSuccess) err = hipEventSynchronize(ev.back());
+        for (int r = 0; r < reps && err == hipSuccess; ++r) {
+            float t = 0.f;
+            err = hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]);
+            sum += t;
+        }
+    }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    HCHECK(err);
+    return sum / reps;
+}
+
 void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s) {
     const fem_options& o = *a.opt;
     const double dt = o.dt;

@@ -77,6 +77,11 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
 // rows already zero).
 void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
 
+// Average milliseconds of one application of the element-by-element operator over every colour (q += A p on
+// the CG scratch vectors p = vec[3], q = vec[4]): the FP32 cached operator (table or per-element cache, as the
+// solves use it) or the FP64 matrix-free one; reps applications, each between its own pair of events.
+double hex_time_operator(HexData& h, const HexStepArgs& a, bool fp32, int reps, hipStream_t s);
+
 // FP64 element outputs for tests: K (24x24 row-major) and f = -K (x - x0).
 void hex_eval_elements(HexData& h, int first, int count, const double* x0, const double* x, const double4* mat,
                        double* R, double* K, double* f, hipStream_t s);

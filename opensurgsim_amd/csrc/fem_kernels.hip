@@ -930,6 +930,51 @@ void launch_cg_vector(const CgArgs& a, int k, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Newton-Raphson convergence test of OdeSolverEulerImplicit::solve (Math/OdeSolverEulerImplicit.cpp:101-112):
+// ||dv - dv_prev||_inf. The grid kernel writes each workgroup's maximum to partial[blockIdx.x] and sets
+// prev = dv for the next iteration (the loop leaves prev unused once the test passes); one workgroup then
+// takes the maximum of the kCgBlocks partials into *out.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kCgThreads) void newton_variation_kernel(int64_t n, const double* __restrict__ dv,
+                                                                      double* __restrict__ prev,
+                                                                      double* __restrict__ partial) {
+    __shared__ double lds[kCgThreads / 64];
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = dv[i];
+        m = fmax(m, fabs(d - prev[i]));
+        prev[i] = d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = fmax(t, lds[w]);
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void max_partials_kernel(const double* __restrict__ partial, int count,
+                                                           double* __restrict__ out) {
+    __shared__ double lds[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) m = fmax(m, partial[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+void launch_newton_variation(int64_t n, const double* dv, double* prev, double* partial, double* out,
+                             hipStream_t s) {
+    launch_k(newton_variation_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, dv, prev, partial);
+    launch_k(max_partials_kernel, dim3(1), dim3(256), 0, s, (const double*)partial, (int)kCgBlocks, out);
+}
+
+// ------------------------------------------------------------------------------------------
 // FP64 iterative refinement around the FP32 CG: r64 = b - S64 dv in FP64 (the first sweep takes
 // r64 = b) with partial |r64|^2; the check kernel then applies the outer test
 // |r64|^2 < max(tol^2 |b|^2, realmin) and marks the inner solve done when it passes. Once the outer

@@ -162,6 +162,9 @@ void launch_cg_vector(const CgArgs& a, int k, hipStream_t s);
 
 // Scalars of a solve from the partials of an init kernel (|b|^2 in [0, B), r.z in [B, 2B)).
 void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s);
+// *out = ||dv - prev||_inf and prev = dv (partial: kCgBlocks doubles of scratch).
+void launch_newton_variation(int64_t n, const double* dv, double* prev, double* partial, double* out,
+                             hipStream_t s);
 // The vector part of iteration k on raw buffers (used by the element-by-element operators).
 void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, const void* p, const void* q,
                           const void* dinv, double* partial, const CgScalars* sc, hipStream_t s);

@@ -1,0 +1,44 @@
+// Direct solver of the assembled operator: banded LU factorization without pivoting in FP64, the GPU form of
+// LinearSparseSolveAndInverseLU (Math/LinearSparseSolveAndInverse.cpp:31-97, Eigen's SparseLU there). The
+// operator of this path (M/dt + D + dt K with boundary rows and columns replaced by the identity) has a positive
+// definite symmetric part, for which elimination without pivoting is stable; the node numbering of a block mesh
+// keeps the bandwidth at 3 (max |I - J| over coupled nodes) + 2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace femgpu {
+
+// Band storage, row-major: entry (i, j) with |i - j| <= bw at ab[i * ld + (j - i + bw)], ld = 2 bw + 1. After
+// lu_factor it holds the unit lower factor L (below the diagonal) and U (on and above it).
+struct DeviceBandLU {
+    int64_t n = 0;
+    int bw = 0;
+    int64_t ld = 0;
+    double* ab = nullptr;
+    size_t capacity = 0;  // doubles allocated
+};
+
+// Bytes the band storage of an n x n matrix of bandwidth bw needs.
+size_t band_bytes(int64_t n, int bw);
+
+// ab = the 3x3 block CSR operator S (9 doubles per block, row-major in the block; block k in block row
+// blk_row[k], block column colidx[k]) scattered into band storage of bandwidth bw (zero elsewhere).
+void launch_band_fill(DeviceBandLU& lu, int64_t num_blocks, const int* blk_row, const int* colidx, const double* S,
+                      hipStream_t s);
+// In-place LU of the band storage: panels of kLuPanel columns factored by one workgroup, each followed by the
+// rank-kLuPanel update of the trailing bw x bw block by LDS-tiled workgroups.
+void launch_band_factor(DeviceBandLU& lu, hipStream_t s);
+// x = A^-1 b for nrhs columns (column j at b + j * ldb and x + j * ldx; x may alias b): forward substitution
+// with L and back substitution with U, one workgroup per column.
+void launch_band_solve(const DeviceBandLU& lu, int nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
+                       hipStream_t s);
+// Columns [first, first + count) of the identity (n rows each, column-major with leading dimension n).
+void launch_identity_columns(int64_t n, int64_t first, int count, double* out, hipStream_t s);
+// Rows and columns of the boundary DOFs (mask != 0) of an n x count column block (columns first.. of an n x n
+// matrix) set to zero: OdeState::applyBoundaryConditionsToMatrix(&C, false).
+void launch_zero_bc_rows_cols(int64_t n, int64_t first, int count, const uint8_t* mask, double* out, hipStream_t s);
+
+}  // namespace femgpu

@@ -54,6 +54,7 @@ class FemOptions(ctypes.Structure):
         ("rayleigh_mass", ctypes.c_double),
         ("rayleigh_stiffness", ctypes.c_double),
         ("mass_lumping", ctypes.c_int32),
+        ("newton_epsilon", ctypes.c_double),
     ]
 
 
@@ -69,6 +70,7 @@ class FemStats(ctypes.Structure):
         ("fp64_polar_fallbacks", ctypes.c_int64),
         ("active", ctypes.c_int32),
         ("converged", ctypes.c_int32),
+        ("last_newton_iterations", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -397,6 +399,10 @@ class FemGpu:
         ms = ctypes.c_double(0.0)
         self._check(self._L.fem_time_operator(self.h, int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def num_nonzeros(self):
+        """Scalar nonzeros of the sparsity pattern (9 per 3x3 node block)."""
+        return int(self._L.fem_num_nonzeros(self.h))
 
     def pattern(self):
         nnz = self._L.fem_num_nonzeros(self.h)

@@ -151,6 +151,11 @@ void OdeSolver::setNewtonRaphsonMaximumIteration(size_t maxIterations) {
     m_newtonMaxIterations = maxIterations;
 }
 
+void OdeSolver::setNewtonRaphsonEpsilonConvergence(double epsilon) {
+    if (!(epsilon >= 0.0)) fail("The epsilon convergence cannot be negative");
+    m_newtonEpsilon = epsilon;
+}
+
 }  // namespace Math
 
 // ---------------------------------------------------------------------------------------------
@@ -339,6 +344,7 @@ void Fem3DRepresentation::applyOptions(double dt) {
     o.mass_lumping = m_massLumping ? 1 : 0;
     o.precision = (int32_t)m_precision;
     o.newton_max_iterations = (int32_t)m_odeSolver->getNewtonRaphsonMaximumIteration();
+    o.newton_epsilon = m_odeSolver->getNewtonRaphsonEpsilonConvergence();
     auto solver = m_odeSolver->getLinearSolver();
     if (auto cg = std::dynamic_pointer_cast<Math::LinearSparseSolveAndInverseCG>(solver)) {
         o.cg_tolerance = cg->getTolerance();

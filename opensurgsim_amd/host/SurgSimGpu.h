@@ -142,12 +142,16 @@ public:
     void setLinearSolver(std::shared_ptr<LinearSparseSolveAndInverse> linearSolver);
     void setNewtonRaphsonMaximumIteration(size_t maxIterations);
     size_t getNewtonRaphsonMaximumIteration() const { return m_newtonMaxIterations; }
+    /// Math/OdeSolverEulerImplicit.cpp:43-48: the iterations stop once ||dv - dv_prev||_inf < epsilon.
+    void setNewtonRaphsonEpsilonConvergence(double epsilon);
+    double getNewtonRaphsonEpsilonConvergence() const { return m_newtonEpsilon; }
 
 private:
     IntegrationScheme m_scheme;
     std::string m_name;
     std::shared_ptr<LinearSparseSolveAndInverse> m_linearSolver;
     size_t m_newtonMaxIterations = 1;
+    double m_newtonEpsilon = 1e-5;
 };
 
 }  // namespace Math

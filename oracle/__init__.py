@@ -30,6 +30,20 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def build_native(out_dir):
+    """Compile the oracle with -march=native for the host it runs on (the CPU baseline's build) into
+    out_dir; returns the library path."""
+    subprocess.check_call(["make", "-s", "-C", _HERE, "native", "NATIVE_OUT=%s" % out_dir])
+    return os.path.join(out_dir, "libfem_oracle.so")
+
+
+def use_library(path):
+    """Load the oracle from `path` from now on (objects made with the previous library must not be used)."""
+    global _lib, _LIB_PATH
+    _LIB_PATH = path
+    _lib = None
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -42,6 +56,15 @@ def lib():
         L.orc_cube_km.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.orc_coro_element.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        _dp, _dp, _dp, _dp, _ip]
+        L.orc_coro_element_given_rotation.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                      _dp, _dp, _dp, _dp, _ip]
+        L.orc_ode_complex_nonlinear.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, _dp,
+                                                ctypes.POINTER(ctypes.c_long)]
+        L.orc_sim_last_newton_iters.restype = ctypes.c_long
+        L.orc_sim_last_newton_iters.argtypes = [ctypes.c_void_p]
+        L.orc_sim_set_newton_eps.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_sim_set_compliance_at_first_linear_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_sim_compliance_matrix.argtypes = [ctypes.c_void_p, _dp]
         L.orc_sim_create.restype = ctypes.c_void_p
         L.orc_sim_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp]
         L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
@@ -139,6 +162,33 @@ def coro_element(x0, x, E, nu, rho):
     return R, M, K, f, flags.value
 
 
+def coro_element_given_rotation(x0, x, E, nu, rho, R):
+    """coro_element with the polar factor R supplied (for configurations where it is not unique). Returns
+    M, K, f, flags."""
+    x0 = _f64(x0).reshape(12)
+    x = _f64(x).reshape(12)
+    R = np.ascontiguousarray(_f64(R).reshape(9))
+    M = np.zeros((12, 12))
+    K = np.zeros((12, 12))
+    f = np.zeros(12)
+    flags = ctypes.c_int(0)
+    _check(lib().orc_coro_element_given_rotation(_d(x0), _d(x), E, nu, rho, _d(R), _d(M), _d(K), _d(f),
+                                                 ctypes.byref(flags)))
+    return M, K, f, flags.value
+
+
+def ode_complex_nonlinear(linear_variant, max_iter, eps):
+    """Two steps of the reference's OdeComplexNonLinear test equation (a = x |v|^2) through the oracle's
+    Newton-Raphson loop (OdeSolverEulerImplicit) or its linear variant; dict of x1, v1, x2, v2, f2 = f(x2, v2)
+    and the number of linear solves."""
+    out = np.zeros(15)
+    solves = ctypes.c_long(0)
+    _check(lib().orc_ode_complex_nonlinear(int(linear_variant), int(max_iter), float(eps), _d(out),
+                                           ctypes.byref(solves)))
+    o = out.reshape(5, 3)
+    return dict(x1=o[0], v1=o[1], x2=o[2], v2=o[3], f2=o[4], solves=solves.value)
+
+
 def cg_solve(outer, inner, vals, b, tol, max_iter=-1):
     outer = np.ascontiguousarray(outer, dtype=np.int32)
     inner = np.ascontiguousarray(inner, dtype=np.int32)
@@ -220,8 +270,9 @@ class Sim:
         return out[0] if vec else out.T
 
     def set_solver(self, scheme=EULER_IMPLICIT, linsolver=CG, tol=2.220446049250313e-16, max_iter=-1,
-                   newton_max_iter=1, faithful_bc=False):
+                   newton_max_iter=1, faithful_bc=False, newton_eps=1e-5):
         lib().orc_sim_set_solver(self.h, scheme, linsolver, tol, max_iter, newton_max_iter, int(faithful_bc))
+        lib().orc_sim_set_newton_eps(self.h, float(newton_eps))
 
     def set_state(self, x=None, v=None):
         xa = _f64(x).reshape(-1) if x is not None else None
@@ -289,6 +340,18 @@ class Sim:
 
     def total_cg_iters(self):
         return lib().orc_sim_total_cg_iters(self.h)
+
+    def set_compliance_at_first_linear_step(self, on=True):
+        lib().orc_sim_set_compliance_at_first_linear_step(self.h, int(on))
+
+    def compliance_matrix(self):
+        """Dense compliance of the last system matrix, boundary rows and columns zero (D x D)."""
+        C = np.zeros((self.ndof, self.ndof), order="F")
+        _check(lib().orc_sim_compliance_matrix(self.h, C.ctypes.data_as(_dp)))
+        return C
+
+    def last_newton_iters(self):
+        return lib().orc_sim_last_newton_iters(self.h)
 
     def counters(self):
         a = ctypes.c_long(0)

@@ -380,7 +380,7 @@ struct Element {
 // :276 (Ke == &m_K on the FMDK path, :83-97). The dense 12x12 products are kept as written
 // (the reference's own cost model).
 static void coro_rotation_mass_stiffness(Element& e, const double x[12], bool updR, bool updM, bool updK,
-                                         Counters* cnt) {
+                                         Counters* cnt, const M3* R_given = nullptr) {
     double P[4][4];
     for (int a = 0; a < 4; ++a) {
         for (int c = 0; c < 3; ++c) P[c][a] = x[3 * a + c];
@@ -397,7 +397,16 @@ static void coro_rotation_mass_stiffness(Element& e, const double x[12], bool up
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) F.m[i][j] = PV[i][j];
     M3 R, S;
-    rotation_scaling(F, &R, &S);
+    if (R_given) {
+        // A caller-supplied polar factor (F = R S): S = sym(R^T F). Used where computeRotationScaling's R is
+        // not unique (tied singular values), to check everything downstream of R against another R.
+        R = *R_given;
+        const M3 RtF = m3_mul(m3_t(R), F);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) S.m[i][j] = 0.5 * (RtF.m[i][j] + RtF.m[j][i]);
+    } else {
+        rotation_scaling(F, &R, &S);
+    }
     if (updR) e.R = R;
     if (std::fabs(m3_det(R) - 1.0) > 1e-8) {
         if (cnt) cnt->det_r_invalid++;
@@ -689,7 +698,8 @@ static void element_init(Element& e, const double* xall, Counters* cnt) {
 }
 
 // updateFMDK(state, FMDK) for one element (doUpdateFMDK of each type).
-static void element_update(Element& e, const double* xall, int options, Counters* cnt) {
+static void element_update(Element& e, const double* xall, int options, Counters* cnt,
+                           const M3* R_given = nullptr) {
     int n = 3 * e.nn;
     double x[24];
     for (int a = 0; a < e.nn; ++a)
@@ -700,7 +710,7 @@ static void element_update(Element& e, const double* xall, int options, Counters
         bool updR = options & F;
         bool updK = (options & F) || (options & K);
         bool updM = options & M;
-        coro_rotation_mass_stiffness(e, x, updR, updM, updK, cnt);
+        coro_rotation_mass_stiffness(e, x, updR, updM, updK, cnt, R_given);
         if (options & F) {
             double Rx0[12];
             for (int a = 0; a < 4; ++a)
@@ -930,6 +940,36 @@ struct BandLU {
     }
 };
 
+// The Newton-Raphson loop of OdeSolverEulerImplicit::solve (Math/OdeSolverEulerImplicit.cpp:55-122), written
+// once over the equation: assemble(xn, vn) forms the system matrix and right-hand side of assembleLinearSystem
+// (:124-161) at the estimate (xn, vn), solve(sol) solves that system. newState starts as the current state
+// (xn = xs, vn = vs on entry); each iteration adds the solution to vn and sets xn = xs + dt vn; with more than one
+// iteration allowed, the loop stops once ||sol - sol_prev||_inf < eps, sol_prev starting at 0. Returns the
+// number of linear solves. Sim::solve_implicit and the harness of the reference's non-linear ODE test both use
+// it.
+template <class Assemble, class Solve>
+static long newton_implicit_euler(double dt, long max_iter, double eps, const std::vector<double>& xs,
+                                  std::vector<double>& xn, std::vector<double>& vn, std::vector<double>& sol,
+                                  Assemble&& assemble, Solve&& solve) {
+    std::vector<double> prev(xs.size(), 0.0);
+    long it = 0, solves = 0;
+    while (it < max_iter) {
+        assemble(xn, vn);
+        solve(sol);
+        ++solves;
+        for (size_t i = 0; i < vn.size(); ++i) vn[i] += sol[i];
+        for (size_t i = 0; i < xn.size(); ++i) xn[i] = xs[i] + dt * vn[i];
+        if (max_iter > 1) {
+            double var = 0.0;
+            for (size_t i = 0; i < sol.size(); ++i) var = std::max(var, std::fabs(sol[i] - prev[i]));
+            if (var < eps) break;
+            prev = sol;
+        }
+        it++;
+    }
+    return solves;
+}
+
 // Math/OdeSolver.h:36-49 (numbered as the GPU library numbers them: the two implicit schemes first)
 enum Scheme {
     EULER_IMPLICIT = 0,
@@ -982,11 +1022,16 @@ struct Sim {
     double newton_eps = 1e-5;
     bool faithful_bc = false;   // O(#BC * n) zeroRow/zeroColumn as in Math/SparseMatrix.h:352-377
     bool linear_initialized = false;
+    // OdeSolverLinearEulerImplicit forms the dense compliance on its first step (solve(..., true),
+    // Math/OdeSolverLinearEulerImplicit.cpp:49); off unless asked for (it is D x D)
+    bool compliance_at_first_linear_step = false;
+    std::vector<double> compliance;  // D x D, column-major, of the last system matrix
     Csc Kconst;
     BandLU lu;
     Counters cnt;
     CgResult last_cg;
     long total_cg_iters = 0;
+    long last_newton_iters = 0;  // linear solves of the last implicit step's Newton-Raphson loop
     bool valid = true;
 
     void init() {
@@ -1252,25 +1297,35 @@ struct Sim {
         }
     }
 
+    // OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76): the inverse of the last
+    // system matrix (LinearSparseSolveAndInverseLU::getInverse solves the identity through the factorization,
+    // Math/LinearSparseSolveAndInverse.cpp:86-90; the CG solver's getInverse is the dense inverse, :25-29, here
+    // through the same LU), then boundary rows and columns zeroed with a zero diagonal
+    // (OdeState::applyBoundaryConditionsToMatrix(&C, false), Math/OdeState.cpp:175-195).
+    void compute_compliance() {
+        const int n = A.n;
+        if (linsolver != LU) lu.factor(A);
+        compliance.assign((size_t)n * n, 0.0);
+        std::vector<double> e(n, 0.0);
+        for (int j = 0; j < n; ++j) {
+            e[j] = 1.0;
+            lu.solve(e.data(), compliance.data() + (size_t)j * n);
+            e[j] = 0.0;
+        }
+        for (int dof : bc_dofs) {
+            for (int j = 0; j < n; ++j) compliance[(size_t)j * n + dof] = 0.0;
+            for (int i = 0; i < n; ++i) compliance[(size_t)dof * n + i] = 0.0;
+        }
+    }
+
     // OdeSolverEulerImplicit::solve (Math/OdeSolverEulerImplicit.cpp:55-122), no compliance.
     void solve_implicit(double dt) {
         std::vector<double> xs = x, vs = v;
         std::vector<double> xn = x, vn = v;
-        std::vector<double> prev(x.size(), 0.0);
-        long it = 0;
-        while (it < newton_max_iter) {
-            assemble(dt, xs, vs, xn, vn);
-            linear_solve(rhs, solution);
-            for (size_t i = 0; i < vn.size(); ++i) vn[i] += solution[i];
-            for (size_t i = 0; i < xn.size(); ++i) xn[i] = xs[i] + dt * vn[i];
-            if (newton_max_iter > 1) {
-                double var = 0.0;
-                for (size_t i = 0; i < solution.size(); ++i) var = std::max(var, std::fabs(solution[i] - prev[i]));
-                if (var < newton_eps) break;
-                prev = solution;
-            }
-            it++;
-        }
+        last_newton_iters = newton_implicit_euler(
+            dt, newton_max_iter, newton_eps, xs, xn, vn, solution,
+            [&](const std::vector<double>& xe, const std::vector<double>& ve) { assemble(dt, xs, vs, xe, ve); },
+            [&](std::vector<double>& sol) { linear_solve(rhs, sol); });
         x = xn;
         v = vn;
     }
@@ -1282,6 +1337,7 @@ struct Sim {
             solve_implicit(dt);
             Kconst = K;  // pruned(): drops exact zeros only, which leaves every product unchanged
             linear_initialized = true;
+            if (compliance_at_first_linear_step) compute_compliance();
             return;
         }
         update_parts(x, v, UPD_F);
@@ -1682,6 +1738,129 @@ int orc_coro_element(const double* x0, const double* x, double E, double nu, dou
     }
 }
 
+// orc_coro_element with the polar factor R9 (row-major) supplied instead of computed.
+int orc_coro_element_given_rotation(const double* x0, const double* x, double E, double nu, double rho,
+                                    const double* R9, double* M, double* K, double* f, int* flags) {
+    try {
+        Element e;
+        e.kind = TET_COROTATIONAL;
+        e.nn = 4;
+        for (int a = 0; a < 4; ++a) e.ids[a] = a;
+        e.E = E;
+        e.nu = nu;
+        e.rho = rho;
+        Counters c0;
+        element_init(e, x0, &c0);
+        M3 R;
+        for (int i = 0; i < 9; ++i) R.m[i / 3][i % 3] = R9[i];
+        Counters c;
+        element_update(e, x, 1 | 2 | 4 | 8, &c, &R);
+        std::memcpy(M, e.M.data(), 144 * sizeof(double));
+        std::memcpy(K, e.K.data(), 144 * sizeof(double));
+        std::memcpy(f, e.f.data(), 12 * sizeof(double));
+        *flags = (c.det_r_invalid ? 1 : 0) | (c.g_singular ? 2 : 0);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// The reference's OdeComplexNonLinear test equation (Math/UnitTests/MockObject.h:267-338: a = x |v|^2 with
+// M = I, f = |v|^2 x, D = 2 x v^T, K = |v|^2 I, no boundary conditions) stepped twice from its test's initial
+// state (x = LinSpaced(1.4, 5.67), v = LinSpaced(-0.4, -0.3), dt = 1e-3;
+// Math/UnitTests/OdeSolverEulerImplicitTests.cpp:139-220) with the reference's default direct solver, by
+// OdeSolverEulerImplicit::solve (linear_variant 0) or OdeSolverLinearEulerImplicit::solve (1: the first step
+// as the non-linear solver, then the first step's K and system matrix kept, Math/OdeSolverLinearEulerImplicit.cpp:43-64).
+// out: x1, v1, x2, v2, f(x2, v2) (3 each); solves: linear solves of the two steps.
+int orc_ode_complex_nonlinear(int linear_variant, long max_iter, double eps, double* out, long* solves) {
+    try {
+        const double dt = 1e-3;
+        const int n = 3;
+        auto force = [](const std::vector<double>& x, const std::vector<double>& v, std::vector<double>& f) {
+            const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            for (int i = 0; i < 3; ++i) f[i] = v2 * x[i];
+        };
+        BandLU lu;
+        std::vector<double> Kd(9), rhs(n);
+        // assembleLinearSystem at (xn, vn) for the step from (xs, vs)
+        auto assemble = [&](const std::vector<double>& xs, const std::vector<double>& vs, const std::vector<double>& xn,
+                            const std::vector<double>& vn) {
+            const double v2 = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+            Csc A;
+            A.n = n;
+            A.outer = {0, 3, 6, 9};
+            A.inner = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+            A.val.assign(9, 0.0);
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    const double M = i == j ? 1.0 : 0.0, D = 2.0 * xn[i] * vn[j], K = i == j ? v2 : 0.0;
+                    Kd[3 * i + j] = K;
+                    A.val[3 * j + i] = M * (1.0 / dt) + D + K * dt;
+                }
+            lu.factor(A);
+            std::vector<double> f(n);
+            force(xn, vn, f);
+            for (int i = 0; i < n; ++i) {
+                double Kt = 0.0;
+                for (int k = 0; k < n; ++k) Kt += Kd[3 * i + k] * ((xn[k] - xs[k]) - vn[k] * dt);
+                rhs[i] = f[i] + Kt - (vn[i] - vs[i]) / dt;
+            }
+        };
+        std::vector<double> x0(n), v0(n);
+        for (int i = 0; i < n; ++i) {
+            x0[i] = 1.4 + (5.67 - 1.4) * i / 2.0;
+            v0[i] = -0.4 + (-0.3 + 0.4) * i / 2.0;
+        }
+        std::vector<double> sol(n);
+        auto nonlinear_step = [&](const std::vector<double>& xs, const std::vector<double>& vs, std::vector<double>& xn,
+                                  std::vector<double>& vn) {
+            xn = xs;
+            vn = vs;
+            return newton_implicit_euler(
+                dt, max_iter, eps, xs, xn, vn, sol,
+                [&](const std::vector<double>& xe, const std::vector<double>& ve) { assemble(xs, vs, xe, ve); },
+                [&](std::vector<double>& s) { lu.solve(rhs.data(), s.data()); });
+        };
+        std::vector<double> x1, v1, x2, v2;
+        long ns = nonlinear_step(x0, v0, x1, v1);
+        if (!linear_variant) {
+            ns += nonlinear_step(x1, v1, x2, v2);
+        } else {
+            // f(x1, v1) - K_const (v1 dt), solved against the first step's last system matrix
+            std::vector<double> f(n), b(n), dv(n);
+            force(x1, v1, f);
+            for (int i = 0; i < n; ++i) {
+                double Kv = 0.0;
+                for (int k = 0; k < n; ++k) Kv += Kd[3 * i + k] * (v1[k] * dt);
+                b[i] = f[i] - Kv;
+            }
+            lu.solve(b.data(), dv.data());
+            ns += 1;
+            v2.resize(n);
+            x2.resize(n);
+            for (int i = 0; i < n; ++i) {
+                v2[i] = v1[i] + dv[i];
+                x2[i] = x1[i] + dt * v2[i];
+            }
+        }
+        std::vector<double> f2(n);
+        force(x2, v2, f2);
+        for (int i = 0; i < n; ++i) {
+            out[i] = x1[i];
+            out[3 + i] = v1[i];
+            out[6 + i] = x2[i];
+            out[9 + i] = v2[i];
+            out[12 + i] = f2[i];
+        }
+        *solves = ns;
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 void* orc_sim_create(int kind, int num_nodes, const double* x0, int num_elems, const int* conn, const double* E,
                      const double* nu, const double* rho) {
     try {
@@ -1850,6 +2029,25 @@ void orc_sim_mass_per_node(void* h, double* out) {
 long orc_sim_last_cg_iters(void* h) { return static_cast<Sim*>(h)->last_cg.iters; }
 double orc_sim_last_cg_error(void* h) { return static_cast<Sim*>(h)->last_cg.error; }
 long orc_sim_total_cg_iters(void* h) { return static_cast<Sim*>(h)->total_cg_iters; }
+long orc_sim_last_newton_iters(void* h) { return static_cast<Sim*>(h)->last_newton_iters; }
+void orc_sim_set_compliance_at_first_linear_step(void* h, int on) {
+    static_cast<Sim*>(h)->compliance_at_first_linear_step = on != 0;
+}
+// The dense compliance of the last system matrix (computes it now); out: D x D column-major.
+int orc_sim_compliance_matrix(void* h, double* out) {
+    try {
+        Sim* s = static_cast<Sim*>(h);
+        if (s->A.n == 0) throw std::runtime_error("no system matrix yet");
+        s->compute_compliance();
+        std::memcpy(out, s->compliance.data(), s->compliance.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+// OdeSolverEulerImplicit::setNewtonRaphsonEpsilonConvergence (Math/OdeSolverEulerImplicit.cpp:43-48)
+void orc_sim_set_newton_eps(void* h, double eps) { static_cast<Sim*>(h)->newton_eps = eps; }
 void orc_sim_counters(void* h, long* det_r, long* g_sing) {
     Sim* s = static_cast<Sim*>(h);
     *det_r = s->cnt.det_r_invalid;

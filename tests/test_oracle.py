@@ -11,6 +11,7 @@ import scipy.sparse as sp
 
 import oracle
 from opensurgsim_amd import meshgen as mg
+from helpers import make_oracle
 
 
 def isapprox(a, b, prec=1e-12):
@@ -580,3 +581,71 @@ def test_oracle_adjoint_matches_central_differences(kind):
 
     fd = (run_v(1e-5) - run_v(-1e-5)) / 2e-5
     assert abs(lv[3 * node + 1] - fd) <= 1e-6 * abs(fd)
+
+
+def _is_approx(a, b, prec=1e-12):
+    """Eigen's DenseBase::isApprox: |a - b| <= prec * min(|a|, |b|) (prec: dummy_precision of double)."""
+    return np.linalg.norm(a - b) <= prec * min(np.linalg.norm(a), np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("linear_variant,max_iter,exact", [(0, 1, False), (0, 10, True), (1, 1, False),
+                                                          (1, 10, False)])
+def test_newton_loop_complex_nonlinear_kat(linear_variant, max_iter, exact):
+    """The Newton-Raphson loop the oracle's implicit step runs (newton_implicit_euler), on the reference's own
+    known-answer test: the non-linear ODE a = x |v|^2 stepped twice with epsilon 1e-13
+    (Math/UnitTests/OdeSolverEulerImplicitTests.cpp:139-220, Math/UnitTests/MockObject.h:267-338). With 10
+    iterations the second step satisfies x2 = x1 + dt v2, v2 = v1 + dt f(x2, v2) to Eigen's isApprox; one
+    iteration does not, and the linear variant never does."""
+    dt = 1e-3
+    r = oracle.ode_complex_nonlinear(linear_variant, max_iter, 1e-13)
+    assert not np.allclose(r["x1"], np.linspace(1.4, 5.67, 3)) and not np.allclose(r["x2"], r["x1"])
+    assert _is_approx(r["x2"], r["x1"] + dt * r["v2"])  # true by construction
+    v_exp = r["v1"] + dt * r["f2"]
+    assert _is_approx(r["x2"], r["x1"] + dt * v_exp) == exact
+    assert _is_approx(r["v2"], v_exp) == exact
+    if linear_variant == 0 and max_iter == 10:
+        # quadratic convergence stops the loop on the epsilon test well before the cap
+        assert 2 * 2 <= r["solves"] < 2 * 10
+        # and the implicit equations hold to rounding, far inside isApprox's 1e-12
+        assert np.linalg.norm(r["v2"] - v_exp) <= 1e-14 * np.linalg.norm(v_exp)
+    if max_iter == 1:
+        assert r["solves"] == 2
+
+
+def test_newton_loop_linear_tets_needs_no_second_correction():
+    """For linear tetrahedra the implicit step is linear, so Newton iterations after the first change the
+    solution only by the CG error: three iterations (epsilon 0: the loop runs to its cap) equal one to the
+    solver's tolerance, and the loop reports three solves."""
+    cfg = mg.config_corotational((3, 3, 3), seed=8)
+    cfg["kind"] = oracle.TET_LINEAR
+    s1 = make_oracle(cfg, tol=1e-14)
+    s3 = make_oracle(cfg, tol=1e-14)
+    s3.set_solver(tol=1e-14, newton_max_iter=3, newton_eps=0.0)
+    s1.step(cfg.dt, 4)
+    s3.step(cfg.dt, 4)
+    assert s3.last_newton_iters() == 3 and s1.last_newton_iters() == 1
+    (x1, v1), (x3, v3) = s1.state(), s3.state()
+    x0 = cfg.x0.reshape(-1)
+    assert np.linalg.norm((x3 - x0) - (x1 - x0)) <= 1e-11 * np.linalg.norm(x1 - x0)
+    assert np.linalg.norm(v3 - v1) <= 1e-11 * np.linalg.norm(v1)
+
+
+@pytest.mark.parametrize("linsolver", [oracle.LU, oracle.CG])
+def test_dense_compliance_is_the_inverse_with_boundary_rows_zeroed(linsolver):
+    """OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76): the inverse of the last
+    system matrix with boundary rows and columns zero, diagonal included (applyBoundaryConditionsToMatrix with
+    hasCompliance = false, Math/OdeState.cpp:175-195); the linear scheme forms it on its first step."""
+    cfg = mg.config_linear_cpu((3, 2, 3))
+    s = make_oracle(cfg, scheme=oracle.LINEAR_EULER_IMPLICIT, linsolver=linsolver, tol=1e-14)
+    s.set_compliance_at_first_linear_step(True)
+    s.step(cfg.dt, 2)
+    C = s.compliance_matrix()
+    outer, inner = s.pattern()
+    n = cfg.x0.size
+    A = sp.csc_matrix((s.matrix_values(2), inner, outer), shape=(n, n)).toarray()
+    Cref = np.linalg.inv(A)
+    bc = np.concatenate([3 * np.asarray(cfg.bc_nodes)[:, None] + np.arange(3)]).ravel()
+    Cref[bc, :] = 0.0
+    Cref[:, bc] = 0.0
+    assert np.abs(C - Cref).max() <= 1e-12 * np.abs(Cref).max()
+    assert np.all(C[bc, :] == 0.0) and np.all(C[:, bc] == 0.0)
