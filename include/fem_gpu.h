@@ -57,6 +57,15 @@ enum {
     FEM_SCHEME_LINEAR_RUNGE_KUTTA_4 = 9            /* INTEGRATIONSCHEME_LINEAR_RUNGE_KUTTA_4 */
 };
 
+/* Linear solver of the step (Math/LinearSparseSolveAndInverse.h:41-52, same numbering). The C ABI's default is
+ * CG; the reference's DeformableRepresentation defaults to LU (Physics/DeformableRepresentation.cpp:53), which
+ * the C++ adapters keep. LU: banded LU without pivoting of the assembled operator in FP64, factored after each
+ * assembly (once for the linear schemes), whatever the precision option; tetrahedral meshes of one instance. */
+enum {
+    FEM_LINEAR_SOLVER_LU = 0,   /* LINEARSOLVER_LU (LinearSparseSolveAndInverseLU, .cpp:31-97) */
+    FEM_LINEAR_SOLVER_CG = 1    /* LINEARSOLVER_CONJUGATEGRADIENT (LinearSparseSolveAndInverseCG, .cpp:99-129) */
+};
+
 /* Arithmetic of the element kernels, assembly and CG. */
 enum {
     FEM_PRECISION_FP64 = 0,         /* everything in FP64 */
@@ -87,6 +96,7 @@ typedef struct fem_options {
     double newton_epsilon;       /* Newton-Raphson stops once ||dv - dv_prev||_inf < newton_epsilon, default 1e-5
                                     (OdeSolverEulerImplicit::setNewtonRaphsonEpsilonConvergence,
                                     Math/OdeSolverEulerImplicit.cpp:26, 43-48, 101-112) */
+    int32_t linear_solver;       /* FEM_LINEAR_SOLVER_*, default FEM_LINEAR_SOLVER_CG */
 } fem_options;
 
 typedef struct fem_stats {
@@ -175,6 +185,15 @@ int fem_apply_compliance_device(fem_ctx* ctx, const double* b, int64_t num_colum
  * overlap on one GPU (other streams or threads) race. Blocks the calling thread until the columns are done. */
 #define FEM_NOALLOC_SCRATCH_BYTES (96ull << 20)
 int fem_apply_compliance_device_noalloc(fem_ctx* ctx, const double* b, int64_t num_columns, double* out);
+/* The dense compliance matrix of the last step's system matrix: C = A^-1 with the boundary rows and columns
+ * zero, their diagonal entries included (ndof x ndof, column-major). Replaces OdeSolver::getComplianceMatrix after
+ * OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76), which the linear schemes run on
+ * their first step (Math/OdeSolverLinearEulerImplicit.cpp:49). Computed on demand from the band LU of the
+ * operator (factored here when the step used CG, whose reference getInverse is the dense inverse,
+ * Math/LinearSparseSolveAndInverse.cpp:25-29). Tetrahedral meshes of one instance; FEM_ERR_ORDER before the first
+ * step; FEM_ERR_INVALID when the band storage does not fit the device. */
+int fem_get_compliance_matrix(fem_ctx* ctx, double* out);         /* host output */
+int fem_get_compliance_matrix_device(fem_ctx* ctx, double* out);  /* device output, ordered on the context stream */
 /* The compliance terms of a constraint localized in one element, as the constraint builders form them for
  * the MLCP (Physics/FemConstraintFixedPoint.cpp:84-106, Physics/FemConstraintFrictionlessContact.cpp:84-112,
  * consumed by MlcpPhysicsProblem::updateConstraint, Physics/MlcpPhysicsProblem.cpp:46-76). H has

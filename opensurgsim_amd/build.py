@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "_lib", "libfem_gpu.so")
 HOST_LIB = os.path.join(HERE, "_lib", "libsurgsim_gpu.so")
 SOURCES = ["fem_kernels.hip", "fem_hex.hip", "fem_adjoint.hip", "fem_batch.hip", "fem_probe.hip", "fem_mcg.hip",
-           "fem_api.cpp"]
+           "fem_lu.hip", "fem_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -13,6 +13,7 @@
 #include "fem_adjoint.hpp"
 #include "fem_batch.hpp"
 #include "fem_hex.hpp"
+#include "fem_lu.hpp"
 #include "fem_mcg.hpp"
 
 #include <hip/hip_runtime.h>
@@ -259,6 +260,10 @@ struct fem_ctx {
     DevBuf<McgColumn> mcg_col;
     McgColumn* h_mcg = nullptr;  // pinned [8]
     void* scratch = nullptr;     // device address of this device's allocation-free workspace
+    // Direct solver (FEM_LINEAR_SOLVER_LU): band LU of the assembled operator, refactored after each assembly
+    DeviceBandLU lu;
+    int lu_bw = 0;               // bandwidth of the pattern in DOFs: 3 max|I - J| + 2
+    bool lu_current = false;     // lu holds the factors of the current S64
 
     // hexahedra (element-by-element path)
     HexData hex;
@@ -527,7 +532,47 @@ int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
 
 // Linear solve S dv = b (b in c->b, FP64). FP64: Eigen-CG on the FP64 operator. FP32: FP32 CG inside
 // FP64 iterative refinement on the implicit-step residual. Result in c->dv.
+// The band LU of the current operator S64 (factored once per assembly; the linear schemes keep the first one).
+void ensure_lu(fem_ctx* c, bool may_allocate = true) {
+    if (c->lu_current) return;
+    if (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0 || !c->S64.p)
+        throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
+    DeviceBandLU& lu = c->lu;
+    const size_t need = band_bytes(c->ndof, c->lu_bw);
+    if (lu.capacity * sizeof(double) < need) {
+        if (!may_allocate) throw InvalidArg("no band LU storage allocated yet: step once with the LU solver first");
+        size_t free_b = 0, total_b = 0;
+        HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+        if (need + (size_t(1) << 30) > free_b + lu.capacity * sizeof(double))
+            throw InvalidArg("the banded LU of " + std::to_string(c->ndof) + " DOFs (bandwidth " +
+                             std::to_string(c->lu_bw) + ") needs " + std::to_string(need >> 20) + " MiB; " +
+                             std::to_string(free_b >> 20) + " MiB free");
+        if (lu.ab) HIPCHECK(hipFree(lu.ab));
+        lu.ab = nullptr;
+        lu.capacity = 0;
+        HIPCHECK(hipMalloc(&lu.ab, need));
+        lu.capacity = need / sizeof(double);
+    }
+    lu.n = c->ndof;
+    lu.bw = c->lu_bw;
+    lu.ld = 2 * (int64_t)c->lu_bw + 1;
+    launch_band_fill(lu, c->num_blocks, c->blk_row.p, c->colidx_b.p, c->S64.p, c->stream);
+    launch_band_factor(lu, c->stream);
+    c->lu_current = true;
+}
+
+// dv = A^-1 b through the band LU (LinearSparseSolveAndInverseLU::solve, Math/LinearSparseSolveAndInverse.cpp:92-97).
+void lu_linear_solve(fem_ctx* c) {
+    ensure_lu(c);
+    TIMED(c, FEM_PHASE_CG_INIT, 0, launch_band_solve(c->lu, 1, c->b.p, c->ndof, c->dv.p, c->ndof, c->stream));
+    c->stats.last_cg_iterations = 0;
+    c->stats.last_cg_error = 0.0;
+    c->stats.converged = 1;
+    c->stats.last_refinements = 0;
+}
+
 void linear_solve(fem_ctx* c) {
+    if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) return lu_linear_solve(c);
     const bool fp32 = c->opt.precision == FEM_PRECISION_FP32_REFINED;
     const int max_iter = (int)max_iters(c);
     size_t n = (size_t)c->ndof;
@@ -694,6 +739,7 @@ void assemble_tet(fem_ctx* c) {
                     (int)c->nctr_off, c->rec.p, c->S64.p, s32, c->stream);
     if (c->ext_nblk) launch_ext_operator(c->ext_nblk, c->ext_blk.p, c->ext_add.p, c->S64.p, s32, c->stream);
     if (c->timer.enabled) c->timer.end(c->stream);
+    c->lu_current = false;
 }
 
 // rhs = (staged element rows + gravity + external forces) / div with boundary rows zeroed, plus the step's
@@ -1349,6 +1395,12 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
     const auto& rp = c->h_rowptr_b;
     const auto& ci = c->h_colidx_b;
     c->num_blocks = (int64_t)ci.size();
+    {
+        int64_t maxd = 0;
+        for (int64_t I = 0; I < N; ++I)
+            for (int32_t k = rp[I]; k < rp[I + 1]; ++k) maxd = std::max<int64_t>(maxd, std::llabs((int64_t)ci[k] - I));
+        c->lu_bw = (int)(3 * maxd + 2);
+    }
     auto find_blk = [&](int32_t I, int32_t J) -> int32_t {
         auto b = ci.begin() + rp[I], en = ci.begin() + rp[I + 1];
         auto it = std::lower_bound(b, en, J);
@@ -1484,6 +1536,7 @@ void fem_default_options(fem_options* o) {
     o->max_refinements = 8;
     o->fail_on_nonconvergence = 0;
     o->newton_epsilon = 1e-5;
+    o->linear_solver = FEM_LINEAR_SOLVER_CG;
 }
 
 int fem_create(int device, fem_ctx** out) {
@@ -1541,6 +1594,7 @@ void fem_destroy(fem_ctx* c) {
         if (c->b_graph[i]) (void)hipGraphDestroy(c->b_graph[i]);
     }
     hex_release(c->hex);
+    if (c->lu.ab) (void)hipFree(c->lu.ab);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1702,6 +1756,10 @@ int fem_set_options(fem_ctx* c, const fem_options* o) {
             throw InvalidArg("FP32 precision needs at least one refinement sweep (max_refinements >= 1)");
         if (o->newton_max_iterations < 1) throw InvalidArg("The maximum iteration needs to be at least 1");
         if (!(o->newton_epsilon >= 0.0)) throw InvalidArg("The epsilon convergence cannot be negative");
+        if (o->linear_solver != FEM_LINEAR_SOLVER_LU && o->linear_solver != FEM_LINEAR_SOLVER_CG)
+            throw InvalidArg("unknown linear solver");
+        if (o->linear_solver == FEM_LINEAR_SOLVER_LU && c->mesh_ready && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
+            throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
         if (!(o->cg_tolerance >= 0.0)) throw InvalidArg("negative tolerance");
         bool grav = o->gravity_enabled != c->opt.gravity_enabled || o->gravity[0] != c->opt.gravity[0] ||
                     o->gravity[1] != c->opt.gravity[1] || o->gravity[2] != c->opt.gravity[2];
@@ -1849,12 +1907,34 @@ void mcg_columns(fem_ctx* c, const double* b, int64_t ncols, double* out, const 
     }
 }
 
+// Compliance columns through the band LU (LinearSparseSolveAndInverseLU::solve): each column of b with its
+// boundary rows zeroed into tmp (tmp_cols columns of ndof at a time), solved into out, boundary rows of the
+// solutions zeroed (DeformableRepresentation::applyCompliance, Physics/DeformableRepresentation.cpp:175-190).
+void compliance_lu(fem_ctx* c, const double* b, int64_t ncols, double* out, double* tmp, int64_t tmp_cols) {
+    const int64_t n = c->ndof;
+    for (int64_t j0 = 0; j0 < ncols; j0 += tmp_cols) {
+        const int cnt = (int)std::min<int64_t>(tmp_cols, ncols - j0);
+        for (int j = 0; j < cnt; ++j)
+            launch_masked_copy(n, b + (size_t)(j0 + j) * n, c->bcmask.p, tmp + (size_t)j * n, c->stream);
+        launch_band_solve(c->lu, cnt, tmp, n, out + (size_t)j0 * n, n, c->stream);
+        launch_zero_masked_rows(n, cnt, c->bcmask.p, out + (size_t)j0 * n, c->stream);
+    }
+}
+
 // Compliance columns of a tetrahedral mesh through the multi-column FP64 CG (mcg_columns) on the context's
 // own work buffers. In the FP32 + refinement mode a remainder of fewer than 5 columns goes column by column
 // through the step's FP32 solver instead: on the 40^3 block a fixed-point constraint's 3 columns take 2.04 ms
 // that way against 2.19 ms in one 4-column FP64 pass, while a full 8-column pass costs 0.35 ms per column.
 void compliance_tet(fem_ctx* c, const double* b, int64_t ncols, double* out) {
     const size_t n = (size_t)c->ndof;
+    if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) {
+        ensure_lu(c);
+        const int64_t tc = std::max<int64_t>(1, std::min<int64_t>(ncols, 64));
+        DevBuf<double> tmp;
+        tmp.alloc(n * (size_t)tc);
+        compliance_lu(c, b, ncols, out, tmp.p, tc);
+        return;
+    }
     if (c->opt.precision == FEM_PRECISION_FP32_REFINED) {
         const int64_t rem = ncols % 8, full = ncols - rem;
         if (rem > 0 && rem < 5) {
@@ -1953,11 +2033,72 @@ int fem_apply_compliance_device_noalloc(fem_ctx* c, const double* b, int64_t num
             return fail(c, FEM_ERR_ORDER, "boundary conditions changed since the last step: step first");
         HIPCHECK(hipSetDevice(c->device));
         char* base = (char*)c->scratch;
+        if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) {
+            ensure_lu(c, false);  // refactoring reuses the band storage of the last step's factorization
+            const int64_t tc = (int64_t)(kNoAllocScratchBytes / ((size_t)c->ndof * sizeof(double)));
+            compliance_lu(c, b, num_columns, out, (double*)base, tc);
+            return FEM_OK;
+        }
         McgColumn status[8];
         const McgWork w{(double*)(base + L.b), (double*)(base + L.x), (double*)(base + L.r),
                         (double*)(base + L.z), (double*)(base + L.p), (double*)(base + L.q),
                         (double*)(base + L.dinv), (double*)(base + L.partial), (McgColumn*)(base + L.col), status};
         mcg_columns(c, b, num_columns, out, w);
+        return FEM_OK;
+    });
+}
+
+namespace {
+// Columns [j0, j0 + cnt) of the dense compliance into out (ndof x cnt, column-major): identity columns solved
+// through the band LU of the operator, then the boundary rows and columns zeroed.
+void compliance_columns_dense(fem_ctx* c, int64_t j0, int cnt, double* out) {
+    const int64_t n = c->ndof;
+    launch_identity_columns(n, j0, cnt, out, c->stream);
+    launch_band_solve(c->lu, cnt, out, n, out, n, c->stream);
+    launch_zero_bc_rows_cols(n, j0, cnt, c->bcmask.p, out, c->stream);
+}
+
+void check_dense_compliance(fem_ctx* c) {
+    if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
+    if (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0)
+        throw InvalidArg("the dense compliance needs the assembled operator of a tetrahedral mesh of one instance");
+}
+}  // namespace
+
+int fem_get_compliance_matrix(fem_ctx* c, double* out) {
+    return guarded(c, [&] {
+        check_dense_compliance(c);
+        if (!out) throw InvalidArg("null output");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_bc(c);
+        ensure_lu(c);
+        const int64_t n = c->ndof;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)((size_t(256) << 20) / (8 * n))));
+        DevBuf<double> buf;
+        buf.alloc((size_t)n * chunk);
+        for (int64_t j0 = 0; j0 < n; j0 += chunk) {
+            const int cnt = (int)std::min(chunk, n - j0);
+            compliance_columns_dense(c, j0, cnt, buf.p);
+            HIPCHECK(hipMemcpyAsync(out + (size_t)j0 * n, buf.p, (size_t)n * cnt * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+        }
+        return FEM_OK;
+    });
+}
+
+int fem_get_compliance_matrix_device(fem_ctx* c, double* out) {
+    return guarded(c, [&] {
+        check_dense_compliance(c);
+        if (!out) throw InvalidArg("null output");
+        if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
+        HIPCHECK(hipSetDevice(c->device));
+        ensure_bc(c);
+        ensure_lu(c);
+        const int64_t n = c->ndof;
+        for (int64_t j0 = 0; j0 < n; j0 += 65536)
+            compliance_columns_dense(c, j0, (int)std::min<int64_t>(65536, n - j0), out + (size_t)j0 * n);
         return FEM_OK;
     });
 }
@@ -2081,6 +2222,8 @@ int fem_step(fem_ctx* c, int32_t nsteps) {
         alloc_precision_buffers(c);
         if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
             throw InvalidArg("the static, explicit and Runge-Kutta schemes step one tetrahedral representation");
+        if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
+            throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
         if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
             c->opt.rayleigh_stiffness != 0.0)
             throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
@@ -2206,6 +2349,8 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
     return guarded(c, [&] {
         if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "upload the mesh before declaring instances");
         if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("batched instances cover tetrahedral meshes");
+        if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU)
+            throw InvalidArg("batched instances solve with CG (the direct solver covers one representation)");
         if (num_instances < 1 || !node_offsets) throw InvalidArg("need at least one instance and node offsets");
         const int M = num_instances;
         if (node_offsets[0] != 0 || node_offsets[M] != c->num_nodes)

@@ -41,4 +41,7 @@ void launch_identity_columns(int64_t n, int64_t first, int count, double* out, h
 // matrix) set to zero: OdeState::applyBoundaryConditionsToMatrix(&C, false).
 void launch_zero_bc_rows_cols(int64_t n, int64_t first, int count, const uint8_t* mask, double* out, hipStream_t s);
 
+// Rows of the boundary DOFs (mask != 0) of count columns of n values (column-major) set to zero.
+void launch_zero_masked_rows(int64_t n, int count, const uint8_t* mask, double* x, hipStream_t s);
+
 }  // namespace femgpu

@@ -20,6 +20,7 @@ EULER_EXPLICIT, LINEAR_EULER_EXPLICIT = 4, 5
 EULER_EXPLICIT_MODIFIED, LINEAR_EULER_EXPLICIT_MODIFIED = 6, 7
 RUNGE_KUTTA_4, LINEAR_RUNGE_KUTTA_4 = 8, 9
 FP64, FP32_REFINED = 0, 1
+LINEAR_SOLVER_LU, LINEAR_SOLVER_CG = 0, 1
 
 # Names of the exported entry points (tests/test_abi.py compares this list with include/fem_gpu.h).
 EXPORTS = [
@@ -27,6 +28,7 @@ EXPORTS = [
     "fem_add_boundary_condition_node", "fem_add_boundary_condition_dof", "fem_default_options",
     "fem_set_options", "fem_set_external_force", "fem_add_external_generalized_force", "fem_apply_compliance",
     "fem_apply_compliance_device", "fem_apply_compliance_device_noalloc", "fem_constraint_compliance",
+    "fem_get_compliance_matrix", "fem_get_compliance_matrix_device",
     "fem_set_state", "fem_get_state", "fem_set_state_device",
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
@@ -55,6 +57,7 @@ class FemOptions(ctypes.Structure):
         ("rayleigh_stiffness", ctypes.c_double),
         ("mass_lumping", ctypes.c_int32),
         ("newton_epsilon", ctypes.c_double),
+        ("linear_solver", ctypes.c_int32),
     ]
 
 
@@ -118,6 +121,8 @@ def load():
     L.fem_apply_compliance.argtypes = [_vp, _dp, ctypes.c_int64, _dp]
     L.fem_apply_compliance_device.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
     L.fem_apply_compliance_device_noalloc.argtypes = [_vp, _vp, ctypes.c_int64, _vp]
+    L.fem_get_compliance_matrix.argtypes = [_vp, _dp]
+    L.fem_get_compliance_matrix_device.argtypes = [_vp, _vp]
     L.fem_constraint_compliance.argtypes = [_vp, _i64, _dp, ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
     L.fem_set_state.argtypes = [_vp, _dp, _dp]
     L.fem_get_state.argtypes = [_vp, _dp, _dp]
@@ -288,6 +293,17 @@ class FemGpu:
         """apply_compliance_device through the entry point that allocates nothing (fixed per-GPU workspace)."""
         self._check(self._L.fem_apply_compliance_device_noalloc(self.h, ctypes.c_void_p(b_ptr), int(num_columns),
                                                                 ctypes.c_void_p(out_ptr)))
+
+    def compliance_matrix(self):
+        """Dense compliance of the last system matrix (ndof x ndof), boundary rows and columns zero
+        (OdeSolver::getComplianceMatrix)."""
+        C = np.zeros((self.ndof, self.ndof), order="F")
+        self._check(self._L.fem_get_compliance_matrix(self.h, C.ctypes.data_as(_dp)))
+        return C
+
+    def compliance_matrix_device(self, out_ptr):
+        """The same into device memory (ndof x ndof FP64, column-major), ordered on the context stream."""
+        self._check(self._L.fem_get_compliance_matrix_device(self.h, ctypes.c_void_p(out_ptr)))
 
     def constraint_compliance(self, element, coordinate, directions, scale, want_cht=True):
         """(C H^T of shape (ndof, m) or None, H C H^T of shape (m, m)) for a constraint localized in one element:

@@ -349,8 +349,11 @@ void Fem3DRepresentation::applyOptions(double dt) {
     if (auto cg = std::dynamic_pointer_cast<Math::LinearSparseSolveAndInverseCG>(solver)) {
         o.cg_tolerance = cg->getTolerance();
         o.cg_max_iterations = cg->getMaxIterations();
+    } else if (elementKind(m_elementType) != FEM_ELEMENT_HEX) {
+        o.linear_solver = FEM_LINEAR_SOLVER_LU;  // the banded LU of the assembled operator
     } else {
-        o.cg_tolerance = 1e-13;  // LU stand-in
+        // hexahedra keep no assembled operator (element-by-element products): CG to 1e-13 stands in for the LU
+        o.cg_tolerance = 1e-13;
         o.cg_max_iterations = -1;
     }
     check(fem_set_options(m_ctx, &o), "options");
@@ -396,6 +399,14 @@ Math::Vector Fem3DRepresentation::applyCompliance(const Math::Vector& b, size_t 
     if (b.size() != n * numColumns) fail("applyCompliance: b needs getNumDof() values per column");
     Math::Vector out(b.size(), 0.0);
     check(fem_apply_compliance(m_ctx, b.data(), (int64_t)numColumns, out.data()), "applyCompliance");
+    return out;
+}
+
+Math::Vector Fem3DRepresentation::getComplianceMatrix() const {
+    if (!m_ctx) fail("Ode solver not initialized, it should have been initialized on wake-up");
+    const size_t n = m_initialState->getNumDof();
+    Math::Vector out(n * n, 0.0);
+    check(fem_get_compliance_matrix(m_ctx, out.data()), "getComplianceMatrix");
     return out;
 }
 

@@ -11,10 +11,9 @@
 // Framework/Assert.h:77-95); a state that becomes non-finite deactivates the representation
 // (Physics/FemRepresentation.cpp:285-293).
 //
-// Only what the stepping path and its compliance consumers need is mirrored. Not provided: dense compliance
-// matrices (OdeSolver::computeComplianceMatrixFromSystemMatrix; the compliance is applied to columns instead),
-// compliance warping, Localization objects (node and element localizations are passed as ids and natural
-// coordinates).
+// Only what the stepping path and its compliance consumers need is mirrored. The dense compliance matrix is
+// formed on request (getComplianceMatrix), not on every update; compliance warping and Localization objects are
+// not provided (node and element localizations are passed as ids and natural coordinates).
 #pragma once
 
 #include <array>
@@ -305,6 +304,10 @@ public:
     /// columns of b (getNumDof() values each, column-major): boundary rows zeroed, one CG solve per column
     /// against the last update's system matrix, boundary rows of the result zeroed.
     Math::Vector applyCompliance(const Math::Vector& b, size_t numColumns) const;
+    /// OdeSolver::getComplianceMatrix after computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76):
+    /// the inverse of the last update's system matrix with boundary rows and columns zero, getNumDof()^2
+    /// values, column-major (tetrahedral meshes).
+    Math::Vector getComplianceMatrix() const;
     /// The compliance terms a constraint localized in one element hands to the MLCP
     /// (Physics/FemConstraintFixedPoint.cpp:84-106, Physics/FemConstraintFrictionlessContact.cpp:84-112,
     /// Physics/MlcpPhysicsProblem.cpp:46-76): row j of H is scale * naturalCoordinate[a] * directions[j] at

@@ -30,20 +30,6 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def build_native(out_dir):
-    """Compile the oracle with -march=native for the host it runs on (the CPU baseline's build) into
-    out_dir; returns the library path."""
-    subprocess.check_call(["make", "-s", "-C", _HERE, "native", "NATIVE_OUT=%s" % out_dir])
-    return os.path.join(out_dir, "libfem_oracle.so")
-
-
-def use_library(path):
-    """Load the oracle from `path` from now on (objects made with the previous library must not be used)."""
-    global _lib, _LIB_PATH
-    _LIB_PATH = path
-    _lib = None
-
-
 def lib():
     global _lib
     if _lib is None:

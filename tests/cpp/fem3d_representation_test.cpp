@@ -37,6 +37,7 @@ void orc_sim_set_damping(void* h, double rayleigh_mass, double rayleigh_stiffnes
 int orc_sim_add_localized_force(void* h, int element, const double* coord, const double* force, const double* K,
                                 const double* D);
 int orc_sim_apply_compliance(void* h, const double* b, int num_columns, double* out);
+int orc_sim_compliance_matrix(void* h, double* out);
 }
 
 // ---- minimal test harness ------------------------------------------------------------------------
@@ -513,6 +514,31 @@ void gpuTests() {
     run("Fem3DRepresentationGpuTest.LinearTetLinearEulerImplicitLU", [] {
         compareWithOracle(kTet, 0, Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT, Math::LINEARSOLVER_LU, 1e-13, 10,
                           1e-9);
+    });
+    run("Fem3DRepresentationGpuTest.ComplianceMatrixMatchesOracle", [] {
+        // getComplianceMatrix after a linear implicit Euler step with the direct solver: the oracle's dense
+        // inverse with boundary rows and columns zeroed (Math/OdeSolver.cpp:69-76)
+        const Block b = makeBlock(3, 2, 3);
+        const Material m{5e4, 0.45, 1000.0};
+        auto fem = makeRepresentation(b, kTet, m, {});
+        fem->setIntegrationScheme(Math::INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT);
+        fem->setLinearSolver(Math::LINEARSOLVER_LU);
+        fem->initialize();
+        fem->update(1e-3);
+        fem->afterUpdate(1e-3);
+        const Math::Vector C = fem->getComplianceMatrix();
+        void* o = makeOracle(b, 0, m, {}, 1, 0, 1e-13);
+        EXPECT_TRUE(orc_sim_step(o, 1e-3, 1) == 0);
+        std::vector<double> Co(C.size());
+        EXPECT_TRUE(orc_sim_compliance_matrix(o, Co.data()) == 0);
+        double err = 0.0, mx = 0.0;
+        for (size_t i = 0; i < C.size(); ++i) {
+            err = std::max(err, std::fabs(C[i] - Co[i]));
+            mx = std::max(mx, std::fabs(Co[i]));
+        }
+        std::printf("  compliance matrix max abs err / max %.3e\n", err / mx);
+        EXPECT_TRUE(err <= 1e-12 * mx);
+        orc_sim_destroy(o);
     });
     run("Fem3DRepresentationGpuTest.CubeImplicitEulerCG", [] {
         compareWithOracle(kCube, 2, Math::INTEGRATIONSCHEME_EULER_IMPLICIT, Math::LINEARSOLVER_CONJUGATEGRADIENT,

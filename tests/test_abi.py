@@ -35,9 +35,10 @@ def test_default_options_match_reference_defaults():
     assert o.rayleigh_mass == 0.0 and o.rayleigh_stiffness == 0.0  # Physics/FemRepresentation.cpp:38-43
     assert o.mass_lumping == 0
     assert o.newton_epsilon == 1e-5                        # Math/OdeSolverEulerImplicit.cpp:26
+    assert o.linear_solver == femgpu.LINEAR_SOLVER_CG       # the C ABI's default (the adapters keep LU)
     assert o.scheme == femgpu.EULER_IMPLICIT
 
 
 def test_options_struct_layout():
-    assert ctypes.sizeof(femgpu.FemOptions) == 112
+    assert ctypes.sizeof(femgpu.FemOptions) == 120
     assert ctypes.sizeof(femgpu.FemStats) == 80
