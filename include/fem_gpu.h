@@ -42,8 +42,8 @@ enum {
     FEM_ELEMENT_HEX = 2               /* "SurgSim::Physics::Fem3DElementCube" */
 };
 
-/* Integration schemes (Math/OdeSolver.h:36-49). The schemes after the two implicit ones step
- * tetrahedral meshes of one representation (not hexahedra, not batched instances). */
+/* Integration schemes (Math/OdeSolver.h:36-49). The schemes after the two implicit ones step one
+ * representation (tetrahedral or hexahedral), not batched instances. */
 enum {
     FEM_SCHEME_EULER_IMPLICIT = 0,                 /* INTEGRATIONSCHEME_EULER_IMPLICIT */
     FEM_SCHEME_LINEAR_EULER_IMPLICIT = 1,          /* INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT */

@@ -1263,7 +1263,50 @@ HexStepArgs hex_args(fem_ctx* c) {
 // Math/OdeSolverLinear*.cpp). The system matrix is M / dt (K for static); the linear variants keep the first
 // step's matrix and solve later steps against it (the compliance of DeformableRepresentation::applyCompliance,
 // Physics/DeformableRepresentation.cpp:175-190: boundary rows of the right-hand side and of the solution zero).
+// The other schemes on a hexahedral mesh: the same updates as step_other_scheme with the element-by-element
+// operator (static: K; explicit and Runge-Kutta 4: M / dt, never lumped, Physics/FemRepresentation.cpp:412-421)
+// and the force f(x, v) with the Rayleigh term as the right-hand side. A hexahedron's K and M are constant, so
+// the linear variants (first step's operator kept) take the same arithmetic.
+void step_other_scheme_hex(fem_ctx* c) {
+    const int sc = c->opt.scheme;
+    const double dt = c->opt.dt;
+    const int n = (int)c->ndof;
+    HexStepArgs a = hex_args(c);
+    const bool stat = sc == FEM_SCHEME_STATIC || sc == FEM_SCHEME_LINEAR_STATIC;
+    hex_set_operator(c->hex, a, stat ? 0.0 : 1.0 / dt, stat ? 1.0 : 0.0, 0, c->stream);
+    const double kv = c->opt.rayleigh_stiffness, am = c->opt.rayleigh_mass;
+    double* x = c->x.p;
+    double* v = c->v.p;
+    if (sc == FEM_SCHEME_RUNGE_KUTTA_4 || sc == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4) {
+        for (DevBuf<double>* b : {&c->rk_x, &c->rk_k2v, &c->rk_k3v, &c->rk_k4v, &c->rk_a1, &c->rk_a2, &c->rk_a3})
+            if (b->n != (size_t)n) b->alloc(n);
+        const size_t bytes = (size_t)n * sizeof(double);
+        // k1 = (v, M^-1 F(x, v)): f / dt against M / dt
+        hex_rhs(c->hex, a, x, v, kv, am, 0, dt, c->stream);
+        hex_solve(c->hex, a, c->stats, c->stream);
+        HIPCHECK(hipMemcpyAsync(c->rk_a1.p, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        auto stage = [&](const double* kvs, const double* kas, double div, double* vstage, double* aout) {
+            launch_rk4_stage(n, x, v, kvs, kas, dt, div, c->rk_x.p, vstage, c->stream);
+            hex_rhs(c->hex, a, c->rk_x.p, vstage, kv, am, 0, dt, c->stream);
+            hex_solve(c->hex, a, c->stats, c->stream);
+            if (aout) HIPCHECK(hipMemcpyAsync(aout, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        };
+        stage(v, c->rk_a1.p, 2.0, c->rk_k2v.p, c->rk_a2.p);
+        stage(c->rk_k2v.p, c->rk_a2.p, 2.0, c->rk_k3v.p, c->rk_a3.p);
+        stage(c->rk_k3v.p, c->rk_a3.p, 1.0, c->rk_k4v.p, nullptr);  // a4 stays in dv
+        TIMED(c, FEM_PHASE_UPDATE, 0,
+              launch_rk4_final(n, x, v, c->rk_k2v.p, c->rk_k3v.p, c->rk_k4v.p, c->rk_a1.p, c->rk_a2.p, c->rk_a3.p,
+                               c->dv.p, dt, c->counters.p + 3, c->stream));
+        return;
+    }
+    hex_rhs(c->hex, a, x, v, kv, am, 0, 1.0, c->stream);
+    hex_solve(c->hex, a, c->stats, c->stream);
+    const int umode = stat ? 2 : (sc == FEM_SCHEME_EULER_EXPLICIT || sc == FEM_SCHEME_LINEAR_EULER_EXPLICIT) ? 1 : 0;
+    TIMED(c, FEM_PHASE_UPDATE, 0, launch_update_scheme(n, umode, c->dv.p, x, x, v, dt, c->counters.p + 3, c->stream));
+}
+
 void step_other_scheme(fem_ctx* c) {
+    if (c->kind == FEM_ELEMENT_HEX) return step_other_scheme_hex(c);
     const int sc = c->opt.scheme;
     const double dt = c->opt.dt;
     const int n = (int)c->ndof;
@@ -1319,11 +1362,11 @@ int step_once(fem_ctx* c) {
     const size_t n = (size_t)c->ndof;
     const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
     const int newton = std::max(1, c->opt.newton_max_iterations);
-    if (c->kind == FEM_ELEMENT_HEX) {
+    if (c->opt.scheme >= FEM_SCHEME_STATIC) {
+        step_other_scheme(c);
+    } else if (c->kind == FEM_ELEMENT_HEX) {
         HexStepArgs a = hex_args(c);
         hex_step(c->hex, a, c->stats, c->stream);
-    } else if (c->opt.scheme >= FEM_SCHEME_STATIC) {
-        step_other_scheme(c);
     } else if (linear_scheme && c->linear_matrix_ready && c->matrix_dt == dt) {
         // Steps >= 2 of OdeSolverLinearEulerImplicit: element forces only, constant operator.
         tet_elements(c, false, c->x.p, c->v.p);
@@ -2220,8 +2263,8 @@ int fem_step(fem_ctx* c, int32_t nsteps) {
         HIPCHECK(hipSetDevice(c->device));
         ensure_bc(c);
         alloc_precision_buffers(c);
-        if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
-            throw InvalidArg("the static, explicit and Runge-Kutta schemes step one tetrahedral representation");
+        if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->num_instances > 0)
+            throw InvalidArg("the static, explicit and Runge-Kutta schemes step one representation");
         if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
             throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
         if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&

@@ -371,10 +371,11 @@ __global__ __launch_bounds__(256) void hex_rhs_init_kernel(int n, const double* 
         b[i] = v;
     }
 }
-__global__ __launch_bounds__(256) void hex_rhs_bc_kernel(int n, const uint8_t* __restrict__ mask,
+// Boundary rows of the right-hand side zeroed, the others divided by div (Runge-Kutta 4 solves M/dt a = f/dt).
+__global__ __launch_bounds__(256) void hex_rhs_bc_kernel(int n, const uint8_t* __restrict__ mask, double div,
                                                          double* __restrict__ b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && mask[i]) b[i] = 0.0;
+    if (i < n) b[i] = mask[i] ? 0.0 : (div == 1.0 ? b[i] : b[i] / div);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -892,9 +893,21 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
     hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.fstep, a.b);
     apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt + o.rayleigh_stiffness, nullptr, o.rayleigh_mass, 1.0, lumped, -1.0,
             a.b, s);
-    hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, a.b);
+    hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, 1.0, a.b);
     hex_solve(h, a, stats, s);
     launch_update(n, a.dv, a.x, a.x, a.v, dt, a.invalid, s);
+}
+
+void hex_set_operator(HexData& h, const HexStepArgs& a, double cm, double ck, int lumped, hipStream_t s) {
+    if (h.a32_cm != cm || h.a32_ck != ck || h.a32_lumped != lumped) hex_prepare(h, a.x0, a.mat, cm, ck, lumped, s);
+}
+
+void hex_rhs(HexData& h, const HexStepArgs& a, const double* x, const double* v, double kv, double am, int lumped,
+             double div, hipStream_t s) {
+    const int n = (int)(3 * h.num_nodes);
+    hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.fstep, a.b);
+    apply64(h, a.x0, a.mat, nullptr, x, v, kv, nullptr, am, 1.0, lumped, -1.0, a.b, s);
+    hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, div, a.b);
 }
 
 void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s) {

@@ -77,6 +77,14 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
 // rows already zero).
 void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
 
+// The operator the solves apply: A = cm M + ck K (lumped: M's row sums on the diagonal); the FP32 cache and the
+// diagonals are rebuilt only when the coefficients change.
+void hex_set_operator(HexData& h, const HexStepArgs& a, double cm, double ck, int lumped, hipStream_t s);
+// a.b = (fg + fext + fstep - K ((x - x0) + kv v) - am M v) / div with boundary rows zero: the force f(x, v) of the
+// schemes (kv = alpha_K, am = alpha_M: the Rayleigh force -D v).
+void hex_rhs(HexData& h, const HexStepArgs& a, const double* x, const double* v, double kv, double am, int lumped,
+             double div, hipStream_t s);
+
 // Average milliseconds of one application of the element-by-element operator over every colour (q += A p on
 // the CG scratch vectors p = vec[3], q = vec[4]): the FP32 cached operator (table or per-element cache, as the
 // solves use it) or the FP64 matrix-free one; reps applications, each between its own pair of events.

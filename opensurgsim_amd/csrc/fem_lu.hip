@@ -10,8 +10,8 @@
 // of the trailing block summed before they are applied, so results agree to rounding.
 //
 // Substitution: one workgroup per right-hand side. Rows go in blocks of 64: the block's products with the
-// already solved entries (up to bw of them per row) are summed by 4 threads per row, then one wavefront solves
-// the 64 x 64 triangle with the solved entries broadcast lane to lane.
+// already solved entries (up to bw of them per row) are summed by 16 lanes per row, then one wavefront solves
+// the 64 x 64 triangle, staged in LDS, with the solved entries broadcast lane to lane.
 #include "fem_lu.hpp"
 #include "fem_kernels.hpp"
 
@@ -129,30 +129,56 @@ __global__ __launch_bounds__(256) void band_update_kernel(double* __restrict__ a
     }
 }
 
-// x = U^-1 L^-1 b for column blockIdx.x. 256 threads: 4 per row of a 64-row block for the products with solved
-// entries, then wavefront 0 for the in-block triangle.
-__global__ __launch_bounds__(256) void band_solve_kernel(const double* __restrict__ ab, int64_t n, int bw, int64_t ld,
-                                                         const double* b, int64_t ldb, double* x, int64_t ldx) {
-    __shared__ double part[kTile][4];
+// x = U^-1 L^-1 b for column blockIdx.x, 1024 threads. Rows go in blocks of 64: 16 lanes per row sum the row's
+// products with the entries solved before the block (a contiguous stretch of the band row, read coalesced, four
+// independent accumulators per lane, then a butterfly over the 16 lanes), while the block's own triangle is
+// staged in LDS; wavefront 0 then solves the 64 x 64 triangle from LDS with the solved entries broadcast lane to
+// lane. Sums are in a fixed order.
+constexpr int kSolveThreads = 1024;
+__global__ __launch_bounds__(kSolveThreads) void band_solve_kernel(const double* __restrict__ ab, int64_t n, int bw,
+                                                                   int64_t ld, const double* b, int64_t ldb, double* x,
+                                                                   int64_t ldx) {
+    __shared__ double T[kTile][kTile + 1];
+    __shared__ double rs[kTile];
     const double* bc = b + (int64_t)blockIdx.x * ldb;
     double* xc = x + (int64_t)blockIdx.x * ldx;
-    const int tid = threadIdx.x, row = tid >> 2, q = tid & 3, lane = tid & 63;
+    const int tid = threadIdx.x, row = tid >> 4, q = tid & 15, lane = tid & 63;
+    auto row_sum = [&](double s) {
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (q == 0) rs[row] = s;
+    };
     // forward: y = L^-1 b (unit diagonal), y written to xc
     for (int64_t r0 = 0; r0 < n; r0 += kTile) {
+        for (int t = tid; t < kTile * kTile; t += kSolveThreads) {
+            const int l = t >> 6, k = t & 63;
+            const int64_t ii = r0 + l, kk = r0 + k;
+            T[l][k] = (ii < n && kk < ii && ii - kk <= bw) ? atc(ab, ld, bw, ii, kk) : 0.0;
+        }
         const int64_t i = r0 + row;
-        double s = 0.0;
-        if (i < n)
-            for (int64_t k = max((int64_t)0, i - bw) + q; k < r0; k += 4) s += atc(ab, ld, bw, i, k) * xc[k];
-        part[row][q] = s;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (i < n) {
+            const int64_t kb = max((int64_t)0, i - bw);
+            const double* arow = ab + i * ld + (kb - i + bw);  // L(i, kb), L(i, kb + 1), ...
+            const double* xk = xc + kb;
+            const int cnt = (int)(r0 - kb);  // solved entries k in [kb, r0)
+            int k = q;
+            for (; k + 48 < cnt; k += 64) {
+                s0 += arow[k] * xk[k];
+                s1 += arow[k + 16] * xk[k + 16];
+                s2 += arow[k + 32] * xk[k + 32];
+                s3 += arow[k + 48] * xk[k + 48];
+            }
+            for (; k < cnt; k += 16) s0 += arow[k] * xk[k];
+        }
+        row_sum((s0 + s1) + (s2 + s3));
         __syncthreads();
         if (tid < 64) {
             const int64_t ii = r0 + lane;
-            double y = 0.0;
-            if (ii < n) y = bc[ii] - ((part[lane][0] + part[lane][1]) + (part[lane][2] + part[lane][3]));
-            const int64_t r1 = min(n, r0 + kTile);
-            for (int64_t k = r0; k < r1; ++k) {
-                const double yk = __shfl(y, (int)(k - r0), 64);
-                if (ii < n && ii > k && ii - k <= bw) y -= atc(ab, ld, bw, ii, k) * yk;
+            double y = ii < n ? bc[ii] - rs[lane] : 0.0;
+            for (int k = 0; k < kTile; ++k) {
+                const double yk = __shfl(y, k, 64);
+                if (lane > k) y -= T[lane][k] * yk;
             }
             if (ii < n) xc[ii] = y;
         }
@@ -162,21 +188,38 @@ __global__ __launch_bounds__(256) void band_solve_kernel(const double* __restric
     const int64_t nblk = (n + kTile - 1) / kTile;
     for (int64_t blk = nblk - 1; blk >= 0; --blk) {
         const int64_t r0 = blk * kTile, r1 = min(n, r0 + kTile);
+        for (int t = tid; t < kTile * kTile; t += kSolveThreads) {
+            const int l = t >> 6, k = t & 63;
+            const int64_t ii = r0 + l, kk = r0 + k;
+            T[l][k] = (ii < r1 && kk < r1 && kk >= ii && kk - ii <= bw) ? atc(ab, ld, bw, ii, kk) : 0.0;
+        }
         const int64_t i = r0 + row;
-        double s = 0.0;
-        if (i < r1)
-            for (int64_t k = r1 + q; k <= min(n - 1, i + bw); k += 4) s += atc(ab, ld, bw, i, k) * xc[k];
-        part[row][q] = s;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (i < r1) {
+            const int64_t ke = min(n - 1, i + bw);
+            const double* arow = ab + i * ld + (r1 - i + bw);  // U(i, r1), U(i, r1 + 1), ...
+            const double* xk = xc + r1;
+            const int cnt = (int)(ke - r1 + 1);  // solved entries k in [r1, ke]
+            int k = q;
+            for (; k + 48 < cnt; k += 64) {
+                s0 += arow[k] * xk[k];
+                s1 += arow[k + 16] * xk[k + 16];
+                s2 += arow[k + 32] * xk[k + 32];
+                s3 += arow[k + 48] * xk[k + 48];
+            }
+            for (; k < cnt; k += 16) s0 += arow[k] * xk[k];
+        }
+        row_sum((s0 + s1) + (s2 + s3));
         __syncthreads();
         if (tid < 64) {
             const int64_t ii = r0 + lane;
-            double acc = 0.0;
-            if (ii < r1) acc = xc[ii] - ((part[lane][0] + part[lane][1]) + (part[lane][2] + part[lane][3]));
+            const int m = (int)(r1 - r0);
+            double acc = ii < r1 ? xc[ii] - rs[lane] : 0.0;
             double xv = 0.0;
-            for (int64_t k = r1 - 1; k >= r0; --k) {
-                if (ii == k) xv = acc / atc(ab, ld, bw, ii, ii);
-                const double xk = __shfl(xv, (int)(k - r0), 64);
-                if (ii < k && k - ii <= bw) acc -= atc(ab, ld, bw, ii, k) * xk;
+            for (int k = m - 1; k >= 0; --k) {
+                if (lane == k) xv = acc / T[k][k];
+                const double xkv = __shfl(xv, k, 64);
+                if (lane < k) acc -= T[lane][k] * xkv;
             }
             if (ii < r1) xc[ii] = xv;
         }
@@ -242,8 +285,8 @@ void launch_band_factor(DeviceBandLU& lu, hipStream_t s) {
 void launch_band_solve(const DeviceBandLU& lu, int nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
                        hipStream_t s) {
     if (nrhs <= 0 || lu.n == 0) return;
-    launch_k(band_solve_kernel, dim3((unsigned)nrhs), dim3(256), 0, s, (const double*)lu.ab, lu.n, lu.bw, lu.ld, b,
-             ldb, x, ldx);
+    launch_k(band_solve_kernel, dim3((unsigned)nrhs), dim3(kSolveThreads), 0, s, (const double*)lu.ab, lu.n, lu.bw,
+             lu.ld, b, ldb, x, ldx);
 }
 
 void launch_identity_columns(int64_t n, int64_t first, int count, double* out, hipStream_t s) {

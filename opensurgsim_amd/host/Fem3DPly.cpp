@@ -1,4 +1,4 @@
-// Fem3D PLY reading (ASCII) and Fem3DRepresentation::loadFem / setFem.
+// Fem3D PLY reading (ascii, binary_little_endian, binary_big_endian) and Fem3DRepresentation::loadFem / setFem.
 //
 // Semantics follow Physics/FemPlyReaderDelegate.cpp:31-145 and Physics/Fem3DPlyReaderDelegate.cpp:52-96:
 //   - required: element "vertex" with x, y, z; element "3d_element" with list property vertex_indices;
@@ -7,8 +7,13 @@
 //     same three properties per 3d_element; the "material" element wins when both are present; no material
 //     leaves zeros;
 //   - other elements (e.g. "face") are read and skipped.
+// The file formats and property types are those of the PLY library the reference reads through
+// (DataStructures/ply.c:61-72, 760-762): char, short, int, uchar, ushort, uint, float, double (and their sized
+// names int8 ... float64); binary data in either byte order.
 #include "SurgSimGpu.h"
 
+#include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -19,9 +24,35 @@ namespace {
 
 [[noreturn]] void fail(const std::string& m) { throw Framework::AssertionFailure(m); }
 
+enum PlyType { kNoType = 0, kChar, kShort, kInt, kUChar, kUShort, kUInt, kFloat, kDouble };
+
+PlyType plyType(const std::string& t) {
+    if (t == "char" || t == "int8") return kChar;
+    if (t == "short" || t == "int16") return kShort;
+    if (t == "int" || t == "int32") return kInt;
+    if (t == "uchar" || t == "uint8") return kUChar;
+    if (t == "ushort" || t == "uint16") return kUShort;
+    if (t == "uint" || t == "uint32") return kUInt;
+    if (t == "float" || t == "float32") return kFloat;
+    if (t == "double" || t == "float64") return kDouble;
+    return kNoType;
+}
+
+size_t typeSize(PlyType t) {
+    switch (t) {
+        case kChar: case kUChar: return 1;
+        case kShort: case kUShort: return 2;
+        case kInt: case kUInt: case kFloat: return 4;
+        case kDouble: return 8;
+        default: return 0;
+    }
+}
+
 struct PlyProperty {
     std::string name;
     bool list = false;
+    PlyType type = kNoType;       // scalar type, or the list items' type
+    PlyType countType = kNoType;  // list length type
 };
 struct PlyElement {
     std::string name;
@@ -34,24 +65,77 @@ struct PlyElement {
     }
 };
 
+// Values of the data section, row by row: ascii rows are lines of whitespace-separated numbers; binary rows are
+// the properties' values back to back in the file's byte order.
+class PlyValues {
+public:
+    PlyValues(std::ifstream& in, int format, const std::string& file) : m_in(in), m_format(format), m_file(file) {}
+    void beginRow(const std::string& element) {
+        if (m_format != 0) return;
+        std::string line;
+        if (!std::getline(m_in, line)) fail("'" + m_file + "': unexpected end of data in element " + element);
+        m_line.clear();
+        m_line.str(line);
+    }
+    double read(PlyType t, const std::string& element) {
+        if (m_format == 0) {
+            double v;
+            if (!(m_line >> v)) fail("'" + m_file + "': bad value in element " + element);
+            return v;
+        }
+        unsigned char b[8];
+        const size_t n = typeSize(t);
+        if (!m_in.read(reinterpret_cast<char*>(b), (std::streamsize)n))
+            fail("'" + m_file + "': unexpected end of data in element " + element);
+        const bool big = m_format == 2;
+        const uint16_t probe = 1;
+        const bool hostBig = *reinterpret_cast<const unsigned char*>(&probe) == 0;
+        if (big != hostBig)
+            for (size_t i = 0; i < n / 2; ++i) std::swap(b[i], b[n - 1 - i]);
+        switch (t) {
+            case kChar: { int8_t v; std::memcpy(&v, b, 1); return v; }
+            case kUChar: { uint8_t v; std::memcpy(&v, b, 1); return v; }
+            case kShort: { int16_t v; std::memcpy(&v, b, 2); return v; }
+            case kUShort: { uint16_t v; std::memcpy(&v, b, 2); return v; }
+            case kInt: { int32_t v; std::memcpy(&v, b, 4); return v; }
+            case kUInt: { uint32_t v; std::memcpy(&v, b, 4); return v; }
+            case kFloat: { float v; std::memcpy(&v, b, 4); return v; }
+            case kDouble: { double v; std::memcpy(&v, b, 8); return v; }
+            default: fail("'" + m_file + "': property of unknown type in element " + element);
+        }
+    }
+
+private:
+    std::ifstream& m_in;
+    int m_format;  // 0 ascii, 1 binary little endian, 2 binary big endian
+    std::string m_file;
+    std::istringstream m_line;
+};
+
 }  // namespace
 
 void Fem3D::load(const std::string& fileName) {
-    std::ifstream in(fileName);
+    std::ifstream in(fileName, std::ios::binary);
     if (!in) fail("Could not open '" + fileName + "'");
     std::string line;
-    std::getline(in, line);
+    auto next_line = [&]() {
+        if (!std::getline(in, line)) return false;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        return true;
+    };
+    next_line();
     if (line.rfind("ply", 0) != 0) fail("'" + fileName + "' is not a ply file");
     std::vector<PlyElement> elems;
-    bool ascii = false;
-    while (std::getline(in, line)) {
+    int format = -1;
+    while (next_line()) {
         std::istringstream ls(line);
         std::string key;
         ls >> key;
         if (key == "format") {
             std::string f;
             ls >> f;
-            ascii = f == "ascii";
+            format = f == "ascii" ? 0 : f == "binary_little_endian" ? 1 : f == "binary_big_endian" ? 2 : -1;
+            if (format < 0) fail("'" + fileName + "': unknown ply format " + f);
         } else if (key == "element") {
             PlyElement e;
             ls >> e.name >> e.count;
@@ -65,6 +149,13 @@ void Fem3D::load(const std::string& fileName) {
                 std::string ct, it;
                 ls >> ct >> it;
                 p.list = true;
+                p.countType = plyType(ct);
+                p.type = plyType(it);
+                if (p.countType == kNoType || p.type == kNoType)
+                    fail("'" + fileName + "': unknown list property type " + ct + " " + it);
+            } else {
+                p.type = plyType(t);
+                if (p.type == kNoType) fail("'" + fileName + "': unknown property type " + t);
             }
             ls >> p.name;
             elems.back().props.push_back(p);
@@ -72,7 +163,7 @@ void Fem3D::load(const std::string& fileName) {
             break;
         }
     }
-    if (!ascii) fail("'" + fileName + "': only ASCII ply files are read");
+    if (format < 0) fail("'" + fileName + "': no format line");
     const PlyElement* ev = nullptr;
     const PlyElement* ee = nullptr;
     const PlyElement* em = nullptr;
@@ -96,21 +187,25 @@ void Fem3D::load(const std::string& fileName) {
     m_elements.clear();
     m_boundaryConditions.clear();
     Element material;  // zeros unless a material is given
+    PlyValues values(in, format, fileName);
     for (const auto& e : elems) {
         for (size_t r = 0; r < e.count; ++r) {
-            if (!std::getline(in, line)) fail("'" + fileName + "': unexpected end of data in element " + e.name);
-            std::istringstream ls(line);
+            values.beginRow(e.name);
             std::vector<double> scal(e.props.size(), 0.0);
             std::vector<size_t> list;
             for (size_t p = 0; p < e.props.size(); ++p) {
                 if (e.props[p].list) {
-                    size_t n = 0;
-                    if (!(ls >> n)) fail("'" + fileName + "': bad list in element " + e.name);
+                    const double nd = values.read(e.props[p].countType, e.name);
+                    if (!(nd >= 0.0)) fail("'" + fileName + "': bad list in element " + e.name);
+                    const size_t n = (size_t)nd;
                     list.resize(n);
-                    for (size_t k = 0; k < n; ++k)
-                        if (!(ls >> list[k])) fail("'" + fileName + "': bad list in element " + e.name);
-                } else if (!(ls >> scal[p])) {
-                    fail("'" + fileName + "': bad value in element " + e.name);
+                    for (size_t k = 0; k < n; ++k) {
+                        const double v = values.read(e.props[p].type, e.name);
+                        if (!(v >= 0.0)) fail("'" + fileName + "': bad list in element " + e.name);
+                        list[k] = (size_t)v;
+                    }
+                } else {
+                    scal[p] = values.read(e.props[p].type, e.name);
                 }
             }
             if (&e == ev) {

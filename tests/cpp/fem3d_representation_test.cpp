@@ -8,8 +8,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <functional>
+#include <iterator>
 #include <string>
 #include <vector>
 
@@ -213,8 +216,84 @@ std::string dataPath(const std::string& name) { return g_data_dir + "/" + name; 
 
 bool near(double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(1.0, std::fabs(b)); }
 
+// A Fem3D written as a binary PLY (ply.c's binary formats, DataStructures/ply.c:760-762): vertex x, y, z as
+// `coord` ("double" or "float"), 3d_element vertex_indices as a list of `count` / `index` types, a material row in
+// double, boundary conditions as uint. bigEndian swaps every value's bytes.
+template <typename Coord, typename Count, typename Index>
+void writeBinaryPly(const Physics::Fem3D& fem, const std::string& path, bool bigEndian, const char* coord,
+                    const char* count, const char* index) {
+    std::ofstream out(path, std::ios::binary);
+    out << "ply\nformat " << (bigEndian ? "binary_big_endian" : "binary_little_endian") << " 1.0\n"
+        << "comment written by the adapter tests\n"
+        << "element vertex " << fem.getNumVertices() << "\n"
+        << "property " << coord << " x\nproperty " << coord << " y\nproperty " << coord << " z\n"
+        << "element 3d_element " << fem.getNumElements() << "\n"
+        << "property list " << count << " " << index << " vertex_indices\n"
+        << "element material 1\nproperty double mass_density\nproperty double poisson_ratio\n"
+        << "property double young_modulus\n"
+        << "element boundary_condition " << fem.getBoundaryConditions().size() << "\n"
+        << "property uint vertex_index\nend_header\n";
+    auto put = [&](auto v) {
+        unsigned char b[sizeof(v)];
+        std::memcpy(b, &v, sizeof(v));
+        if (bigEndian) std::reverse(b, b + sizeof(v));
+        out.write(reinterpret_cast<const char*>(b), sizeof(v));
+    };
+    for (size_t i = 0; i < fem.getNumVertices(); ++i)
+        for (double c : fem.getVertexPosition(i)) put((Coord)c);
+    for (const auto& e : fem.getElements()) {
+        put((Count)e.nodeIds.size());
+        for (size_t id : e.nodeIds) put((Index)id);
+    }
+    const auto& e0 = fem.getElement(0);
+    put(e0.massDensity);
+    put(e0.poissonRatio);
+    put(e0.youngModulus);
+    for (size_t b : fem.getBoundaryConditions()) put((uint32_t)b);
+}
+
 // ---- CPU-only tests (no GPU context is created) ----------------------------------------------------
 void plyTests() {
+    run("Fem3DRepresentationReaderTests.BinaryFormats", [] {
+        // the reference's wound mesh written back in ply.c's two binary formats reads as the ascii original
+        Physics::Fem3D ascii;
+        ascii.load(dataPath("wound_deformable.ply"));
+        const std::string dir = "/tmp";
+        const std::string le = dir + "/fem3d_wound_le.ply", be = dir + "/fem3d_wound_be.ply";
+        writeBinaryPly<double, uint8_t, int32_t>(ascii, le, false, "double", "uchar", "int");
+        writeBinaryPly<float, uint16_t, uint32_t>(ascii, be, true, "float", "ushort", "uint");
+        Physics::Fem3D a, b;
+        a.load(le);
+        b.load(be);
+        EXPECT_TRUE(a.getVertexPositions() == ascii.getVertexPositions());
+        bool floatsMatch = b.getNumVertices() == ascii.getNumVertices();
+        for (size_t i = 0; floatsMatch && i < 3 * ascii.getNumVertices(); ++i)
+            floatsMatch = b.getVertexPositions()[i] == (double)(float)ascii.getVertexPositions()[i];
+        EXPECT_TRUE(floatsMatch);
+        for (const Physics::Fem3D* f : {&a, &b}) {
+            EXPECT_TRUE(f->getNumElements() == ascii.getNumElements());
+            bool same = true;
+            for (size_t e = 0; same && e < ascii.getNumElements(); ++e)
+                same = f->getElement(e).nodeIds == ascii.getElement(e).nodeIds &&
+                       f->getElement(e).youngModulus == ascii.getElement(e).youngModulus &&
+                       f->getElement(e).poissonRatio == ascii.getElement(e).poissonRatio &&
+                       f->getElement(e).massDensity == ascii.getElement(e).massDensity;
+            EXPECT_TRUE(same);
+            EXPECT_TRUE(f->getBoundaryConditions() == ascii.getBoundaryConditions());
+        }
+        // a truncated binary file is rejected
+        {
+            std::ifstream src(le, std::ios::binary);
+            std::string all((std::istreambuf_iterator<char>(src)), std::istreambuf_iterator<char>());
+            std::ofstream cut(dir + "/fem3d_wound_cut.ply", std::ios::binary);
+            cut.write(all.data(), (std::streamsize)(all.size() - 40));
+        }
+        Physics::Fem3D c;
+        EXPECT_THROW_ASSERT(c.load(dir + "/fem3d_wound_cut.ply"));
+        std::remove(le.c_str());
+        std::remove(be.c_str());
+        std::remove((dir + "/fem3d_wound_cut.ply").c_str());
+    });
     // Physics/UnitTests/Fem3DPlyReaderDelegateTests.cpp, on the reference's own data files
     run("Fem3DRepresentationReaderTests.TetrahedronMeshDelegateTest", [] {
         Physics::Fem3D fem;

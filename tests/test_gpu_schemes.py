@@ -81,6 +81,22 @@ def test_linear_variants_vs_oracle(scheme):
     assert e["u"] < 1e-9 and e["v"] < 1e-9
 
 
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+@pytest.mark.parametrize("scheme", NONLINEAR + LINEAR)
+def test_hex_schemes_vs_oracle(scheme, precision):
+    """The static, explicit, modified explicit and Runge-Kutta 4 schemes and their linear variants on hexahedra
+    (the element-by-element operator is K, or M / dt never lumped), damped, with the generator's loads."""
+    cfg = mg.config_hex((4, 3, 4))
+    cfg["E"] = np.full(cfg.conn.shape[0], 5e3)
+    cfg["dt"] = 1e-4
+    steps = 2 if scheme in (femgpu.STATIC, femgpu.LINEAR_STATIC) else 8
+    say("hex scheme", scheme, "precision", precision)
+    e = run_pair(cfg, steps, scheme, precision=precision, am=AM, ak=AK)
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-8
+    assert e["u"] < bound and e["v"] < bound
+
+
 @pytest.mark.parametrize("scheme", [femgpu.EULER_EXPLICIT, femgpu.RUNGE_KUTTA_4])
 def test_explicit_schemes_fp32_refined(scheme):
     cfg = soft_config(oracle.TET_COROTATIONAL)
@@ -187,8 +203,16 @@ def test_unsupported_combinations_rejected():
     with pytest.raises(femgpu.FemError):
         g.add_external_generalized_force(0, np.full(8, 0.125), [0, 0, 1.0], np.eye(3), None)
     g.set_options(dt=cfg.dt, scheme=femgpu.RUNGE_KUTTA_4)
+    assert g.step(1) == femgpu.FEM_OK  # the other schemes step hexahedra too
+    cfgs = mg.config_batched(2, seed=3, lo=2, hi=3)
+    b = mg.concat_instances(cfgs)
+    gb = femgpu.FemGpu(0)
+    gb.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    gb.set_instances(b.node_offsets, b.dt)
+    gb.set_options(dt=1e-3, scheme=femgpu.EULER_EXPLICIT)
     with pytest.raises(femgpu.FemError):
-        g.step(1)
+        gb.step(1)  # batched instances keep the implicit schemes
+    gb.close()
     t = make_gpu(soft_config(oracle.TET_LINEAR))
     with pytest.raises(femgpu.FemError):
         t.set_options(dt=1e-3, scheme=femgpu.EULER_EXPLICIT, rayleigh_mass=1.0, mass_lumping=1)
