@@ -215,7 +215,12 @@ struct fem_ctx {
     DevBuf<double> rec;         // element records
     DevBuf<double> rhs_rec;
     DevBuf<int> lower_blk, mirror_blk, blk_row, colidx_b, rowptr_b, ctr_ptr, node_ptr, node_list, diagblk;
-    DevBuf<int> slot_off, rhs_off;  // staging positions of each element's block contributions / rhs rows
+    // per element group (the element kernel's workgroup): its combined block records and node rows (see
+    // build_tet_maps); rhs_ptr: the rhs records of each node
+    DevBuf<int> grp_q, q_out, q_src, grp_n, n_out, n_src, rhs_ptr;
+    DevBuf<uint16_t> src16, nsrc16;
+    int64_t num_groups = 0;
+    int64_t rhs_len = 0;    // doubles of the rhs staging
     DevBuf<int4> lb_info;           // per lower block: block, mirror block, boundary bits (row DOFs 0-2, column 3-5)
     std::vector<int4> h_lb_info;
     std::vector<int32_t> h_lb_row, h_lb_col;
@@ -726,8 +731,15 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.general = (a.mode == 0 && (xs != a.xn || vs != a.vn)) ? 1 : 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
-    a.slot_off = c->slot_off.p;
-    a.rhs_off = c->rhs_off.p;
+    a.grp_q = c->grp_q.p;
+    a.q_out = c->q_out.p;
+    a.q_src = c->q_src.p;
+    a.src16 = c->src16.p;
+    a.grp_n = c->grp_n.p;
+    a.n_out = c->n_out.p;
+    a.n_src = c->n_src.p;
+    a.nsrc16 = c->nsrc16.p;
+    a.num_groups = (int)c->num_groups;
     a.flag_counts = c->counters.p;
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
 }
@@ -746,7 +758,7 @@ void assemble_tet(fem_ctx* c) {
 // localized external rows; (xn, vn, xs, vs) as for tet_elements.
 void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const double* vn = nullptr,
              const double* xs = nullptr, const double* vs = nullptr) {
-    TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->node_ptr.p, c->rhs_rec.p, c->fg.p,
+    TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->rhs_ptr.p, c->rhs_rec.p, c->fg.p,
                                           c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream, div));
     if (c->ext_nrows) {
         xn = xn ? xn : c->x.p;
@@ -1191,8 +1203,15 @@ int batch_step(fem_ctx* c) {
     a.general = 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
-    a.slot_off = c->slot_off.p;
-    a.rhs_off = c->rhs_off.p;
+    a.grp_q = c->grp_q.p;
+    a.q_out = c->q_out.p;
+    a.q_src = c->q_src.p;
+    a.src16 = c->src16.p;
+    a.grp_n = c->grp_n.p;
+    a.n_out = c->n_out.p;
+    a.n_src = c->n_src.p;
+    a.nsrc16 = c->nsrc16.p;
+    a.num_groups = (int)c->num_groups;
     a.flag_counts = c->counters.p;
     a.elem_inst = c->elem_inst.p;
     a.inst_dt = c->b_inst_dt.p;
@@ -1496,31 +1515,82 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
             cptr[t + 1]++;
         }
     }
+    // The element kernel sums, per workgroup of kElemGroup consecutive elements, the contributions its elements
+    // make to one lower block (or one node's rhs row) in LDS, in element order, and stores one record per
+    // (group, block): records of block t are ordered by group, i.e. by element, so the assembly's sum over them
+    // keeps the element order. Per group: its unique blocks q (grp_q), each with the staging offset of its record
+    // (q_out) and the LDS sources (element in group * 10 + slot) of its contributions in element order
+    // (q_src / src16); the same for the node rows (grp_n, n_out, n_src / nsrc16) with sources element * 4 + corner.
+    const int64_t G = (E + kElemGroup - 1) / kElemGroup;
+    c->num_groups = G;
+    std::fill(cptr.begin(), cptr.end(), 0);
+    std::vector<int32_t> rptr(N + 1, 0);
+    std::vector<std::array<int32_t, 3>> ctb, ntb;  // (block or node, element in group, source)
+    auto group_lists = [&](int64_t g) {
+        const int64_t e0 = g * kElemGroup, e1 = std::min<int64_t>(E, e0 + kElemGroup);
+        ctb.clear();
+        ntb.clear();
+        for (int64_t e = e0; e < e1; ++e) {
+            const int le = (int)(e - e0);
+            for (int sl = 0; sl < 10; ++sl) ctb.push_back({slot_t[10 * e + sl], le, 10 * le + sl});
+            for (int q = 0; q < 4; ++q) ntb.push_back({conn[4 * e + q], le, 4 * le + q});
+        }
+        std::sort(ctb.begin(), ctb.end());
+        std::sort(ntb.begin(), ntb.end());
+    };
+    for (int64_t g = 0; g < G; ++g) {
+        group_lists(g);
+        for (size_t k = 0; k < ctb.size(); ++k)
+            if (k == 0 || ctb[k][0] != ctb[k - 1][0]) cptr[ctb[k][0] + 1]++;
+        for (size_t k = 0; k < ntb.size(); ++k)
+            if (k == 0 || ntb[k][0] != ntb[k - 1][0]) rptr[ntb[k][0] + 1]++;
+    }
     for (int t = 0; t < c->num_lower; ++t) cptr[t + 1] += cptr[t];
+    for (int64_t i = 0; i < N; ++i) rptr[i + 1] += rptr[i];
     c->nctr_off = cptr[c->num_lower_off];
     c->stage_len = kPairRec * c->nctr_off + kDiagRec * ((int64_t)cptr[c->num_lower] - c->nctr_off);
-    std::vector<int32_t> slot_off(10 * E);
+    c->rhs_len = kRhsRec * (int64_t)rptr[N];
+    if (c->stage_len > (int64_t)std::numeric_limits<int32_t>::max())
+        throw InvalidArg("mesh too large for int32 staging offsets");
+    std::vector<int32_t> grp_q(G + 1, 0), q_out, q_src(1, 0), grp_n(G + 1, 0), n_out, n_src(1, 0);
+    std::vector<uint16_t> src16, nsrc16;
     {
-        std::vector<int64_t> p(cptr.begin(), cptr.end() - 1);
-        for (int64_t i = 0; i < 10 * E; ++i) {
-            const int64_t pos = p[slot_t[i]]++;
-            slot_off[i] = (int32_t)(pos < c->nctr_off ? kPairRec * pos
-                                                      : kPairRec * c->nctr_off + kDiagRec * (pos - c->nctr_off));
+        std::vector<int64_t> pb(cptr.begin(), cptr.end() - 1), pn(rptr.begin(), rptr.end() - 1);
+        for (int64_t g = 0; g < G; ++g) {
+            group_lists(g);
+            for (size_t k = 0; k < ctb.size(); ++k) {
+                const int32_t t = ctb[k][0];
+                if (k == 0 || t != ctb[k - 1][0]) {
+                    if (k > 0) q_src.push_back((int32_t)src16.size());
+                    const int64_t pos = pb[t]++;
+                    q_out.push_back((int32_t)(pos < c->nctr_off ? kPairRec * pos
+                                                                : kPairRec * c->nctr_off + kDiagRec * (pos - c->nctr_off)));
+                }
+                src16.push_back((uint16_t)ctb[k][2]);
+            }
+            if (!ctb.empty()) q_src.push_back((int32_t)src16.size());
+            grp_q[g + 1] = (int32_t)q_out.size();
+            for (size_t k = 0; k < ntb.size(); ++k) {
+                const int32_t nd = ntb[k][0];
+                if (k == 0 || nd != ntb[k - 1][0]) {
+                    if (k > 0) n_src.push_back((int32_t)nsrc16.size());
+                    n_out.push_back((int32_t)(kRhsRec * pn[nd]++));
+                }
+                nsrc16.push_back((uint16_t)ntb[k][2]);
+            }
+            if (!ntb.empty()) n_src.push_back((int32_t)nsrc16.size());
+            grp_n[g + 1] = (int32_t)n_out.size();
         }
     }
-    // Node rhs contributions, element order; the staging offset of each element's node rows.
-    std::vector<int32_t> nptr(N + 1, 0), nlist(4 * E), rhs_off(4 * E);
+    // Node lists of (element, corner) codes in element order, for the adjoint's gather.
+    std::vector<int32_t> nptr(N + 1, 0), nlist(4 * E);
     for (int64_t e = 0; e < E; ++e)
         for (int a = 0; a < 4; ++a) nptr[conn[4 * e + a] + 1]++;
     for (int64_t i = 0; i < N; ++i) nptr[i + 1] += nptr[i];
     {
         std::vector<int32_t> p(nptr.begin(), nptr.end() - 1);
         for (int64_t e = 0; e < E; ++e)
-            for (int a = 0; a < 4; ++a) {
-                const int32_t pos = p[conn[4 * e + a]]++;
-                nlist[pos] = (int32_t)(e << 2) | a;
-                rhs_off[4 * e + a] = kRhsRec * pos;
-            }
+            for (int a = 0; a < 4; ++a) nlist[p[conn[4 * e + a]]++] = (int32_t)(e << 2) | a;
     }
     hipStream_t s = c->stream;
     c->rowptr_b.alloc(N + 1);
@@ -1547,10 +1617,23 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
     c->mirror_blk.upload(mirror.data(), c->num_lower, s);
     c->ctr_ptr.alloc(c->num_lower + 1);
     c->ctr_ptr.upload(cptr.data(), c->num_lower + 1, s);
-    c->slot_off.alloc(10 * E);
-    c->slot_off.upload(slot_off.data(), 10 * E, s);
-    c->rhs_off.alloc(4 * E);
-    c->rhs_off.upload(rhs_off.data(), 4 * E, s);
+    auto up32 = [&](DevBuf<int>& d, const std::vector<int32_t>& h) {
+        d.alloc(std::max<size_t>(1, h.size()));
+        d.upload(h.data(), h.size(), s);
+    };
+    auto up16 = [&](DevBuf<uint16_t>& d, const std::vector<uint16_t>& h) {
+        d.alloc(std::max<size_t>(1, h.size()));
+        d.upload(h.data(), h.size(), s);
+    };
+    up32(c->grp_q, grp_q);
+    up32(c->q_out, q_out);
+    up32(c->q_src, q_src);
+    up16(c->src16, src16);
+    up32(c->grp_n, grp_n);
+    up32(c->n_out, n_out);
+    up32(c->n_src, n_src);
+    up16(c->nsrc16, nsrc16);
+    up32(c->rhs_ptr, rptr);
     c->node_ptr.alloc(N + 1);
     c->node_ptr.upload(nptr.data(), N + 1, s);
     c->node_list.alloc(4 * E);
@@ -1722,7 +1805,7 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
                 c4[e] = make_int4(conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]);
             c->conn4.alloc(num_elements);
             c->conn4.upload(c4.data(), num_elements, c->stream);
-            c->rhs_rec.alloc((size_t)num_elements * 4 * kRhsRec);
+            c->rhs_rec.alloc((size_t)c->rhs_len);
         } else {
             hex_setup(c->hex, num_nodes, x0, num_elements, conn, mat.data(), c->h_mass_per_node, c->h_rowptr_b,
                       c->h_colidx_b, c->stream);
@@ -2562,8 +2645,15 @@ int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const dou
             ea.general = 0;
             ea.rec = c->rec.p;
             ea.rhs_rec = c->rhs_rec.p;
-            ea.slot_off = c->slot_off.p;
-            ea.rhs_off = c->rhs_off.p;
+            ea.grp_q = c->grp_q.p;
+            ea.q_out = c->q_out.p;
+            ea.q_src = c->q_src.p;
+            ea.src16 = c->src16.p;
+            ea.grp_n = c->grp_n.p;
+            ea.n_out = c->n_out.p;
+            ea.n_src = c->n_src.p;
+            ea.nsrc16 = c->nsrc16.p;
+            ea.num_groups = (int)c->num_groups;
             ea.flag_counts = c->counters.p;
             launch_tet_elements(c->kind, ea, s);
         }

@@ -51,6 +51,9 @@ struct CgScalars {
 constexpr int kPairRec = 10;
 constexpr int kDiagRec = 6;
 constexpr int kRhsRec = 4;
+// Elements per workgroup of the tetrahedron element kernel (256 threads, four lanes per element): the group
+// whose contributions are summed in LDS before they are staged.
+constexpr int kElemGroup = 64;
 
 struct TetElementArgs {
     int num_elements;
@@ -64,12 +67,23 @@ struct TetElementArgs {
     double dt;
     int write_matrix;     // 0: only the rhs record
     int general;          // 0: xn == xs and vn == vs (first Newton iteration)
-    double* rec;          // lower-block contributions staged by block: kPairRec records of the off-diagonal
-                          // blocks, then kDiagRec records of the diagonal blocks
-    double* rhs_rec;      // element rhs rows staged by node, kRhsRec doubles each
-    const int* slot_off;  // [num_elements][10] offset in rec of the contribution to the pair slots 0..5
-                          // ((0,1), (0,2), (0,3), (1,2), (1,3), (2,3)) and the diagonal blocks 6..9
-    const int* rhs_off;   // [num_elements][4] offset in rhs_rec of node a's row
+    double* rec;          // lower-block contributions staged by block, one record per element group touching
+                          // the block, in group order: kPairRec records of the off-diagonal blocks, then kDiagRec
+                          // records of the diagonal blocks
+    double* rhs_rec;      // rhs rows staged by node, one kRhsRec record per element group touching the node
+    // Per element group g (kElemGroup consecutive elements): records q in [grp_q[g], grp_q[g + 1]) go to offset
+    // q_out[q] of rec and sum the LDS contributions src16[q_src[q] .. q_src[q + 1]) (element in group * 10 +
+    // slot; slots 0..5 the pairs (0,1), (0,2), (0,3), (1,2), (1,3), (2,3), 6..9 the diagonal blocks), in element
+    // order; node rows likewise (grp_n, n_out into rhs_rec, n_src / nsrc16 = element in group * 4 + corner).
+    const int* grp_q;
+    const int* q_out;
+    const int* q_src;
+    const uint16_t* src16;
+    const int* grp_n;
+    const int* n_out;
+    const int* n_src;
+    const uint16_t* nsrc16;
+    int num_groups;
     int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
     const int* elem_inst = nullptr;     // batched instances: instance of each element (null: one dt)
     const double* inst_dt = nullptr;    // time step per instance
