@@ -214,6 +214,12 @@ int fem_get_state_device(fem_ctx* ctx, double* x, double* v);
  * Math/OdeSolverEulerImplicit.cpp:55-161, Math/OdeSolverLinearEulerImplicit.cpp:43-64).
  * Returns FEM_INACTIVE once the state becomes non-finite (further calls do nothing). */
 int fem_step(fem_ctx* ctx, int32_t nsteps);
+
+/* fem_step that also records the states: x_hist and v_hist (device, (nsteps + 1) x ndof doubles each, ordered
+ * on the context stream) receive the state at the call's start (row 0) and after each step (row i). No
+ * counterpart in the reference; it serves the adjoint's recomputation of a checkpoint segment, whose states
+ * fem_adjoint_steps reads back. */
+int fem_step_record(fem_ctx* ctx, int32_t nsteps, double* x_hist, double* v_hist);
 int fem_synchronize(fem_ctx* ctx);
 int fem_query(fem_ctx* ctx, fem_stats* stats);
 
@@ -261,6 +267,15 @@ int fem_set_young_modulus(fem_ctx* ctx, const double* young);
 int fem_adjoint_step(fem_ctx* ctx, const double* x_n, const double* v_n, const double* v_next,
                      const double* lam_x, const double* lam_v, double* lam_x_out, double* lam_v_out,
                      double* grad_young);
+
+/* nsteps fem_adjoint_step calls backwards through a recorded history (x_hist, v_hist: rows 0 .. nsteps as
+ * fem_step_record writes them): step j = nsteps - 1 .. 0 reverses the step from row j to row j + 1, with
+ * lam_x, lam_v updated in place and grad_young accumulated. No counterpart in the reference (see
+ * fem_adjoint_step). In FP64 CG mode every transpose solve of the call after the first starts from the
+ * solution of the step after it (Eigen's solveWithGuess: same tolerance, relative to the right-hand side);
+ * the result depends only on the arguments. */
+int fem_adjoint_steps(fem_ctx* ctx, int32_t nsteps, const double* x_hist, const double* v_hist, double* lam_x,
+                      double* lam_v, double* grad_young);
 
 /* Lumped node masses (num_nodes values): each element adds rho V / (4 | 8) to its nodes
  * (Physics/FemRepresentation.cpp:102-117); their sum is FemRepresentation::getTotalMass

@@ -45,6 +45,7 @@ class Rollout:
         # order the context's work with torch's on the same stream
         sim.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
         self.checkpoints = []
+        self.final = None
         self.young = None
 
     def _get_state(self):
@@ -75,7 +76,9 @@ class Rollout:
             done += n
             if done < self.steps:
                 self.checkpoints.append(self._get_state())
-        return self._get_state()
+        xf, vf = self._get_state()
+        self.final = (xf.clone(), vf.clone())
+        return xf, vf
 
     def backward(self, grad_x, grad_v):
         """dL/dE (ne), dL/dx_0, dL/dv_0 for the adjoints grad_x, grad_v of the final state."""
@@ -90,14 +93,17 @@ class Rollout:
             length = min(self.k, self.steps - start)
             x, v = self.checkpoints[seg]
             self._set_state(x, v)
-            states = [(x, v)]
-            for _ in range(length):
-                self._advance(1)
-                states.append(self._get_state())
-            for j in range(length - 1, -1, -1):
-                xj, vj = states[j]
-                self.sim.adjoint_step(xj.data_ptr(), vj.data_ptr(), states[j + 1][1].data_ptr(), lx.data_ptr(),
-                                      lv.data_ptr(), lx.data_ptr(), lv.data_ptr(), gE.data_ptr())
+            # the segment's states (rows 0 .. length): recomputed from its checkpoint up to row length - 1; row
+            # length is the next checkpoint (or the final state), which the same steps reproduce bit for bit
+            xh = torch.empty((length + 1, self.n), dtype=torch.float64, device=self.device)
+            vh = torch.empty_like(xh)
+            rc = self.sim.step_record(length - 1, xh.data_ptr(), vh.data_ptr())
+            if rc != femgpu.FEM_OK:
+                raise FloatingPointError("the rollout's state became non-finite")
+            xe, ve = self.checkpoints[seg + 1] if seg + 1 < len(self.checkpoints) else self.final
+            xh[length].copy_(xe)
+            vh[length].copy_(ve)
+            self.sim.adjoint_steps(length, xh.data_ptr(), vh.data_ptr(), lx.data_ptr(), lv.data_ptr(), gE.data_ptr())
         self.sim.synchronize()
         return gE, lx, lv
 

@@ -65,45 +65,144 @@ __device__ __forceinline__ T bilinear_lower(const double* l, const T* K, const d
     return s;
 }
 
+// Stiffness block K_pq of coro_block in dual arithmetic from the element-wide factors (R, W_p in duals; the
+// gradients g and V, lambda, mu constants): K = V [lambda h_p h_q^T + mu h_q h_p^T + mu (g_p . g_q) I] plus
+// W_p [g_q]x R^T when the correction applies; h = R g.
+__device__ __forceinline__ void coro_block_dual(const Dual* R, const Dual* W, const Dual* hp, const Dual* hq,
+                                                const double* gp, const double* gq, double V, double lam,
+                                                double mu, bool correction, Dual* K) {
+    const double gab = gp[0] * gq[0] + gp[1] * gq[1] + gp[2] * gq[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            K[3 * i + j] = V * (lam * hp[i] * hq[j] + mu * hq[i] * hp[j] + Dual((i == j) ? mu * gab : 0.0));
+    if (correction) {
+        Dual WG[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            WG[3 * i + 0] = W[3 * i + 1] * gq[2] - W[3 * i + 2] * gq[1];
+            WG[3 * i + 1] = -W[3 * i + 0] * gq[2] + W[3 * i + 2] * gq[0];
+            WG[3 * i + 2] = W[3 * i + 0] * gq[1] - W[3 * i + 1] * gq[0];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                K[3 * i + j] += WG[3 * i] * R[3 * j] + WG[3 * i + 1] * R[3 * j + 1] + WG[3 * i + 2] * R[3 * j + 2];
+    }
+}
+
+// Row a's three values of a 12-vector (or of the 4 x 3 gradients) without indexing a register array by a
+// loop variable (which would place the array in scratch memory).
+__device__ __forceinline__ void row3(const double* v, int a, double* out) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = a == 0 ? v[i] : a == 1 ? v[3 + i] : a == 2 ? v[6 + i] : v[9 + i];
+}
+
+// One lane per (element, direction): directions 0..11 carry the tangent of x_e's coordinate dir and write
+// d phi_e / d x_(e, dir); direction 12 (tangent zero) adds phi_e / E_e to dL/dE_e and writes -dt K_e^T mu~_e.
+// The element-wide factors are formed once; the per-row factor W_p and the blocks K_pq follow row by row in
+// loops kept rolled, and the node vectors (mu~, v, dv) are read where they are used, so the lane's live set
+// stays in registers.
 __global__ __launch_bounds__(256) void adj_coro_kernel(AdjointArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int e = t / kAdjLanes, dir = t % kAdjLanes;
     if (e >= a.num_elements) return;
-    int id[4];
-    double X0[12], xv[12], vv[12], mt[12], dv[12];
-    load_element(a, e, id, X0, xv, vv, mt, dv);
+    const int4 c4 = a.conn[e];
+    double X0[12];
+    Dual X[12];
+    {
+        const int id[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const size_t d = 3 * (size_t)id[n] + k;
+                X0[3 * n + k] = a.x0[d];
+                X[3 * n + k] = Dual(a.x[d], 3 * n + k == dir ? 1.0 : 0.0);
+            }
+    }
     double g[4][3];
     const double V = tet_rest_gradients(X0, g);
+    const double* gf = &g[0][0];
     const double4 m = a.mat[e];
-    Dual X[12];
+    const double lam = m.x, mu = m.y, dt = a.dt;
+    CoroCommon<Dual> cc;
+    coro_common<Dual>(X0, X, g, lam, mu, cc);
+    auto node = [&](int p) { return p == 0 ? c4.x : p == 1 ? c4.y : p == 2 ? c4.z : c4.w; };
+    auto mtilde = [&](int p, double* out) {  // mu~ of node p (zero on BC rows)
 #pragma unroll
-    for (int i = 0; i < 12; ++i) X[i] = Dual(xv[i], i == dir ? 1.0 : 0.0);
-    Dual f[12];
-    CoroFactors<Dual> cf;
-    coro_factors<Dual, Dual>(X0, X, g, V, m.x, m.y, f, cf);
+        for (int k = 0; k < 3; ++k) {
+            const size_t d = 3 * (size_t)node(p) + k;
+            out[k] = a.bcmask[d] ? 0.0 : a.mu[d];
+        }
+    };
+    // mu~ . f(x)
     Dual phi(0.0);
-#pragma unroll
-    for (int i = 0; i < 12; ++i) phi += mt[i] * f[i];
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+        double gp[3], mp[3];
+        row3(gf, p, gp);
+        mtilde(p, mp);
+        Dual fp[3];
+        coro_force<Dual>(cc, gp, V, fp);
+        phi += mp[0] * fp[0] + mp[1] * fp[1] + mp[2] * fp[2];
+    }
     double gv[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) gv[i] = 0.0;
-    const double dt = a.dt;
-    for (int p = 0; p < 4; ++p)
-        for (int q = 0; q < 4; ++q) {
-            Dual K[9];
-            coro_block<Dual>(cf, p, q, K);
-            phi -= dt * bilinear(mt + 3 * p, K, vv + 3 * q);
-            if (p == q) {
-                phi -= dt * bilinear_lower(mt + 3 * p, K, dv + 3 * p);
-            } else if (id[p] > id[q]) {
-                // block (I, J), I > J, and its mirror (J, I) = K_pq^T in the self-adjoint view
-                phi -= dt * (bilinear(mt + 3 * p, K, dv + 3 * q) + bilinear(dv + 3 * p, K, mt + 3 * q));
-            }
-            if (dir == 12)
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+        double gp[3], mp[3], dvp[3];
+        row3(gf, p, gp);
+        mtilde(p, mp);
+        const int ip = node(p);
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    gv[3 * q + j] -= dt * (K[j].v * mt[3 * p] + K[3 + j].v * mt[3 * p + 1] + K[6 + j].v * mt[3 * p + 2]);
+        for (int k = 0; k < 3; ++k) {
+            const size_t d = 3 * (size_t)ip + k;
+            dvp[k] = a.v_next[d] - a.v[d];
         }
+        Dual W[9], hp[3];
+        coro_weight<Dual>(cc, gp, V, lam, mu, W);
+        const Dual gpd[3] = {Dual(gp[0]), Dual(gp[1]), Dual(gp[2])};
+        mv3(cc.R, gpd, hp);
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            double gq[3], mq[3], vq[3], dvq[3];
+            row3(gf, q, gq);
+            mtilde(q, mq);
+            const int iq = node(q);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const size_t d = 3 * (size_t)iq + k;
+                vq[k] = a.v[d];
+                dvq[k] = a.v_next[d] - vq[k];
+            }
+            Dual hq[3], K[9];
+            const Dual gqd[3] = {Dual(gq[0]), Dual(gq[1]), Dual(gq[2])};
+            mv3(cc.R, gqd, hq);
+            coro_block_dual(cc.R, W, hp, hq, gp, gq, V, lam, mu, cc.correction, K);
+            phi -= dt * bilinear(mp, K, vq);
+            if (p == q) {
+                phi -= dt * bilinear_lower(mp, K, dvp);
+            } else if (ip > iq) {
+                // block (I, J), I > J, and its mirror (J, I) = K_pq^T in the self-adjoint view
+                phi -= dt * (bilinear(mp, K, dvq) + bilinear(dvp, K, mq));
+            }
+            if (dir == 12) {
+                double kq[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) kq[j] = -dt * (K[j].v * mp[0] + K[3 + j].v * mp[1] + K[6 + j].v * mp[2]);
+                // gv[3 q + j] += kq[j], q selected without a dynamic register index
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+                    if (qq == q)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) gv[3 * qq + j] += kq[j];
+            }
+        }
+    }
     if (dir < 12) {
         a.gx_rec[12 * (size_t)e + dir] = phi.d;
     } else {

@@ -315,6 +315,9 @@ struct fem_ctx {
     double matrix_dt = 0.0;
     int last_precision = -1;
     int predicted_iters = 16;     // FP64 solve / current inner solve
+    int predicted_adj = 16;       // FP64 transpose solve of the adjoint step
+    bool solving_adjoint = false; // linear_solve serves fem_adjoint_step (its own iteration predictor)
+    bool adj_warm = false;        // the FP64 transpose solve starts from dv (the previous adjoint step's solution)
     int cg_kernels = 0;           // 0, 1: fused single kernel, 2: apply + vector kernels (FEM_CG_KERNELS)
     int predicted_sweep[16];       // FP32 inner solves, per refinement sweep
     int predicted_sweeps = 3;      // FP32 refinement sweeps queued without a host round trip
@@ -492,8 +495,9 @@ void launch_solver_init(fem_ctx* c, const CgArgs& a, const double* b, const CgSc
     t.r = a.r1;
     t.z = a.z1;
     // FP32 inner solves of the refinement: block-diagonal preconditioner (the outer FP64 test sets the
-    // accuracy); FP64: Eigen's diagonal preconditioner
+    // accuracy); FP64: Eigen's diagonal preconditioner, from zero or (a warm adjoint solve) from the guess in x
     if (a.fp32) launch_cg_init_block(t, b, skip, c->stream);
+    else if (c->adj_warm && c->solving_adjoint) launch_cg_init_guess(t, b, c->stream);
     else launch_cg_init(t, b, skip, c->stream);
 }
 
@@ -515,23 +519,24 @@ int64_t max_iters(fem_ctx* c) {
 // Runs the CG iterations of an initialised solve: launches a predicted number of iterations, then
 // polls the device 'done' flag between chunks. Returns the iteration count.
 int run_cg_iterations(fem_ctx* c, const CgArgs& a, int max_iter) {
+    int& pred = c->solving_adjoint ? c->predicted_adj : c->predicted_iters;
     int k = 0;
     // iteration k = convergence test of k-1 + p update + operator application, then vector updates;
     // the last launch (k == max_iter) only tests.
     auto iteration = [&](int kk) { launch_iteration(c, a, kk, max_iter, kk); };
     // iteration k tests iteration k - 1: a converged solve with iters = i needs launches 0 .. i + 1
-    int first = std::max(1, std::min(c->predicted_iters + 1, max_iter + 1));
+    int first = std::max(1, std::min(pred + 1, max_iter + 1));
     for (; k < first; ++k) iteration(k);
     for (;;) {
         HIPCHECK(hipMemcpyAsync(c->h_sc, a.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         if (c->h_sc->done || k > max_iter) break;
-        int chunk = std::min(std::max(2, c->predicted_iters / 8), max_iter + 1 - k);
+        int chunk = std::min(std::max(2, pred / 8), max_iter + 1 - k);
         for (int j = 0; j < chunk; ++j, ++k) iteration(k);
     }
     int it = c->h_sc->iters;
     if (c->timer.enabled) c->timer.resolve(it);
-    c->predicted_iters = it + 1;
+    pred = it + 1;
     return it;
 }
 
@@ -2338,41 +2343,81 @@ int fem_get_state_device(fem_ctx* c, double* x, double* v) {
     });
 }
 
+namespace {
+
+int step_checks(fem_ctx* c, int32_t nsteps) {
+    if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "Initial state has not been set yet");
+    if (nsteps < 0) throw InvalidArg("negative step count");
+    if (!c->stats.active) return FEM_INACTIVE;
+    HIPCHECK(hipSetDevice(c->device));
+    ensure_bc(c);
+    alloc_precision_buffers(c);
+    if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->num_instances > 0)
+        throw InvalidArg("the static, explicit and Runge-Kutta schemes step one representation");
+    if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
+        throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
+    if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
+        c->opt.rayleigh_stiffness != 0.0)
+        throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
+    return FEM_OK;
+}
+
+// State after step i of a call into the history (device, rows of ndof), i = 0 for the call's start.
+void record_state(fem_ctx* c, double* x_hist, double* v_hist, int i) {
+    if (!x_hist) return;
+    const size_t n = (size_t)c->ndof;
+    HIPCHECK(hipMemcpyAsync(x_hist + i * n, c->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(v_hist + i * n, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Element flag counts and the invalid-state flag of the call (read into h_counters by the caller).
+int finish_steps(fem_ctx* c) {
+    c->timer.resolve();
+    c->timer.enabled = false;
+    c->stats.det_r_invalid = c->h_counters[0];
+    c->stats.g_singular = c->h_counters[1];
+    c->stats.fp64_polar_fallbacks = c->h_counters[2];
+    if (c->h_counters[3]) {
+        c->stats.active = 0;
+        return FEM_INACTIVE;
+    }
+    return FEM_OK;
+}
+
+// The synchronous step loop: the host reads each FP64 solve's outcome (each FP32 refinement's per step) before
+// the next one goes out.
+int step_loop(fem_ctx* c, int32_t nsteps, double* x_hist, double* v_hist) {
+    record_state(c, x_hist, v_hist, 0);
+    for (int i = 0; i < nsteps; ++i) {
+        c->timer.enabled = c->timer_every > 0 && (c->timer_count++ % c->timer_every) == 0;
+        prepare_external(c);
+        step_once(c);
+        clear_external(c);
+        record_state(c, x_hist, v_hist, i + 1);
+        if (c->opt.fail_on_nonconvergence && !c->stats.converged)
+            return fail(c, FEM_ERR_NOT_CONVERGED, "linear solve did not converge");
+    }
+    HIPCHECK(hipMemcpyAsync(c->h_counters, c->counters.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return finish_steps(c);
+}
+
+}  // namespace
+
 int fem_step(fem_ctx* c, int32_t nsteps) {
     return guarded(c, [&] {
-        if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "Initial state has not been set yet");
-        if (nsteps < 0) throw InvalidArg("negative step count");
-        if (!c->stats.active) return FEM_INACTIVE;
-        HIPCHECK(hipSetDevice(c->device));
-        ensure_bc(c);
-        alloc_precision_buffers(c);
-        if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->num_instances > 0)
-            throw InvalidArg("the static, explicit and Runge-Kutta schemes step one representation");
-        if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
-            throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
-        if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
-            c->opt.rayleigh_stiffness != 0.0)
-            throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
-        for (int i = 0; i < nsteps; ++i) {
-            c->timer.enabled = c->timer_every > 0 && (c->timer_count++ % c->timer_every) == 0;
-            prepare_external(c);
-            step_once(c);
-            clear_external(c);
-            if (c->opt.fail_on_nonconvergence && !c->stats.converged)
-                return fail(c, FEM_ERR_NOT_CONVERGED, "linear solve did not converge");
-        }
-        HIPCHECK(hipMemcpyAsync(c->h_counters, c->counters.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        c->timer.resolve();
-        c->timer.enabled = false;
-        c->stats.det_r_invalid = c->h_counters[0];
-        c->stats.g_singular = c->h_counters[1];
-        c->stats.fp64_polar_fallbacks = c->h_counters[2];
-        if (c->h_counters[3]) {
-            c->stats.active = 0;
-            return FEM_INACTIVE;
-        }
-        return FEM_OK;
+        const int rc = step_checks(c, nsteps);
+        if (rc != FEM_OK) return rc;
+        return step_loop(c, nsteps, nullptr, nullptr);
+    });
+}
+
+int fem_step_record(fem_ctx* c, int32_t nsteps, double* x_hist, double* v_hist) {
+    return guarded(c, [&] {
+        if (!x_hist || !v_hist) throw InvalidArg("null state history");
+        const int rc = step_checks(c, nsteps);
+        if (rc != FEM_OK) return rc;
+        return step_loop(c, nsteps, x_hist, v_hist);
     });
 }
 
@@ -2606,83 +2651,140 @@ int fem_set_young_modulus(fem_ctx* c, const double* young) {
     });
 }
 
+namespace {
+
+int adjoint_checks(fem_ctx* c) {
+    if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "mesh not uploaded");
+    if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("the adjoint covers tetrahedral meshes");
+    if (c->num_instances > 0) throw InvalidArg("the adjoint covers single (non-batched) meshes");
+    if (c->opt.rayleigh_mass != 0.0 || c->opt.rayleigh_stiffness != 0.0 || c->opt.mass_lumping ||
+        scheme_mode(c->opt.scheme) != 0)
+        throw InvalidArg("the adjoint reverses undamped implicit Euler steps with the consistent mass");
+    if (c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
+        (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1))
+        throw InvalidArg("the corotational adjoint reverses one-Newton-iteration implicit Euler steps");
+    HIPCHECK(hipSetDevice(c->device));
+    ensure_bc(c);
+    alloc_precision_buffers(c);
+    return FEM_OK;
+}
+
+// One adjoint step (the transpose of the implicit-Euler step from (x_n, v_n)): the operator at the step's
+// start, S mu = lam_v + dt lam_x, then the element tangents and the gather of the new adjoints.
+void adjoint_step_impl(fem_ctx* c, const double* x_n, const double* v_n, const double* v_next, const double* lam_x,
+                       const double* lam_v, double* lam_x_out, double* lam_v_out, double* grad_young) {
+    const size_t n = (size_t)c->ndof;
+    hipStream_t s = c->stream;
+    // operator S at the step's starting state (element kernel + assembly; the rhs is not needed)
+    HIPCHECK(hipMemcpyAsync(c->xs.p, x_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->vs.p, v_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    {
+        // element matrices evaluated at (x_n, v_n): the kernel's xn/vn are the evaluation point
+        // (the reference's newState), xs/vs the step's start state; both are x_n, v_n here
+        TetElementArgs ea;
+        ea.num_elements = (int)c->num_elements;
+        ea.conn = c->conn4.p;
+        ea.x0 = c->x0.p;
+        ea.xn = c->xs.p;
+        ea.vn = c->vs.p;
+        ea.xs = c->xs.p;
+        ea.vs = c->vs.p;
+        ea.mat = c->mat.p;
+        ea.dt = c->opt.dt;
+        ea.write_matrix = 1;
+        ea.general = 0;
+        ea.rec = c->rec.p;
+        ea.rhs_rec = c->rhs_rec.p;
+        ea.grp_q = c->grp_q.p;
+        ea.q_out = c->q_out.p;
+        ea.q_src = c->q_src.p;
+        ea.src16 = c->src16.p;
+        ea.grp_n = c->grp_n.p;
+        ea.n_out = c->n_out.p;
+        ea.n_src = c->n_src.p;
+        ea.nsrc16 = c->nsrc16.p;
+        ea.num_groups = (int)c->num_groups;
+        ea.flag_counts = c->counters.p;
+        launch_tet_elements(c->kind, ea, s);
+    }
+    assemble_tet(c);
+    // S mu = lv + dt lx (transpose solve on the symmetric operator)
+    launch_adjoint_rhs((int)n, lam_x, lam_v, c->opt.dt, c->b.p, s);
+    c->solving_adjoint = true;  // its own iteration predictor
+    try {
+        linear_solve(c);
+    } catch (...) {
+        c->solving_adjoint = false;
+        throw;
+    }
+    c->solving_adjoint = false;
+    if (!c->adj_gx.p) {
+        c->adj_gx.alloc(12 * (size_t)c->num_elements);
+        c->adj_gv.alloc(12 * (size_t)c->num_elements);
+    }
+    AdjointArgs a;
+    a.num_elements = (int)c->num_elements;
+    a.conn = c->conn4.p;
+    a.x0 = c->x0.p;
+    a.x = c->xs.p;
+    a.v = c->vs.p;
+    a.v_next = v_next;
+    a.mu = c->dv.p;
+    a.bcmask = c->bcmask.p;
+    a.mat = c->mat.p;
+    a.dt = c->opt.dt;
+    a.gx_rec = c->adj_gx.p;
+    a.gv_rec = c->adj_gv.p;
+    a.grad_E = grad_young;
+    launch_adjoint_elements(c->kind, a, s);
+    launch_adjoint_gather((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->adj_gx.p, c->adj_gv.p, lam_x,
+                          c->b.p, lam_x_out, lam_v_out, s);
+    HIPCHECK(hipGetLastError());
+}
+
+// nsteps adjoint steps backwards through a history (rows of ndof: states 0 .. nsteps), lam_x, lam_v updated
+// in place: step j = nsteps - 1 .. 0 reverses the step from state j to state j + 1.
+// The call's first transpose solve starts from zero; each later one from the solution of the step after it
+// (still in dv: nothing between two adjoint solves writes it), which lies close to its own. The result of a call
+// depends only on its arguments.
+void adjoint_loop(fem_ctx* c, int32_t nsteps, const double* x_hist, const double* v_hist, double* lam_x,
+                  double* lam_v, double* grad_young) {
+    const size_t n = (size_t)c->ndof;
+    for (int j = nsteps - 1; j >= 0; --j) {
+        c->adj_warm = j < nsteps - 1;
+        try {
+            adjoint_step_impl(c, x_hist + j * n, v_hist + j * n, v_hist + (j + 1) * n, lam_x, lam_v, lam_x, lam_v,
+                              grad_young);
+        } catch (...) {
+            c->adj_warm = false;
+            throw;
+        }
+    }
+    c->adj_warm = false;
+}
+
+}  // namespace
+
 int fem_adjoint_step(fem_ctx* c, const double* x_n, const double* v_n, const double* v_next, const double* lam_x,
                      const double* lam_v, double* lam_x_out, double* lam_v_out, double* grad_young) {
     return guarded(c, [&] {
-        if (!c->mesh_ready) return fail(c, FEM_ERR_ORDER, "mesh not uploaded");
-        if (c->kind == FEM_ELEMENT_HEX) throw InvalidArg("the adjoint covers tetrahedral meshes");
-        if (c->num_instances > 0) throw InvalidArg("the adjoint covers single (non-batched) meshes");
-        if (c->opt.rayleigh_mass != 0.0 || c->opt.rayleigh_stiffness != 0.0 || c->opt.mass_lumping ||
-            scheme_mode(c->opt.scheme) != 0)
-            throw InvalidArg("the adjoint reverses undamped implicit Euler steps with the consistent mass");
-        if (c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
-            (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1))
-            throw InvalidArg("the corotational adjoint reverses one-Newton-iteration implicit Euler steps");
         if (!x_n || !v_n || !v_next || !lam_x || !lam_v || !lam_x_out || !lam_v_out || !grad_young)
             throw InvalidArg("null device pointer");
-        HIPCHECK(hipSetDevice(c->device));
-        ensure_bc(c);
-        alloc_precision_buffers(c);
-        const size_t n = (size_t)c->ndof;
-        hipStream_t s = c->stream;
-        // operator S at the step's starting state (element kernel + assembly; the rhs is not needed)
-        HIPCHECK(hipMemcpyAsync(c->xs.p, x_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(c->vs.p, v_n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        {
-            // element matrices evaluated at (x_n, v_n): the kernel's xn/vn are the evaluation point
-            // (the reference's newState), xs/vs the step's start state; both are x_n, v_n here
-            TetElementArgs ea;
-            ea.num_elements = (int)c->num_elements;
-            ea.conn = c->conn4.p;
-            ea.x0 = c->x0.p;
-            ea.xn = c->xs.p;
-            ea.vn = c->vs.p;
-            ea.xs = c->xs.p;
-            ea.vs = c->vs.p;
-            ea.mat = c->mat.p;
-            ea.dt = c->opt.dt;
-            ea.write_matrix = 1;
-            ea.general = 0;
-            ea.rec = c->rec.p;
-            ea.rhs_rec = c->rhs_rec.p;
-            ea.grp_q = c->grp_q.p;
-            ea.q_out = c->q_out.p;
-            ea.q_src = c->q_src.p;
-            ea.src16 = c->src16.p;
-            ea.grp_n = c->grp_n.p;
-            ea.n_out = c->n_out.p;
-            ea.n_src = c->n_src.p;
-            ea.nsrc16 = c->nsrc16.p;
-            ea.num_groups = (int)c->num_groups;
-            ea.flag_counts = c->counters.p;
-            launch_tet_elements(c->kind, ea, s);
-        }
-        assemble_tet(c);
-        // S mu = lv + dt lx (transpose solve on the symmetric operator)
-        launch_adjoint_rhs((int)n, lam_x, lam_v, c->opt.dt, c->b.p, s);
-        linear_solve(c);
-        if (!c->adj_gx.p) {
-            c->adj_gx.alloc(12 * (size_t)c->num_elements);
-            c->adj_gv.alloc(12 * (size_t)c->num_elements);
-        }
-        AdjointArgs a;
-        a.num_elements = (int)c->num_elements;
-        a.conn = c->conn4.p;
-        a.x0 = c->x0.p;
-        a.x = c->xs.p;
-        a.v = c->vs.p;
-        a.v_next = v_next;
-        a.mu = c->dv.p;
-        a.bcmask = c->bcmask.p;
-        a.mat = c->mat.p;
-        a.dt = c->opt.dt;
-        a.gx_rec = c->adj_gx.p;
-        a.gv_rec = c->adj_gv.p;
-        a.grad_E = grad_young;
-        launch_adjoint_elements(c->kind, a, s);
-        launch_adjoint_gather((int)c->num_nodes, c->node_ptr.p, c->node_list.p, c->adj_gx.p, c->adj_gv.p, lam_x,
-                              c->b.p, lam_x_out, lam_v_out, s);
-        HIPCHECK(hipGetLastError());
+        const int rc = adjoint_checks(c);
+        if (rc != FEM_OK) return rc;
+        adjoint_step_impl(c, x_n, v_n, v_next, lam_x, lam_v, lam_x_out, lam_v_out, grad_young);
+        return FEM_OK;
+    });
+}
+
+int fem_adjoint_steps(fem_ctx* c, int32_t nsteps, const double* x_hist, const double* v_hist, double* lam_x,
+                      double* lam_v, double* grad_young) {
+    return guarded(c, [&] {
+        if (!x_hist || !v_hist || !lam_x || !lam_v || !grad_young) throw InvalidArg("null device pointer");
+        if (nsteps < 0) throw InvalidArg("negative step count");
+        const int rc = adjoint_checks(c);
+        if (rc != FEM_OK) return rc;
+        adjoint_loop(c, nsteps, x_hist, v_hist, lam_x, lam_v, grad_young);
         return FEM_OK;
     });
 }

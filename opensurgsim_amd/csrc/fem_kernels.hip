@@ -531,6 +531,84 @@ __global__ __launch_bounds__(kCgThreads) void cg_init_kernel(Operator op, const 
     }
 }
 
+// FP64 start from the guess in x (Eigen's solveWithGuess): r = b - S x, dinv as cg_init_kernel, z = dinv r;
+// partial |b|^2, r.z and |r|^2 (slots 0, 1, 2 of the partials).
+__global__ __launch_bounds__(kCgThreads) void cg_init_guess_kernel(Operator op, const double* __restrict__ S,
+                                                                   const double* __restrict__ b,
+                                                                   const double* __restrict__ x, double* __restrict__ r,
+                                                                   double* __restrict__ z, double* __restrict__ dinv,
+                                                                   double* __restrict__ partial) {
+    __shared__ double lds[64];
+    double bb = 0.0, rz = 0.0, rr = 0.0;
+    group_spmv<double, double, kRowGroup>(
+        op, S, [&](int J, double& p0, double& p1, double& p2) {
+            p0 = x[3 * J];
+            p1 = x[3 * J + 1];
+            p2 = x[3 * J + 2];
+        },
+        [&](int I, int lane, double y0, double y1, double y2) {
+            if (lane < 3) {
+                const int i = 3 * I + lane;
+                const int db = op.diagblk[I];
+                const double sd = db >= 0 ? S[9 * (int64_t)db + 4 * lane] : 0.0;
+                const double d = sd != 0.0 ? 1.0 / sd : 1.0;
+                const double bi = b[i];
+                const double ri = bi - (lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+                const double zi = d * ri;
+                r[i] = ri;
+                z[i] = zi;
+                dinv[i] = d;
+                bb += bi * bi;
+                rz += ri * zi;
+                rr += ri * ri;
+            }
+        });
+    block_sum2(bb, rz, lds);  // ends with a barrier
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bb;
+        partial[kCgBlocks + blockIdx.x] = rz;
+    }
+    __syncthreads();
+    const double srr = block_sum(rr, lds);
+    if (threadIdx.x == 0) partial[2 * kCgBlocks + blockIdx.x] = srr;
+}
+
+// Scalars of a solve started from a guess: threshold max(tol^2 |b|^2, realmin); Eigen's exit before the first
+// iteration when |r|^2 is already below it (err = |r| / |b|), or when b = 0 (x = 0).
+__global__ __launch_bounds__(kCgThreads) void cg_init2_guess_kernel(const double* __restrict__ partial, CgScalars* sc,
+                                                                    double tol, double* __restrict__ x, int n) {
+    __shared__ double lds[64];
+    __shared__ int exit_zero;
+    double bb, rz;
+    reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
+    __syncthreads();
+    const double rr = reduce_partials(partial + 2 * kCgBlocks, lds);
+    if (threadIdx.x == 0) {
+        sc->bb = bb;
+        sc->rz[0] = rz;
+        sc->rz[1] = 0.0;
+        sc->threshold = fmax(tol * tol * bb, DBL_MIN);
+        sc->rr = rr;
+        sc->iters = 0;
+        sc->err = 0.0;
+        sc->converged = 0;
+        sc->done = 0;
+        exit_zero = 0;
+        if (bb == 0.0) {
+            sc->done = 1;
+            sc->converged = 1;
+            exit_zero = 1;
+        } else if (rr < sc->threshold) {
+            sc->done = 1;
+            sc->converged = 1;
+            sc->err = sqrt(rr / bb);
+        }
+    }
+    __syncthreads();
+    if (exit_zero)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.0;
+}
+
 // |b|^2 and r.z -> threshold max(tol^2 |b|^2, realmin) and Eigen's early exits.
 __global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __restrict__ partial, CgScalars* sc,
                                                               const CgScalars* __restrict__ skip, double tol) {
@@ -972,6 +1050,12 @@ void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hip
         launch_k((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
                            (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial, skip);
     launch_k(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
+}
+
+void launch_cg_init_guess(const CgArgs& a, const double* b, hipStream_t s) {
+    launch_k(cg_init_guess_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.op, (const double*)a.S, b,
+             (const double*)a.x, (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial);
+    launch_k(cg_init2_guess_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, a.sc, a.tol, (double*)a.x, a.n);
 }
 
 void launch_cg_init_block(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {

@@ -166,6 +166,9 @@ constexpr int kFusedSums = 7;
 void launch_cg_fused(const CgArgs& a, int k, int max_iter, hipStream_t s);
 // x = 0, r = (TV) b, z = dinv r and the solve's scalars; skipped when skip && skip->done.
 void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);
+// FP64: the same from the guess a.x (not zeroed): r = b - S x, z = dinv r; the solve's scalars as Eigen's
+// solveWithGuess sets them (the tolerance relative to |b|, no iteration when |r| is already below it).
+void launch_cg_init_guess(const CgArgs& a, const double* b, hipStream_t s);
 // The same for the FP32 one-kernel iteration with the 3x3 block-diagonal preconditioner: a.binv gets
 // blockdiag(S)^-1 (9 floats per node), z = binv r.
 void launch_cg_init_block(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s);

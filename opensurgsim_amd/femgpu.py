@@ -33,7 +33,8 @@ EXPORTS = [
     "fem_get_state_device", "fem_step", "fem_synchronize", "fem_query", "fem_num_nonzeros", "fem_get_pattern",
     "fem_get_system_matrix", "fem_get_rhs", "fem_evaluate_elements", "fem_enable_phase_timers",
     "fem_get_phase_times", "fem_probe_copy", "fem_time_operator", "fem_get_node_masses",
-    "fem_set_young_modulus", "fem_adjoint_step", "fem_set_instances", "fem_query_instances",
+    "fem_set_young_modulus", "fem_adjoint_step", "fem_adjoint_steps", "fem_step_record", "fem_set_instances",
+    "fem_query_instances",
     "fem_set_batch_tuning", "fem_get_batch_tuning", "fem_hex_distinct_matrices",
 ]
 
@@ -149,6 +150,8 @@ def load():
     L.fem_get_batch_tuning.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_int32), _dp]
     L.fem_adjoint_step.argtypes = [_vp] + [_vp] * 8
+    L.fem_adjoint_steps.argtypes = [_vp, ctypes.c_int32] + [_vp] * 5
+    L.fem_step_record.argtypes = [_vp, ctypes.c_int32, _vp, _vp]
     L.fem_time_operator.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.fem_probe_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -340,6 +343,12 @@ class FemGpu:
     def step(self, n=1):
         return self._check(self._L.fem_step(self.h, int(n)), allow_inactive=True)
 
+    def step_record(self, n, x_hist, v_hist):
+        """n steps; device pointers (ints) x_hist, v_hist ((n + 1) x ndof) receive the start and each step's
+        state (fem_step_record)."""
+        return self._check(self._L.fem_step_record(self.h, int(n), _vp(int(x_hist)), _vp(int(v_hist))),
+                           allow_inactive=True)
+
     def hex_distinct_matrices(self):
         """Distinct FP32 element matrices the hexahedral inner solve reads from its table (0: per element)."""
         return int(self._L.fem_hex_distinct_matrices(self.h))
@@ -404,6 +413,12 @@ class FemGpu:
         """Device pointers (ints): reverse of one step; grad_young accumulated (fem_adjoint_step)."""
         args = [_vp(int(p)) for p in (x_n, v_n, v_next, lam_x, lam_v, lam_x_out, lam_v_out, grad_young)]
         self._check(self._L.fem_adjoint_step(self.h, *args))
+
+    def adjoint_steps(self, n, x_hist, v_hist, lam_x, lam_v, grad_young):
+        """Device pointers (ints): n adjoint steps backwards through a recorded history, lam_x, lam_v updated
+        in place, grad_young accumulated (fem_adjoint_steps)."""
+        args = [_vp(int(p)) for p in (x_hist, v_hist, lam_x, lam_v, grad_young)]
+        self._check(self._L.fem_adjoint_steps(self.h, int(n), *args))
 
     def node_masses(self):
         m = np.zeros(self.ndof // 3)
