@@ -200,6 +200,12 @@ struct fem_ctx {
     // host copies needed after set-up
     std::vector<int32_t> h_rowptr_b, h_colidx_b;   // block CSR
     std::vector<double> h_mass_per_node;
+    // OdeEquation::m_initState (Math/OdeEquation.cpp:51-104): F, M, D, K computed at least once (bits 1, 2, 4,
+    // 8), and whether the last global M was lumped (computeFMDK with lumping) or consistent (computeM)
+    int ode_init = 0;
+    bool m_lumped = false;
+    // diag(M) per DOF of the tetrahedral mesh: consistent (element diagonals 2 rho V / 20) and lumped (row sums)
+    DevBuf<double> mdiag_cons, mdiag_lump;
     std::vector<double> h_nu, h_rho;  // per element, for fem_set_young_modulus
     DevBuf<double> adj_gx, adj_gv;     // adjoint per-element records [E][12]
     std::vector<uint8_t> h_bcmask;
@@ -715,6 +721,34 @@ bool scheme_is_linear(int scheme) {
            scheme == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4;
 }
 
+// The Rayleigh mass damping force of computeF (Physics/FemRepresentation.cpp:396-410), which calls
+// addRayleighDampingForce with useGlobalMassMatrix = hasF() (:565-593): once F has been computed, the global M is
+// used, and with lumping only its diagonal: the row sums after an implicit step's computeFMDK (which lumps,
+// :499-508), the consistent diagonal after an explicit or Runge-Kutta step's computeM (which does not lump,
+// :412-421). True when a non-implicit scheme's force takes that diagonal form; otherwise the elements add
+// -alpha_M M_e v_e (the element loop, or the global M v, which is the same sum).
+bool diag_mass_damping(const fem_ctx* c) {
+    return scheme_mode(c->opt.scheme) != 0 && c->opt.mass_lumping && c->opt.rayleigh_mass != 0.0 &&
+           (c->ode_init & 1) != 0;
+}
+
+// OdeEquation::updateFMDK's bookkeeping (Math/OdeEquation.cpp:71-104) after F (and M or K) were formed: an
+// implicit step's computeFMDK sets every bit and leaves M lumped as configured; computeM leaves it consistent.
+void mark_fmdk(fem_ctx* c) {
+    c->ode_init = 15;
+    c->m_lumped = c->opt.mass_lumping != 0;
+}
+void mark_f(fem_ctx* c, bool matrix) {
+    c->ode_init |= 1;
+    if (!matrix) return;
+    if (scheme_mode(c->opt.scheme) == 2) {
+        c->ode_init |= 8;  // computeK
+    } else {
+        c->ode_init |= 2;  // computeM
+        c->m_lumped = false;
+    }
+}
+
 // Element kernel at state (xn, vn) (null: the current state); xs, vs: the step's start state (Newton terms).
 void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs, const double* xn = nullptr,
                   const double* vn = nullptr) {
@@ -730,7 +764,7 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.dt = c->opt.dt;
     a.write_matrix = write_matrix ? 1 : 0;
     a.mode = scheme_mode(c->opt.scheme);
-    a.ray_mass = c->opt.rayleigh_mass;
+    a.ray_mass = diag_mass_damping(c) ? 0.0 : c->opt.rayleigh_mass;  // the diagonal form goes with the rhs
     a.ray_stiff = c->opt.rayleigh_stiffness;
     a.lumped = (a.mode == 0 && c->opt.mass_lumping) ? 1 : 0;
     a.general = (a.mode == 0 && (xs != a.xn || vs != a.vn)) ? 1 : 0;
@@ -763,8 +797,11 @@ void assemble_tet(fem_ctx* c) {
 // localized external rows; (xn, vn, xs, vs) as for tet_elements.
 void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const double* vn = nullptr,
              const double* xs = nullptr, const double* vs = nullptr) {
+    const bool dmd = diag_mass_damping(c);
+    const double* dd = dmd ? (c->m_lumped ? c->mdiag_lump.p : c->mdiag_cons.p) : nullptr;
     TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->rhs_ptr.p, c->rhs_rec.p, c->fg.p,
-                                          c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream, div));
+                                          c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream, div, dd,
+                                          vn ? vn : c->v.p, c->opt.rayleigh_mass));
     if (c->ext_nrows) {
         xn = xn ? xn : c->x.p;
         vn = vn ? vn : c->v.p;
@@ -1301,17 +1338,28 @@ void step_other_scheme_hex(fem_ctx* c) {
     const double kv = c->opt.rayleigh_stiffness, am = c->opt.rayleigh_mass;
     double* x = c->x.p;
     double* v = c->v.p;
+    // the step forms its matrix (computeM or computeK) unless a linear variant keeps the first one
+    const bool forms = !(scheme_is_linear(sc) && c->linear_matrix_ready && c->matrix_dt == dt);
+    auto damp = [&]() -> const double* {  // computeF's diagonal Rayleigh mass form, when it applies
+        return diag_mass_damping(c) ? hex_mass_diagonal(c->hex, c->x0.p, c->mat.p, c->m_lumped, c->stream) : nullptr;
+    };
+    if (scheme_is_linear(sc) && forms) {
+        c->linear_matrix_ready = true;
+        c->matrix_dt = dt;
+    }
     if (sc == FEM_SCHEME_RUNGE_KUTTA_4 || sc == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4) {
         for (DevBuf<double>* b : {&c->rk_x, &c->rk_k2v, &c->rk_k3v, &c->rk_k4v, &c->rk_a1, &c->rk_a2, &c->rk_a3})
             if (b->n != (size_t)n) b->alloc(n);
         const size_t bytes = (size_t)n * sizeof(double);
         // k1 = (v, M^-1 F(x, v)): f / dt against M / dt
-        hex_rhs(c->hex, a, x, v, kv, am, 0, dt, c->stream);
+        hex_rhs(c->hex, a, x, v, kv, am, 0, dt, c->stream, damp());
+        mark_f(c, forms);
         hex_solve(c->hex, a, c->stats, c->stream);
         HIPCHECK(hipMemcpyAsync(c->rk_a1.p, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         auto stage = [&](const double* kvs, const double* kas, double div, double* vstage, double* aout) {
             launch_rk4_stage(n, x, v, kvs, kas, dt, div, c->rk_x.p, vstage, c->stream);
-            hex_rhs(c->hex, a, c->rk_x.p, vstage, kv, am, 0, dt, c->stream);
+            hex_rhs(c->hex, a, c->rk_x.p, vstage, kv, am, 0, dt, c->stream, damp());
+            mark_f(c, false);
             hex_solve(c->hex, a, c->stats, c->stream);
             if (aout) HIPCHECK(hipMemcpyAsync(aout, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         };
@@ -1323,7 +1371,8 @@ void step_other_scheme_hex(fem_ctx* c) {
                                c->dv.p, dt, c->counters.p + 3, c->stream));
         return;
     }
-    hex_rhs(c->hex, a, x, v, kv, am, 0, 1.0, c->stream);
+    hex_rhs(c->hex, a, x, v, kv, am, 0, 1.0, c->stream, damp());
+    mark_f(c, forms);
     hex_solve(c->hex, a, c->stats, c->stream);
     const int umode = stat ? 2 : (sc == FEM_SCHEME_EULER_EXPLICIT || sc == FEM_SCHEME_LINEAR_EULER_EXPLICIT) ? 1 : 0;
     TIMED(c, FEM_PHASE_UPDATE, 0, launch_update_scheme(n, umode, c->dv.p, x, x, v, dt, c->counters.p + 3, c->stream));
@@ -1346,12 +1395,14 @@ void step_other_scheme(fem_ctx* c) {
         tet_elements(c, !reuse, x, v);
         if (!reuse) assemble_tet(c);
         rhs_tet(c, dt);
+        mark_f(c, !reuse);
         linear_solve(c);
         HIPCHECK(hipMemcpyAsync(c->rk_a1.p, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         auto stage = [&](const double* kv, const double* ka, double div, double* vstage, double* aout) {
             launch_rk4_stage(n, x, v, kv, ka, dt, div, c->rk_x.p, vstage, c->stream);
             tet_elements(c, false, c->rk_x.p, vstage, c->rk_x.p, vstage);
             rhs_tet(c, dt, c->rk_x.p, vstage);
+            mark_f(c, false);
             linear_solve(c);
             if (aout) HIPCHECK(hipMemcpyAsync(aout, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         };
@@ -1365,6 +1416,7 @@ void step_other_scheme(fem_ctx* c) {
         tet_elements(c, !reuse, x, v);
         if (!reuse) assemble_tet(c);
         rhs_tet(c);
+        mark_f(c, !reuse);
         linear_solve(c);
         const int umode = (sc == FEM_SCHEME_STATIC || sc == FEM_SCHEME_LINEAR_STATIC) ? 2
                           : (sc == FEM_SCHEME_EULER_EXPLICIT || sc == FEM_SCHEME_LINEAR_EULER_EXPLICIT) ? 1 : 0;
@@ -1391,16 +1443,19 @@ int step_once(fem_ctx* c) {
     } else if (c->kind == FEM_ELEMENT_HEX) {
         HexStepArgs a = hex_args(c);
         hex_step(c->hex, a, c->stats, c->stream);
+        mark_fmdk(c);
     } else if (linear_scheme && c->linear_matrix_ready && c->matrix_dt == dt) {
         // Steps >= 2 of OdeSolverLinearEulerImplicit: element forces only, constant operator.
         tet_elements(c, false, c->x.p, c->v.p);
         rhs_tet(c);
+        mark_f(c, false);
         linear_solve(c);
         state_update(c, c->x.p);
     } else if (newton == 1 || linear_scheme) {
         tet_elements(c, true, c->x.p, c->v.p);
         assemble_tet(c);
         rhs_tet(c);
+        mark_fmdk(c);
         linear_solve(c);
         state_update(c, c->x.p);
         if (linear_scheme) {
@@ -1417,6 +1472,7 @@ int step_once(fem_ctx* c) {
             tet_elements(c, true, c->xs.p, c->vs.p);
             assemble_tet(c);
             rhs_tet(c, 1.0, c->x.p, c->v.p, c->xs.p, c->vs.p);
+            mark_fmdk(c);
             linear_solve(c);
             state_update(c, c->xs.p);
             // ||dv - dv_prev||_inf < epsilon (default 1e-5, Math/OdeSolverEulerImplicit.cpp:26, 101-112), reduced on
@@ -1804,6 +1860,25 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
                 double mass = vol * density[e];
                 for (int a = 0; a < 4; ++a) c->h_mass_per_node[id[a]] += mass / 4;
             }
+            {
+                // diag(M): consistent, the elements' diagonal entries 2 rho V / 20 in element order; lumped, the
+                // row sums rho V / 4 (the node masses)
+                std::vector<double> dc((size_t)3 * num_nodes, 0.0), dl((size_t)3 * num_nodes, 0.0);
+                for (int64_t e = 0; e < num_elements; ++e) {
+                    const int32_t* id = conn + 4 * e;
+                    const double m = tet_volume(x0 + 3 * id[0], x0 + 3 * id[1], x0 + 3 * id[2], x0 + 3 * id[3]) *
+                                     density[e] / 20.0;
+                    for (int a = 0; a < 4; ++a)
+                        for (int k = 0; k < 3; ++k) dc[3 * (size_t)id[a] + k] += 2.0 * m;
+                }
+                for (int64_t i = 0; i < num_nodes; ++i)
+                    for (int k = 0; k < 3; ++k) dl[3 * (size_t)i + k] = c->h_mass_per_node[i];
+                c->mdiag_cons.alloc(dc.size());
+                c->mdiag_cons.upload(dc.data(), dc.size(), c->stream);
+                c->mdiag_lump.alloc(dl.size());
+                c->mdiag_lump.upload(dl.data(), dl.size(), c->stream);
+                HIPCHECK(hipStreamSynchronize(c->stream));  // the host tables are freed on return
+            }
             build_tet_maps(c, conn);
             std::vector<int4> c4(num_elements);
             for (int64_t e = 0; e < num_elements; ++e)
@@ -1840,6 +1915,8 @@ int fem_upload_mesh(fem_ctx* c, int kind, int64_t num_nodes, const double* x0, i
         c->fg_dirty = true;
         HIPCHECK(hipStreamSynchronize(c->stream));
         c->mesh_ready = true;
+        c->ode_init = 0;  // a new representation: no F, M, D, K yet
+        c->m_lumped = false;
         c->last_precision = -1;
         alloc_precision_buffers(c);
         return FEM_OK;
@@ -1879,8 +1956,6 @@ int fem_set_options(fem_ctx* c, const fem_options* o) {
             throw InvalidArg("unknown integration scheme");
         if (!std::isfinite(o->rayleigh_mass) || !std::isfinite(o->rayleigh_stiffness))
             throw InvalidArg("Rayleigh damping coefficients must be finite");
-        if (scheme_mode(o->scheme) != 0 && o->mass_lumping && o->rayleigh_mass != 0.0)
-            throw InvalidArg("mass lumping with Rayleigh mass damping needs an implicit Euler scheme");
         if (o->precision != FEM_PRECISION_FP64 && o->precision != FEM_PRECISION_FP32_REFINED)
             throw InvalidArg("unknown precision");
         if (o->precision == FEM_PRECISION_FP32_REFINED && o->max_refinements < 1)
@@ -2359,6 +2434,14 @@ int step_checks(fem_ctx* c, int32_t nsteps) {
     if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
         c->opt.rayleigh_stiffness != 0.0)
         throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
+    // Once K exists (an implicit or static step formed it), computeF takes the Rayleigh stiffness force from that
+    // global K (addRayleighDampingForce with useGlobalStiffnessMatrix = hasK(), Physics/FemRepresentation.cpp:
+    // 596-612), which the explicit and Runge-Kutta schemes never reform: stale for corotational elements, and not
+    // restated here (their current element K is used).
+    if (scheme_mode(c->opt.scheme) == 1 && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
+        c->opt.rayleigh_stiffness != 0.0 && (c->ode_init & 8) != 0)
+        throw InvalidArg("Rayleigh stiffness damping in an explicit or Runge-Kutta step after an implicit or static "
+                         "one of corotational elements (the reference's stale global K) is not supported");
     return FEM_OK;
 }
 

@@ -196,10 +196,11 @@ __global__ __launch_bounds__(128) void hex_apply64_kernel(int count, const int* 
         for (int c = 0; c < 3; ++c) y[3 * (size_t)ids[i] + c] += sign * out[3 * i + c];
 }
 
-// FP32 cached operator over one colour: q[nodes] += A_e p_e (p masked on BC DOFs), one element per
-// lane; the 36 blocks stream in 9 groups of 4 (36 floats = 9 float4 chunks per group). TABLE: A_e is
-// entry table_idx[t] of the distinct-matrix table, staged in LDS by every workgroup (the same floats and
-// the same arithmetic as the per-position cache).
+// FP32 cached operator over one colour: q[nodes] += A_e p_e, one element per lane. p is zero on BC DOFs without
+// a mask: hex_cg_init_kernel zeroes r there, and the BC rows of the operator are the identity (q = p on them),
+// so r, z and p stay zero on those rows through the solve. The 36 blocks stream in 9 groups of 4 (36 floats =
+// 9 float4 chunks per group). TABLE: A_e is entry table_idx[t] of the distinct-matrix table, staged in LDS by
+// every workgroup (the same floats and the same arithmetic as the per-position cache).
 template <bool TABLE>
 __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const int* __restrict__ order_pad,
                                                           const int* __restrict__ conn,
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int d = 3 * ids[a] + c;
-            pv[3 * a + c] = mask[d] ? 0.f : p[d];
+            pv[3 * a + c] = p[d];  // zero on BC DOFs (the solve starts from r = 0 there; BC rows are the identity)
             y[3 * a + c] = 0.f;
         }
     const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
@@ -353,6 +354,15 @@ __global__ __launch_bounds__(128) void hex_diag_kernel(int count, const int* __r
         for (int c = 0; c < 3; ++c) diag[3 * (size_t)ids[a] + c] += d[3 * a + c];
 }
 
+// b -= alpha diag(M) v per DOF (the Rayleigh mass force with the global lumped form's diagonal,
+// Physics/FemRepresentation.cpp:583-587).
+__global__ __launch_bounds__(256) void hex_diag_damp_kernel(int n, const double* __restrict__ d,
+                                                            const double* __restrict__ v, double alpha,
+                                                            double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] -= alpha * (d[i] * v[i]);
+}
+
 __global__ __launch_bounds__(256) void hex_cast_diag_kernel(int n, const double* __restrict__ d64,
                                                             float* __restrict__ d32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -394,7 +404,7 @@ __global__ __launch_bounds__(kCgThreads) void hex_cg_init_kernel(int n, const do
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const TV s = mask[i] ? TV(1) : diag[i];  // BC rows of the system matrix are the identity
         const TV d = (s != TV(0)) ? TV(1) / s : TV(1);
-        const TV bi = (TV)b[i];
+        const TV bi = mask[i] ? TV(0) : (TV)b[i];  // zero on BC rows: r, z, p and x stay zero there
         x[i] = TV(0);
         r[i] = bi;
         dinv[i] = d;
@@ -581,8 +591,9 @@ double host_hex_volume(const double* X, int64_t e) {
 
 void hex_release(HexData& h) {
     for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.order_pad, (void*)h.A32, (void*)h.diag64, (void*)h.diag32,
-                    (void*)h.table, (void*)h.table_idx})
+                    (void*)h.table, (void*)h.table_idx, (void*)h.mdiag_cons, (void*)h.mdiag_lump})
         if (p) (void)hipFree(p);
+    h.mdiag_cons = h.mdiag_lump = nullptr;
     h.conn = h.order = h.order_pad = nullptr;
     h.table = nullptr;
     h.table_idx = nullptr;
@@ -902,11 +913,26 @@ void hex_set_operator(HexData& h, const HexStepArgs& a, double cm, double ck, in
     if (h.a32_cm != cm || h.a32_ck != ck || h.a32_lumped != lumped) hex_prepare(h, a.x0, a.mat, cm, ck, lumped, s);
 }
 
+const double* hex_mass_diagonal(HexData& h, const double* x0, const double4* mat, bool lumped, hipStream_t s) {
+    double*& d = lumped ? h.mdiag_lump : h.mdiag_cons;
+    if (d) return d;
+    const int n = (int)(3 * h.num_nodes);
+    HCHECK(hipMalloc(&d, (size_t)n * sizeof(double)));
+    HCHECK(hipMemsetAsync(d, 0, (size_t)n * sizeof(double), s));
+    for (int c = 0; c < h.num_colors; ++c) {
+        const int cnt = h.color_ptr[c + 1] - h.color_ptr[c];
+        hipLaunchKernelGGL(hex_diag_kernel, blocks_for(cnt, 128), dim3(128), 0, s, cnt, h.order + h.color_ptr[c],
+                           h.conn, x0, mat, 1.0, 0.0, lumped ? 1 : 0, d);
+    }
+    return d;
+}
+
 void hex_rhs(HexData& h, const HexStepArgs& a, const double* x, const double* v, double kv, double am, int lumped,
-             double div, hipStream_t s) {
+             double div, hipStream_t s, const double* damp_d) {
     const int n = (int)(3 * h.num_nodes);
     hipLaunchKernelGGL(hex_rhs_init_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.fg, a.fext, a.fstep, a.b);
-    apply64(h, a.x0, a.mat, nullptr, x, v, kv, nullptr, am, 1.0, lumped, -1.0, a.b, s);
+    if (damp_d) hipLaunchKernelGGL(hex_diag_damp_kernel, blocks_for(n, 256), dim3(256), 0, s, n, damp_d, v, am, a.b);
+    apply64(h, a.x0, a.mat, nullptr, x, v, kv, nullptr, damp_d ? 0.0 : am, 1.0, lumped, -1.0, a.b, s);
     hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, div, a.b);
 }
 

@@ -37,6 +37,8 @@ struct HexData {
     int* table_idx = nullptr;           // [padded positions] table entry of each position
     int num_distinct = 0;               // 0: A32 per position
     std::vector<int> h_order_pad;       // host copy of order_pad (padding positions are -1)
+    double* mdiag_cons = nullptr;       // diag(M) per DOF, consistent and lumped (row sums); built on first use
+    double* mdiag_lump = nullptr;
     double a32_cm = 0.0, a32_ck = 0.0;  // coefficients of M and K the cached A32 / diagonal were built for
     int a32_lumped = -1;                // lumped mass flag of the cache (-1: not built)
     bool ready = false;
@@ -82,8 +84,11 @@ void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s
 void hex_set_operator(HexData& h, const HexStepArgs& a, double cm, double ck, int lumped, hipStream_t s);
 // a.b = (fg + fext + fstep - K ((x - x0) + kv v) - am M v) / div with boundary rows zero: the force f(x, v) of the
 // schemes (kv = alpha_K, am = alpha_M: the Rayleigh force -D v).
+// With damp_d (diag(M) per DOF): the Rayleigh mass term as -am diag(M) v per DOF instead of -am M v.
 void hex_rhs(HexData& h, const HexStepArgs& a, const double* x, const double* v, double kv, double am, int lumped,
-             double div, hipStream_t s);
+             double div, hipStream_t s, const double* damp_d = nullptr);
+// diag(M) per DOF (lumped: M's row sums), built on the first call.
+const double* hex_mass_diagonal(HexData& h, const double* x0, const double4* mat, bool lumped, hipStream_t s);
 
 // Average milliseconds of one application of the element-by-element operator over every colour (q += A p on
 // the CG scratch vectors p = vec[3], q = vec[4]): the FP32 cached operator (table or per-element cache, as the

@@ -387,15 +387,19 @@ void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* 
 
 // Right-hand side, one lane per DOF: the node's staged element rows in element order, gravity m_i g
 // (Physics/FemRepresentation.cpp:641-662), external force; zero on BC DOFs (Math/OdeState.cpp:159-172).
+// With damp_d: the Rayleigh mass damping force as the global form with a lumped mass takes it,
+// -alpha_M diag(M) v (Physics/FemRepresentation.cpp:583-587), damp_d = diag(M) per DOF.
 __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __restrict__ node_ptr,
                                                   const double* __restrict__ stage, const double* __restrict__ fg,
                                                   const double* __restrict__ fext, const uint8_t* __restrict__ bcmask,
-                                                  double* __restrict__ b, double div) {
+                                                  double* __restrict__ b, double div, const double* __restrict__ damp_d,
+                                                  const double* __restrict__ damp_v, double damp_alpha) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (i >= 3 * (int64_t)num_nodes) return;
     const int I = (int)(i / 3), k = (int)(i - 3 * (int64_t)I);
     const double acc = sum_strided(stage + k, kRhsRec, node_ptr[I], node_ptr[I + 1]);
     double v = acc + fg[i];
+    if (damp_d) v -= damp_alpha * (damp_d[i] * damp_v[i]);
     if (fext) v += fext[i];
     b[i] = bcmask[i] ? 0.0 : v / div;
 }
@@ -444,10 +448,11 @@ void launch_ext_rhs(int rows, const int* row_dof, const int* row_ptr, const int*
 }
 
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
-                const uint8_t* bcmask, double* b, hipStream_t s, double div) {
+                const uint8_t* bcmask, double* b, hipStream_t s, double div, const double* damp_d,
+                const double* damp_v, double damp_alpha) {
     if (num_nodes == 0) return;
     launch_k(rhs_kernel, xcd_grid(3 * (int64_t)num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, stage, fg,
-             fext, bcmask, b, div);
+             fext, bcmask, b, div, damp_d, damp_v, damp_alpha);
 }
 
 // ------------------------------------------------------------------------------------------

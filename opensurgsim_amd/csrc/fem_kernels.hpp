@@ -113,8 +113,11 @@ struct Operator {
 // num_off off-diagonal lower blocks (9 lanes each), then num_diag diagonal ones (6 lanes each).
 void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* ctr_ptr, int nctr_off,
                      const double* stage, double* S64, float* S32, hipStream_t s);
+// With damp_d (diag(M) per DOF): the Rayleigh mass damping force -damp_alpha diag(M) v (v = damp_v) added per DOF
+// instead of by the elements.
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
-                const uint8_t* bcmask, double* b, hipStream_t s, double div = 1.0);
+                const uint8_t* bcmask, double* b, hipStream_t s, double div = 1.0, const double* damp_d = nullptr,
+                const double* damp_v = nullptr, double damp_alpha = 0.0);
 // Localized external stiffness and damping folded into assembled blocks: S[9 blk[e] + k] += add[9 e + k]
 // (the FP32 copy rewritten from the FP64 sum when S32 != null).
 void launch_ext_operator(int count, const int* blk, const double* add, double* S64, float* S32, hipStream_t s);

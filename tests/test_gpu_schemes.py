@@ -106,6 +106,64 @@ def test_explicit_schemes_fp32_refined(scheme):
     assert e["u"] < 1e-8 and e["v"] < 1e-8
 
 
+@pytest.mark.parametrize("scheme", [femgpu.EULER_EXPLICIT, femgpu.EULER_EXPLICIT_MODIFIED, femgpu.RUNGE_KUTTA_4,
+                                    femgpu.LINEAR_EULER_EXPLICIT, femgpu.LINEAR_RUNGE_KUTTA_4, femgpu.STATIC])
+def test_lumped_rayleigh_mass_other_schemes_vs_oracle(scheme):
+    """Mass lumping with Rayleigh mass damping on the non-implicit schemes: the first F takes -alpha_M M v element
+    by element, every later one -alpha_M diag(M) v with the consistent diagonal (computeF's global form,
+    Physics/FemRepresentation.cpp:396-410, 583-587; pinned in tests/test_oracle_schemes.py)."""
+    kind = oracle.TET_LINEAR if scheme in (femgpu.LINEAR_EULER_EXPLICIT, femgpu.LINEAR_RUNGE_KUTTA_4) else \
+        oracle.TET_COROTATIONAL
+    cfg = soft_config(kind)
+    steps = 3 if scheme == femgpu.STATIC else 10
+    say("lumped Rayleigh mass, scheme", scheme)
+    e = run_pair(cfg, steps, scheme, am=AM, ak=AK, lumping=True)
+    say("  rel err", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+@pytest.mark.parametrize("scheme", [femgpu.EULER_EXPLICIT, femgpu.RUNGE_KUTTA_4, femgpu.LINEAR_EULER_EXPLICIT_MODIFIED])
+def test_hex_lumped_rayleigh_mass_other_schemes_vs_oracle(scheme):
+    """The same on hexahedra: diag(M) per DOF from the element quadrature, consistent or the row sums."""
+    cfg = mg.config_hex((4, 3, 4))
+    cfg["E"] = np.full(cfg.conn.shape[0], 5e3)
+    cfg["dt"] = 1e-4
+    say("hex lumped Rayleigh mass, scheme", scheme)
+    e = run_pair(cfg, 8, scheme, am=AM, ak=AK, lumping=True)
+    say("  rel err", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_lumped_rayleigh_mass_after_implicit_steps_vs_oracle():
+    """Implicit steps with lumping, then explicit ones: the first explicit F takes the row sums of the implicit
+    step's lumped M, the later ones the consistent diagonal of computeM. Rayleigh stiffness damping is left out:
+    after an implicit step the reference damps explicit steps with that step's global K, stale for corotational
+    elements, which the GPU rejects (checked at the end)."""
+    cfg = soft_config(oracle.TET_COROTATIONAL)
+    o = make_oracle(cfg, scheme=oracle.EULER_IMPLICIT, tol=1e-12)
+    o.set_damping(AM, 0.0, True)
+    g = make_gpu(cfg, scheme=femgpu.EULER_IMPLICIT, tol=1e-12, rayleigh_mass=AM, mass_lumping=1)
+    for k in range(8):
+        if k == 3:
+            o.set_solver(scheme=oracle.EULER_EXPLICIT, tol=1e-12)
+            g.set_options(dt=cfg.dt, cg_tolerance=1e-12, scheme=femgpu.EULER_EXPLICIT, gravity=cfg.gravity,
+                          rayleigh_mass=AM, mass_lumping=1)
+        assert o.step(cfg.dt, 1) == 0
+        assert g.step(1) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - x0, xr - x0), v=rel(v, vr))
+    say("implicit then explicit, lumped Rayleigh mass", e)
+    assert e["u"] < 1e-9 and e["v"] < 1e-9
+    g.set_options(dt=cfg.dt, cg_tolerance=1e-12, scheme=femgpu.EULER_EXPLICIT, gravity=cfg.gravity,
+                  rayleigh_mass=AM, rayleigh_stiffness=AK, mass_lumping=1)
+    with pytest.raises(femgpu.FemError) as err:
+        g.step(1)
+    assert err.value.code == femgpu.FEM_ERR_INVALID
+    g.close()
+
+
 @pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
 @pytest.mark.parametrize("lumping", [False, True])
 def test_implicit_rayleigh_and_lumping_vs_oracle(precision, lumping):
@@ -214,8 +272,6 @@ def test_unsupported_combinations_rejected():
         gb.step(1)  # batched instances keep the implicit schemes
     gb.close()
     t = make_gpu(soft_config(oracle.TET_LINEAR))
-    with pytest.raises(femgpu.FemError):
-        t.set_options(dt=1e-3, scheme=femgpu.EULER_EXPLICIT, rayleigh_mass=1.0, mass_lumping=1)
     t.add_external_generalized_force(0, np.full(4, 0.25), [0, 0, 1.0], np.eye(3), None)
     t.set_options(dt=1e-3, scheme=femgpu.LINEAR_EULER_IMPLICIT)
     with pytest.raises(femgpu.FemError):

@@ -177,6 +177,42 @@ def test_explicit_ignores_mass_lumping():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+def test_explicit_lumped_rayleigh_mass_uses_the_global_diagonal():
+    """computeF calls addRayleighDampingForce with useGlobalMassMatrix = hasF() (Physics/FemRepresentation.cpp:
+    396-410, 565-593): a representation's first F takes -alpha_M M v element by element; once F exists the global
+    M is used, and with lumping only its diagonal, which is the consistent one after computeM (explicit schemes,
+    :412-421) and the row sums after an implicit step's lumping computeFMDK (:499-508). The force differences of
+    two oracles that differ only in alpha_M, at the same state, against those closed forms (M assembled here from
+    the pinned element matrices)."""
+    x0, tets, bc, K, M, free, u, v = setup(5)
+    x = x0.reshape(-1) + u
+
+    def force_difference(first_scheme, steps_before):
+        fs = []
+        for am in (AM, 0.0):
+            s = make_sim(x0, tets, [], first_scheme, am, 0.0, lumping=True)
+            for _ in range(steps_before):
+                s.set_state(x=x, v=v)
+                s.step(DT, 1)
+            s.set_solver(scheme=oracle.EULER_EXPLICIT, linsolver=oracle.LU)
+            s.set_state(x=x, v=v)
+            s.step(DT, 1)
+            fs.append(s.force())
+        return fs[0] - fs[1]
+
+    scale = AM * np.abs(M @ v).max()
+    # the first F of the representation: -alpha_M M v (consistent, element by element)
+    d = force_difference(oracle.EULER_EXPLICIT, 0)
+    assert np.abs(d - (-AM * (M @ v))).max() <= 1e-12 * scale
+    # after an explicit step (computeM, never lumped): -alpha_M diag(M) v
+    d = force_difference(oracle.EULER_EXPLICIT, 1)
+    assert np.abs(d - (-AM * np.diag(M) * v)).max() <= 1e-12 * scale
+    assert np.abs(d - (-AM * (M @ v))).max() > 1e-3 * scale  # the diagonal form differs from M v
+    # after an implicit step (lumping computeFMDK): -alpha_M rowsum(M) v
+    d = force_difference(oracle.EULER_IMPLICIT, 1)
+    assert np.abs(d - (-AM * M.sum(axis=1) * v)).max() <= 1e-12 * scale
+
+
 def test_rayleigh_terms_and_lumped_mass_assembled():
     x0, tets, bc, K, M, free, u, v = setup(4)
     base = make_sim(x0, tets, [], oracle.EULER_IMPLICIT)
