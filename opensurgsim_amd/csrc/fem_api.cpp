@@ -780,6 +780,10 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.nsrc16 = c->nsrc16.p;
     a.num_groups = (int)c->num_groups;
     a.flag_counts = c->counters.p;
+    if (c->num_instances > 0) {  // each instance's own time step
+        a.elem_inst = c->elem_inst.p;
+        a.inst_dt = c->b_inst_dt.p;
+    }
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
 }
 
@@ -799,9 +803,12 @@ void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const dou
              const double* xs = nullptr, const double* vs = nullptr) {
     const bool dmd = diag_mass_damping(c);
     const double* dd = dmd ? (c->m_lumped ? c->mdiag_lump.p : c->mdiag_cons.p) : nullptr;
+    // batched instances: a divisor other than 1 is the step size (the Runge-Kutta rhs F / dt), each instance's own
+    const bool inst_div = c->num_instances > 0 && div != 1.0;
     TIMED(c, FEM_PHASE_RHS, 0, launch_rhs((int)c->num_nodes, c->rhs_ptr.p, c->rhs_rec.p, c->fg.p,
                                           c->have_fext ? c->fext.p : nullptr, c->bcmask.p, c->b.p, c->stream, div, dd,
-                                          vn ? vn : c->v.p, c->opt.rayleigh_mass));
+                                          vn ? vn : c->v.p, c->opt.rayleigh_mass, inst_div ? c->node_inst.p : nullptr,
+                                          inst_div ? c->b_inst_dt.p : nullptr));
     if (c->ext_nrows) {
         xn = xn ? xn : c->x.p;
         vn = vn ? vn : c->v.p;
@@ -809,7 +816,7 @@ void rhs_tet(fem_ctx* c, double div = 1.0, const double* xn = nullptr, const dou
         vs = vs ? vs : vn;
         const int general = (xs != xn || vs != vn) ? 1 : 0;
         launch_ext_rhs(c->ext_nrows, c->ext_row_dof.p, c->ext_row_ptr.p, c->ext_col_dof.p, c->ext_kval.p,
-                       c->ext_fval.p, c->ext_row_dt.p, xn, xs, vn, general, div, c->b.p, c->stream);
+                       c->ext_fval.p, c->ext_row_dt.p, xn, xs, vn, general, div, c->b.p, c->stream, inst_div ? 1 : 0);
     }
 }
 
@@ -1227,9 +1234,38 @@ void batch_linear_solve(fem_ctx* c) {
     c->stats.last_refinements = sweeps;
 }
 
+// The batched linear solve of a step, with each instance's count of unconverged solves.
+void batch_solve(fem_ctx* c) {
+    batch_linear_solve(c);
+    for (int i = 0; i < c->num_instances; ++i)
+        if (c->h_inst_active[i] && !c->h_inst_conv[i]) c->h_inst_unconv[i]++;
+}
+
+// After a batched update: instances whose new state is not finite are deactivated
+// (Physics/FemRepresentation.cpp:285-293), the others go on; the batch is inactive only when every instance is.
+void batch_deactivate(fem_ctx* c) {
+    std::vector<int> inv(c->num_instances);
+    HIPCHECK(hipMemcpyAsync(inv.data(), c->b_invalid.p, inv.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    bool changed = false, any = false;
+    for (int i = 0; i < c->num_instances; ++i) {
+        if (inv[i] && c->h_inst_active[i]) {
+            c->h_inst_active[i] = 0;
+            changed = true;
+        }
+        any = any || c->h_inst_active[i];
+    }
+    if (changed) c->b_active.upload(c->h_inst_active.data(), c->num_instances, c->stream);
+    if (!any) HIPCHECK(hipMemsetAsync(c->counters.p + 3, 0xff, sizeof(int), c->stream));
+}
+
+// One implicit Euler step of every instance (one Newton iteration); the linear scheme keeps the first step's
+// operator (Math/OdeSolverLinearEulerImplicit.cpp:43-64).
 int batch_step(fem_ctx* c) {
-    if (c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT || c->opt.newton_max_iterations != 1)
+    const bool linear = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
+    if ((c->opt.scheme != FEM_SCHEME_EULER_IMPLICIT && !linear) || c->opt.newton_max_iterations != 1)
         throw InvalidArg("batched instances step with implicit Euler, one Newton iteration");
+    const bool reuse = linear && c->linear_matrix_ready && c->matrix_dt == c->opt.dt;
     refresh_batch_params(c);
     TetElementArgs a;
     a.num_elements = (int)c->num_elements;
@@ -1241,7 +1277,7 @@ int batch_step(fem_ctx* c) {
     a.vs = c->v.p;
     a.mat = c->mat.p;
     a.dt = c->opt.dt;
-    a.write_matrix = 1;
+    a.write_matrix = reuse ? 0 : 1;
     a.general = 0;
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
@@ -1261,29 +1297,22 @@ int batch_step(fem_ctx* c) {
     a.ray_stiff = c->opt.rayleigh_stiffness;
     a.lumped = c->opt.mass_lumping ? 1 : 0;
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
-    assemble_tet(c);
+    if (!reuse) assemble_tet(c);
     rhs_tet(c);
-    batch_linear_solve(c);
-    for (int i = 0; i < c->num_instances; ++i)
-        if (c->h_inst_active[i] && !c->h_inst_conv[i]) c->h_inst_unconv[i]++;
+    if (reuse) {
+        mark_f(c, false);
+    } else {
+        mark_fmdk(c);
+    }
+    batch_solve(c);
     TIMED(c, FEM_PHASE_UPDATE, 0,
           launch_update((int)c->ndof, c->dv.p, c->x.p, c->x.p, c->v.p, c->opt.dt, c->counters.p + 3, c->stream,
                         c->node_inst.p, c->b_inst_dt.p, c->b_active.p, c->b_invalid.p));
-    // instances whose new state is not finite are deactivated (Physics/FemRepresentation.cpp:285-293), the
-    // others go on; the batch is inactive only when every instance is
-    std::vector<int> inv(c->num_instances);
-    HIPCHECK(hipMemcpyAsync(inv.data(), c->b_invalid.p, inv.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    bool changed = false, any = false;
-    for (int i = 0; i < c->num_instances; ++i) {
-        if (inv[i] && c->h_inst_active[i]) {
-            c->h_inst_active[i] = 0;
-            changed = true;
-        }
-        any = any || c->h_inst_active[i];
+    batch_deactivate(c);
+    if (linear && !reuse) {
+        c->linear_matrix_ready = true;
+        c->matrix_dt = c->opt.dt;
     }
-    if (changed) c->b_active.upload(c->h_inst_active.data(), c->num_instances, c->stream);
-    if (!any) HIPCHECK(hipMemsetAsync(c->counters.p + 3, 0xff, sizeof(int), c->stream));
     c->stats.steps++;
     return FEM_OK;
 }
@@ -1387,6 +1416,17 @@ void step_other_scheme(fem_ctx* c) {
     const bool reuse = linear && c->linear_matrix_ready && c->matrix_dt == dt;
     double* x = c->x.p;
     double* v = c->v.p;
+    // batched instances: every instance steps with its own dt; inactive ones keep their state
+    const bool batched = c->num_instances > 0;
+    if (batched) refresh_batch_params(c);
+    const int* ni = batched ? c->node_inst.p : nullptr;
+    const double* idt = batched ? c->b_inst_dt.p : nullptr;
+    const int* act = batched ? c->b_active.p : nullptr;
+    int* inv = batched ? c->b_invalid.p : nullptr;
+    auto solve = [&]() {
+        if (batched) batch_solve(c);
+        else linear_solve(c);
+    };
     if (sc == FEM_SCHEME_RUNGE_KUTTA_4 || sc == FEM_SCHEME_LINEAR_RUNGE_KUTTA_4) {
         for (DevBuf<double>* b : {&c->rk_x, &c->rk_k2v, &c->rk_k3v, &c->rk_k4v, &c->rk_a1, &c->rk_a2, &c->rk_a3})
             if (b->n != (size_t)n) b->alloc(n);
@@ -1396,14 +1436,14 @@ void step_other_scheme(fem_ctx* c) {
         if (!reuse) assemble_tet(c);
         rhs_tet(c, dt);
         mark_f(c, !reuse);
-        linear_solve(c);
+        solve();
         HIPCHECK(hipMemcpyAsync(c->rk_a1.p, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         auto stage = [&](const double* kv, const double* ka, double div, double* vstage, double* aout) {
-            launch_rk4_stage(n, x, v, kv, ka, dt, div, c->rk_x.p, vstage, c->stream);
+            launch_rk4_stage(n, x, v, kv, ka, dt, div, c->rk_x.p, vstage, c->stream, ni, idt);
             tet_elements(c, false, c->rk_x.p, vstage, c->rk_x.p, vstage);
             rhs_tet(c, dt, c->rk_x.p, vstage);
             mark_f(c, false);
-            linear_solve(c);
+            solve();
             if (aout) HIPCHECK(hipMemcpyAsync(aout, c->dv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
         };
         stage(v, c->rk_a1.p, 2.0, c->rk_k2v.p, c->rk_a2.p);
@@ -1411,29 +1451,30 @@ void step_other_scheme(fem_ctx* c) {
         stage(c->rk_k3v.p, c->rk_a3.p, 1.0, c->rk_k4v.p, nullptr);  // a4 stays in dv
         TIMED(c, FEM_PHASE_UPDATE, 0,
               launch_rk4_final(n, x, v, c->rk_k2v.p, c->rk_k3v.p, c->rk_k4v.p, c->rk_a1.p, c->rk_a2.p, c->rk_a3.p,
-                               c->dv.p, dt, c->counters.p + 3, c->stream));
+                               c->dv.p, dt, c->counters.p + 3, c->stream, ni, idt, act, inv));
     } else {
         tet_elements(c, !reuse, x, v);
         if (!reuse) assemble_tet(c);
         rhs_tet(c);
         mark_f(c, !reuse);
-        linear_solve(c);
+        solve();
         const int umode = (sc == FEM_SCHEME_STATIC || sc == FEM_SCHEME_LINEAR_STATIC) ? 2
                           : (sc == FEM_SCHEME_EULER_EXPLICIT || sc == FEM_SCHEME_LINEAR_EULER_EXPLICIT) ? 1 : 0;
         TIMED(c, FEM_PHASE_UPDATE, 0,
-              launch_update_scheme(n, umode, c->dv.p, x, x, v, dt, c->counters.p + 3, c->stream));
+              launch_update_scheme(n, umode, c->dv.p, x, x, v, dt, c->counters.p + 3, c->stream, ni, idt, act, inv));
     }
     if (linear && !reuse) {
         c->linear_matrix_ready = true;
         c->matrix_dt = dt;
     }
+    if (batched) batch_deactivate(c);
 }
 
 // One step of the configured scheme (OdeSolver::solve with computeCompliance = false). Returns FEM_OK or
 // FEM_INACTIVE.
 int step_once(fem_ctx* c) {
     c->stats.last_newton_iterations = 1;  // the Newton-Raphson loop below sets its own count
-    if (c->num_instances > 0) return batch_step(c);
+    if (c->num_instances > 0 && c->opt.scheme < FEM_SCHEME_STATIC) return batch_step(c);
     const double dt = c->opt.dt;
     const size_t n = (size_t)c->ndof;
     const bool linear_scheme = c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT;
@@ -2427,8 +2468,6 @@ int step_checks(fem_ctx* c, int32_t nsteps) {
     HIPCHECK(hipSetDevice(c->device));
     ensure_bc(c);
     alloc_precision_buffers(c);
-    if (c->opt.scheme > FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->num_instances > 0)
-        throw InvalidArg("the static, explicit and Runge-Kutta schemes step one representation");
     if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU && (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0))
         throw InvalidArg("the direct solver needs the assembled operator of a tetrahedral mesh of one instance");
     if (c->opt.scheme == FEM_SCHEME_LINEAR_EULER_IMPLICIT && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
@@ -2616,9 +2655,6 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
                 if (!(dt[i] > 0.0)) throw InvalidArg("instance time steps must be positive");
         HIPCHECK(hipSetDevice(c->device));
         c->h_node_off.assign(node_offsets, node_offsets + M + 1);
-        auto inst_of = [&](int64_t node) {
-            return (int)(std::upper_bound(c->h_node_off.begin(), c->h_node_off.end(), node) - c->h_node_off.begin()) - 1;
-        };
         std::vector<int> ni(c->num_nodes), ei(c->num_elements);
         for (int i = 0; i < M; ++i)
             for (int64_t a = node_offsets[i]; a < node_offsets[i + 1]; ++a) ni[a] = i;
@@ -2630,8 +2666,8 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
                 throw InvalidArg("element " + std::to_string(e) + " joins two instances");
             ei[e] = i;
         }
-        (void)inst_of;
         c->num_instances = M;
+        c->linear_matrix_ready = false;
         c->elem_inst.alloc(c->num_elements);
         c->elem_inst.upload(ei.data(), c->num_elements, c->stream);
         c->node_inst.alloc(c->num_nodes);

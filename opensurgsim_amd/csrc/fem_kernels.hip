@@ -393,7 +393,9 @@ __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __re
                                                   const double* __restrict__ stage, const double* __restrict__ fg,
                                                   const double* __restrict__ fext, const uint8_t* __restrict__ bcmask,
                                                   double* __restrict__ b, double div, const double* __restrict__ damp_d,
-                                                  const double* __restrict__ damp_v, double damp_alpha) {
+                                                  const double* __restrict__ damp_v, double damp_alpha,
+                                                  const int* __restrict__ node_inst,
+                                                  const double* __restrict__ inst_div) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (i >= 3 * (int64_t)num_nodes) return;
     const int I = (int)(i / 3), k = (int)(i - 3 * (int64_t)I);
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(int num_nodes, const int* __re
     double v = acc + fg[i];
     if (damp_d) v -= damp_alpha * (damp_d[i] * damp_v[i]);
     if (fext) v += fext[i];
-    b[i] = bcmask[i] ? 0.0 : v / div;
+    b[i] = bcmask[i] ? 0.0 : v / (inst_div ? inst_div[node_inst[I]] : div);
 }
 
 __global__ __launch_bounds__(256) void ext_operator_kernel(int count, const int* __restrict__ blk,
@@ -420,7 +422,8 @@ __global__ __launch_bounds__(256) void ext_rhs_kernel(int rows, const int* __res
                                                       const double* __restrict__ kval, const double* __restrict__ fval,
                                                       const double* __restrict__ row_dt, const double* __restrict__ xn,
                                                       const double* __restrict__ xs, const double* __restrict__ vn,
-                                                      int general, double div, double* __restrict__ b) {
+                                                      int general, double div, double* __restrict__ b,
+                                                      int div_by_dt) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const double dt = row_dt[r];
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(256) void ext_rhs_kernel(int rows, const int* __res
         const double w = general ? (xn[j] - xs[j]) - vn[j] * dt : -(vn[j] * dt);
         acc += kval[p] * w;
     }
-    b[row_dof[r]] += (fval[r] + acc) / div;
+    b[row_dof[r]] += (fval[r] + acc) / (div_by_dt ? dt : div);
 }
 
 void launch_ext_operator(int count, const int* blk, const double* add, double* S64, float* S32, hipStream_t s) {
@@ -441,18 +444,18 @@ void launch_ext_operator(int count, const int* blk, const double* add, double* S
 
 void launch_ext_rhs(int rows, const int* row_dof, const int* row_ptr, const int* col_dof, const double* kval,
                     const double* fval, const double* row_dt, const double* xn, const double* xs, const double* vn,
-                    int general, double div, double* b, hipStream_t s) {
+                    int general, double div, double* b, hipStream_t s, int div_by_dt) {
     if (rows <= 0) return;
     launch_k(ext_rhs_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, rows, row_dof, row_ptr, col_dof, kval, fval,
-             row_dt, xn, xs, vn, general, div, b);
+             row_dt, xn, xs, vn, general, div, b, div_by_dt);
 }
 
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
                 const uint8_t* bcmask, double* b, hipStream_t s, double div, const double* damp_d,
-                const double* damp_v, double damp_alpha) {
+                const double* damp_v, double damp_alpha, const int* node_inst, const double* inst_div) {
     if (num_nodes == 0) return;
     launch_k(rhs_kernel, xcd_grid(3 * (int64_t)num_nodes, 256), dim3(256), 0, s, num_nodes, node_ptr, stage, fg,
-             fext, bcmask, b, div, damp_d, damp_v, damp_alpha);
+             fext, bcmask, b, div, damp_d, damp_v, damp_alpha, node_inst, inst_div);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1410,10 +1413,15 @@ void launch_update(int n, const double* dv, const double* xs, double* x, double*
 // implicit and modified explicit update (Math/OdeSolverEulerImplicit.cpp:98-99,
 // Math/OdeSolverEulerExplicitModified.cpp:38-39).
 __global__ __launch_bounds__(256) void update_scheme_kernel(int n, int mode, const double* __restrict__ dv,
-                                                            const double* xs, double* x, double* v, double dt,
-                                                            int* invalid) {
+                                                            const double* xs, double* x, double* v, double dt_all,
+                                                            int* invalid, const int* __restrict__ node_inst,
+                                                            const double* __restrict__ inst_dt,
+                                                            const int* __restrict__ inst_active, int* inst_invalid) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int inst = node_inst ? node_inst[i / 3] : 0;
+    if (inst_active && !inst_active[inst]) return;
+    const double dt = node_inst ? inst_dt[inst] : dt_all;
     double vi, xi;
     if (mode == 1) {
         xi = xs[i] + dt * v[i];
@@ -1427,22 +1435,27 @@ __global__ __launch_bounds__(256) void update_scheme_kernel(int n, int mode, con
     }
     v[i] = vi;
     x[i] = xi;
-    if (!isfinite(vi) || !isfinite(xi)) atomicOr(invalid, 1);
+    if (!isfinite(vi) || !isfinite(xi)) atomicOr(inst_invalid ? &inst_invalid[inst] : invalid, 1);
 }
 
 void launch_update_scheme(int n, int mode, const double* dv, const double* xs, double* x, double* v, double dt,
-                          int* invalid, hipStream_t s) {
+                          int* invalid, hipStream_t s, const int* node_inst, const double* inst_dt,
+                          const int* inst_active, int* inst_invalid) {
     if (n == 0) return;
-    launch_k(update_scheme_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, mode, dv, xs, x, v, dt, invalid);
+    launch_k(update_scheme_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, mode, dv, xs, x, v, dt, invalid,
+             node_inst, inst_dt, inst_active, inst_invalid);
 }
 
-// Math/OdeSolverRungeKutta4.cpp:52-87: stage states y + k dt / 2 and y + k dt.
+// Math/OdeSolverRungeKutta4.cpp:52-87: stage states y + k dt / 2 and y + k dt (dt per instance with node_inst).
 __global__ __launch_bounds__(256) void rk4_stage_kernel(int n, const double* __restrict__ x,
                                                         const double* __restrict__ v, const double* __restrict__ kv,
-                                                        const double* __restrict__ ka, double dt, double div,
-                                                        double* __restrict__ xo, double* __restrict__ vo) {
+                                                        const double* __restrict__ ka, double dt_all, double div,
+                                                        double* __restrict__ xo, double* __restrict__ vo,
+                                                        const int* __restrict__ node_inst,
+                                                        const double* __restrict__ inst_dt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double dt = node_inst ? inst_dt[node_inst[i / 3]] : dt_all;
     xo[i] = x[i] + kv[i] * dt / div;
     vo[i] = v[i] + ka[i] * dt / div;
 }
@@ -1452,28 +1465,36 @@ __global__ __launch_bounds__(256) void rk4_final_kernel(int n, double* __restric
                                                         const double* __restrict__ k2v, const double* __restrict__ k3v,
                                                         const double* __restrict__ k4v, const double* __restrict__ a1,
                                                         const double* __restrict__ a2, const double* __restrict__ a3,
-                                                        const double* __restrict__ a4, double dt, int* invalid) {
+                                                        const double* __restrict__ a4, double dt_all, int* invalid,
+                                                        const int* __restrict__ node_inst,
+                                                        const double* __restrict__ inst_dt,
+                                                        const int* __restrict__ inst_active, int* inst_invalid) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int inst = node_inst ? node_inst[i / 3] : 0;
+    if (inst_active && !inst_active[inst]) return;
+    const double dt = node_inst ? inst_dt[inst] : dt_all;
     const double xi = x[i] + (v[i] + k4v[i] + 2.0 * (k2v[i] + k3v[i])) * dt / 6.0;
     const double vi = v[i] + (a1[i] + a4[i] + 2.0 * (a2[i] + a3[i])) * dt / 6.0;
     x[i] = xi;
     v[i] = vi;
-    if (!isfinite(vi) || !isfinite(xi)) atomicOr(invalid, 1);
+    if (!isfinite(vi) || !isfinite(xi)) atomicOr(inst_invalid ? &inst_invalid[inst] : invalid, 1);
 }
 
 void launch_rk4_stage(int n, const double* x, const double* v, const double* kv, const double* ka, double dt,
-                      double div, double* xo, double* vo, hipStream_t s) {
+                      double div, double* xo, double* vo, hipStream_t s, const int* node_inst, const double* inst_dt) {
     if (n == 0) return;
-    launch_k(rk4_stage_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, v, kv, ka, dt, div, xo, vo);
+    launch_k(rk4_stage_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, v, kv, ka, dt, div, xo, vo, node_inst,
+             inst_dt);
 }
 
 void launch_rk4_final(int n, double* x, double* v, const double* k2v, const double* k3v, const double* k4v,
                       const double* a1, const double* a2, const double* a3, const double* a4, double dt, int* invalid,
-                      hipStream_t s) {
+                      hipStream_t s, const int* node_inst, const double* inst_dt, const int* inst_active,
+                      int* inst_invalid) {
     if (n == 0) return;
     launch_k(rk4_final_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, v, k2v, k3v, k4v, a1, a2, a3, a4, dt,
-             invalid);
+             invalid, node_inst, inst_dt, inst_active, inst_invalid);
 }
 
 // ------------------------------------------------------------------------------------------

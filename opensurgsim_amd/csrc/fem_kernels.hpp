@@ -115,28 +115,35 @@ void launch_assemble(int num_off, int num_diag, const int4* lb_info, const int* 
                      const double* stage, double* S64, float* S32, hipStream_t s);
 // With damp_d (diag(M) per DOF): the Rayleigh mass damping force -damp_alpha diag(M) v (v = damp_v) added per DOF
 // instead of by the elements.
+// With node_inst and inst_div (batched instances), each instance's rows are divided by inst_div of the instance.
 void launch_rhs(int num_nodes, const int* node_ptr, const double* stage, const double* fg, const double* fext,
                 const uint8_t* bcmask, double* b, hipStream_t s, double div = 1.0, const double* damp_d = nullptr,
-                const double* damp_v = nullptr, double damp_alpha = 0.0);
+                const double* damp_v = nullptr, double damp_alpha = 0.0, const int* node_inst = nullptr,
+                const double* inst_div = nullptr);
 // Localized external stiffness and damping folded into assembled blocks: S[9 blk[e] + k] += add[9 e + k]
 // (the FP32 copy rewritten from the FP64 sum when S32 != null).
 void launch_ext_operator(int count, const int* blk, const double* add, double* S64, float* S32, hipStream_t s);
-// Localized external rows of the right-hand side: b[row_dof[r]] += (f[r] + sum_p kval[p] w[col_dof[p]]) / div,
-// w = (xn - xs) - vn row_dt[r] when general, -vn row_dt[r] otherwise.
+// Localized external rows of the right-hand side: b[row_dof[r]] += (f[r] + sum_p kval[p] w[col_dof[p]]) / div
+// (div_by_dt: divided by row_dt[r] instead), w = (xn - xs) - vn row_dt[r] when general, -vn row_dt[r] otherwise.
 void launch_ext_rhs(int rows, const int* row_dof, const int* row_ptr, const int* col_dof, const double* kval,
                     const double* fval, const double* row_dt, const double* xn, const double* xs, const double* vn,
-                    int general, double div, double* b, hipStream_t s);
+                    int general, double div, double* b, hipStream_t s, int div_by_dt = 0);
 // State update by scheme: 0 (implicit Euler, modified explicit Euler): v += dv, x = xs + dt v;
-// 1 (explicit Euler): x = xs + dt v, v += dv; 2 (static): x = xs + dv, v = 0.
+// 1 (explicit Euler): x = xs + dt v, v += dv; 2 (static): x = xs + dv, v = 0. With node_inst (batched instances):
+// dt per instance, inactive instances left as they are, non-finite states flagged per instance.
 void launch_update_scheme(int n, int mode, const double* dv, const double* xs, double* x, double* v, double dt,
-                          int* invalid, hipStream_t s);
-// Runge-Kutta 4 stage state: xo = x + kv dt / div, vo = v + ka dt / div.
+                          int* invalid, hipStream_t s, const int* node_inst = nullptr, const double* inst_dt = nullptr,
+                          const int* inst_active = nullptr, int* inst_invalid = nullptr);
+// Runge-Kutta 4 stage state: xo = x + kv dt / div, vo = v + ka dt / div (dt per instance with node_inst).
 void launch_rk4_stage(int n, const double* x, const double* v, const double* kv, const double* ka, double dt,
-                      double div, double* xo, double* vo, hipStream_t s);
-// Runge-Kutta 4 update: x += (v + k4v + 2 (k2v + k3v)) dt / 6, v += (a1 + a4 + 2 (a2 + a3)) dt / 6.
+                      double div, double* xo, double* vo, hipStream_t s, const int* node_inst = nullptr,
+                      const double* inst_dt = nullptr);
+// Runge-Kutta 4 update: x += (v + k4v + 2 (k2v + k3v)) dt / 6, v += (a1 + a4 + 2 (a2 + a3)) dt / 6; batched
+// instances as launch_update_scheme.
 void launch_rk4_final(int n, double* x, double* v, const double* k2v, const double* k3v, const double* k4v,
                       const double* a1, const double* a2, const double* a3, const double* a4, double dt, int* invalid,
-                      hipStream_t s);
+                      hipStream_t s, const int* node_inst = nullptr, const double* inst_dt = nullptr,
+                      const int* inst_active = nullptr, int* inst_invalid = nullptr);
 
 // Jacobi-preconditioned CG (Eigen recurrence) on the operator: one fused kernel per iteration, or the
 // operator kernel + vector kernel pair.

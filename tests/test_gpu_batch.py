@@ -177,3 +177,113 @@ def test_non_finite_instance_is_deactivated_alone():
         (xo, vo), _ = run_oracle(cfgs[i], steps, tol=1e-12)
         sl = slice(3 * off[i], 3 * off[i + 1])
         assert rel(v[sl], vo) < 1e-9
+
+
+AM, AK = 2.0, 1e-4
+SCHEMES = [(femgpu.STATIC, 2), (femgpu.EULER_EXPLICIT, 8), (femgpu.EULER_EXPLICIT_MODIFIED, 8),
+           (femgpu.RUNGE_KUTTA_4, 6), (femgpu.LINEAR_STATIC, 3), (femgpu.LINEAR_EULER_EXPLICIT, 8),
+           (femgpu.LINEAR_EULER_EXPLICIT_MODIFIED, 8), (femgpu.LINEAR_RUNGE_KUTTA_4, 6),
+           (femgpu.LINEAR_EULER_IMPLICIT, 6)]
+LINEAR_SCHEMES = {femgpu.LINEAR_STATIC, femgpu.LINEAR_EULER_EXPLICIT, femgpu.LINEAR_EULER_EXPLICIT_MODIFIED,
+                  femgpu.LINEAR_RUNGE_KUTTA_4, femgpu.LINEAR_EULER_IMPLICIT}
+
+
+def soft_batch(count, seed, kind):
+    """Soft instances (E 2-8 kPa) with time steps of 0.1, 0.2 and 0.4 ms, so that the explicit schemes stay
+    bounded over the steps, and a tangential velocity on one face of each."""
+    cfgs = mg.config_batched(count, seed=seed, lo=2, hi=4)
+    for i, c in enumerate(cfgs):
+        c["kind"] = kind
+        c["E"] = np.full(c.conn.shape[0], 2e3 * (1 + i % 4))
+        c["dt"] = [1e-4, 2e-4, 4e-4][i % 3]
+        top = mg.face_nodes(c.nx, c.ny, c.nz, 1, 1)
+        v0 = np.zeros((c.x0.shape[0], 3))
+        v0[top, 0] = 0.02 * (i + 1)
+        c["v0"] = v0.reshape(-1)
+        rng = np.random.default_rng(seed + i)
+        c["fext"] = 0.01 * rng.normal(size=c.x0.size)
+    return cfgs, mg.concat_instances(cfgs)
+
+
+@pytest.mark.parametrize("lumping", [False, True])
+@pytest.mark.parametrize("scheme,steps", SCHEMES)
+def test_batch_other_schemes_per_instance(scheme, steps, lumping):
+    """The static, explicit, modified explicit and Runge-Kutta 4 schemes, their linear variants and linear
+    implicit Euler on batched instances, each instance with its own time step, Rayleigh damping and a
+    constant external force: every instance against the oracle stepped alone with that scheme
+    (Math/OdeSolver*.cpp). The Runge-Kutta right-hand side F / dt and the explicit updates take each
+    instance's dt."""
+    kind = oracle.TET_LINEAR if scheme in LINEAR_SCHEMES else oracle.TET_COROTATIONAL
+    cfgs, b = soft_batch(4, seed=29, kind=kind)
+    say("batched scheme", scheme, "lumping", lumping, "dts", list(b.dt))
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    g.add_bc_nodes(b.bc_nodes)
+    g.set_options(dt=1e-3, cg_tolerance=1e-12, scheme=scheme, rayleigh_mass=AM, rayleigh_stiffness=AK,
+                  mass_lumping=int(lumping))
+    g.set_external_force(b.fext)
+    g.set_state(v=b.v0)
+    g.set_instances(b.node_offsets, b.dt)
+    for _ in range(steps):
+        assert g.step(1) == femgpu.FEM_OK
+    x, v = g.state()
+    g.instance_stats()
+    assert g.last_instance_active.tolist() == [1] * len(cfgs)
+    g.close()
+    off = b.node_offsets
+    for i, c in enumerate(cfgs):
+        s = oracle.Sim(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
+        for n in c.bc_nodes:
+            s.add_bc_node(n)
+        s.set_solver(scheme=scheme, tol=1e-12)
+        s.set_damping(AM, AK, lumping)
+        s.set_external_force(c.fext)
+        s.set_state(v=c.v0)
+        for _ in range(steps):
+            assert s.step(c.dt, 1) == 0
+        xo, vo = s.state()
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        x0 = c.x0.reshape(-1)
+        e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo) if np.linalg.norm(vo) > 0 else np.linalg.norm(v[sl]))
+        say("  instance", i, "dt", c.dt, e)
+        assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_batch_linear_scheme_keeps_operator_until_instances_change():
+    """Linear implicit Euler on batched instances keeps the first step's operator; declaring the instances
+    again (new time steps) forms it anew, so the next steps match the oracle at the new time steps."""
+    cfgs, b = soft_batch(3, seed=41, kind=oracle.TET_LINEAR)
+    g = femgpu.FemGpu(0)
+    g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+    g.add_bc_nodes(b.bc_nodes)
+    g.set_options(dt=1e-3, cg_tolerance=1e-12, scheme=femgpu.LINEAR_EULER_IMPLICIT)
+    g.set_state(v=b.v0)
+    g.set_instances(b.node_offsets, b.dt)
+    assert g.step(3) == femgpu.FEM_OK
+    dt2 = b.dt * 2.0
+    g.set_instances(b.node_offsets, dt2)
+    assert g.step(3) == femgpu.FEM_OK
+    x, v = g.state()
+    g.close()
+    off = b.node_offsets
+    for i, c in enumerate(cfgs):
+        s = oracle.Sim(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
+        for n in c.bc_nodes:
+            s.add_bc_node(n)
+        s.set_solver(scheme=femgpu.LINEAR_EULER_IMPLICIT, tol=1e-12)
+        s.set_state(v=c.v0)
+        s.step(c.dt, 3)
+        # the oracle's linear solver keeps its matrix; a fresh one at the new dt continues from the state
+        xo, vo = s.state()
+        s2 = oracle.Sim(c.kind, c.x0, c.conn, c.E, c.nu, c.rho)
+        for n in c.bc_nodes:
+            s2.add_bc_node(n)
+        s2.set_solver(scheme=femgpu.LINEAR_EULER_IMPLICIT, tol=1e-12)
+        s2.set_state(x=xo, v=vo)
+        s2.step(dt2[i], 3)
+        xo, vo = s2.state()
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        x0 = c.x0.reshape(-1)
+        e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo))
+        say("  instance", i, "dt", c.dt, "then", dt2[i], e)
+        assert e["u"] < 1e-9 and e["v"] < 1e-9
