@@ -262,6 +262,14 @@ struct fem_ctx {
     DevBuf<double> ext_add, ext_kval, ext_fval, ext_row_dt, fstep;
     int ext_nblk = 0, ext_nrows = 0;
     bool fstep_set = false;
+    // hexahedra: the localized external operator rows of the last step (kept for compliance after it)
+    DevBuf<int> hx_row, hx_ptr, hx_col;
+    DevBuf<double> hx_val;
+    DevBuf<float> hx_val32;
+    int hx_rows = 0;
+    bool ext_has_k = false;  // this step's localized terms include a stiffness
+    bool k_has_ext = false;  // the last global K formed (implicit or static step) included one
+    bool v_static_zero = false;  // v was zeroed by a static step and not set since
 
     // Runge-Kutta 4 stage vectors
     DevBuf<double> rk_x, rk_k2v, rk_k3v, rk_k4v, rk_a1, rk_a2, rk_a3;
@@ -737,12 +745,14 @@ bool diag_mass_damping(const fem_ctx* c) {
 void mark_fmdk(fem_ctx* c) {
     c->ode_init = 15;
     c->m_lumped = c->opt.mass_lumping != 0;
+    c->k_has_ext = c->ext_has_k;
 }
 void mark_f(fem_ctx* c, bool matrix) {
     c->ode_init |= 1;
     if (!matrix) return;
     if (scheme_mode(c->opt.scheme) == 2) {
         c->ode_init |= 8;  // computeK
+        c->k_has_ext = c->ext_has_k;
     } else {
         c->ode_init |= 2;  // computeM
         c->m_lumped = false;
@@ -824,27 +834,32 @@ void clear_external(fem_ctx* c);
 
 // Device lists of the localized external terms for the next step (Physics/Fem3DRepresentation.cpp:133-198 and
 // their use in computeFMDK / computeF / computeK, Physics/FemRepresentation.cpp:396-410, 460-477, 527-545):
-//  - operator blocks: D + dt K (implicit Euler) or K (static) under Eigen's lower-triangle view (strictly
+//  - operator entries: D + dt K (implicit Euler) or K (static) under Eigen's lower-triangle view (strictly
 //    lower node blocks and their mirrors; the diagonal block's lower part mirrored), boundary rows and
-//    columns left as assembled;
-//  - right-hand-side rows: the force, plus K (xn - xs - vn dt) with the full external K (implicit Euler).
+//    columns left as assembled; tetrahedra add them to the assembled blocks, hexahedra apply them as rows
+//    beside the element-by-element operator;
+//  - right-hand-side rows: the force, plus K (xn - xs - vn dt) with the full external K (implicit Euler);
+//    hexahedra take the force from the dense fstep vector and only the K rows here.
 void prepare_external(fem_ctx* c) {
     c->ext_nblk = c->ext_nrows = 0;
+    c->hx_rows = 0;
     c->fstep_set = false;
+    c->ext_has_k = false;
     if (!c->loc_pending) return;
     const size_t n = (size_t)c->ndof;
+    const bool hex = c->kind == FEM_ELEMENT_HEX;
     if (c->h_loc_f.size() != n) c->h_loc_f.assign(n, 0.0);
-    if (c->kind == FEM_ELEMENT_HEX) {
-        c->fstep.alloc(n);
-        c->fstep.upload(c->h_loc_f.data(), n, c->stream);
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        c->fstep_set = true;
-        return;
-    }
     const int mode = scheme_mode(c->opt.scheme);
     if (!c->h_loc_blk.empty() && scheme_is_linear(c->opt.scheme)) {
         clear_external(c);  // the rejected terms do not linger into later steps
         throw InvalidArg("external stiffness and damping are not supported by the linear (constant-operator) schemes");
+    }
+    for (const auto& kv : c->h_loc_blk)
+        for (int k = 0; k < 9; ++k) c->ext_has_k = c->ext_has_k || kv.second[k] != 0.0;
+    if (hex) {
+        c->fstep.alloc(n);
+        c->fstep.upload(c->h_loc_f.data(), n, c->stream);
+        c->fstep_set = true;
     }
     auto node_dt = [&](int64_t node) {
         if (c->num_instances > 0 && !c->h_inst_dt.empty()) {
@@ -854,48 +869,65 @@ void prepare_external(fem_ctx* c) {
         }
         return c->opt.dt;
     };
-    auto block_of = [&](int64_t I, int64_t J) -> int64_t {
-        const int32_t* b = c->h_colidx_b.data() + c->h_rowptr_b[I];
-        const int32_t* e = c->h_colidx_b.data() + c->h_rowptr_b[I + 1];
-        const int32_t* it = std::lower_bound(b, e, (int32_t)J);
-        if (it == e || *it != J) throw InvalidArg("external block outside the mesh pattern");
-        return (int64_t)(it - c->h_colidx_b.data());
-    };
     const uint8_t* bc = c->h_bcmask.data();
-    std::map<int64_t, std::array<double, 9>> add;
-    if (mode != 1) {
+    // operator entries (node I, node J, row r, column q of the 3x3 block, value) under the lower-triangle view
+    auto for_each_operator_entry = [&](auto&& emit) {
+        if (mode == 1) return;  // the explicit schemes' operator is M / dt alone
         for (const auto& kv : c->h_loc_blk) {
             const int64_t I = kv.first.first, J = kv.first.second;
             if (I < J) continue;
             const double dt = node_dt(I);
             double W[9];
             for (int k = 0; k < 9; ++k) W[k] = mode == 0 ? kv.second[9 + k] + dt * kv.second[k] : kv.second[k];
-            if (I > J) {
-                auto& lo = add[block_of(I, J)];
-                auto& up = add[block_of(J, I)];
-                for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q)
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) {
+                    if (I > J) {
                         if (!bc[3 * I + r] && !bc[3 * J + q]) {
-                            lo[3 * r + q] += W[3 * r + q];
-                            up[3 * q + r] += W[3 * r + q];
+                            emit(I, J, r, q, W[3 * r + q]);
+                            emit(J, I, q, r, W[3 * r + q]);
                         }
-            } else {
-                auto& d = add[block_of(I, I)];
-                for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q)
-                        if (!bc[3 * I + r] && !bc[3 * I + q]) d[3 * r + q] += r >= q ? W[3 * r + q] : W[3 * q + r];
-            }
+                    } else if (!bc[3 * I + r] && !bc[3 * I + q]) {
+                        emit(I, I, r, q, r >= q ? W[3 * r + q] : W[3 * q + r]);
+                    }
+                }
         }
-    }
+    };
     std::vector<int> blk;
     std::vector<double> vals;
-    for (const auto& kv : add) {
-        blk.push_back((int)kv.first);
-        vals.insert(vals.end(), kv.second.begin(), kv.second.end());
+    std::vector<int> op_row, op_ptr(1, 0), op_col;
+    std::vector<double> op_val;
+    if (hex) {
+        std::map<std::pair<int64_t, int64_t>, double> ent;
+        for_each_operator_entry([&](int64_t I, int64_t J, int r, int q, double w) { ent[{3 * I + r, 3 * J + q}] += w; });
+        for (const auto& kv : ent) {
+            if (op_row.empty() || op_row.back() != (int)kv.first.first) {
+                if (!op_row.empty()) op_ptr.push_back((int)op_col.size());
+                op_row.push_back((int)kv.first.first);
+            }
+            op_col.push_back((int)kv.first.second);
+            op_val.push_back(kv.second);
+        }
+        if (!op_row.empty()) op_ptr.push_back((int)op_col.size());
+    } else {
+        auto block_of = [&](int64_t I, int64_t J) -> int64_t {
+            const int32_t* b = c->h_colidx_b.data() + c->h_rowptr_b[I];
+            const int32_t* e = c->h_colidx_b.data() + c->h_rowptr_b[I + 1];
+            const int32_t* it = std::lower_bound(b, e, (int32_t)J);
+            if (it == e || *it != J) throw InvalidArg("external block outside the mesh pattern");
+            return (int64_t)(it - c->h_colidx_b.data());
+        };
+        std::map<int64_t, std::array<double, 9>> add;
+        for_each_operator_entry(
+            [&](int64_t I, int64_t J, int r, int q, double w) { add[block_of(I, J)][3 * r + q] += w; });
+        for (const auto& kv : add) {
+            blk.push_back((int)kv.first);
+            vals.insert(vals.end(), kv.second.begin(), kv.second.end());
+        }
     }
     std::map<int64_t, std::vector<std::pair<int, double>>> rows;
-    for (size_t d = 0; d < n; ++d)
-        if (c->h_loc_f[d] != 0.0) rows[(int64_t)d];
+    if (!hex)
+        for (size_t d = 0; d < n; ++d)
+            if (c->h_loc_f[d] != 0.0) rows[(int64_t)d];
     if (mode == 0)
         for (const auto& kv : c->h_loc_blk) {
             const int64_t I = kv.first.first, J = kv.first.second;
@@ -908,7 +940,7 @@ void prepare_external(fem_ctx* c) {
     for (const auto& kv : rows) {
         if (bc[kv.first]) continue;
         row_dof.push_back((int)kv.first);
-        fval.push_back(c->h_loc_f[kv.first]);
+        fval.push_back(hex ? 0.0 : c->h_loc_f[kv.first]);
         row_dt.push_back(node_dt(kv.first / 3));
         for (const auto& e : kv.second) {
             col_dof.push_back(e.first);
@@ -921,6 +953,19 @@ void prepare_external(fem_ctx* c) {
         c->ext_blk.upload(blk.data(), blk.size(), c->stream);
         c->ext_add.alloc(vals.size());
         c->ext_add.upload(vals.data(), vals.size(), c->stream);
+    }
+    std::vector<float> op_val32(op_val.begin(), op_val.end());
+    if (!op_row.empty()) {
+        c->hx_row.alloc(op_row.size());
+        c->hx_row.upload(op_row.data(), op_row.size(), c->stream);
+        c->hx_ptr.alloc(op_ptr.size());
+        c->hx_ptr.upload(op_ptr.data(), op_ptr.size(), c->stream);
+        c->hx_col.alloc(op_col.size());
+        c->hx_col.upload(op_col.data(), op_col.size(), c->stream);
+        c->hx_val.alloc(op_val.size());
+        c->hx_val.upload(op_val.data(), op_val.size(), c->stream);
+        c->hx_val32.alloc(op_val32.size());
+        c->hx_val32.upload(op_val32.data(), op_val32.size(), c->stream);
     }
     if (!row_dof.empty()) {
         c->ext_row_dof.alloc(row_dof.size());
@@ -941,6 +986,7 @@ void prepare_external(fem_ctx* c) {
     HIPCHECK(hipStreamSynchronize(c->stream));  // host tables are freed on return
     c->ext_nblk = (int)blk.size();
     c->ext_nrows = (int)row_dof.size();
+    c->hx_rows = (int)op_row.size();
 }
 
 // DeformableRepresentation::afterUpdate (Physics/DeformableRepresentation.cpp:236-253): the localized
@@ -1326,6 +1372,23 @@ HexStepArgs hex_args(fem_ctx* c) {
     a.fg = c->fg.p;
     a.fext = c->have_fext ? c->fext.p : nullptr;
     a.fstep = c->fstep_set ? c->fstep.p : nullptr;
+    a.ext_rows = c->hx_rows;
+    if (c->hx_rows) {
+        a.ext_row = c->hx_row.p;
+        a.ext_ptr = c->hx_ptr.p;
+        a.ext_col = c->hx_col.p;
+        a.ext_val = c->hx_val.p;
+        a.ext_val32 = c->hx_val32.p;
+    }
+    a.rhs_rows = c->ext_nrows;
+    if (c->ext_nrows) {
+        a.rhs_row = c->ext_row_dof.p;
+        a.rhs_ptr = c->ext_row_ptr.p;
+        a.rhs_col = c->ext_col_dof.p;
+        a.rhs_k = c->ext_kval.p;
+        a.rhs_f = c->ext_fval.p;
+        a.rhs_dt = c->ext_row_dt.p;
+    }
     a.bcmask = c->bcmask.p;
     a.mat = c->mat.p;
     a.b = c->b.p;
@@ -2044,8 +2107,6 @@ int fem_add_external_generalized_force(fem_ctx* c, int64_t element, const double
         if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
         if (!coordinate || !force) throw InvalidArg("Invalid localization (null coordinate or force)");
         if (element < 0 || element >= c->num_elements) throw InvalidArg("Invalid localization (element out of range)");
-        if ((stiffness || damping) && c->kind == FEM_ELEMENT_HEX)
-            throw InvalidArg("external stiffness and damping need a tetrahedral mesh");
         if ((stiffness || damping) && scheme_is_linear(c->opt.scheme))
             throw InvalidArg("external stiffness and damping are not supported by the linear (constant-operator) schemes");
         const int nn = c->nn;
@@ -2420,6 +2481,7 @@ int fem_set_state(fem_ctx* c, const double* x, const double* v) {
         HIPCHECK(hipSetDevice(c->device));
         if (x) c->x.upload(x, (size_t)c->ndof, c->stream);
         if (v) c->v.upload(v, (size_t)c->ndof, c->stream);
+        if (v) c->v_static_zero = false;
         HIPCHECK(hipStreamSynchronize(c->stream));
         return FEM_OK;
     });
@@ -2444,6 +2506,7 @@ int fem_set_state_device(fem_ctx* c, const double* x, const double* v) {
         size_t bytes = (size_t)c->ndof * sizeof(double);
         if (x) HIPCHECK(hipMemcpyAsync(c->x.p, x, bytes, hipMemcpyDeviceToDevice, c->stream));
         if (v) HIPCHECK(hipMemcpyAsync(c->v.p, v, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (v) c->v_static_zero = false;
         return FEM_OK;
     });
 }
@@ -2475,12 +2538,16 @@ int step_checks(fem_ctx* c, int32_t nsteps) {
         throw InvalidArg("Rayleigh stiffness damping with the linear implicit scheme needs linear elements");
     // Once K exists (an implicit or static step formed it), computeF takes the Rayleigh stiffness force from that
     // global K (addRayleighDampingForce with useGlobalStiffnessMatrix = hasK(), Physics/FemRepresentation.cpp:
-    // 596-612), which the explicit and Runge-Kutta schemes never reform: stale for corotational elements, and not
-    // restated here (their current element K is used).
-    if (scheme_mode(c->opt.scheme) == 1 && c->kind == FEM_ELEMENT_TET_COROTATIONAL &&
-        c->opt.rayleigh_stiffness != 0.0 && (c->ode_init & 8) != 0)
-        throw InvalidArg("Rayleigh stiffness damping in an explicit or Runge-Kutta step after an implicit or static "
-                         "one of corotational elements (the reference's stale global K) is not supported");
+    // 596-612), which the explicit and Runge-Kutta schemes never reform and the static scheme reforms only after
+    // F (Math/OdeEquation.cpp:78-103): stale for corotational elements and holding the localized stiffness of the
+    // step that formed it, neither restated here (the current element K is used). A static step after a static
+    // one has v = 0 and no such force.
+    const int mode = scheme_mode(c->opt.scheme);
+    if (mode != 0 && c->opt.rayleigh_stiffness != 0.0 && (c->ode_init & 8) != 0 &&
+        (c->kind == FEM_ELEMENT_TET_COROTATIONAL || c->k_has_ext) && !(mode == 2 && c->v_static_zero))
+        throw InvalidArg("Rayleigh stiffness damping in an explicit, Runge-Kutta or static step after a step whose "
+                         "global K differs from the current element K (corotational elements, or a localized "
+                         "stiffness; the reference's stale global K) is not supported");
     return FEM_OK;
 }
 
@@ -2514,6 +2581,7 @@ int step_loop(fem_ctx* c, int32_t nsteps, double* x_hist, double* v_hist) {
         c->timer.enabled = c->timer_every > 0 && (c->timer_count++ % c->timer_every) == 0;
         prepare_external(c);
         step_once(c);
+        c->v_static_zero = scheme_mode(c->opt.scheme) == 2;  // OdeSolverStatic zeroes the velocities
         clear_external(c);
         record_state(c, x_hist, v_hist, i + 1);
         if (c->opt.fail_on_nonconvergence && !c->stats.converged)

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -369,6 +370,34 @@ __global__ __launch_bounds__(256) void hex_cast_diag_kernel(int n, const double*
     if (i < n) d32[i] = (float)d64[i];
 }
 
+// q[row] += sum_k val[k] p[col[k]] over the localized external operator rows: one lane per row, every row listed
+// once, launched after the colour passes on the same stream.
+template <typename TV>
+__global__ __launch_bounds__(256) void hex_ext_apply_kernel(int rows, const int* __restrict__ row,
+                                                            const int* __restrict__ ptr, const int* __restrict__ col,
+                                                            const TV* __restrict__ val, const TV* __restrict__ p,
+                                                            TV* __restrict__ q, const CgScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    TV acc = TV(0);
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k) acc += val[k] * p[col[k]];
+    q[row[r]] += acc;
+}
+
+// The diagonal entries of those rows added to d (a copy of the element operator's diagonal).
+__global__ __launch_bounds__(256) void hex_ext_diag_kernel(int rows, const int* __restrict__ row,
+                                                           const int* __restrict__ ptr, const int* __restrict__ col,
+                                                           const double* __restrict__ val, double* __restrict__ d) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int i = row[r];
+    double acc = d[i];
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k)
+        if (col[k] == i) acc += val[k];
+    d[i] = acc;
+}
+
 // b = fg + fext (+ the step's localized force) before the element forces are subtracted; BC DOFs zeroed
 // afterwards.
 __global__ __launch_bounds__(256) void hex_rhs_init_kernel(int n, const double* __restrict__ fg,
@@ -591,9 +620,12 @@ double host_hex_volume(const double* X, int64_t e) {
 
 void hex_release(HexData& h) {
     for (void* p : {(void*)h.conn, (void*)h.order, (void*)h.order_pad, (void*)h.A32, (void*)h.diag64, (void*)h.diag32,
-                    (void*)h.table, (void*)h.table_idx, (void*)h.mdiag_cons, (void*)h.mdiag_lump})
+                    (void*)h.table, (void*)h.table_idx, (void*)h.mdiag_cons, (void*)h.mdiag_lump, (void*)h.diagx64,
+                    (void*)h.diagx32})
         if (p) (void)hipFree(p);
     h.mdiag_cons = h.mdiag_lump = nullptr;
+    h.diagx64 = nullptr;
+    h.diagx32 = nullptr;
     h.conn = h.order = h.order_pad = nullptr;
     h.table = nullptr;
     h.table_idx = nullptr;
@@ -787,6 +819,19 @@ void apply64(HexData& h, const double* x0, const double4* mat, const double* p, 
     }
 }
 
+// q += A_ext p with the step's localized external operator rows (none: nothing launched).
+template <typename TV>
+void ext_apply(const HexStepArgs& a, const TV* p, TV* q, const CgScalars* sc, hipStream_t s) {
+    if (a.ext_rows <= 0) return;
+    const TV* val;
+    if constexpr (std::is_same<TV, float>::value)
+        val = a.ext_val32;
+    else
+        val = a.ext_val;
+    hipLaunchKernelGGL((hex_ext_apply_kernel<TV>), blocks_for(a.ext_rows, 256), dim3(256), 0, s, a.ext_rows, a.ext_row,
+                       a.ext_ptr, a.ext_col, val, p, q, sc);
+}
+
 // Runs CG iterations with the element-by-element operator until the device reports done.
 template <typename TV>
 int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgScalars* h_sc, int& predicted,
@@ -816,6 +861,7 @@ int run_hex_cg(HexData& h, const HexStepArgs& a, bool fp32, CgScalars* sc, CgSca
                                    0.0, a.bcmask, h.a32_cm, h.a32_ck, h.a32_lumped, 1.0, (double*)q,
                                    (const CgScalars*)sc);
         }
+        ext_apply<TV>(a, p, q, sc, s);
         hipLaunchKernelGGL((hex_cg_bc_pq_kernel<TV>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, p, q,
                            a.partial, sc);
         launch_cg_vector_raw(fp32, n, k, x, r, z, p, q, dinv, a.partial, sc, s);
@@ -905,6 +951,8 @@ void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s)
     apply64(h, a.x0, a.mat, nullptr, a.x, a.v, dt + o.rayleigh_stiffness, nullptr, o.rayleigh_mass, 1.0, lumped, -1.0,
             a.b, s);
     hipLaunchKernelGGL(hex_rhs_bc_kernel, blocks_for(n, 256), dim3(256), 0, s, n, a.bcmask, 1.0, a.b);
+    launch_ext_rhs(a.rhs_rows, a.rhs_row, a.rhs_ptr, a.rhs_col, a.rhs_k, a.rhs_f, a.rhs_dt, a.x, a.x, a.v, 0, 1.0,
+                   a.b, s);  // - dt K_ext v
     hex_solve(h, a, stats, s);
     launch_update(n, a.dv, a.x, a.x, a.v, dt, a.invalid, s);
 }
@@ -943,9 +991,23 @@ void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s
     const double cm = h.a32_cm, ck = h.a32_ck;
     const int lumped = h.a32_lumped;
     CgScalars* sc = a.sc;
+    const double* diag64 = h.diag64;
+    const float* diag32 = h.diag32;
+    if (a.ext_rows > 0) {  // the Jacobi diagonal of the operator with the localized external entries
+        if (!h.diagx64) {
+            HCHECK(hipMalloc(&h.diagx64, (size_t)n * sizeof(double)));
+            HCHECK(hipMalloc(&h.diagx32, (size_t)n * sizeof(float)));
+        }
+        HCHECK(hipMemcpyAsync(h.diagx64, h.diag64, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(hex_ext_diag_kernel, blocks_for(a.ext_rows, 256), dim3(256), 0, s, a.ext_rows, a.ext_row,
+                           a.ext_ptr, a.ext_col, a.ext_val, h.diagx64);
+        hipLaunchKernelGGL(hex_cast_diag_kernel, blocks_for(n, 256), dim3(256), 0, s, n, h.diagx64, h.diagx32);
+        diag64 = h.diagx64;
+        diag32 = h.diagx32;
+    }
     if (!fp32) {
         double* x = a.dv;
-        hipLaunchKernelGGL((hex_cg_init_kernel<double>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.b, h.diag64,
+        hipLaunchKernelGGL((hex_cg_init_kernel<double>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.b, diag64,
                            x, (double*)a.vec[1], (double*)a.vec[2], (double*)a.vec[5], a.bcmask, a.partial,
                            (const CgScalars*)nullptr);
         launch_cg_init2(a.partial, sc, nullptr, o.cg_tolerance, s);
@@ -969,6 +1031,7 @@ void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s
                 double* q64 = (double*)a.vec[6];
                 HCHECK(hipMemsetAsync(q64, 0, (size_t)n * sizeof(double), s));
                 apply64(h, a.x0, a.mat, a.dv, nullptr, nullptr, 0.0, a.bcmask, cm, ck, lumped, 1.0, q64, s);
+                ext_apply<double>(a, a.dv, q64, nullptr, s);
                 hipLaunchKernelGGL(hex_residual_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.bcmask, a.b,
                                    a.dv, q64, a.r64, a.partial);
             }
@@ -976,7 +1039,7 @@ void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s
                                o.cg_tolerance, outer, sc);
             if (sweep == o.max_refinements) break;
             hipLaunchKernelGGL((hex_cg_init_kernel<float>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.r64,
-                               h.diag32, (float*)a.vec[0], (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[5],
+                               diag32, (float*)a.vec[0], (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[5],
                                a.bcmask, a.partial, (const CgScalars*)outer);
             launch_cg_init2(a.partial, sc, outer, o.inner_tolerance, s);
             const int ps = 1 + std::min(sweep, 15);

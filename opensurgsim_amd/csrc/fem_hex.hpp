@@ -39,6 +39,8 @@ struct HexData {
     std::vector<int> h_order_pad;       // host copy of order_pad (padding positions are -1)
     double* mdiag_cons = nullptr;       // diag(M) per DOF, consistent and lumped (row sums); built on first use
     double* mdiag_lump = nullptr;
+    double* diagx64 = nullptr;          // diag64 plus the localized external diagonal of a step that has one
+    float* diagx32 = nullptr;
     double a32_cm = 0.0, a32_ck = 0.0;  // coefficients of M and K the cached A32 / diagonal were built for
     int a32_lumped = -1;                // lumped mass flag of the cache (-1: not built)
     bool ready = false;
@@ -73,6 +75,25 @@ struct HexStepArgs {
     int* invalid;
     int max_iter;
     int* predicted;         // [17]: FP64 solve, then per refinement sweep
+    // Localized external stiffness and damping of the step (Physics/Fem3DRepresentation.cpp:133-198): operator
+    // rows (row DOF, then column DOFs and values in ext_ptr[r] .. ext_ptr[r + 1]) under the lower-triangle view,
+    // added to every application of the operator; their diagonal entries join the Jacobi diagonal. Boundary
+    // rows and columns carry none.
+    int ext_rows = 0;
+    const int* ext_row = nullptr;
+    const int* ext_ptr = nullptr;
+    const int* ext_col = nullptr;
+    const double* ext_val = nullptr;
+    const float* ext_val32 = nullptr;
+    // Right-hand-side rows of implicit Euler, -dt K_ext v with the full external K (fval zero: the force itself
+    // is in fstep), in launch_ext_rhs's layout.
+    int rhs_rows = 0;
+    const int* rhs_row = nullptr;
+    const int* rhs_ptr = nullptr;
+    const int* rhs_col = nullptr;
+    const double* rhs_k = nullptr;
+    const double* rhs_f = nullptr;
+    const double* rhs_dt = nullptr;
 };
 void hex_step(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s);
 // The linear solve of hex_step alone: dv = A^-1 b with the operator cached by the last step (b's boundary

@@ -255,11 +255,66 @@ def test_hex_localized_force():
     assert e["u"] < 1e-9 and e["v"] < 1e-9
 
 
+def _hex_localized(k, o, g):
+    """Localized force, stiffness and damping on three hexahedra before the first and third steps."""
+    if k not in (0, 2):
+        return
+    rng = np.random.default_rng(200 + k)
+    for e in (3, 20, 41):
+        coord = rng.dirichlet(np.ones(8))
+        F = rng.normal(size=3) * 0.05
+        K = rng.normal(size=(3, 3)) * 20.0 + 60.0 * np.eye(3)
+        D = rng.normal(size=(3, 3)) * 0.05 + 0.1 * np.eye(3)
+        o.add_localized_force(e, coord, F, K, D)
+        g.add_external_generalized_force(e, coord, F, K, D)
+
+
+@pytest.mark.parametrize("scheme,precision", [(femgpu.EULER_IMPLICIT, femgpu.FP64),
+                                              (femgpu.EULER_IMPLICIT, femgpu.FP32_REFINED),
+                                              (femgpu.STATIC, femgpu.FP64), (femgpu.RUNGE_KUTTA_4, femgpu.FP64)])
+def test_hex_localized_stiffness_damping(scheme, precision):
+    """Localized stiffness and damping on hexahedra: the lower-triangle view of D + dt K (implicit Euler) or K
+    (static) applied beside the element-by-element operator and in its Jacobi diagonal, -dt K v in the implicit
+    right-hand side; the explicit schemes take the force alone. Rayleigh damping on the implicit cases."""
+    cfg = mg.config_hex((4, 3, 4))
+    cfg["E"] = np.full(cfg.conn.shape[0], 5e3)
+    cfg["dt"] = 1e-4 if scheme == femgpu.RUNGE_KUTTA_4 else 1e-3
+    damped = scheme == femgpu.EULER_IMPLICIT
+    say("hex localized stiffness and damping, scheme", scheme, "precision", precision)
+    e = run_pair(cfg, 4, scheme, precision=precision, am=AM if damped else 0.0, ak=AK if damped else 0.0,
+                 before_step=_hex_localized)
+    say("  rel err", e)
+    bound = 1e-9 if precision == femgpu.FP64 else 1e-8
+    assert e["u"] < bound and e["v"] < bound
+
+
+@pytest.mark.parametrize("precision", [femgpu.FP64, femgpu.FP32_REFINED])
+def test_hex_compliance_after_localized_step(precision):
+    """The compliance after a hexahedral step with localized stiffness and damping uses that step's operator
+    with the external entries (applyCompliance of the last system matrix)."""
+    cfg = mg.config_hex((4, 3, 4))
+    o = make_oracle(cfg, tol=1e-12)
+    g = make_gpu(cfg, precision=precision, tol=1e-12)
+    _hex_localized(0, o, g)
+    assert o.step(cfg.dt, 1) == 0
+    assert g.step(1) == femgpu.FEM_OK
+    rng = np.random.default_rng(6)
+    B = rng.normal(size=(cfg.x0.size, 3))
+    X = g.apply_compliance(B)
+    Xr = o.apply_compliance(B)
+    err = np.linalg.norm(X - Xr) / np.linalg.norm(Xr)
+    say("hex compliance after a localized step, precision", precision, "rel err", err)
+    assert err < (1e-9 if precision == femgpu.FP64 else 1e-6)
+
+
 def test_unsupported_combinations_rejected():
     cfg = mg.config_hex((2, 2, 2))
-    g = make_gpu(cfg)
+    g = make_gpu(cfg, rayleigh_stiffness=AK)
+    g.add_external_generalized_force(0, np.full(8, 0.125), [0, 0, 1.0], np.eye(3), None)
+    assert g.step(1) == femgpu.FEM_OK  # localized stiffness on hexahedra
+    g.set_options(dt=cfg.dt, scheme=femgpu.RUNGE_KUTTA_4, rayleigh_stiffness=AK)
     with pytest.raises(femgpu.FemError):
-        g.add_external_generalized_force(0, np.full(8, 0.125), [0, 0, 1.0], np.eye(3), None)
+        g.step(1)  # Rayleigh stiffness force from the reference's global K holding that localized stiffness
     g.set_options(dt=cfg.dt, scheme=femgpu.RUNGE_KUTTA_4)
     assert g.step(1) == femgpu.FEM_OK  # the other schemes step hexahedra too
     t = make_gpu(soft_config(oracle.TET_LINEAR))
