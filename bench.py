@@ -424,8 +424,9 @@ def bench_compliance(args):
 def bench_adjoint(args):
     """Adjoint material identification workload: 20x20x20 corotational tet block (40,000 tets), per-cube
     log-normal ground-truth E, top pressure 200 Pa, 50 implicit steps at dt = 2 ms; loss = squared
-    displacement mismatch at 500 seeded surface nodes; full dL/dE (all element moduli) by the adjoint with
-    checkpoints every 10 steps, at a uniform guess E = 3e4 Pa; checked against central differences along
+    displacement mismatch at 500 seeded surface nodes; full dL/dE (all element moduli) by the adjoint over
+    the whole state history kept in HBM (22 MB; --adj-checkpoint k recomputes k-step segments from
+    checkpoints instead), at a uniform guess E = 3e4 Pa; checked against central differences along
     32 seeded unit directions."""
     log("importing torch (the adjoint harness is a torch.autograd.Function)")
     import torch
@@ -442,7 +443,8 @@ def bench_adjoint(args):
     g.add_bc_nodes(cfg.bc_nodes)
     g.set_options(dt=cfg.dt, cg_tolerance=args.adj_tol)
     g.set_external_force(cfg.fext)
-    r = adjoint.Rollout(g, cfg.x0, np.zeros(cfg.x0.size), steps, checkpoint_every=10)
+    r = adjoint.Rollout(g, cfg.x0, np.zeros(cfg.x0.size), steps, checkpoint_every=args.adj_checkpoint or None)
+    keep = "whole state history in HBM" if r.full_history else "checkpoints every %d" % r.k
     xr = torch.as_tensor(cfg.x0.reshape(-1), device="cuda")
     obs = torch.as_tensor((3 * cfg.obs_nodes[:, None] + np.arange(3)).reshape(-1), device="cuda")
     with torch.no_grad():
@@ -504,7 +506,7 @@ def bench_adjoint(args):
            "warmup": 1, "ms_per_step": 1e3 * t / reps, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "adjoint dL/dE, corotational tet block %dx%dx%d, %d implicit steps dt=2ms, "
-                                  "checkpoints every 10, CG %.0e" % (n + (steps, args.adj_tol)),
+                                  "%s, CG %.0e" % (n + (steps, keep, args.adj_tol)),
                       "elements": int(cfg.conn.shape[0]), "dof": int(cfg.x0.size), "observed_nodes":
                       int(cfg.obs_nodes.size), "parallelism": "single-gpu"},
            "fd_check": {"directions": len(errs), "step_rel": args.adj_fd_h, "max_rel_err": max(errs),
@@ -883,6 +885,8 @@ def main():
     ap.add_argument("--adj-fd-dirs", type=int, default=32)
     ap.add_argument("--adj-fd-h", type=float, default=1e-4, help="FD step relative to mean E")
     ap.add_argument("--adj-ident", type=int, default=0, help="identification iterations to run (0: none)")
+    ap.add_argument("--adj-checkpoint", type=int, default=0,
+                    help="adjoint: checkpoint every k steps and recompute segments (0: keep the whole history)")
     ap.add_argument("--hex-n", type=int, default=128, help="hex block: n x n/2 x n cubes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also time a copy of this many MiB (probe_copy_kernel)")

@@ -1,10 +1,12 @@
 """Differentiable implicit-Euler rollouts on the GPU, for material identification.
 
 `ImplicitEulerRollout` is a torch.autograd.Function over the C ABI. Its forward advances a tetrahedral
-mesh N implicit steps (fem_step) from a given state with per-element Young's moduli E, and keeps
-the state every k steps (checkpoints). Its backward recomputes each k-step segment from its checkpoint
-and runs fem_adjoint_step backwards through it. That returns dL/dE together with dL/dx_0 and dL/dv_0
-of the initial state.
+mesh N implicit steps from a given state with per-element Young's moduli E. When the whole state history
+(2 (N + 1) ndof doubles) fits the rollout's history budget, the forward records every state in device
+memory (fem_step_record) and the backward runs fem_adjoint_steps over it in one call, with no forward
+step repeated. Otherwise the forward keeps the state every k steps (checkpoints) and the backward
+recomputes each k-step segment from its checkpoint before running the adjoint steps through it. Either
+way the backward returns dL/dE together with dL/dx_0 and dL/dv_0 of the initial state.
 
 The forward map is the reference's implicit Euler step (Math/OdeSolverEulerImplicit.cpp:55-161, one
 Newton iteration, Jacobi CG on the lower-triangle view) with the elements of
@@ -29,17 +31,23 @@ def _dev_zeros(n, device):
 
 class Rollout:
     """A FemGpu context (mesh, boundary conditions, loads and options already set) bound to torch
-    tensors on its device. The initial state is (x_init, v_init)."""
+    tensors on its device. The initial state is (x_init, v_init).
 
-    def __init__(self, sim, x_init, v_init, steps, checkpoint_every=10, device="cuda"):
-        if steps < 1 or checkpoint_every < 1:
+    checkpoint_every=None keeps the whole state history when it fits history_bytes of device memory
+    (no recomputation in the backward) and checkpoints every 10 steps otherwise; an integer k always
+    checkpoints every k steps."""
+
+    def __init__(self, sim, x_init, v_init, steps, checkpoint_every=None, device="cuda", history_bytes=4 << 30):
+        if steps < 1 or (checkpoint_every is not None and checkpoint_every < 1):
             raise ValueError("steps and checkpoint_every must be positive")
         self.sim = sim
         self.device = torch.device(device)
         self.n = sim.ndof
         self.ne = sim.num_elements
         self.steps = int(steps)
-        self.k = int(checkpoint_every)
+        self.full_history = checkpoint_every is None and 2 * (self.steps + 1) * self.n * 8 <= history_bytes
+        self.k = self.steps if self.full_history else int(checkpoint_every or 10)
+        self.history = None
         self.x_init = torch.as_tensor(np.asarray(x_init, dtype=np.float64).reshape(-1), device=self.device).clone()
         self.v_init = torch.as_tensor(np.asarray(v_init, dtype=np.float64).reshape(-1), device=self.device).clone()
         # order the context's work with torch's on the same stream
@@ -68,6 +76,15 @@ class Rollout:
         self.young = E.copy()
         self.sim.set_young_modulus(E)
         self._set_state(self.x_init, self.v_init)
+        if self.full_history:
+            if self.history is None:
+                self.history = (torch.empty((self.steps + 1, self.n), dtype=torch.float64, device=self.device),
+                                torch.empty((self.steps + 1, self.n), dtype=torch.float64, device=self.device))
+            xh, vh = self.history
+            if self.sim.step_record(self.steps, xh.data_ptr(), vh.data_ptr()) != femgpu.FEM_OK:
+                raise FloatingPointError("the rollout's state became non-finite")
+            self.final = (xh[self.steps].clone(), vh[self.steps].clone())
+            return self.final[0].clone(), self.final[1].clone()
         self.checkpoints = [(self.x_init.clone(), self.v_init.clone())]
         done = 0
         while done < self.steps:
@@ -88,6 +105,12 @@ class Rollout:
         lx = (grad_x.detach().to(torch.float64).clone() if grad_x is not None else _dev_zeros(self.n, self.device))
         lv = (grad_v.detach().to(torch.float64).clone() if grad_v is not None else _dev_zeros(self.n, self.device))
         gE = _dev_zeros(self.ne, self.device)
+        if self.full_history:
+            xh, vh = self.history
+            self.sim.adjoint_steps(self.steps, xh.data_ptr(), vh.data_ptr(), lx.data_ptr(), lv.data_ptr(),
+                                   gE.data_ptr())
+            self.sim.synchronize()
+            return gE, lx, lv
         for seg in range(len(self.checkpoints) - 1, -1, -1):
             start = seg * self.k
             length = min(self.k, self.steps - start)

@@ -85,12 +85,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+@pytest.mark.parametrize("k", [2, None])
 @pytest.mark.parametrize("kind", [femgpu.TET_COROTATIONAL, femgpu.TET_LINEAR])
-def test_adjoint_matches_oracle(kind):
+def test_adjoint_matches_oracle(kind, k):
+    """Checkpoints every 2 steps, and the whole history kept (k None)."""
     p = problem(kind, (2, 3, 2), 5)
-    say("adjoint vs oracle, kind", kind)
+    say("adjoint vs oracle, kind", kind, "checkpoints", k)
     gE_o, lx_o, lv_o, x_o = oracle_gradient(p)
-    gE, lx, lv, x = gpu_gradient(p, k=2)
+    gE, lx, lv, x = gpu_gradient(p, k=k)
     e = dict(x=rel(x, x_o), gE=rel(gE, gE_o), lx0=rel(lx, lx_o), lv0=rel(lv, lv_o))
     say("  rel err", e)
     assert e["x"] < 1e-10
@@ -129,10 +131,12 @@ def test_adjoint_vs_central_differences():
 
 def test_checkpoint_interval_does_not_change_gradient():
     p = problem(femgpu.TET_COROTATIONAL, (3, 3, 3), 9)
-    say("checkpoint intervals 1, 4, 20")
+    say("checkpoint intervals 1, 4, 20 and the whole history")
     ref = gpu_gradient(p, k=1)[0]
-    for k in (4, 20):
-        assert rel(gpu_gradient(p, k=k)[0], ref) < 1e-12
+    for k in (4, 20, None):
+        e = rel(gpu_gradient(p, k=k)[0], ref)
+        say("  k", k, "rel", e)
+        assert e < 1e-12
 
 
 def test_autograd_function_and_identification():
