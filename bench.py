@@ -257,12 +257,12 @@ def bench_hex(args):
                     v=float(np.linalg.norm(r["v"] - r64["v"]) / np.linalg.norm(r64["v"])))
     e32, et = err(r32), err(rt)
     log("fp32+refine vs fp64 rel err", e32, "table", et, "identical", bool(np.array_equal(r32["x"], rt["x"])))
-    # one application of the FP32 operator (8 colours on a block mesh): per element the cached blocks
-    # (1,296 B; table: a 4 B index), 32 B of node ids and its 4 B slot; per colour p read, q read and written
-    # and the boundary mask over the DOFs
+    # one application of the FP32 operator (8 colours on a block mesh): per element the cached packed upper
+    # triangle (300 floats, 1,200 B; table: a 4 B index), 32 B of node ids and its 4 B slot; per colour p read
+    # and q read and written over the DOFs
     colours = 8
-    cache_bytes = E * (1296 + 32 + 4) + colours * 13 * D
-    table_bytes = E * (4 + 32 + 4) + colours * 13 * D
+    cache_bytes = E * (1200 + 32 + 4) + colours * 12 * D
+    table_bytes = E * (4 + 32 + 4) + colours * 12 * D
     return {"metric": METRIC, "value": D * steps / r32["seconds"], "unit": "DOF-steps/s", "n_gpus": 1,
             "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",

@@ -627,6 +627,7 @@ void linear_solve(fem_ctx* c) {
         c->refine_iters_cap = max_ref + 1;
     }
     CgArgs a = cg_args(c, true, c->opt.inner_tolerance);
+    a.floor_tol = kInnerFloor * c->opt.cg_tolerance;
     CgScalars* outer = c->scal.p + 1;
     // One-kernel CG path: sweep s is one launch for dv_s = dv_{s-1} + x_{s-1}, the FP64 residual and the inner
     // solve's start, and one for the outer test with the inner scalars; dv_s and x_s alternate between two

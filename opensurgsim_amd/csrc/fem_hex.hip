@@ -67,9 +67,7 @@ __host__ __device__ __forceinline__ double hex_gp(const double* X, int gi, int g
     return d;
 }
 
-__device__ __forceinline__ constexpr int blk_index(int a, int b) { return 8 * a - a * (a - 1) / 2 + (b - a); }
-
-// Packed block k of the upper block triangle (a <= b), row-major over a: k = blk_index(a, b).
+// Block k (0..35) of the upper block triangle (a <= b), row-major over a.
 __host__ __device__ constexpr int blk_a(int k) {
     int a = 0;
     while (k >= 8 - a) {
@@ -88,10 +86,34 @@ __host__ __device__ constexpr int blk_b(int k) {
 }
 
 // Cached FP32 element matrices, interleaved for one element per lane: position t of the padded colour
-// order (tile T = t / 64, lane l = t % 64) keeps its 324 floats (36 blocks x 9, row-major within a
-// block) as 81 float4 chunks at A4[(81 T + k) 64 + l], so each wave-wide load reads 1 KiB contiguous.
-constexpr int kHexChunks = 81;
-constexpr int kHexMaxDistinct = 32;  // distinct-matrix table: at most 41.5 KB of LDS per workgroup
+// order (tile T = t / 64, lane l = t % 64) keeps its 300 floats as 75 float4 chunks at A4[(75 T + k) 64 + l],
+// so each wave-wide load reads 1 KiB contiguous. Floats 0..251: the 28 strictly upper node blocks (a < b,
+// row-major over a, then b), 9 each, row-major; floats 252..299: the 8 diagonal blocks, the 6 entries
+// (0,0) (0,1) (0,2) (1,1) (1,2) (2,2) of each (A_e is symmetric).
+constexpr int kHexChunks = 75;
+constexpr int kHexOffFloats = 252;
+constexpr int kHexMaxDistinct = 32;  // distinct-matrix table: at most 38.4 KB of LDS per workgroup
+
+// Strictly upper node block o (0..27) of the packed order: its nodes (a, b), a < b.
+__host__ __device__ constexpr int off_a(int o) {
+    int a = 0;
+    while (o >= 7 - a) {
+        o -= 7 - a;
+        ++a;
+    }
+    return a;
+}
+__host__ __device__ constexpr int off_b(int o) {
+    int a = 0;
+    while (o >= 7 - a) {
+        o -= 7 - a;
+        ++a;
+    }
+    return a + 1 + o;
+}
+__host__ __device__ constexpr int off_index(int a, int b) { return 7 * a - a * (a - 1) / 2 + (b - a - 1); }
+// Packed float of entry (i, j) of a diagonal block (upper triangle, i <= j).
+__host__ __device__ constexpr int sym_index(int i, int j) { return i == 0 ? j : (i == 1 ? 2 + j : 5); }
 
 // ------------------------------------------------------------------------------------------
 // Set-up: A_e = cm M_e + ck K_e in FP32 (implicit Euler with Rayleigh damping: cm = 1/dt + alpha_M,
@@ -128,9 +150,12 @@ __global__ __launch_bounds__(256) void hex_prepare_kernel(int count_pad, const i
                                         (i == j ? ms : 0.0);
             }
     float* base = A32 + ((size_t)(t >> 6) * kHexChunks * 64) * 4 + (size_t)(t & 63) * 4;
-    for (int i = 0; i < 9; ++i) {
-        const int f = 9 * kb + i;
-        base[(size_t)(f >> 2) * 256 + (f & 3)] = (float)A[i];
+    auto put = [&](int f, double v) { base[(size_t)(f >> 2) * 256 + (f & 3)] = (float)v; };
+    if (a < b) {
+        for (int i = 0; i < 9; ++i) put(9 * off_index(a, b) + i, A[i]);
+    } else {
+        for (int i = 0; i < 3; ++i)
+            for (int j = i; j < 3; ++j) put(kHexOffFloats + 6 * a + sym_index(i, j), A[3 * i + j]);
     }
 }
 
@@ -235,8 +260,9 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
         }
     const float4* At = A4 + (size_t)(t >> 6) * kHexChunks * 64 + (t & 63);
     const float4* Tt = TABLE ? lds_table + table_idx[t] * kHexChunks : nullptr;
+    // the strictly upper blocks, 4 per group of 9 chunks: y_a += B p_b and y_b += B^T p_a
 #pragma unroll
-    for (int g = 0; g < 9; ++g) {
+    for (int g = 0; g < 7; ++g) {
         float w[36];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -248,15 +274,48 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
         }
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int a = blk_a(4 * g + m), b = blk_b(4 * g + m);
+            const int a = off_a(4 * g + m), b = off_b(4 * g + m);
             const float* B = w + 9 * m;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 y[3 * a + i] += B[3 * i] * pv[3 * b] + B[3 * i + 1] * pv[3 * b + 1] + B[3 * i + 2] * pv[3 * b + 2];
-            if (b != a)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    y[3 * b + j] += B[j] * pv[3 * a] + B[3 + j] * pv[3 * a + 1] + B[6 + j] * pv[3 * a + 2];
+            for (int j = 0; j < 3; ++j)
+                y[3 * b + j] += B[j] * pv[3 * a] + B[3 + j] * pv[3 * a + 1] + B[6 + j] * pv[3 * a + 2];
+        }
+        // The next group's loads take their address from a value this group produced (a zero the compiler
+        // cannot see through), so they are not hoisted above this group's products: without it the compiler
+        // keeps most of the 75 chunks in flight at once (over 300 registers, 1 wave per SIMD) instead of about
+        // two groups (210 registers, 2 waves per SIMD).
+        {
+            int zero;
+            asm volatile("v_mov_b32 %0, 0" : "=v"(zero) : "v"(y[3 * off_b(4 * g + 3) + 2]));
+            if (TABLE)
+                Tt += zero;
+            else
+                At += zero;
+        }
+    }
+    // the diagonal blocks, 4 per group of 6 chunks (symmetric, 6 floats each)
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        float w[24];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float4 v = TABLE ? Tt[kHexOffFloats / 4 + 6 * g + k] : At[(size_t)(kHexOffFloats / 4 + 6 * g + k) * 64];
+            w[4 * k] = v.x;
+            w[4 * k + 1] = v.y;
+            w[4 * k + 2] = v.z;
+            w[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int a = 4 * g + m;
+            const float* B = w + 6 * m;
+            const float p0 = pv[3 * a], p1 = pv[3 * a + 1], p2 = pv[3 * a + 2];
+            y[3 * a] += B[0] * p0 + B[1] * p1 + B[2] * p2;
+            y[3 * a + 1] += B[1] * p0 + B[3] * p1 + B[4] * p2;
+            y[3 * a + 2] += B[2] * p0 + B[4] * p1 + B[5] * p2;
         }
     }
 #pragma unroll
@@ -265,7 +324,7 @@ __global__ __launch_bounds__(256) void hex_apply32_kernel(int count_pad, const i
         for (int c = 0; c < 3; ++c) q[3 * ids[a] + c] += y[3 * a + c];
 }
 
-// 64-bit FNV-1a hash of each padded position's 324 cached floats (bit patterns, chunk order); 0 for padding.
+// 64-bit FNV-1a hash of each padded position's 300 cached floats (bit patterns, chunk order); 0 for padding.
 __global__ __launch_bounds__(256) void hex_hash_kernel(int count_pad, const int* __restrict__ order_pad,
                                                        const float4* __restrict__ A4,
                                                        unsigned long long* __restrict__ hash) {
@@ -310,7 +369,7 @@ __global__ __launch_bounds__(256) void hex_table_verify_kernel(int count_pad, co
     if (__syncthreads_or(diff) && threadIdx.x == 0) bad[blockIdx.x] = 1;
 }
 
-// table[u] = the cached floats of position rep[u], as 81 consecutive float4 chunks.
+// table[u] = the cached floats of position rep[u], as kHexChunks consecutive float4 chunks.
 __global__ __launch_bounds__(256) void hex_table_fill_kernel(int num_distinct, const int* __restrict__ rep,
                                                              const float4* __restrict__ A4,
                                                              float4* __restrict__ table) {
@@ -705,7 +764,7 @@ void hex_setup(HexData& h, int64_t num_nodes, const double* x0, int64_t num_elem
     HCHECK(hipMalloc(&h.conn, (size_t)num_elements * 8 * sizeof(int)));
     HCHECK(hipMalloc(&h.order, (size_t)num_elements * sizeof(int)));
     HCHECK(hipMalloc(&h.order_pad, order_pad.size() * sizeof(int)));
-    HCHECK(hipMalloc(&h.A32, order_pad.size() * 36 * 9 * sizeof(float)));
+    HCHECK(hipMalloc(&h.A32, order_pad.size() * (size_t)kHexChunks * 4 * sizeof(float)));
     h.h_order_pad = order_pad;
     HCHECK(hipMemcpyAsync(h.order_pad, order_pad.data(), order_pad.size() * sizeof(int), hipMemcpyHostToDevice, s));
     HCHECK(hipMalloc(&h.diag64, (size_t)num_nodes * 3 * sizeof(double)));
@@ -1041,7 +1100,7 @@ void hex_solve(HexData& h, const HexStepArgs& a, fem_stats& stats, hipStream_t s
             hipLaunchKernelGGL((hex_cg_init_kernel<float>), dim3(kCgBlocks), dim3(kCgThreads), 0, s, n, a.r64,
                                diag32, (float*)a.vec[0], (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[5],
                                a.bcmask, a.partial, (const CgScalars*)outer);
-            launch_cg_init2(a.partial, sc, outer, o.inner_tolerance, s);
+            launch_cg_init2(a.partial, sc, outer, o.inner_tolerance, s, kInnerFloor * o.cg_tolerance);
             const int ps = 1 + std::min(sweep, 15);
             const int it = run_hex_cg<float>(h, a, true, sc, a.h_sc, a.predicted[ps], (float*)a.vec[0],
                                              (float*)a.vec[1], (float*)a.vec[2], (float*)a.vec[3], (float*)a.vec[4],

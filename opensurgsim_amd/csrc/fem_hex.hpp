@@ -26,14 +26,15 @@ struct HexData {
     int* order = nullptr;         // elements sorted by color
     std::vector<int> pad_ptr;     // colour c: order_pad[pad_ptr[c] .. pad_ptr[c + 1]), multiples of 64
     int* order_pad = nullptr;     // colour order padded to 64-element tiles (-1 = padding)
-    float* A32 = nullptr;         // packed upper block triangle of M/dt + dt K (FP32), 324 floats per padded
-                                  // position, interleaved by 64-position tiles in float4 chunks
+    float* A32 = nullptr;         // packed upper triangle of M/dt + dt K (FP32): 28 upper node blocks of 9 and
+                                  // 8 symmetric diagonal blocks of 6, 300 floats per padded position,
+                                  // interleaved by 64-position tiles in float4 chunks
     double* diag64 = nullptr;     // diagonal of A (per DOF)
     float* diag32 = nullptr;
     // Distinct cached matrices (a structured mesh repeats a few element matrices): when every padded
-    // position's 324 floats equal one of at most kHexMaxDistinct representatives bit for bit, the FP32
-    // operator reads table[u] (81 float4 chunks each, staged in LDS) through table_idx instead of A32.
-    float* table = nullptr;             // [num_distinct][324]
+    // position's 300 floats equal one of at most kHexMaxDistinct representatives bit for bit, the FP32
+    // operator reads table[u] (75 float4 chunks each, staged in LDS) through table_idx instead of A32.
+    float* table = nullptr;             // [num_distinct][300]
     int* table_idx = nullptr;           // [padded positions] table entry of each position
     int num_distinct = 0;               // 0: A32 per position
     std::vector<int> h_order_pad;       // host copy of order_pad (padding positions are -1)

@@ -617,18 +617,23 @@ __global__ __launch_bounds__(kCgThreads) void cg_init2_guess_kernel(const double
         for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.0;
 }
 
-// |b|^2 and r.z -> threshold max(tol^2 |b|^2, realmin) and Eigen's early exits.
+// |b|^2 and r.z -> threshold max(tol^2 |b|^2, realmin) and Eigen's early exits. An inner solve of the FP64
+// refinement (skip = the outer scalars) with floor_tol > 0 stops at the larger of that and floor_tol |b_outer|:
+// its solution only has to bring the outer residual below the outer tolerance.
 __global__ __launch_bounds__(kCgThreads) void cg_init2_kernel(const double* __restrict__ partial, CgScalars* sc,
-                                                              const CgScalars* __restrict__ skip, double tol) {
+                                                              const CgScalars* __restrict__ skip, double tol,
+                                                              double floor_tol) {
     __shared__ double lds[64];
     if (skip && skip->done) return;
     double bb, rz;
     reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
     if (threadIdx.x == 0) {
+        double th = tol * tol * bb;
+        if (skip && floor_tol > 0.0) th = fmax(th, floor_tol * floor_tol * skip->bb);
         sc->bb = bb;
         sc->rz[0] = rz;
         sc->rz[1] = 0.0;
-        sc->threshold = fmax(tol * tol * bb, DBL_MIN);
+        sc->threshold = fmax(th, DBL_MIN);
         sc->rr = bb;
         sc->iters = 0;
         sc->err = 0.0;
@@ -1057,7 +1062,7 @@ void launch_cg_init(const CgArgs& a, const double* b, const CgScalars* skip, hip
     else
         launch_k((cg_init_kernel<double>), grid, block, 0, s, a.op, (const double*)a.S, b, (double*)a.x,
                            (double*)a.r, (double*)a.z, (double*)a.dinv, a.partial, skip);
-    launch_k(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol);
+    launch_k(cg_init2_kernel, dim3(1), block, 0, s, a.partial, a.sc, skip, a.tol, a.floor_tol);
 }
 
 void launch_cg_init_guess(const CgArgs& a, const double* b, hipStream_t s) {
@@ -1069,7 +1074,7 @@ void launch_cg_init_guess(const CgArgs& a, const double* b, hipStream_t s) {
 void launch_cg_init_block(const CgArgs& a, const double* b, const CgScalars* skip, hipStream_t s) {
     launch_k(cg_init_block_kernel, dim3(kCgBlocks), dim3(kCgThreads), 0, s, a.op, (const float*)a.S, b, (float*)a.x,
              (float*)a.r, (float*)a.z, (float*)a.binv, a.partial, skip);
-    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, a.sc, skip, a.tol);
+    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, a.partial, a.sc, skip, a.tol, a.floor_tol);
 }
 
 void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
@@ -1084,8 +1089,9 @@ void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s) {
                            (const double*)a.z, (const double*)po, (double*)pn, (double*)a.q, a.partial, a.sc);
 }
 
-void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s) {
-    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sc, skip, tol);
+void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s,
+                     double floor_tol) {
+    launch_k(cg_init2_kernel, dim3(1), dim3(kCgThreads), 0, s, partial, sc, skip, tol, floor_tol);
 }
 
 void launch_cg_vector_raw(bool fp32, int n, int k, void* x, void* r, void* z, const void* p, const void* q,
@@ -1332,7 +1338,9 @@ __global__ __launch_bounds__(kCgThreads) void refine_check_init_kernel(const dou
             inner->bb = bb;
             inner->rz[0] = rz;
             inner->rz[1] = 0.0;
-            inner->threshold = fmax(inner_tol * inner_tol * bb, DBL_MIN);
+            // the inner solve only has to bring the outer residual below the outer tolerance (half of it)
+            const double fl = kInnerFloor * tol;
+            inner->threshold = fmax(fmax(inner_tol * inner_tol * bb, fl * fl * outer->bb), DBL_MIN);
             inner->rr = bb;
             inner->iters = 0;
             inner->err = 0.0;

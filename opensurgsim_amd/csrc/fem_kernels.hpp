@@ -161,6 +161,7 @@ struct CgArgs {
     double* partial;    // [4 * kCgBlocks]
     CgScalars* sc;
     double tol;
+    double floor_tol = 0.0;  // inner solve of the refinement: absolute floor relative to the outer |b| (0: none)
     // fused iteration (one kernel per iteration): second buffers of r, z, q and both of w = dinv q,
     // selected by iteration parity; 7 partial sums per block
     void* r1 = nullptr;
@@ -188,7 +189,12 @@ void launch_cg_apply(const CgArgs& a, int k, int max_iter, hipStream_t s);
 void launch_cg_vector(const CgArgs& a, int k, hipStream_t s);
 
 // Scalars of a solve from the partials of an init kernel (|b|^2 in [0, B), r.z in [B, 2B)).
-void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s);
+void launch_cg_init2(const double* partial, CgScalars* sc, const CgScalars* skip, double tol, hipStream_t s,
+                     double floor_tol = 0.0);
+// Fraction of the outer tolerance an inner FP32 solve of the refinement reduces the outer residual to: its
+// threshold is max(inner_tol |r_s|, kInnerFloor tol |b|), so a sweep that starts near the outer tolerance
+// stops after the few iterations it needs.
+constexpr double kInnerFloor = 0.5;
 // *out = ||dv - prev||_inf and prev = dv (partial: kCgBlocks doubles of scratch).
 void launch_newton_variation(int64_t n, const double* dv, double* prev, double* partial, double* out,
                              hipStream_t s);
