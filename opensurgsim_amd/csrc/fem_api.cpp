@@ -760,25 +760,15 @@ void mark_f(fem_ctx* c, bool matrix) {
     }
 }
 
-// Element kernel at state (xn, vn) (null: the current state); xs, vs: the step's start state (Newton terms).
-void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs, const double* xn = nullptr,
-                  const double* vn = nullptr) {
-    TetElementArgs a;
-    a.num_elements = (int)c->num_elements;
+// The element kernel's per-element inputs (connectivity, materials, instances), the group tables, the record
+// buffers and the element flag counters (every launch of the kernel takes them from here).
+void set_staging(const fem_ctx* c, TetElementArgs& a) {
     a.conn = c->conn4.p;
-    a.x0 = c->x0.p;
-    a.xn = xn ? xn : c->x.p;
-    a.vn = vn ? vn : c->v.p;
-    a.xs = xs;
-    a.vs = vs;
     a.mat = c->mat.p;
-    a.dt = c->opt.dt;
-    a.write_matrix = write_matrix ? 1 : 0;
-    a.mode = scheme_mode(c->opt.scheme);
-    a.ray_mass = diag_mass_damping(c) ? 0.0 : c->opt.rayleigh_mass;  // the diagonal form goes with the rhs
-    a.ray_stiff = c->opt.rayleigh_stiffness;
-    a.lumped = (a.mode == 0 && c->opt.mass_lumping) ? 1 : 0;
-    a.general = (a.mode == 0 && (xs != a.xn || vs != a.vn)) ? 1 : 0;
+    if (c->num_instances > 0) {  // each instance's own time step
+        a.elem_inst = c->elem_inst.p;
+        a.inst_dt = c->b_inst_dt.p;
+    }
     a.rec = c->rec.p;
     a.rhs_rec = c->rhs_rec.p;
     a.grp_q = c->grp_q.p;
@@ -791,10 +781,26 @@ void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double*
     a.nsrc16 = c->nsrc16.p;
     a.num_groups = (int)c->num_groups;
     a.flag_counts = c->counters.p;
-    if (c->num_instances > 0) {  // each instance's own time step
-        a.elem_inst = c->elem_inst.p;
-        a.inst_dt = c->b_inst_dt.p;
-    }
+}
+
+// Element kernel at state (xn, vn) (null: the current state); xs, vs: the step's start state (Newton terms).
+void tet_elements(fem_ctx* c, bool write_matrix, const double* xs, const double* vs, const double* xn = nullptr,
+                  const double* vn = nullptr) {
+    TetElementArgs a;
+    a.num_elements = (int)c->num_elements;
+    a.x0 = c->x0.p;
+    a.xn = xn ? xn : c->x.p;
+    a.vn = vn ? vn : c->v.p;
+    a.xs = xs;
+    a.vs = vs;
+    a.dt = c->opt.dt;
+    a.write_matrix = write_matrix ? 1 : 0;
+    a.mode = scheme_mode(c->opt.scheme);
+    a.ray_mass = diag_mass_damping(c) ? 0.0 : c->opt.rayleigh_mass;  // the diagonal form goes with the rhs
+    a.ray_stiff = c->opt.rayleigh_stiffness;
+    a.lumped = (a.mode == 0 && c->opt.mass_lumping) ? 1 : 0;
+    a.general = (a.mode == 0 && (xs != a.xn || vs != a.vn)) ? 1 : 0;
+    set_staging(c, a);
     TIMED(c, FEM_PHASE_ELEMENT, 0, launch_tet_elements(c->kind, a, c->stream));
 }
 
@@ -1316,30 +1322,15 @@ int batch_step(fem_ctx* c) {
     refresh_batch_params(c);
     TetElementArgs a;
     a.num_elements = (int)c->num_elements;
-    a.conn = c->conn4.p;
     a.x0 = c->x0.p;
     a.xn = c->x.p;
     a.vn = c->v.p;
     a.xs = c->x.p;
     a.vs = c->v.p;
-    a.mat = c->mat.p;
     a.dt = c->opt.dt;
     a.write_matrix = reuse ? 0 : 1;
     a.general = 0;
-    a.rec = c->rec.p;
-    a.rhs_rec = c->rhs_rec.p;
-    a.grp_q = c->grp_q.p;
-    a.q_out = c->q_out.p;
-    a.q_src = c->q_src.p;
-    a.src16 = c->src16.p;
-    a.grp_n = c->grp_n.p;
-    a.n_out = c->n_out.p;
-    a.n_src = c->n_src.p;
-    a.nsrc16 = c->nsrc16.p;
-    a.num_groups = (int)c->num_groups;
-    a.flag_counts = c->counters.p;
-    a.elem_inst = c->elem_inst.p;
-    a.inst_dt = c->b_inst_dt.p;
+    set_staging(c, a);
     a.ray_mass = c->opt.rayleigh_mass;
     a.ray_stiff = c->opt.rayleigh_stiffness;
     a.lumped = c->opt.mass_lumping ? 1 : 0;
@@ -1722,29 +1713,43 @@ void build_tet_maps(fem_ctx* c, const int32_t* conn) {
     std::vector<uint16_t> src16, nsrc16;
     {
         std::vector<int64_t> pb(cptr.begin(), cptr.end() - 1), pn(rptr.begin(), rptr.end() - 1);
+        // A group's records go out longest source list first: one lane sums one record, so lanes of a wave
+        // then loop over lists of similar lengths. The sum within a record keeps the element order.
+        struct Run {
+            size_t first, count;
+            int32_t out;
+        };
+        std::vector<Run> runs;
+        auto emit = [&](const std::vector<std::array<int32_t, 3>>& tb, std::vector<int32_t>& out,
+                        std::vector<int32_t>& src_ptr, std::vector<uint16_t>& src) {
+            std::stable_sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.count > b.count; });
+            for (const Run& r : runs) {
+                out.push_back(r.out);
+                for (size_t k = r.first; k < r.first + r.count; ++k) src.push_back((uint16_t)tb[k][2]);
+                src_ptr.push_back((int32_t)src.size());
+            }
+        };
         for (int64_t g = 0; g < G; ++g) {
             group_lists(g);
+            runs.clear();
             for (size_t k = 0; k < ctb.size(); ++k) {
                 const int32_t t = ctb[k][0];
                 if (k == 0 || t != ctb[k - 1][0]) {
-                    if (k > 0) q_src.push_back((int32_t)src16.size());
                     const int64_t pos = pb[t]++;
-                    q_out.push_back((int32_t)(pos < c->nctr_off ? kPairRec * pos
-                                                                : kPairRec * c->nctr_off + kDiagRec * (pos - c->nctr_off)));
+                    runs.push_back({k, 0, (int32_t)(pos < c->nctr_off ? kPairRec * pos
+                                                                      : kPairRec * c->nctr_off + kDiagRec * (pos - c->nctr_off))});
                 }
-                src16.push_back((uint16_t)ctb[k][2]);
+                runs.back().count++;
             }
-            if (!ctb.empty()) q_src.push_back((int32_t)src16.size());
+            emit(ctb, q_out, q_src, src16);
             grp_q[g + 1] = (int32_t)q_out.size();
+            runs.clear();
             for (size_t k = 0; k < ntb.size(); ++k) {
                 const int32_t nd = ntb[k][0];
-                if (k == 0 || nd != ntb[k - 1][0]) {
-                    if (k > 0) n_src.push_back((int32_t)nsrc16.size());
-                    n_out.push_back((int32_t)(kRhsRec * pn[nd]++));
-                }
-                nsrc16.push_back((uint16_t)ntb[k][2]);
+                if (k == 0 || nd != ntb[k - 1][0]) runs.push_back({k, 0, (int32_t)(kRhsRec * pn[nd]++)});
+                runs.back().count++;
             }
-            if (!ntb.empty()) n_src.push_back((int32_t)nsrc16.size());
+            emit(ntb, n_out, n_src, nsrc16);
             grp_n[g + 1] = (int32_t)n_out.size();
         }
     }
@@ -2871,28 +2876,15 @@ void adjoint_step_impl(fem_ctx* c, const double* x_n, const double* v_n, const d
         // (the reference's newState), xs/vs the step's start state; both are x_n, v_n here
         TetElementArgs ea;
         ea.num_elements = (int)c->num_elements;
-        ea.conn = c->conn4.p;
         ea.x0 = c->x0.p;
         ea.xn = c->xs.p;
         ea.vn = c->vs.p;
         ea.xs = c->xs.p;
         ea.vs = c->vs.p;
-        ea.mat = c->mat.p;
         ea.dt = c->opt.dt;
         ea.write_matrix = 1;
         ea.general = 0;
-        ea.rec = c->rec.p;
-        ea.rhs_rec = c->rhs_rec.p;
-        ea.grp_q = c->grp_q.p;
-        ea.q_out = c->q_out.p;
-        ea.q_src = c->q_src.p;
-        ea.src16 = c->src16.p;
-        ea.grp_n = c->grp_n.p;
-        ea.n_out = c->n_out.p;
-        ea.n_src = c->n_src.p;
-        ea.nsrc16 = c->nsrc16.p;
-        ea.num_groups = (int)c->num_groups;
-        ea.flag_counts = c->counters.p;
+        set_staging(c, ea);
         launch_tet_elements(c->kind, ea, s);
     }
     assemble_tet(c);

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <stdexcept>
 
 namespace femgpu {
 
@@ -299,6 +300,8 @@ __global__ __launch_bounds__(256) void tet_element_kernel(TetElementArgs A) {
 
 void launch_tet_elements(int kind, const TetElementArgs& a, hipStream_t s) {
     if (a.num_elements == 0) return;
+    if (!a.conn || !a.mat || !a.rhs_rec || !a.grp_n || (a.write_matrix && (!a.rec || !a.grp_q)))
+        throw std::invalid_argument("element kernel launched without its inputs or staging tables");
     const dim3 grid = xcd_grid(4 * (int64_t)a.num_elements, 256), block(256);
     if (kind == 1) {
         if (a.general) launch_k((tet_element_kernel<true, true>), grid, block, 0, s, a);
@@ -833,7 +836,7 @@ __device__ __forceinline__ void reduce7(const double* __restrict__ part, double*
 }
 
 template <typename TV>
-__global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(8))) void cg_fused_kernel(Operator op, const TV* __restrict__ S, int k,
+__global__ __launch_bounds__(kCgThreads) __attribute__((amdgpu_waves_per_eu(sizeof(TV) == 4 ? 8 : 7))) void cg_fused_kernel(Operator op, const TV* __restrict__ S, int k,
                                                               int max_iter, TV* __restrict__ x, TV* r0, TV* r1,
                                                               TV* z0, TV* z1, TV* w0, TV* w1, TV* p0, TV* p1, TV* q0,
                                                               TV* q1, const TV* __restrict__ dinv,

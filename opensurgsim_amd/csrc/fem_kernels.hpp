@@ -56,35 +56,35 @@ constexpr int kRhsRec = 4;
 constexpr int kElemGroup = 64;
 
 struct TetElementArgs {
-    int num_elements;
-    const int4* conn;
-    const double* x0;
-    const double* xn;     // state the element is evaluated at (Newton estimate)
-    const double* vn;
-    const double* xs;     // start-of-step state
-    const double* vs;
-    const double4* mat;   // lambda, mu, rho, E
-    double dt;
-    int write_matrix;     // 0: only the rhs record
-    int general;          // 0: xn == xs and vn == vs (first Newton iteration)
-    double* rec;          // lower-block contributions staged by block, one record per element group touching
-                          // the block, in group order: kPairRec records of the off-diagonal blocks, then kDiagRec
-                          // records of the diagonal blocks
-    double* rhs_rec;      // rhs rows staged by node, one kRhsRec record per element group touching the node
+    int num_elements = 0;
+    const int4* conn = nullptr;
+    const double* x0 = nullptr;
+    const double* xn = nullptr;     // state the element is evaluated at (Newton estimate)
+    const double* vn = nullptr;
+    const double* xs = nullptr;     // start-of-step state
+    const double* vs = nullptr;
+    const double4* mat = nullptr;   // lambda, mu, rho, E
+    double dt = 0.0;
+    int write_matrix = 0;  // 0: only the rhs record
+    int general = 0;       // 0: xn == xs and vn == vs (first Newton iteration)
+    // lower-block contributions staged by block, one record per element group touching the block, in group
+    // order: kPairRec records of the off-diagonal blocks, then kDiagRec records of the diagonal blocks
+    double* rec = nullptr;
+    double* rhs_rec = nullptr;  // rhs rows staged by node, one kRhsRec record per element group touching the node
     // Per element group g (kElemGroup consecutive elements): records q in [grp_q[g], grp_q[g + 1]) go to offset
     // q_out[q] of rec and sum the LDS contributions src16[q_src[q] .. q_src[q + 1]) (element in group * 10 +
     // slot; slots 0..5 the pairs (0,1), (0,2), (0,3), (1,2), (1,3), (2,3), 6..9 the diagonal blocks), in element
     // order; node rows likewise (grp_n, n_out into rhs_rec, n_src / nsrc16 = element in group * 4 + corner).
-    const int* grp_q;
-    const int* q_out;
-    const int* q_src;
-    const uint16_t* src16;
-    const int* grp_n;
-    const int* n_out;
-    const int* n_src;
-    const uint16_t* nsrc16;
-    int num_groups;
-    int* flag_counts;     // [3]: det R invalid, G singular, eigen polar path
+    const int* grp_q = nullptr;
+    const int* q_out = nullptr;
+    const int* q_src = nullptr;
+    const uint16_t* src16 = nullptr;
+    const int* grp_n = nullptr;
+    const int* n_out = nullptr;
+    const int* n_src = nullptr;
+    const uint16_t* nsrc16 = nullptr;
+    int num_groups = 0;
+    int* flag_counts = nullptr;  // [3]: det R invalid, G singular, eigen polar path
     const int* elem_inst = nullptr;     // batched instances: instance of each element (null: one dt)
     const double* inst_dt = nullptr;    // time step per instance
     // Operator and right-hand side by integration scheme:
