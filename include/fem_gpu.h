@@ -42,8 +42,8 @@ enum {
     FEM_ELEMENT_HEX = 2               /* "SurgSim::Physics::Fem3DElementCube" */
 };
 
-/* Integration schemes (Math/OdeSolver.h:36-49). The schemes after the two implicit ones step one
- * representation (tetrahedral or hexahedral), not batched instances. */
+/* Integration schemes (Math/OdeSolver.h:36-49). Every scheme steps tetrahedral and hexahedral meshes and
+ * batched instances (each instance with its own time step). */
 enum {
     FEM_SCHEME_EULER_IMPLICIT = 0,                 /* INTEGRATIONSCHEME_EULER_IMPLICIT */
     FEM_SCHEME_LINEAR_EULER_IMPLICIT = 1,          /* INTEGRATIONSCHEME_LINEAR_EULER_IMPLICIT */
@@ -152,7 +152,7 @@ int fem_set_external_force(fem_ctx* ctx, const double* force);
  * coordinate[a] * coordinate[b]. Calls accumulate; the terms are cleared after the next step, as
  * DeformableRepresentation::afterUpdate clears them. Replaces Fem3DRepresentation::addExternalGeneralizedForce
  * (Physics/Fem3DRepresentation.cpp:133-198; Physics/DeformableRepresentation.cpp:236-253). Stiffness and
- * damping need a tetrahedral mesh and a scheme that is not one of the linear (constant-operator) variants. */
+ * damping need a scheme that is not one of the linear (constant-operator) variants. */
 int fem_add_external_generalized_force(fem_ctx* ctx, int64_t element, const double* coordinate, const double* force,
                                        const double* stiffness, const double* damping);
 
@@ -226,10 +226,10 @@ int fem_query(fem_ctx* ctx, fem_stats* stats);
 /* Batched independent instances. The uploaded tetrahedral mesh is declared to consist of num_instances
  * disconnected meshes (no element joins two of them): instance i owns nodes
  * [node_offsets[i], node_offsets[i + 1]). Each instance is one representation of the reference advanced
- * by its own OdeSolverEulerImplicit step (one Newton iteration) with its own time step dt[i] (NULL: the
- * options' dt for all) and its own Eigen CG (own tolerance test, iteration count and cap), as if
- * FemRepresentation::update ran once per instance (Physics/FreeMotion.cpp:52-63 runs one task per
- * representation). All instances advance in the same launches: per-instance offset tables, per-instance
+ * by its own step of the configured scheme (implicit Euler with one Newton iteration, or any other scheme)
+ * with its own time step dt[i] (NULL: the options' dt for all) and its own Eigen CG (own tolerance test,
+ * iteration count and cap), as if FemRepresentation::update ran once per instance
+ * (Physics/FreeMotion.cpp:52-63 runs one task per representation). All instances advance in the same launches: per-instance offset tables, per-instance
  * CG scalars with convergence masks, iteration chunks replayed as a captured hipGraph, and row-chunk and
  * lane shapes picked by an autotuner on the first step. Call after fem_upload_mesh. */
 int fem_set_instances(fem_ctx* ctx, int32_t num_instances, const int64_t* node_offsets, const double* dt);
