@@ -242,6 +242,10 @@ typedef struct fem_instance_stats {
     int64_t unconverged_steps;   /* steps whose solve stopped at its iteration cap */
     int32_t converged;           /* last step's solve met its tolerance */
     int32_t active;
+    int32_t solved_in_fp64;      /* FP32 + refinement mode: 1 once FP32 CG proved inadequate for the instance (its
+                                    refinement stopped short of the tolerance, or an inner solve took more than a
+                                    quarter of its iteration cap); its later solves run in FP64 */
+    int32_t reserved;
 } fem_instance_stats;
 int fem_query_instances(fem_ctx* ctx, fem_instance_stats* stats);   /* num_instances entries */
 /* Fix the chunk shape (chunk_rows > 0 block rows per workgroup, lanes_per_row 4, 8 or 16) or return to

@@ -297,6 +297,11 @@ struct fem_ctx {
     DevBuf<CgScalars> b_sc, b_outer;
     DevBuf<int> b_kA, b_kV;
     DevBuf<int> b_active, b_invalid;   // [M] per-instance activity / non-finite flag of the last update
+    // FP32 + refinement mode: instances for which FP32 CG proved inadequate are solved in FP64 from then on
+    // (h_inst_fp64, the FP32 pass's skip flags in b_skip32); b_skip64 flags the instances a step's FP64 pass
+    // leaves alone
+    std::vector<int32_t> h_inst_fp64;
+    DevBuf<int> b_skip32, b_skip64;
     DevBuf<int> b_live;                // compacted chunk list of instances still iterating
     int* h_live = nullptr;             // pinned staging for b_live
     std::vector<int32_t> h_inst_active;
@@ -437,8 +442,10 @@ void alloc_precision_buffers(fem_ctx* c) {
         if (fp32) c->S32.alloc((size_t)c->num_blocks * 9);
         else c->S32.release();
     }
+    // batched instances in the FP32 mode solve some instances in an FP64 pass on the same vectors
+    const size_t tvb = c->num_instances > 0 ? 8 : tv;
     for (auto* b : {&c->cg_x, &c->cg_r, &c->cg_z, &c->cg_p0, &c->cg_p1, &c->cg_q, &c->cg_dinv, &c->cg_b})
-        b->alloc(n * tv);
+        b->alloc(n * tvb);
     if (c->kind != FEM_ELEMENT_HEX)
         for (auto* b : {&c->cg_r1, &c->cg_z1, &c->cg_q1, &c->cg_w0, &c->cg_w1}) b->alloc(n * tv);
     if (c->kind != FEM_ELEMENT_HEX && fp32) {
@@ -1250,7 +1257,9 @@ void batch_linear_solve(fem_ctx* c) {
     const size_t n = (size_t)c->ndof;
     HIPCHECK(hipMemsetAsync(c->dv.p, 0, n * sizeof(double), c->stream));
     BatchCg a = batch_args(c, true);
+    a.t.inst_skip = c->b_skip32.p;  // instances solved in FP64 (below) are left alone here
     int sweeps = 0, longest = 0;
+    std::vector<int> sweep_max(M, 0);  // each instance's longest inner solve of this step
     for (int sweep = 0; sweep <= c->opt.max_refinements; ++sweep) {
         launch_batch_residual(a, c->S64.p, c->b.p, c->dv.p, sweep == 0, c->r64.p, c->b_outer.p, c->opt.cg_tolerance,
                               c->b_lanes, c->stream);
@@ -1259,7 +1268,10 @@ void batch_linear_solve(fem_ctx* c) {
         int& pred = c->b_predicted[1 + std::min(sweep, 15)];
         longest += run_batch_iterations(c, a, pred) + 1;
         for (int i = 0; i < M; ++i)
-            if (!c->h_bsc[i].converged || c->h_bsc[i].iters > 0) c->h_inst_iters[i] += c->h_bsc[i].iters;
+            if (!c->h_bsc[i].converged || c->h_bsc[i].iters > 0) {
+                c->h_inst_iters[i] += c->h_bsc[i].iters;
+                sweep_max[i] = std::max(sweep_max[i], c->h_bsc[i].iters);
+            }
         HIPCHECK(hipMemcpyAsync(c->h_bsc, c->b_outer.p, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         bool all = true;
@@ -1270,15 +1282,62 @@ void batch_linear_solve(fem_ctx* c) {
     }
     HIPCHECK(hipMemcpyAsync(c->h_bsc, c->b_outer.p, M * sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    // FP32 CG is not enough for some operators (ill-conditioned or indefinite ones, typically a collapsing mesh):
+    // its inner solves then take many times the iterations of FP64 CG, or reach the cap. An instance whose
+    // FP32 refinement stopped short of the outer tolerance, or one of whose inner solves took more than a
+    // quarter of its iteration cap (at least 64), is solved in FP64 from the next step on, as the FP64 mode
+    // would solve it; one that stopped short is solved in FP64 in this step too. The switch is per instance
+    // and lasts until the instances are declared again.
+    std::vector<int32_t> solve64(M, 0);
+    bool changed = false, any64 = false;
+    for (int i = 0; i < M; ++i) {
+        if (!c->h_inst_active[i]) continue;
+        if (c->h_inst_fp64[i]) {
+            solve64[i] = 1;
+        } else {
+            const int64_t cap = c->opt.cg_max_iterations < 0 ? 6 * (c->h_node_off[i + 1] - c->h_node_off[i])
+                                                             : c->opt.cg_max_iterations;
+            const bool slow = sweep_max[i] > std::max<int64_t>(64, cap / 4);
+            if (!c->h_bsc[i].converged || slow) {
+                c->h_inst_fp64[i] = 1;
+                changed = true;
+            }
+            solve64[i] = c->h_bsc[i].converged ? 0 : 1;
+        }
+        any64 = any64 || solve64[i];
+    }
+    for (int i = 0; i < M; ++i)
+        if (!solve64[i]) {
+            c->h_inst_err[i] = c->h_bsc[i].err;
+            c->h_inst_conv[i] = c->h_bsc[i].converged;
+        }
+    if (changed) {
+        c->b_skip32.upload(c->h_inst_fp64.data(), M, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    if (any64) {
+        std::vector<int32_t> comp(M);
+        for (int i = 0; i < M; ++i) comp[i] = solve64[i] ? 0 : 1;
+        c->b_skip64.upload(comp.data(), M, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));  // comp is freed on return
+        BatchCg a64 = batch_args(c, false);  // FP64 operator, x = dv, the Jacobi preconditioner
+        a64.t.inst_skip = c->b_skip64.p;
+        launch_batch_cg_init(a64, c->b.p, nullptr, c->opt.cg_tolerance, c->b_lanes, c->stream);
+        longest += run_batch_iterations(c, a64, c->b_predicted[0]);
+        for (int i = 0; i < M; ++i)
+            if (solve64[i]) {
+                c->h_inst_iters[i] += c->h_bsc[i].iters;
+                c->h_inst_err[i] = c->h_bsc[i].err;
+                c->h_inst_conv[i] = c->h_bsc[i].converged;
+            }
+    }
     int64_t total = 0;
     bool conv = true;
     double err = 0.0;
     for (int i = 0; i < M; ++i) {
-        c->h_inst_err[i] = c->h_bsc[i].err;
-        c->h_inst_conv[i] = c->h_bsc[i].converged;
         total += c->h_inst_iters[i];
-        conv = conv && c->h_bsc[i].converged;
-        err = std::max(err, c->h_bsc[i].err);
+        conv = conv && c->h_inst_conv[i];
+        err = std::max(err, c->h_inst_err[i]);
     }
     c->stats.last_cg_iterations = longest;
     c->stats.total_cg_iterations += total;
@@ -2762,6 +2821,11 @@ int fem_set_instances(fem_ctx* c, int32_t num_instances, const int64_t* node_off
         c->b_kV.alloc(M);
         c->h_inst_active.assign(M, 1);
         c->h_inst_unconv.assign(M, 0);
+        c->h_inst_fp64.assign(M, 0);
+        c->b_skip32.alloc(M);
+        c->b_skip32.upload(c->h_inst_fp64.data(), M, c->stream);
+        c->b_skip64.alloc(M);
+        c->last_precision = -1;  // the CG vectors are sized for the FP64 pass of a batch at the next step
         c->b_active.alloc(M);
         c->b_active.upload(c->h_inst_active.data(), M, c->stream);
         c->b_invalid.alloc(M);
@@ -2792,6 +2856,8 @@ int fem_query_instances(fem_ctx* c, fem_instance_stats* out) {
             out[i].last_iterations = c->h_inst_iters[i];
             out[i].last_error = c->h_inst_err[i];
             out[i].unconverged_steps = c->h_inst_unconv[i];
+            out[i].solved_in_fp64 = c->h_inst_fp64[i];
+            out[i].reserved = 0;
             out[i].converged = c->h_inst_conv[i];
             out[i].active = c->h_inst_active[i];
         }

@@ -76,10 +76,11 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init_kernel(BatchTables t, 
                                                                  const CgScalars* __restrict__ skip) {
     __shared__ double lds[64];
     const int c = blockIdx.x, inst = t.chunk_inst[c];
+    const bool left = (skip && skip[inst].done) || (t.inst_skip && t.inst_skip[inst]);
     double bb = 0.0, rz = 0.0;
     if constexpr (sizeof(TV) == 4) {
         // FP32 inner solves of the refinement: 3x3 block-diagonal preconditioner, dinv = 9 floats per node
-        if (!(skip && skip[inst].done)) {
+        if (!left) {
             for (int I = t.chunk_row0[c] + (int)threadIdx.x; I < t.chunk_row1[c]; I += blockDim.x) {
                 float Bf[9];
                 block_jacobi_inverse(S, op.diagblk[I], Bf);
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init_kernel(BatchTables t, 
                 }
             }
         }
-    } else if (!(skip && skip[inst].done)) {
+    } else if (!left) {
         const int d0 = 3 * t.chunk_row0[c], d1 = 3 * t.chunk_row1[c];
         for (int i = d0 + (int)threadIdx.x; i < d1; i += blockDim.x) {
             const int I = i / 3, cc = i - 3 * I;
@@ -132,11 +133,21 @@ __global__ __launch_bounds__(kBatchThreads) void bcg_init2_kernel(BatchTables t,
         kA[inst] = 0;
         kV[inst] = 0;
     }
-    if (skip && skip[inst].done) return;
+    if (skip && skip[inst].done) {  // no inner solve this sweep: no iterations of it either
+        if (threadIdx.x == 0) {
+            CgScalars s = {};
+            s.done = 1;
+            s.converged = 1;
+            sc[inst] = s;
+        }
+        return;
+    }
     double bb, rz;
     range_sum2(partial, partial + t.num_chunks, t.inst_chunk[inst], t.inst_chunk[inst + 1], bb, rz, lds);
     if (threadIdx.x == 0) {
-        if (!t.inst_active[inst]) {  // deactivated instance: no solve (dv is zeroed by the init kernel)
+        if (!t.inst_active[inst] || (t.inst_skip && t.inst_skip[inst])) {
+            // deactivated instance: no solve (dv is zeroed by the init kernel); an instance this pass leaves
+            // alone: no solve, its x untouched
             CgScalars s = {};
             s.done = 1;
             s.converged = 1;
@@ -351,7 +362,7 @@ __global__ __launch_bounds__(kBatchThreads) void brefine_check_kernel(BatchTable
         }
         o.rr = rr;
         o.err = o.bb > 0 ? sqrt(rr / o.bb) : 0.0;
-        if (!t.inst_active[inst] || o.bb == 0.0 || rr < o.threshold) {
+        if (!t.inst_active[inst] || (t.inst_skip && t.inst_skip[inst]) || o.bb == 0.0 || rr < o.threshold) {
             o.done = 1;
             o.converged = 1;
             inner[inst].done = 1;

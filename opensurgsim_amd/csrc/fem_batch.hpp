@@ -23,6 +23,9 @@ struct BatchTables {
     const int* inst_chunk;   // [M + 1]
     const int* inst_maxit;   // [M] iteration cap per instance
     const int* inst_active;  // [M] 0 once the instance's state became non-finite (deactivated)
+    // [M] 1: this solve leaves the instance alone (its x untouched, its scalars done); null: none. A batch in
+    // the FP32 + refinement mode solves the instances flagged for FP64 in a second, FP64 pass.
+    const int* inst_skip = nullptr;
     const int* live = nullptr;  // iteration kernels: workgroup b works on chunk live[b] (null: chunk b)
     int grid = 0;               // iteration kernels: workgroups launched (0: num_chunks)
 };

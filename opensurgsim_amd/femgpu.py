@@ -83,7 +83,8 @@ class FemStats(ctypes.Structure):
 
 class FemInstanceStats(ctypes.Structure):
     _fields_ = [("last_iterations", ctypes.c_int64), ("last_error", ctypes.c_double),
-                ("unconverged_steps", ctypes.c_int64), ("converged", ctypes.c_int32), ("active", ctypes.c_int32)]
+                ("unconverged_steps", ctypes.c_int64), ("converged", ctypes.c_int32), ("active", ctypes.c_int32),
+                ("solved_in_fp64", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class FemPhaseTimes(ctypes.Structure):
@@ -379,8 +380,8 @@ class FemGpu:
         self.num_instances = off.size - 1
 
     def instance_stats(self):
-        """Per-instance (last iterations, last error, converged) arrays; also sets last_instance_active and
-        last_instance_unconverged_steps."""
+        """Per-instance (last iterations, last error, converged) arrays; also sets last_instance_active,
+        last_instance_unconverged_steps and last_instance_fp64."""
         m = self.num_instances
         st = (FemInstanceStats * m)()
         self._check(self._L.fem_query_instances(self.h, st))
@@ -389,6 +390,7 @@ class FemGpu:
         conv = np.array([s.converged for s in st], dtype=np.int32)
         self.last_instance_active = np.array([s.active for s in st], dtype=np.int32)
         self.last_instance_unconverged_steps = np.array([s.unconverged_steps for s in st], dtype=np.int64)
+        self.last_instance_fp64 = np.array([s.solved_in_fp64 for s in st], dtype=np.int32)
         return it, err, conv
 
     def set_batch_tuning(self, chunk_rows=0, lanes_per_row=0, use_graphs=True):

@@ -42,3 +42,4 @@ def test_default_options_match_reference_defaults():
 def test_options_struct_layout():
     assert ctypes.sizeof(femgpu.FemOptions) == 120
     assert ctypes.sizeof(femgpu.FemStats) == 80
+    assert ctypes.sizeof(femgpu.FemInstanceStats) == 40

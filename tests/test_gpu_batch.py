@@ -287,3 +287,56 @@ def test_batch_linear_scheme_keeps_operator_until_instances_change():
         e = dict(u=rel(x[sl] - x0, xo - x0), v=rel(v[sl], vo))
         say("  instance", i, "dt", c.dt, "then", dt2[i], e)
         assert e["u"] < 1e-9 and e["v"] < 1e-9
+
+
+def test_fp32_batch_switches_inadequate_instances_to_fp64():
+    """In the FP32 + refinement mode an instance whose FP32 inner solve takes more than a quarter of its
+    iteration cap (or whose refinement stops short of the tolerance) is solved in FP64 from then on; the
+    others stay FP32. Here the cap is 1.5 times the FP64 iteration count of a stiff instance, so its FP32
+    inner solve (about half of that per sweep) passes a quarter of the cap, while a soft instance needs far
+    fewer iterations. Both instances stay within the FP32 bound of the FP64 batch, every step converges, and
+    after the switch the stiff instance takes the FP64 run's iteration counts."""
+    cfgs = mg.config_batched(1, seed=7, lo=24, hi=24) + mg.config_batched(1, seed=8, lo=6, hi=6)
+    cfgs[0]["E"] = np.full(cfgs[0].conn.shape[0], 1e6)
+    cfgs[0]["dt"] = 2e-3
+    cfgs[1]["E"] = np.full(cfgs[1].conn.shape[0], 3e3)
+    cfgs[1]["dt"] = 5e-4
+    b = mg.concat_instances(cfgs)
+
+    def run(precision, cap, steps):
+        g = femgpu.FemGpu(0)
+        g.upload_mesh(b.kind, b.x0, b.conn, b.E, b.nu, b.rho)
+        g.add_bc_nodes(b.bc_nodes)
+        g.set_options(dt=1e-3, cg_tolerance=1e-8, precision=precision, cg_max_iterations=cap)
+        g.set_instances(b.node_offsets, b.dt)
+        its, flags = [], []
+        for _ in range(steps):
+            assert g.step(1) == femgpu.FEM_OK
+            it, _, conv = g.instance_stats()
+            assert conv.all()
+            its.append(it.copy())
+            flags.append(g.last_instance_fp64.copy())
+        x, v = g.state()
+        g.close()
+        return x, v, np.array(its), np.array(flags)
+
+    _, _, its64, _ = run(femgpu.FP64, 100000, 1)
+    n64 = int(its64[0, 0])
+    say("FP64 iterations of the stiff instance", n64, "of the soft one", int(its64[0, 1]))
+    assert n64 >= 160 and its64[0, 1] < 64
+    cap = int(1.5 * n64)
+    steps = 4
+    x64, v64, its64, flags64 = run(femgpu.FP64, cap, steps)
+    x32, v32, its32, flags32 = run(femgpu.FP32_REFINED, cap, steps)
+    say("  cap", cap, "FP64 its", its64.tolist(), "mixed its", its32.tolist(), "flags", flags32.tolist())
+    assert not flags64.any()
+    assert flags32[0].tolist() == [1, 0] and flags32[-1].tolist() == [1, 0]
+    for k in range(1, steps):  # after the switch the stiff instance solves in FP64 from x = 0
+        assert abs(int(its32[k, 0]) - int(its64[k, 0])) <= 1
+    off = b.node_offsets
+    x0 = b.x0.reshape(-1)
+    for i in range(2):
+        sl = slice(3 * off[i], 3 * off[i + 1])
+        e = dict(u=rel(x32[sl] - x0[sl], x64[sl] - x0[sl]), v=rel(v32[sl], v64[sl]))
+        say("  instance", i, e)
+        assert e["u"] < 1e-6 and e["v"] < 1e-6
