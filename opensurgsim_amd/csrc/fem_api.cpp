@@ -283,6 +283,11 @@ struct fem_ctx {
     DeviceBandLU lu;
     int lu_bw = 0;               // bandwidth of the pattern in DOFs: 3 max|I - J| + 2
     bool lu_current = false;     // lu holds the factors of the current S64
+    // The dense compliance kept by the linear implicit scheme with the direct solver (formed on its first step,
+    // as the reference's first pass forms it; n x n column-major, boundary rows and columns zero), valid while the
+    // factors it came from are; gemv_part: the column-chunk partial products that apply it.
+    DevBuf<double> dense_c, gemv_part;
+    bool dense_c_valid = false;
 
     // hexahedra (element-by-element path)
     HexData hex;
@@ -592,10 +597,17 @@ void ensure_lu(fem_ctx* c, bool may_allocate = true) {
     c->lu_current = true;
 }
 
-// dv = A^-1 b through the band LU (LinearSparseSolveAndInverseLU::solve, Math/LinearSparseSolveAndInverse.cpp:92-97).
+// dv = A^-1 b through the band LU (LinearSparseSolveAndInverseLU::solve, Math/LinearSparseSolveAndInverse.cpp:92-97),
+// or, when the dense compliance of the same factors is kept, dv = C b: the same solution to rounding (C has the
+// boundary rows and columns zeroed; b's boundary rows are zero, and the band solve returns them zero too).
 void lu_linear_solve(fem_ctx* c) {
-    ensure_lu(c);
-    TIMED(c, FEM_PHASE_CG_INIT, 0, launch_band_solve(c->lu, 1, c->b.p, c->ndof, c->dv.p, c->ndof, c->stream));
+    if (c->dense_c_valid) {
+        TIMED(c, FEM_PHASE_CG_INIT, 0,
+              launch_dense_gemv(c->ndof, c->dense_c.p, c->b.p, c->gemv_part.p, c->dv.p, c->stream));
+    } else {
+        ensure_lu(c);
+        TIMED(c, FEM_PHASE_CG_INIT, 0, launch_band_solve(c->lu, 1, c->b.p, c->ndof, c->dv.p, c->ndof, c->stream));
+    }
     c->stats.last_cg_iterations = 0;
     c->stats.last_cg_error = 0.0;
     c->stats.converged = 1;
@@ -819,6 +831,7 @@ void assemble_tet(fem_ctx* c) {
     if (c->ext_nblk) launch_ext_operator(c->ext_nblk, c->ext_blk.p, c->ext_add.p, c->S64.p, s32, c->stream);
     if (c->timer.enabled) c->timer.end(c->stream);
     c->lu_current = false;
+    c->dense_c_valid = false;
 }
 
 // rhs = (staged element rows + gravity + external forces) / div with boundary rows zeroed, plus the step's
@@ -1584,6 +1597,8 @@ void step_other_scheme(fem_ctx* c) {
     if (batched) batch_deactivate(c);
 }
 
+void keep_dense_compliance(fem_ctx* c);
+
 // One step of the configured scheme (OdeSolver::solve with computeCompliance = false). Returns FEM_OK or
 // FEM_INACTIVE.
 int step_once(fem_ctx* c) {
@@ -1616,6 +1631,7 @@ int step_once(fem_ctx* c) {
         if (linear_scheme) {
             c->linear_matrix_ready = true;
             c->matrix_dt = dt;
+            if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) keep_dense_compliance(c);
         }
     } else {
         // Newton-Raphson (Math/OdeSolverEulerImplicit.cpp:84-115): newState starts as the current state.
@@ -2431,6 +2447,23 @@ void compliance_columns_dense(fem_ctx* c, int64_t j0, int cnt, double* out) {
     launch_zero_bc_rows_cols(n, j0, cnt, c->bcmask.p, out, c->stream);
 }
 
+// OdeSolverLinearEulerImplicit forms the compliance on its first pass and keeps it (computeCompliance = true,
+// Math/OdeSolverLinearEulerImplicit.cpp:43-51; "the compliance matrix is constant and used in all following
+// calls"). Kept here when it fits kDenseKeepBytes; the later steps then apply it (lu_linear_solve) and
+// fem_get_compliance_matrix returns it.
+constexpr size_t kDenseKeepBytes = size_t(1) << 30;
+
+void keep_dense_compliance(fem_ctx* c) {
+    const int64_t n = c->ndof;
+    if ((size_t)n * (size_t)n * sizeof(double) > kDenseKeepBytes) return;
+    ensure_bc(c);
+    ensure_lu(c);
+    if (c->dense_c.n != (size_t)(n * n)) c->dense_c.alloc((size_t)(n * n));
+    if (c->gemv_part.n != (size_t)(gemv_chunks(n) * n)) c->gemv_part.alloc((size_t)(gemv_chunks(n) * n));
+    compliance_columns_dense(c, 0, (int)n, c->dense_c.p);
+    c->dense_c_valid = true;
+}
+
 void check_dense_compliance(fem_ctx* c) {
     if (!c->mesh_ready) throw InvalidArg("mesh not uploaded");
     if (c->kind == FEM_ELEMENT_HEX || c->num_instances > 0)
@@ -2445,8 +2478,14 @@ int fem_get_compliance_matrix(fem_ctx* c, double* out) {
         if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
         HIPCHECK(hipSetDevice(c->device));
         ensure_bc(c);
-        ensure_lu(c);
         const int64_t n = c->ndof;
+        if (c->dense_c_valid) {  // the kept compliance of the current factors
+            HIPCHECK(hipMemcpyAsync(out, c->dense_c.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            return FEM_OK;
+        }
+        ensure_lu(c);
         const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)((size_t(256) << 20) / (8 * n))));
         DevBuf<double> buf;
         buf.alloc((size_t)n * chunk);
@@ -2468,8 +2507,13 @@ int fem_get_compliance_matrix_device(fem_ctx* c, double* out) {
         if (!c->have_operator) return fail(c, FEM_ERR_ORDER, "no system matrix yet: the solver has not stepped");
         HIPCHECK(hipSetDevice(c->device));
         ensure_bc(c);
-        ensure_lu(c);
         const int64_t n = c->ndof;
+        if (c->dense_c_valid) {  // the kept compliance of the current factors
+            HIPCHECK(hipMemcpyAsync(out, c->dense_c.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice,
+                                    c->stream));
+            return FEM_OK;
+        }
+        ensure_lu(c);
         for (int64_t j0 = 0; j0 < n; j0 += 65536)
             compliance_columns_dense(c, j0, (int)std::min<int64_t>(65536, n - j0), out + (size_t)j0 * n);
         return FEM_OK;

@@ -298,6 +298,48 @@ void launch_zero_bc_rows_cols(int64_t n, int64_t first, int count, const uint8_t
              out);
 }
 
+// Partial products of a dense column-major C: workgroup (row tile, column chunk), one thread per row, the
+// chunk's columns in order; the chunk's x values staged in LDS.
+__global__ __launch_bounds__(256) void dense_gemv_part_kernel(int64_t n, const double* __restrict__ C,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ part) {
+    __shared__ double xs[kGemvCols];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.y * kGemvCols;
+    const int64_t j1 = min(n, j0 + kGemvCols);
+    if (threadIdx.x < kGemvCols && j0 + threadIdx.x < j1) xs[threadIdx.x] = x[j0 + threadIdx.x];
+    __syncthreads();
+    if (i >= n) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const double* col = C + j0 * n + i;
+    const int m = (int)(j1 - j0);
+    int k = 0;
+    for (; k + 3 < m; k += 4) {
+        a0 += col[(int64_t)k * n] * xs[k];
+        a1 += col[(int64_t)(k + 1) * n] * xs[k + 1];
+        a2 += col[(int64_t)(k + 2) * n] * xs[k + 2];
+        a3 += col[(int64_t)(k + 3) * n] * xs[k + 3];
+    }
+    for (; k < m; ++k) a0 += col[(int64_t)k * n] * xs[k];
+    part[(int64_t)blockIdx.y * n + i] = (a0 + a1) + (a2 + a3);
+}
+
+__global__ __launch_bounds__(256) void dense_gemv_sum_kernel(int64_t n, int64_t chunks,
+                                                             const double* __restrict__ part, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) acc += part[c * n + i];
+    y[i] = acc;
+}
+
+void launch_dense_gemv(int64_t n, const double* C, const double* x, double* part, double* y, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t chunks = gemv_chunks(n);
+    launch_k(dense_gemv_part_kernel, dim3(grid_for(n, 256), (unsigned)chunks), dim3(256), 0, s, n, C, x, part);
+    launch_k(dense_gemv_sum_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, chunks, (const double*)part, y);
+}
+
 void launch_zero_masked_rows(int64_t n, int count, const uint8_t* mask, double* x, hipStream_t s) {
     launch_k(zero_masked_rows_kernel, dim3(grid_for(n * (int64_t)count, 256)), dim3(256), 0, s, n, count, mask, x);
 }

@@ -44,4 +44,10 @@ void launch_zero_bc_rows_cols(int64_t n, int64_t first, int count, const uint8_t
 // Rows of the boundary DOFs (mask != 0) of count columns of n values (column-major) set to zero.
 void launch_zero_masked_rows(int64_t n, int count, const uint8_t* mask, double* x, hipStream_t s);
 
+// y = C x for a dense n x n column-major C: partial sums over column chunks of kGemvCols (part holds
+// gemv_chunks(n) x n doubles), then each row's partials added in chunk order (deterministic).
+constexpr int kGemvCols = 128;
+inline int64_t gemv_chunks(int64_t n) { return (n + kGemvCols - 1) / kGemvCols; }
+void launch_dense_gemv(int64_t n, const double* C, const double* x, double* part, double* y, hipStream_t s);
+
 }  // namespace femgpu
