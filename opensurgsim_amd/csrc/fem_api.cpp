@@ -592,6 +592,7 @@ void ensure_lu(fem_ctx* c, bool may_allocate = true) {
     lu.n = c->ndof;
     lu.bw = c->lu_bw;
     lu.ld = 2 * (int64_t)c->lu_bw + 1;
+    lu.inv = lu.ab + lu.n * lu.ld;
     launch_band_fill(lu, c->num_blocks, c->blk_row.p, c->colidx_b.p, c->S64.p, c->stream);
     launch_band_factor(lu, c->stream);
     c->lu_current = true;
@@ -2443,7 +2444,7 @@ namespace {
 void compliance_columns_dense(fem_ctx* c, int64_t j0, int cnt, double* out) {
     const int64_t n = c->ndof;
     launch_identity_columns(n, j0, cnt, out, c->stream);
-    launch_band_solve(c->lu, cnt, out, n, out, n, c->stream);
+    launch_band_solve(c->lu, cnt, out, n, out, n, c->stream, j0);
     launch_zero_bc_rows_cols(n, j0, cnt, c->bcmask.p, out, c->stream);
 }
 

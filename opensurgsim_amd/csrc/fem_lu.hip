@@ -1,17 +1,22 @@
-// Banded LU without pivoting in FP64 (see fem_lu.hpp).
+// Banded factorization without pivoting in FP64 (see fem_lu.hpp).
 //
-// Factorization: right-looking, blocked by panels of kLuPanel columns. For panel [k0, k1):
-//   1. one workgroup factors the panel column by column: multipliers l(i, c) = a(i, c) / a(c, c) for the rows
-//      below the diagonal within the band, and the rank-1 updates restricted to the panel's columns (the L21
-//      block, rows up to k1 - 1 + bw) and the panel's rows (the U12 block, columns up to k1 - 1 + bw);
-//   2. the trailing block a(i, j), i, j in [k1, k1 + bw), takes -= L21 U12, by 64 x 64 output tiles staged
-//      through LDS (the entries of L21 and U12 outside the band are zero).
-// The same elimination as the column-by-column form of the restated reference solver, with the panel's updates
-// of the trailing block summed before they are applied, so results agree to rounding.
+// The operator is symmetric (its lower triangle mirrored), so the elimination runs on the lower triangle only
+// (LDL^T) and writes the LU form the solves read: L (unit lower) below the diagonal, U = D L^T on and above it.
+// Row c of U is column c of the eliminated lower triangle, the value the elimination divides by the pivot.
+// Right-looking, by panels of kLuPanel columns. For panel [k0, k1):
+//   1. its diagonal block is eliminated by one wavefront in registers (for the first panel by its own launch,
+//      for the others by the first workgroup of the previous panel's trailing update);
+//   2. the rows below, k1 .. k1 + bw - 1, are eliminated against it, one lane per row: l(i, c) and u(c, i);
+//   3. the trailing block a(i, j), i >= j in [k1, k1 + bw), takes -= L21 U12 by 64 x 64 output tiles staged
+//      through LDS (lower tiles only: the upper band is rewritten from L by step 2 before anything reads it).
+// Each entry takes the column-by-column updates in the same order as the column form of the restated reference
+// solver, with the panel's updates of the trailing block summed before they are applied: results agree with
+// unblocked LU to rounding.
 //
-// Substitution: one workgroup per right-hand side. Rows go in blocks of 64: the block's products with the
-// already solved entries (up to bw of them per row) are summed by 16 lanes per row, then one wavefront solves
-// the 64 x 64 triangle, staged in LDS, with the solved entries broadcast lane to lane.
+// Substitution: the 64 x 64 diagonal blocks of L and U are inverted once per factorization. A solve walks the
+// rows in blocks of 64: the block's products with the entries solved before it (up to bw per row) as an LDS-tiled
+// product over 64-entry tiles (the next tile prefetched into registers), then the block's triangle as a product
+// with its inverse. One workgroup per 16 (or 4) right-hand sides; sums in a fixed order.
 #include "fem_lu.hpp"
 #include "fem_kernels.hpp"
 
@@ -30,6 +35,14 @@ __device__ __forceinline__ double& at(double* ab, int64_t ld, int bw, int64_t i,
 }
 __device__ __forceinline__ double atc(const double* ab, int64_t ld, int bw, int64_t i, int64_t j) {
     return ab[i * ld + (j - i + bw)];
+}
+
+// Value of v in lane `lane` (a constant after unrolling) for every lane.
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 __global__ __launch_bounds__(256) void band_zero_kernel(double* __restrict__ ab, int64_t count) {
@@ -51,40 +64,88 @@ __global__ __launch_bounds__(256) void band_fill_kernel(double* __restrict__ ab,
         for (int c = 0; c < 3; ++c) at(ab, ld, bw, 3 * I + r, 3 * J + c) = S[9 * k + 3 * r + c];
 }
 
-// Panel [k0, k1) factored in place by one workgroup (global memory; every step ends with a barrier).
-__global__ __launch_bounds__(1024) void band_panel_kernel(double* __restrict__ ab, int64_t n, int bw, int64_t ld,
-                                                          int64_t k0, int64_t k1) {
-    for (int64_t c = k0; c < k1; ++c) {
-        // l(i, c) is non-zero for i in (c, c + bw], u(c, j) for j in (c, c + bw]; every product lands in the band
-        const int64_t iend = min(n - 1, c + bw);  // last row with a multiplier in column c
-        const int64_t jend = min(n - 1, c + bw);  // last column of row c in the band
-        const double piv = atc(ab, ld, bw, c, c);
-        for (int64_t i = c + 1 + threadIdx.x; i <= iend; i += blockDim.x) at(ab, ld, bw, i, c) /= piv;
-        __syncthreads();
-        // rank-1 update restricted to the panel's columns (c, min(k1 - 1, jend)] for rows (c, iend] ...
-        const int64_t ncol = min(k1 - 1, jend) - c, nrow = iend - c;
-        for (int64_t t = threadIdx.x; t < ncol * nrow; t += blockDim.x) {
-            const int64_t i = c + 1 + t % nrow, j = c + 1 + t / nrow;
-            at(ab, ld, bw, i, j) -= atc(ab, ld, bw, i, c) * atc(ab, ld, bw, c, j);
+// The nb x nb diagonal block at k0 eliminated by one wavefront (lanes 0..63 of the calling workgroup): lane i
+// holds row i of the block's lower triangle; at step c the pivot and every lane's column-c value, the eliminated
+// a(j, c) = u(c, j), are broadcast by readlane. Writes L below the diagonal, U on and above it.
+__device__ void factor_diag_block(double* __restrict__ ab, int bw, int64_t ld, int64_t k0, int nb, int lane) {
+    double a[kLuPanel], piv[kLuPanel];
+#pragma unroll
+    for (int j = 0; j < kLuPanel; ++j)
+        a[j] = (lane < nb && j <= lane && lane - j <= bw) ? atc(ab, ld, bw, k0 + lane, k0 + j) : 0.0;
+#pragma unroll
+    for (int c = 0; c < kLuPanel; ++c) {
+        piv[c] = 1.0;
+        if (c < nb) {
+            piv[c] = readlane_d(a[c], c);
+            const double l = a[c] / piv[c];
+#pragma unroll
+            for (int j = c + 1; j < kLuPanel; ++j) {
+                const double ucj = readlane_d(a[c], j);
+                if (lane >= j) a[j] -= l * ucj;
+            }
         }
-        // ... and to the panel's rows (c, min(k1 - 1, iend)] for columns [k1, jend] (U12)
-        const int64_t nrow2 = min(k1 - 1, iend) - c;
-        const int64_t nu = jend >= k1 ? jend - k1 + 1 : 0;
-        for (int64_t t = threadIdx.x; t < nrow2 * nu; t += blockDim.x) {
-            const int64_t i = c + 1 + t / nu, j = k1 + t % nu;
-            at(ab, ld, bw, i, j) -= atc(ab, ld, bw, i, c) * atc(ab, ld, bw, c, j);
+    }
+    if (lane >= nb) return;
+#pragma unroll
+    for (int c = 0; c < kLuPanel; ++c) {
+        if (c < lane && lane - c <= bw) {
+            at(ab, ld, bw, k0 + lane, k0 + c) = a[c] / piv[c];
+            at(ab, ld, bw, k0 + c, k0 + lane) = a[c];
         }
-        __syncthreads();
+        if (c == lane) at(ab, ld, bw, k0 + lane, k0 + lane) = a[c];
     }
 }
 
-// Trailing update a(i, j) -= sum_c l(i, c) u(c, j) over the panel [k0, k1), i, j in [k1, k1 + m): one 64 x 64
-// output tile per workgroup, 4 x 4 outputs per thread, panel factors staged in LDS.
-__global__ __launch_bounds__(256) void band_update_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
-                                                          int64_t k1, int64_t m) {
+__global__ __launch_bounds__(64) void ldl_diag_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
+                                                      int nb) {
+    factor_diag_block(ab, bw, ld, k0, nb, threadIdx.x);
+}
+
+// Rows k1 + 64 blockIdx.x + lane (< k1 + m) of panel [k0, k1 = k0 + nb), eliminated against the panel's factored
+// diagonal block (U11 from the upper band, staged in LDS): s_c = a(i, c) - sum_{k<c} l(i, k) u(k, c) in order of
+// k, then l(i, c) = s_c / u(c, c) into row i's lower band and s_c = u(c, i) into row c's upper band.
+__global__ __launch_bounds__(64) void ldl_panel_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
+                                                       int nb, int64_t m) {
+    __shared__ double Us[kLuPanel][kLuPanel + 1];
+    const int lane = threadIdx.x;
+    for (int e = lane; e < kLuPanel * kLuPanel; e += 64) {
+        const int c = e / kLuPanel, j = e % kLuPanel;
+        Us[c][j] = (c < nb && j < nb && j >= c && j - c <= bw) ? atc(ab, ld, bw, k0 + c, k0 + j) : 0.0;
+    }
+    __syncthreads();
+    const int64_t k1 = k0 + nb;
+    const int64_t i = k1 + (int64_t)blockIdx.x * 64 + lane;
+    if (i >= k1 + m) return;
+    double r[kLuPanel];
+#pragma unroll
+    for (int c = 0; c < kLuPanel; ++c) r[c] = (c < nb && i - (k0 + c) <= bw) ? atc(ab, ld, bw, i, k0 + c) : 0.0;
+#pragma unroll
+    for (int c = 0; c < kLuPanel; ++c) {
+        if (c < nb) {
+            double s = r[c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) s -= r[k] * Us[k][c];
+            r[c] = s / Us[c][c];
+            if (i - (k0 + c) <= bw) {  // beyond the band s and l are zero
+                at(ab, ld, bw, k0 + c, i) = s;
+                at(ab, ld, bw, i, k0 + c) = r[c];
+            }
+        }
+    }
+}
+
+// Trailing update a(i, j) -= sum_c l(i, c) u(c, j) over the panel [k0, k1), i >= j in [k1, k1 + m) by 64 x 64
+// lower output tiles (blockIdx.x enumerates (ti, tj), tj <= ti, row by row), 4 x 4 outputs per thread, the panel's
+// L21 and U12 staged in LDS. The first workgroup then eliminates the next panel's diagonal block (rows
+// [k1, k1 + next_nb), all inside its own tile or untouched by this update).
+__global__ __launch_bounds__(256) void ldl_update_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
+                                                         int64_t k1, int64_t m, int next_nb) {
     __shared__ double Ls[kTile][kLuPanel + 1];
     __shared__ double Us[kLuPanel][kTile + 1];
-    const int64_t i0 = k1 + (int64_t)blockIdx.y * kTile, j0 = k1 + (int64_t)blockIdx.x * kTile;
+    int ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= (int)blockIdx.x) ++ti;
+    const int tj = (int)blockIdx.x - ti * (ti + 1) / 2;
+    const int64_t i0 = k1 + (int64_t)ti * kTile, j0 = k1 + (int64_t)tj * kTile;
     const int64_t iend = k1 + m, jend = k1 + m;
     const int nb = (int)(k1 - k0);
     const int tid = threadIdx.x;
@@ -123,107 +184,175 @@ __global__ __launch_bounds__(256) void band_update_kernel(double* __restrict__ a
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int64_t j = j0 + tx + 16 * b;
-            if (j >= jend) continue;
+            if (j >= jend || j - i > bw || i - j > bw) continue;
             at(ab, ld, bw, i, j) -= acc[a][b];
         }
     }
+    if (blockIdx.x == 0 && next_nb > 0) {
+        __syncthreads();  // this tile's updated values visible to wavefront 0
+        if (tid < 64) factor_diag_block(ab, bw, ld, k1, next_nb, tid);
+    }
 }
 
-// x = U^-1 L^-1 b for column blockIdx.x, 1024 threads. Rows go in blocks of 64: 16 lanes per row sum the row's
-// products with the entries solved before the block (a contiguous stretch of the band row, read coalesced, four
-// independent accumulators per lane, then a butterfly over the 16 lanes), while the block's own triangle is
-// staged in LDS; wavefront 0 then solves the 64 x 64 triangle from LDS with the solved entries broadcast lane to
-// lane. Sums are in a fixed order.
-constexpr int kSolveThreads = 1024;
-__global__ __launch_bounds__(kSolveThreads) void band_solve_kernel(const double* __restrict__ ab, int64_t n, int bw,
-                                                                   int64_t ld, const double* b, int64_t ldb, double* x,
-                                                                   int64_t ldx) {
-    __shared__ double T[kTile][kTile + 1];
-    __shared__ double rs[kTile];
-    const double* bc = b + (int64_t)blockIdx.x * ldb;
-    double* xc = x + (int64_t)blockIdx.x * ldx;
-    const int tid = threadIdx.x, row = tid >> 4, q = tid & 15, lane = tid & 63;
-    auto row_sum = [&](double s) {
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (q == 0) rs[row] = s;
-    };
-    // forward: y = L^-1 b (unit diagonal), y written to xc
-    for (int64_t r0 = 0; r0 < n; r0 += kTile) {
-        for (int t = tid; t < kTile * kTile; t += kSolveThreads) {
-            const int l = t >> 6, k = t & 63;
-            const int64_t ii = r0 + l, kk = r0 + k;
-            T[l][k] = (ii < n && kk < ii && ii - kk <= bw) ? atc(ab, ld, bw, ii, kk) : 0.0;
-        }
-        const int64_t i = r0 + row;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        if (i < n) {
-            const int64_t kb = max((int64_t)0, i - bw);
-            const double* arow = ab + i * ld + (kb - i + bw);  // L(i, kb), L(i, kb + 1), ...
-            const double* xk = xc + kb;
-            const int cnt = (int)(r0 - kb);  // solved entries k in [kb, r0)
-            int k = q;
-            for (; k + 48 < cnt; k += 64) {
-                s0 += arow[k] * xk[k];
-                s1 += arow[k + 16] * xk[k + 16];
-                s2 += arow[k + 32] * xk[k + 32];
-                s3 += arow[k + 48] * xk[k + 48];
-            }
-            for (; k < cnt; k += 16) s0 += arow[k] * xk[k];
-        }
-        row_sum((s0 + s1) + (s2 + s3));
-        __syncthreads();
-        if (tid < 64) {
-            const int64_t ii = r0 + lane;
-            double y = ii < n ? bc[ii] - rs[lane] : 0.0;
-            for (int k = 0; k < kTile; ++k) {
-                const double yk = __shfl(y, k, 64);
-                if (lane > k) y -= T[lane][k] * yk;
-            }
-            if (ii < n) xc[ii] = y;
-        }
-        __syncthreads();
+// Inverse of the 64 x 64 diagonal block blockIdx.x of L (blockIdx.y = 0, unit lower) or U (1, upper), row-major
+// at inv + (2 blockIdx.x + blockIdx.y) 64^2; rows past n padded with the identity. Lane c forms column c by
+// substitution on the identity column.
+__global__ __launch_bounds__(64) void band_block_inverse_kernel(const double* __restrict__ ab, int64_t n, int bw,
+                                                                int64_t ld, double* __restrict__ inv) {
+    __shared__ double T[kTile][kTile];  // read at one address per step by every lane
+    __shared__ double X[kTile][kTile];  // lane c's column
+    const int64_t r0 = (int64_t)blockIdx.x * kTile;
+    const int mb = (int)min((int64_t)kTile, n - r0);
+    const bool upper = blockIdx.y == 1;
+    const int c = threadIdx.x;
+    for (int e = c; e < kTile * kTile; e += 64) {
+        const int r = e >> 6, k = e & 63;
+        double v = (r == k) ? 1.0 : 0.0;
+        if (r < mb && k < mb && (upper ? (k >= r && k - r <= bw) : (k < r && r - k <= bw)))
+            v = atc(ab, ld, bw, r0 + r, r0 + k);
+        T[r][k] = v;
     }
-    // backward: x = U^-1 y
+    __syncthreads();
+    if (!upper) {
+        for (int r = 0; r < kTile; ++r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            for (int k = c; k < r; ++k) s -= T[r][k] * X[k][c];
+            X[r][c] = s;
+        }
+    } else {
+        for (int r = kTile - 1; r >= 0; --r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            for (int k = r + 1; k <= c; ++k) s -= T[r][k] * X[k][c];
+            X[r][c] = s / T[r][r];
+        }
+    }
+    __syncthreads();
+    double* out = inv + ((int64_t)blockIdx.x * 2 + blockIdx.y) * (kTile * kTile);
+    for (int e = c; e < kTile * kTile; e += 64) out[e] = X[e >> 6][e & 63];
+}
+
+// x = U^-1 L^-1 b for columns [R blockIdx.x, R blockIdx.x + R) of nrhs, 256 threads: thread (row l = tid % 64,
+// wavefront g) owns R / 4 columns of the row block. identity_first >= 0: b holds columns identity_first + j of
+// the identity, whose forward solutions are zero above row identity_first + j; the forward pass starts at the
+// row block of the first column (rows above it written zero). x may alias b.
+template <int R>
+__global__ __launch_bounds__(256) void band_solve_blocked_kernel(const double* __restrict__ ab,
+                                                                 const double* __restrict__ inv, int64_t n, int bw,
+                                                                 int64_t ld, int nrhs, const double* b, int64_t ldb,
+                                                                 double* x, int64_t ldx, int64_t identity_first) {
+    constexpr int CPT = R / 4;
+    constexpr int PA = kTile * kTile / 256;  // tile values each thread stages
+    constexpr int PY = kTile * R / 256;
+    __shared__ double At[kTile][kTile + 1];  // At[k][row]
+    __shared__ double Ys[kTile][R + 1];      // Ys[k][column]
+    __shared__ double Rs[kTile][R + 1];      // Rs[row][column]
+    const int tid = threadIdx.x, l = tid & 63, c0 = (tid >> 6) * CPT;
+    const int64_t j0 = (int64_t)blockIdx.x * R;
     const int64_t nblk = (n + kTile - 1) / kTile;
-    for (int64_t blk = nblk - 1; blk >= 0; --blk) {
-        const int64_t r0 = blk * kTile, r1 = min(n, r0 + kTile);
-        for (int t = tid; t < kTile * kTile; t += kSolveThreads) {
-            const int l = t >> 6, k = t & 63;
-            const int64_t ii = r0 + l, kk = r0 + k;
-            T[l][k] = (ii < r1 && kk < r1 && kk >= ii && kk - ii <= bw) ? atc(ab, ld, bw, ii, kk) : 0.0;
+    int64_t sb = 0;
+    if (identity_first >= 0) {
+        sb = (identity_first + j0) / kTile;
+        const int64_t rows = sb * kTile;
+        for (int64_t e = tid; e < rows * R; e += 256) {
+            const int64_t cc = e / rows, i = e - cc * rows;
+            if (j0 + cc < nrhs) x[(j0 + cc) * ldx + i] = 0.0;
         }
-        const int64_t i = r0 + row;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        if (i < r1) {
-            const int64_t ke = min(n - 1, i + bw);
-            const double* arow = ab + i * ld + (r1 - i + bw);  // U(i, r1), U(i, r1 + 1), ...
-            const double* xk = xc + r1;
-            const int cnt = (int)(ke - r1 + 1);  // solved entries k in [r1, ke]
-            int k = q;
-            for (; k + 48 < cnt; k += 64) {
-                s0 += arow[k] * xk[k];
-                s1 += arow[k + 16] * xk[k + 16];
-                s2 += arow[k + 32] * xk[k + 32];
-                s3 += arow[k + 48] * xk[k + 48];
+    }
+    double pa[PA], py[PY];
+    // Loads of the product tile [kt, kt + kn) of row block r0 into registers; the forward pass reads L (k < i),
+    // the backward pass U (k > i).
+    auto load_tile = [&](int64_t r0, int64_t rend, int64_t kt, int kn, bool lower) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + 256 * q, rr = e >> 6, kk = e & 63;
+            const int64_t i = r0 + rr, k = kt + kk;
+            const bool in = kk < kn && i < rend && (lower ? i - k <= bw : k - i <= bw);
+            pa[q] = in ? atc(ab, ld, bw, i, k) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PY; ++q) {
+            const int e = tid + 256 * q, cc = e >> 6, kk = e & 63;
+            py[q] = (kk < kn && j0 + cc < nrhs) ? x[(j0 + cc) * ldx + kt + kk] : 0.0;
+        }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + 256 * q;
+            At[e & 63][e >> 6] = pa[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PY; ++q) {
+            const int e = tid + 256 * q;
+            Ys[e & 63][e >> 6] = py[q];
+        }
+    };
+    // acc = sum over the tiles [kbeg, kend) of the row block's band products with the solved entries
+    auto products = [&](int64_t r0, int64_t rend, int64_t kbeg, int64_t kend, bool lower, double* acc) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) acc[q] = 0.0;
+        if (kbeg >= kend) return;
+        load_tile(r0, rend, kbeg, (int)min((int64_t)kTile, kend - kbeg), lower);
+        for (int64_t kt = kbeg; kt < kend; kt += kTile) {
+            __syncthreads();  // the last tile's readers are done
+            store_tile();
+            __syncthreads();
+            const int64_t kn = kt + kTile;
+            if (kn < kend) load_tile(r0, rend, kn, (int)min((int64_t)kTile, kend - kn), lower);
+#pragma unroll 8
+            for (int kk = 0; kk < kTile; ++kk) {
+                const double a = At[kk][l];
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) acc[q] += a * Ys[kk][c0 + q];
             }
-            for (; k < cnt; k += 16) s0 += arow[k] * xk[k];
         }
-        row_sum((s0 + s1) + (s2 + s3));
-        __syncthreads();
-        if (tid < 64) {
-            const int64_t ii = r0 + lane;
-            const int m = (int)(r1 - r0);
-            double acc = ii < r1 ? xc[ii] - rs[lane] : 0.0;
-            double xv = 0.0;
-            for (int k = m - 1; k >= 0; --k) {
-                if (lane == k) xv = acc / T[k][k];
-                const double xkv = __shfl(xv, k, 64);
-                if (lane < k) acc -= T[lane][k] * xkv;
-            }
-            if (ii < r1) xc[ii] = xv;
+    };
+    // x rows of the block = inverse block (row-major at blk) times Rs
+    auto triangle = [&](const double* __restrict__ blk, int64_t r0, int64_t rend) {
+        __syncthreads();  // the last product tile's readers are done with At; Rs complete
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + 256 * q;
+            At[e & 63][e >> 6] = blk[e];
         }
         __syncthreads();
+        double y[CPT] = {};
+#pragma unroll 8
+        for (int kk = 0; kk < kTile; ++kk) {
+            const double a = At[kk][l];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) y[q] += a * Rs[kk][c0 + q];
+        }
+        const int64_t i = r0 + l;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int64_t j = j0 + c0 + q;
+            if (i < rend && j < nrhs) x[j * ldx + i] = y[q];
+        }
+        __syncthreads();  // x rows written and At free for the next block
+    };
+    double acc[CPT];
+    for (int64_t rb = sb; rb < nblk; ++rb) {  // forward: y = L^-1 b
+        const int64_t r0 = rb * kTile;
+        products(r0, n, std::max(sb * kTile, r0 - bw), r0, true, acc);
+        const int64_t i = r0 + l;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int64_t j = j0 + c0 + q;
+            Rs[l][c0 + q] = (i < n && j < nrhs) ? b[j * ldb + i] - acc[q] : 0.0;
+        }
+        triangle(inv + rb * 2 * (kTile * kTile), r0, n);
+    }
+    for (int64_t rb = nblk - 1; rb >= 0; --rb) {  // backward: x = U^-1 y
+        const int64_t r0 = rb * kTile, r1 = std::min(n, r0 + kTile);
+        products(r0, r1, r1, std::min(n, r1 + bw), false, acc);
+        const int64_t i = r0 + l;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int64_t j = j0 + c0 + q;
+            Rs[l][c0 + q] = (i < r1 && j < nrhs) ? x[j * ldx + i] - acc[q] : 0.0;
+        }
+        triangle(inv + (rb * 2 + 1) * (kTile * kTile), r0, r1);
     }
 }
 
@@ -259,7 +388,11 @@ unsigned grid_for(int64_t count, int per) {
 
 }  // namespace
 
-size_t band_bytes(int64_t n, int bw) { return (size_t)n * (size_t)(2 * bw + 1) * sizeof(double); }
+int64_t band_blocks(int64_t n) { return (n + kTile - 1) / kTile; }
+
+size_t band_bytes(int64_t n, int bw) {
+    return ((size_t)n * (size_t)(2 * bw + 1) + (size_t)band_blocks(n) * 2 * kTile * kTile) * sizeof(double);
+}
 
 void launch_band_fill(DeviceBandLU& lu, int64_t num_blocks, const int* blk_row, const int* colidx, const double* S,
                       hipStream_t s) {
@@ -271,22 +404,35 @@ void launch_band_fill(DeviceBandLU& lu, int64_t num_blocks, const int* blk_row, 
 }
 
 void launch_band_factor(DeviceBandLU& lu, hipStream_t s) {
+    if (lu.n <= 0) return;
+    if (!lu.inv) throw std::runtime_error("band LU without block-inverse storage");
+    launch_k(ldl_diag_kernel, dim3(1), dim3(64), 0, s, lu.ab, lu.bw, lu.ld, (int64_t)0,
+             (int)std::min<int64_t>(lu.n, kLuPanel));
     for (int64_t k0 = 0; k0 < lu.n; k0 += kLuPanel) {
         const int64_t k1 = std::min(lu.n, k0 + kLuPanel);
-        launch_k(band_panel_kernel, dim3(1), dim3(1024), 0, s, lu.ab, lu.n, lu.bw, lu.ld, k0, k1);
         const int64_t m = std::min(lu.n, k1 + lu.bw) - k1;
-        if (m > 0) {
-            const unsigned tiles = (unsigned)((m + kTile - 1) / kTile);
-            launch_k(band_update_kernel, dim3(tiles, tiles), dim3(256), 0, s, lu.ab, lu.bw, lu.ld, k0, k1, m);
-        }
+        if (m <= 0) break;  // the last panel: its diagonal block is all there is
+        launch_k(ldl_panel_kernel, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, s, lu.ab, lu.bw, lu.ld, k0,
+                 (int)(k1 - k0), m);
+        const int64_t tiles = (m + kTile - 1) / kTile;
+        launch_k(ldl_update_kernel, dim3((unsigned)(tiles * (tiles + 1) / 2)), dim3(256), 0, s, lu.ab, lu.bw, lu.ld,
+                 k0, k1, m, (int)(std::min(lu.n, k1 + kLuPanel) - k1));
     }
+    launch_k(band_block_inverse_kernel, dim3((unsigned)band_blocks(lu.n), 2), dim3(64), 0, s, (const double*)lu.ab,
+             lu.n, lu.bw, lu.ld, lu.inv);
 }
 
 void launch_band_solve(const DeviceBandLU& lu, int nrhs, const double* b, int64_t ldb, double* x, int64_t ldx,
-                       hipStream_t s) {
+                       hipStream_t s, int64_t identity_first) {
     if (nrhs <= 0 || lu.n == 0) return;
-    launch_k(band_solve_kernel, dim3((unsigned)nrhs), dim3(kSolveThreads), 0, s, (const double*)lu.ab, lu.n, lu.bw,
-             lu.ld, b, ldb, x, ldx);
+    if (nrhs >= 64)
+        launch_k(band_solve_blocked_kernel<16>, dim3((unsigned)((nrhs + 15) / 16)), dim3(256), 0, s,
+                 (const double*)lu.ab, (const double*)lu.inv, lu.n, lu.bw, lu.ld, nrhs, b, ldb, x, ldx,
+                 identity_first);
+    else
+        launch_k(band_solve_blocked_kernel<4>, dim3((unsigned)((nrhs + 3) / 4)), dim3(256), 0, s,
+                 (const double*)lu.ab, (const double*)lu.inv, lu.n, lu.bw, lu.ld, nrhs, b, ldb, x, ldx,
+                 identity_first);
 }
 
 void launch_identity_columns(int64_t n, int64_t first, int count, double* out, hipStream_t s) {
