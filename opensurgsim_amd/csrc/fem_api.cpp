@@ -1627,12 +1627,13 @@ int step_once(fem_ctx* c) {
         assemble_tet(c);
         rhs_tet(c);
         mark_fmdk(c);
+        // the kept compliance is formed before this step's solve, which then applies it like the later steps
+        if (linear_scheme && c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) keep_dense_compliance(c);
         linear_solve(c);
         state_update(c, c->x.p);
         if (linear_scheme) {
             c->linear_matrix_ready = true;
             c->matrix_dt = dt;
-            if (c->opt.linear_solver == FEM_LINEAR_SOLVER_LU) keep_dense_compliance(c);
         }
     } else {
         // Newton-Raphson (Math/OdeSolverEulerImplicit.cpp:84-115): newState starts as the current state.

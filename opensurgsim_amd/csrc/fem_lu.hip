@@ -4,8 +4,8 @@
 // (LDL^T) and writes the LU form the solves read: L (unit lower) below the diagonal, U = D L^T on and above it.
 // Row c of U is column c of the eliminated lower triangle, the value the elimination divides by the pivot.
 // Right-looking, by panels of kLuPanel columns. For panel [k0, k1):
-//   1. its diagonal block is eliminated by one wavefront in registers (for the first panel by its own launch,
-//      for the others by the first workgroup of the previous panel's trailing update);
+//   1. its diagonal block is eliminated in LDS (for the first panel by its own launch, for the others by the first
+//      workgroup of the previous panel's trailing update, which updates that block itself);
 //   2. the rows below, k1 .. k1 + bw - 1, are eliminated against it, one lane per row: l(i, c) and u(c, i);
 //   3. the trailing block a(i, j), i >= j in [k1, k1 + bw), takes -= L21 U12 by 64 x 64 output tiles staged
 //      through LDS (lower tiles only: the upper band is rewritten from L by step 2 before anything reads it).
@@ -37,14 +37,6 @@ __device__ __forceinline__ double atc(const double* ab, int64_t ld, int bw, int6
     return ab[i * ld + (j - i + bw)];
 }
 
-// Value of v in lane `lane` (a constant after unrolling) for every lane.
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 __global__ __launch_bounds__(256) void band_zero_kernel(double* __restrict__ ab, int64_t count) {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
         ab[t] = 0.0;
@@ -64,91 +56,173 @@ __global__ __launch_bounds__(256) void band_fill_kernel(double* __restrict__ ab,
         for (int c = 0; c < 3; ++c) at(ab, ld, bw, 3 * I + r, 3 * J + c) = S[9 * k + 3 * r + c];
 }
 
-// The nb x nb diagonal block at k0 eliminated by one wavefront (lanes 0..63 of the calling workgroup): lane i
-// holds row i of the block's lower triangle; at step c the pivot and every lane's column-c value, the eliminated
-// a(j, c) = u(c, j), are broadcast by readlane. Writes L below the diagonal, U on and above it.
-__device__ void factor_diag_block(double* __restrict__ ab, int bw, int64_t ld, int64_t k0, int nb, int lane) {
-    double a[kLuPanel], piv[kLuPanel];
-#pragma unroll
-    for (int j = 0; j < kLuPanel; ++j)
-        a[j] = (lane < nb && j <= lane && lane - j <= bw) ? atc(ab, ld, bw, k0 + lane, k0 + j) : 0.0;
-#pragma unroll
-    for (int c = 0; c < kLuPanel; ++c) {
-        piv[c] = 1.0;
-        if (c < nb) {
-            piv[c] = readlane_d(a[c], c);
-            const double l = a[c] / piv[c];
-#pragma unroll
-            for (int j = c + 1; j < kLuPanel; ++j) {
-                const double ucj = readlane_d(a[c], j);
-                if (lane >= j) a[j] -= l * ucj;
-            }
+// Elimination of an nb x nb diagonal block held in LDS (T: kLuPanel rows of kLuPanel + 1 doubles, its lower
+// triangle) by the whole workgroup: step c subtracts (a(i, c) / a(c, c)) u(c, j) from the lower entries (i, j),
+// c < j <= i, where u(c, j) = T[j][c] is the eliminated column value (the division as a product with the pivot's
+// reciprocal). On return T's lower triangle holds the eliminated values: u(c, i) = T[i][c], pivots on the diagonal.
+__device__ void eliminate_diag_block(int nb, double (*T)[kLuPanel + 1]) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int c = 0; c < nb; ++c) {
+        const int w = nb - 1 - c;  // entries below the pivot
+        const double rp = 1.0 / T[c][c];
+        for (int e = tid; e < w * w; e += nt) {
+            const int i = c + 1 + e / w, j = c + 1 + e % w;
+            if (j <= i) T[i][j] -= (T[i][c] * rp) * T[j][c];
         }
-    }
-    if (lane >= nb) return;
-#pragma unroll
-    for (int c = 0; c < kLuPanel; ++c) {
-        if (c < lane && lane - c <= bw) {
-            at(ab, ld, bw, k0 + lane, k0 + c) = a[c] / piv[c];
-            at(ab, ld, bw, k0 + c, k0 + lane) = a[c];
-        }
-        if (c == lane) at(ab, ld, bw, k0 + lane, k0 + lane) = a[c];
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(64) void ldl_diag_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
-                                                      int nb) {
-    factor_diag_block(ab, bw, ld, k0, nb, threadIdx.x);
+// The eliminated block into the band at k0: L (l(i, c) = u(c, i) / u(c, c), as a product with the reciprocal)
+// below the diagonal, U on and above it.
+__device__ void write_diag_block(double* __restrict__ ab, int bw, int64_t ld, int64_t k0, int nb,
+                                 const double (*T)[kLuPanel + 1]) {
+    for (int e = threadIdx.x; e < kLuPanel * kLuPanel; e += blockDim.x) {
+        const int i = e / kLuPanel, c = e % kLuPanel;
+        if (i >= nb || c > i || i - c > bw) continue;
+        if (c == i) {
+            at(ab, ld, bw, k0 + i, k0 + i) = T[i][i];
+        } else {
+            at(ab, ld, bw, k0 + i, k0 + c) = T[i][c] * (1.0 / T[c][c]);
+            at(ab, ld, bw, k0 + c, k0 + i) = T[i][c];
+        }
+    }
+}
+
+// The first panel's diagonal block: staged from the band, eliminated, written back.
+__global__ __launch_bounds__(256) void ldl_diag_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
+                                                       int nb) {
+    __shared__ double T[kLuPanel][kLuPanel + 1];
+    for (int e = threadIdx.x; e < kLuPanel * kLuPanel; e += blockDim.x) {
+        const int i = e / kLuPanel, j = e % kLuPanel;
+        T[i][j] = (i < nb && j <= i && i - j <= bw) ? atc(ab, ld, bw, k0 + i, k0 + j) : 0.0;
+    }
+    __syncthreads();
+    eliminate_diag_block(nb, T);
+    write_diag_block(ab, bw, ld, k0, nb, T);
 }
 
 // Rows k1 + 64 blockIdx.x + lane (< k1 + m) of panel [k0, k1 = k0 + nb), eliminated against the panel's factored
-// diagonal block (U11 from the upper band, staged in LDS): s_c = a(i, c) - sum_{k<c} l(i, k) u(k, c) in order of
-// k, then l(i, c) = s_c / u(c, c) into row i's lower band and s_c = u(c, i) into row c's upper band.
+// diagonal block (U11 from the upper band, staged in LDS with the pivots' reciprocals), column by column:
+// s_k = the row's entry k with the updates of columns < k applied, l(i, k) = s_k / u(k, k) (as a product with the
+// reciprocal) into row i's lower band and s_k = u(k, i) into row k's upper band, then l(i, k) u(k, c) subtracted
+// from the entries c > k (independent of each other).
 __global__ __launch_bounds__(64) void ldl_panel_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
                                                        int nb, int64_t m) {
     __shared__ double Us[kLuPanel][kLuPanel + 1];
+    __shared__ double rp[kLuPanel];
     const int lane = threadIdx.x;
     for (int e = lane; e < kLuPanel * kLuPanel; e += 64) {
         const int c = e / kLuPanel, j = e % kLuPanel;
         Us[c][j] = (c < nb && j < nb && j >= c && j - c <= bw) ? atc(ab, ld, bw, k0 + c, k0 + j) : 0.0;
     }
+    if (lane < kLuPanel) rp[lane] = lane < nb ? 1.0 / atc(ab, ld, bw, k0 + lane, k0 + lane) : 0.0;
     __syncthreads();
     const int64_t k1 = k0 + nb;
     const int64_t i = k1 + (int64_t)blockIdx.x * 64 + lane;
     if (i >= k1 + m) return;
     double r[kLuPanel];
 #pragma unroll
-    for (int c = 0; c < kLuPanel; ++c) r[c] = (c < nb && i - (k0 + c) <= bw) ? atc(ab, ld, bw, i, k0 + c) : 0.0;
-#pragma unroll
     for (int c = 0; c < kLuPanel; ++c) {
-        if (c < nb) {
-            double s = r[c];
+        const bool in = c < nb && i - (k0 + c) <= bw;
+        const double v = ab[in ? i * ld + (k0 + c - i + bw) : 0];
+        r[c] = in ? v : 0.0;
+    }
+    // Row k of U11 and the pivot's reciprocal come from LDS one step ahead (cur: step k, nxt: step k + 1); the
+    // scheduling barrier keeps those loads ahead of step k's products.
+    double ua[kLuPanel + 1], ub[kLuPanel + 1];  // [c]: u(k, c) for c > k; [kLuPanel]: 1 / u(k, k)
+    auto fetch = [&](int k, double (&u)[kLuPanel + 1]) {
 #pragma unroll
-            for (int k = 0; k < c; ++k) s -= r[k] * Us[k][c];
-            r[c] = s / Us[c][c];
-            if (i - (k0 + c) <= bw) {  // beyond the band s and l are zero
-                at(ab, ld, bw, k0 + c, i) = s;
-                at(ab, ld, bw, i, k0 + c) = r[c];
-            }
+        for (int c = k + 1; c < kLuPanel; ++c) u[c] = Us[k][c];
+        u[kLuPanel] = rp[k];
+    };
+    auto step = [&](int k, double (&cur)[kLuPanel + 1], double (&nxt)[kLuPanel + 1]) {
+        if (k >= nb) return;
+        if (k + 1 < kLuPanel) fetch(k + 1, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+        const double sk = r[k];
+        const double l = sk * cur[kLuPanel];
+        if (i - (k0 + k) <= bw) {  // beyond the band s and l are zero
+            at(ab, ld, bw, k0 + k, i) = sk;
+            at(ab, ld, bw, i, k0 + k) = l;
         }
+#pragma unroll
+        for (int c = k + 1; c < kLuPanel; ++c) r[c] -= l * cur[c];
+    };
+    fetch(0, ua);
+#pragma unroll
+    for (int k = 0; k < kLuPanel; k += 2) {
+        step(k, ua, ub);
+        step(k + 1, ub, ua);
     }
 }
 
-// Trailing update a(i, j) -= sum_c l(i, c) u(c, j) over the panel [k0, k1), i >= j in [k1, k1 + m) by 64 x 64
-// lower output tiles (blockIdx.x enumerates (ti, tj), tj <= ti, row by row), 4 x 4 outputs per thread, the panel's
-// L21 and U12 staged in LDS. The first workgroup then eliminates the next panel's diagonal block (rows
-// [k1, k1 + next_nb), all inside its own tile or untouched by this update).
+// Trailing update a(i, j) -= sum_c l(i, c) u(c, j) over the panel [k0, k1), i >= j in [k1, k1 + m). Workgroup 0
+// takes the next panel's diagonal block [k1, k1 + next_nb)^2: its update (the same sums in the same order), then
+// its elimination and its L and U into the band. Workgroups 1.. take the 64 x 64 lower output tiles ((ti, tj),
+// tj <= ti, row by row; tile (0, 0) leaves the diagonal block to workgroup 0), 4 x 4 outputs per thread, the
+// panel's L21 and U12 staged in LDS.
 __global__ __launch_bounds__(256) void ldl_update_kernel(double* __restrict__ ab, int bw, int64_t ld, int64_t k0,
                                                          int64_t k1, int64_t m, int next_nb) {
     __shared__ double Ls[kTile][kLuPanel + 1];
     __shared__ double Us[kLuPanel][kTile + 1];
-    int ti = 0;
-    while ((ti + 1) * (ti + 2) / 2 <= (int)blockIdx.x) ++ti;
-    const int tj = (int)blockIdx.x - ti * (ti + 1) / 2;
-    const int64_t i0 = k1 + (int64_t)ti * kTile, j0 = k1 + (int64_t)tj * kTile;
-    const int64_t iend = k1 + m, jend = k1 + m;
     const int nb = (int)(k1 - k0);
     const int tid = threadIdx.x;
+    const int64_t iend = k1 + m, jend = k1 + m;
+    if (blockIdx.x == 0) {
+        if (next_nb <= 0) return;
+        double (*T)[kLuPanel + 1] = &Ls[kLuPanel];  // rows 32..63 of Ls
+        for (int t = tid; t < kLuPanel * kLuPanel; t += 256) {
+            const int r = t / kLuPanel, c = t % kLuPanel;
+            const int64_t i = k1 + r, col = k0 + c;
+            Ls[r][c] = (c < nb && r < next_nb && i < iend && i - col <= bw) ? atc(ab, ld, bw, i, col) : 0.0;
+            const int64_t row = k0 + r, j = k1 + c;  // U12 entry (r, c) of the block's columns
+            Us[r][c] = (r < nb && c < next_nb && j < jend && j - row <= bw) ? atc(ab, ld, bw, row, j) : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < kLuPanel * kLuPanel; e += 256) {
+            const int i = e / kLuPanel, j = e % kLuPanel;
+            const bool in = i < next_nb && j <= i && i - j <= bw;
+            const double old = ab[in ? (k1 + i) * ld + (j - i + bw) : 0];
+            double acc = 0.0;  // Ls and Us are zero past nb: the sum equals the tiles' sum over c < nb
+#pragma unroll 1
+            for (int c0 = 0; c0 < kLuPanel; c0 += 8) {
+                double l[8], u[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    l[q] = Ls[i][c0 + q];
+                    u[q] = Us[c0 + q][j];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc += l[q] * u[q];
+            }
+            T[i][j] = in ? old - acc : 0.0;
+        }
+        __syncthreads();
+        eliminate_diag_block(next_nb, T);
+        write_diag_block(ab, bw, ld, k1, next_nb, T);
+        return;
+    }
+    const int b = (int)blockIdx.x - 1;
+    int ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+    const int tj = b - ti * (ti + 1) / 2;
+    const int64_t i0 = k1 + (int64_t)ti * kTile, j0 = k1 + (int64_t)tj * kTile;
+    const int tx = tid % 16, ty = tid / 16;
+    // entries (i, j) this tile writes: in the band, inside the trailing block, not in the next diagonal block
+    auto mine = [&](int64_t i, int64_t j) {
+        return i < iend && j < jend && j - i <= bw && i - j <= bw && !(i < k1 + next_nb && j < k1 + next_nb);
+    };
+    double old[4][4];  // the outputs, fetched first
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int64_t i = i0 + ty + 16 * a, j = j0 + tx + 16 * bb;
+            const bool in = mine(i, j);
+            const double v = ab[in ? i * ld + (j - i + bw) : 0];
+            old[a][bb] = in ? v : 0.0;
+        }
     for (int t = tid; t < kTile * kLuPanel; t += 256) {
         const int r = t / kLuPanel, c = t % kLuPanel;
         const int64_t i = i0 + r, col = k0 + c;
@@ -164,34 +238,31 @@ __global__ __launch_bounds__(256) void ldl_update_kernel(double* __restrict__ ab
         Us[c][q] = v;
     }
     __syncthreads();
-    const int tx = tid % 16, ty = tid / 16;
-    double acc[4][4] = {};
-    for (int c = 0; c < nb; ++c) {
-        double l[4], u[4];
+    double acc[4][4] = {};  // Ls and Us are zero past nb, so the loop runs over every staged column
+#pragma unroll 1
+    for (int c0 = 0; c0 < kLuPanel; c0 += 4) {
+        double l[4][4], u[4][4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) l[a] = Ls[ty + 16 * a][c];
+        for (int q = 0; q < 4; ++q) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) u[b] = Us[c][tx + 16 * b];
+            for (int a = 0; a < 4; ++a) l[q][a] = Ls[ty + 16 * a][c0 + q];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * u[b];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int64_t i = i0 + ty + 16 * a;
-        if (i >= iend) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t j = j0 + tx + 16 * b;
-            if (j >= jend || j - i > bw || i - j > bw) continue;
-            at(ab, ld, bw, i, j) -= acc[a][b];
+            for (int bb = 0; bb < 4; ++bb) u[q][bb] = Us[c0 + q][tx + 16 * bb];
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) acc[a][bb] += l[q][a] * u[q][bb];
     }
-    if (blockIdx.x == 0 && next_nb > 0) {
-        __syncthreads();  // this tile's updated values visible to wavefront 0
-        if (tid < 64) factor_diag_block(ab, bw, ld, k1, next_nb, tid);
-    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int64_t i = i0 + ty + 16 * a, j = j0 + tx + 16 * bb;
+            if (mine(i, j)) at(ab, ld, bw, i, j) = old[a][bb] - acc[a][bb];
+        }
 }
 
 // Inverse of the 64 x 64 diagonal block blockIdx.x of L (blockIdx.y = 0, unit lower) or U (1, upper), row-major
@@ -231,22 +302,26 @@ __global__ __launch_bounds__(64) void band_block_inverse_kernel(const double* __
     for (int e = c; e < kTile * kTile; e += 64) out[e] = X[e >> 6][e & 63];
 }
 
-// x = U^-1 L^-1 b for columns [R blockIdx.x, R blockIdx.x + R) of nrhs, 256 threads: thread (row l = tid % 64,
-// wavefront g) owns R / 4 columns of the row block. identity_first >= 0: b holds columns identity_first + j of
-// the identity, whose forward solutions are zero above row identity_first + j; the forward pass starts at the
-// row block of the first column (rows above it written zero). x may alias b.
+// x = U^-1 L^-1 b for columns [R blockIdx.x, R blockIdx.x + R) of nrhs, 256 threads. Products over a 64-entry
+// tile are split by entries: wavefront g takes entries [16 g, 16 g + 16) of every row and column, and the four
+// partial sums of a row block are added in wavefront order; thread (row l = tid % 64, g) then owns columns
+// [g R / 4, (g + 1) R / 4) of the row. identity_first >= 0: b holds columns identity_first + j of the identity,
+// whose forward solutions are zero above row identity_first + j; the forward pass starts at the row block of the
+// first column (rows above it written zero). x may alias b.
 template <int R>
 __global__ __launch_bounds__(256) void band_solve_blocked_kernel(const double* __restrict__ ab,
                                                                  const double* __restrict__ inv, int64_t n, int bw,
                                                                  int64_t ld, int nrhs, const double* b, int64_t ldb,
                                                                  double* x, int64_t ldx, int64_t identity_first) {
-    constexpr int CPT = R / 4;
+    constexpr int CPT = R / 4;               // columns each thread finishes
     constexpr int PA = kTile * kTile / 256;  // tile values each thread stages
     constexpr int PY = kTile * R / 256;
-    __shared__ double At[kTile][kTile + 1];  // At[k][row]
-    __shared__ double Ys[kTile][R + 1];      // Ys[k][column]
-    __shared__ double Rs[kTile][R + 1];      // Rs[row][column]
-    const int tid = threadIdx.x, l = tid & 63, c0 = (tid >> 6) * CPT;
+    constexpr int KW = kTile / 4;            // tile entries per wavefront
+    static_assert(4 * R * kTile <= kTile * (kTile + 1), "partial sums fit in At");
+    __shared__ double At[kTile][kTile + 1];  // At[k][row]; also the wavefronts' partial sums
+    __shared__ double Ys[kTile][R + 2];      // Ys[k][column] (rows 16-byte aligned)
+    __shared__ double Rs[kTile][R + 2];      // Rs[row][column]
+    const int tid = threadIdx.x, l = tid & 63, g = tid >> 6, kk0 = tid & 63;
     const int64_t j0 = (int64_t)blockIdx.x * R;
     const int64_t nblk = (n + kTile - 1) / kTile;
     int64_t sb = 0;
@@ -259,98 +334,115 @@ __global__ __launch_bounds__(256) void band_solve_blocked_kernel(const double* _
         }
     }
     double pa[PA], py[PY];
-    // Loads of the product tile [kt, kt + kn) of row block r0 into registers; the forward pass reads L (k < i),
-    // the backward pass U (k > i).
+    // Tile [kt, kt + kn) of row block r0 into registers (thread: entry kk0, rows g + 4 q; columns g + 4 q of x),
+    // loads issued unconditionally from clamped addresses. The forward pass reads L (k < i), the backward U (k > i).
     auto load_tile = [&](int64_t r0, int64_t rend, int64_t kt, int kn, bool lower) {
+        const int64_t k = kt + kk0;
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
-            const int e = tid + 256 * q, rr = e >> 6, kk = e & 63;
-            const int64_t i = r0 + rr, k = kt + kk;
-            const bool in = kk < kn && i < rend && (lower ? i - k <= bw : k - i <= bw);
-            pa[q] = in ? atc(ab, ld, bw, i, k) : 0.0;
+            const int64_t i = r0 + g + 4 * q;
+            const bool in = kk0 < kn && i < rend && (lower ? i - k <= bw : k - i <= bw);
+            const double v = ab[in ? i * ld + (k - i + bw) : 0];
+            pa[q] = in ? v : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < PY; ++q) {
-            const int e = tid + 256 * q, cc = e >> 6, kk = e & 63;
-            py[q] = (kk < kn && j0 + cc < nrhs) ? x[(j0 + cc) * ldx + kt + kk] : 0.0;
+            const int64_t j = j0 + g + 4 * q;
+            const bool in = kk0 < kn && j < nrhs;
+            const double v = x[in ? j * ldx + k : 0];
+            py[q] = in ? v : 0.0;
         }
     };
     auto store_tile = [&]() {
 #pragma unroll
-        for (int q = 0; q < PA; ++q) {
-            const int e = tid + 256 * q;
-            At[e & 63][e >> 6] = pa[q];
-        }
+        for (int q = 0; q < PA; ++q) At[kk0][g + 4 * q] = pa[q];
 #pragma unroll
-        for (int q = 0; q < PY; ++q) {
-            const int e = tid + 256 * q;
-            Ys[e & 63][e >> 6] = py[q];
-        }
+        for (int q = 0; q < PY; ++q) Ys[kk0][g + 4 * q] = py[q];
     };
-    // acc = sum over the tiles [kbeg, kend) of the row block's band products with the solved entries
-    auto products = [&](int64_t r0, int64_t rend, int64_t kbeg, int64_t kend, bool lower, double* acc) {
+    // acc[c] += sum over this wavefront's entries of At[k][l] S[k][c]
+    auto kpart = [&](const double (*S)[R + 2], double* acc) {
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) acc[q] = 0.0;
-        if (kbeg >= kend) return;
-        load_tile(r0, rend, kbeg, (int)min((int64_t)kTile, kend - kbeg), lower);
-        for (int64_t kt = kbeg; kt < kend; kt += kTile) {
-            __syncthreads();  // the last tile's readers are done
-            store_tile();
-            __syncthreads();
-            const int64_t kn = kt + kTile;
-            if (kn < kend) load_tile(r0, rend, kn, (int)min((int64_t)kTile, kend - kn), lower);
-#pragma unroll 8
-            for (int kk = 0; kk < kTile; ++kk) {
-                const double a = At[kk][l];
-#pragma unroll
-                for (int q = 0; q < CPT; ++q) acc[q] += a * Ys[kk][c0 + q];
-            }
-        }
-    };
-    // x rows of the block = inverse block (row-major at blk) times Rs
-    auto triangle = [&](const double* __restrict__ blk, int64_t r0, int64_t rend) {
-        __syncthreads();  // the last product tile's readers are done with At; Rs complete
-#pragma unroll
-        for (int q = 0; q < PA; ++q) {
-            const int e = tid + 256 * q;
-            At[e & 63][e >> 6] = blk[e];
-        }
-        __syncthreads();
-        double y[CPT] = {};
-#pragma unroll 8
-        for (int kk = 0; kk < kTile; ++kk) {
+        for (int u = 0; u < KW; ++u) {
+            const int kk = g * KW + u;
             const double a = At[kk][l];
 #pragma unroll
-            for (int q = 0; q < CPT; ++q) y[q] += a * Rs[kk][c0 + q];
+            for (int c = 0; c < R; ++c) acc[c] += a * S[kk][c];
         }
+    };
+    // out[q] = the four wavefronts' partials of column g CPT + q of row l, added in wavefront order. At must be
+    // free on entry; it is free again on return.
+    auto reduce = [&](const double* acc, double* out) {
+        double* P = &At[0][0];
+#pragma unroll
+        for (int c = 0; c < R; ++c) P[(g * R + c) * kTile + l] = acc[c];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int c = g * CPT + q;
+            out[q] = ((P[c * kTile + l] + P[(R + c) * kTile + l]) + P[(2 * R + c) * kTile + l]) +
+                     P[(3 * R + c) * kTile + l];
+        }
+        __syncthreads();
+    };
+    // tot = the row block's band products with the solved entries [kbeg, kend); At free on return
+    auto products = [&](int64_t r0, int64_t rend, int64_t kbeg, int64_t kend, bool lower, double* tot) {
+        double acc[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[c] = 0.0;
+        if (kbeg < kend) {
+            load_tile(r0, rend, kbeg, (int)min((int64_t)kTile, kend - kbeg), lower);
+            for (int64_t kt = kbeg; kt < kend; kt += kTile) {
+                __syncthreads();  // the last tile's readers are done
+                store_tile();
+                __syncthreads();
+                const int64_t kn = kt + kTile;
+                if (kn < kend) load_tile(r0, rend, kn, (int)min((int64_t)kTile, kend - kn), lower);
+                kpart(Ys, acc);
+            }
+            __syncthreads();
+        }
+        reduce(acc, tot);
+    };
+    // x rows [r0, rend) of the block's columns = inverse block (row-major at blk) times Rs
+    auto triangle = [&](const double* __restrict__ blk, int64_t r0, int64_t rend) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) At[kk0][g + 4 * q] = blk[(g + 4 * q) * kTile + kk0];
+        __syncthreads();  // At and Rs complete
+        double acc[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[c] = 0.0;
+        kpart(Rs, acc);
+        __syncthreads();
+        double y[CPT];
+        reduce(acc, y);
         const int64_t i = r0 + l;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
-            const int64_t j = j0 + c0 + q;
+            const int64_t j = j0 + g * CPT + q;
             if (i < rend && j < nrhs) x[j * ldx + i] = y[q];
         }
-        __syncthreads();  // x rows written and At free for the next block
+        __syncthreads();  // x rows written before the next block reads them
     };
-    double acc[CPT];
+    double tot[CPT];
     for (int64_t rb = sb; rb < nblk; ++rb) {  // forward: y = L^-1 b
         const int64_t r0 = rb * kTile;
-        products(r0, n, std::max(sb * kTile, r0 - bw), r0, true, acc);
+        products(r0, n, std::max(sb * kTile, r0 - bw), r0, true, tot);
         const int64_t i = r0 + l;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
-            const int64_t j = j0 + c0 + q;
-            Rs[l][c0 + q] = (i < n && j < nrhs) ? b[j * ldb + i] - acc[q] : 0.0;
+            const int64_t j = j0 + g * CPT + q;
+            Rs[l][g * CPT + q] = (i < n && j < nrhs) ? b[j * ldb + i] - tot[q] : 0.0;
         }
         triangle(inv + rb * 2 * (kTile * kTile), r0, n);
     }
     for (int64_t rb = nblk - 1; rb >= 0; --rb) {  // backward: x = U^-1 y
         const int64_t r0 = rb * kTile, r1 = std::min(n, r0 + kTile);
-        products(r0, r1, r1, std::min(n, r1 + bw), false, acc);
+        products(r0, r1, r1, std::min(n, r1 + bw), false, tot);
         const int64_t i = r0 + l;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
-            const int64_t j = j0 + c0 + q;
-            Rs[l][c0 + q] = (i < r1 && j < nrhs) ? x[j * ldx + i] - acc[q] : 0.0;
+            const int64_t j = j0 + g * CPT + q;
+            Rs[l][g * CPT + q] = (i < r1 && j < nrhs) ? x[j * ldx + i] - tot[q] : 0.0;
         }
         triangle(inv + (rb * 2 + 1) * (kTile * kTile), r0, r1);
     }
@@ -406,7 +498,7 @@ void launch_band_fill(DeviceBandLU& lu, int64_t num_blocks, const int* blk_row, 
 void launch_band_factor(DeviceBandLU& lu, hipStream_t s) {
     if (lu.n <= 0) return;
     if (!lu.inv) throw std::runtime_error("band LU without block-inverse storage");
-    launch_k(ldl_diag_kernel, dim3(1), dim3(64), 0, s, lu.ab, lu.bw, lu.ld, (int64_t)0,
+    launch_k(ldl_diag_kernel, dim3(1), dim3(256), 0, s, lu.ab, lu.bw, lu.ld, (int64_t)0,
              (int)std::min<int64_t>(lu.n, kLuPanel));
     for (int64_t k0 = 0; k0 < lu.n; k0 += kLuPanel) {
         const int64_t k1 = std::min(lu.n, k0 + kLuPanel);
@@ -415,7 +507,7 @@ void launch_band_factor(DeviceBandLU& lu, hipStream_t s) {
         launch_k(ldl_panel_kernel, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, s, lu.ab, lu.bw, lu.ld, k0,
                  (int)(k1 - k0), m);
         const int64_t tiles = (m + kTile - 1) / kTile;
-        launch_k(ldl_update_kernel, dim3((unsigned)(tiles * (tiles + 1) / 2)), dim3(256), 0, s, lu.ab, lu.bw, lu.ld,
+        launch_k(ldl_update_kernel, dim3((unsigned)(1 + tiles * (tiles + 1) / 2)), dim3(256), 0, s, lu.ab, lu.bw, lu.ld,
                  k0, k1, m, (int)(std::min(lu.n, k1 + kLuPanel) - k1));
     }
     launch_k(band_block_inverse_kernel, dim3((unsigned)band_blocks(lu.n), 2), dim3(64), 0, s, (const double*)lu.ab,
@@ -470,20 +562,30 @@ __global__ __launch_bounds__(256) void dense_gemv_part_kernel(int64_t n, const d
     part[(int64_t)blockIdx.y * n + i] = (a0 + a1) + (a2 + a3);
 }
 
-__global__ __launch_bounds__(256) void dense_gemv_sum_kernel(int64_t n, int64_t chunks,
-                                                             const double* __restrict__ part, double* __restrict__ y) {
+// y[i] = the chunks' partials of row i added in chunk order; 64 rows per workgroup, loads 8 chunks at a time.
+__global__ __launch_bounds__(64) void dense_gemv_sum_kernel(int64_t n, int64_t chunks,
+                                                            const double* __restrict__ part, double* __restrict__ y) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double acc = 0.0;
-    for (int64_t c = 0; c < chunks; ++c) acc += part[c * n + i];
+    int64_t c = 0;
+    for (; c + 8 <= chunks; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = part[(c + q) * n + i];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += v[q];
+    }
+    for (; c < chunks; ++c) acc += part[c * n + i];
     y[i] = acc;
 }
 
 void launch_dense_gemv(int64_t n, const double* C, const double* x, double* part, double* y, hipStream_t s) {
     if (n <= 0) return;
+    if (n > (int64_t)65536 * 64) throw std::runtime_error("dense GEMV: more rows than one grid covers");
     const int64_t chunks = gemv_chunks(n);
     launch_k(dense_gemv_part_kernel, dim3(grid_for(n, 256), (unsigned)chunks), dim3(256), 0, s, n, C, x, part);
-    launch_k(dense_gemv_sum_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, chunks, (const double*)part, y);
+    launch_k(dense_gemv_sum_kernel, dim3(grid_for(n, 64)), dim3(64), 0, s, n, chunks, (const double*)part, y);
 }
 
 void launch_zero_masked_rows(int64_t n, int count, const uint8_t* mask, double* x, hipStream_t s) {
