@@ -59,8 +59,9 @@ enum {
 
 /* Linear solver of the step (Math/LinearSparseSolveAndInverse.h:41-52, same numbering). The C ABI's default is
  * CG; the reference's DeformableRepresentation defaults to LU (Physics/DeformableRepresentation.cpp:53), which
- * the C++ adapters keep. LU: banded LU without pivoting of the assembled operator in FP64, factored after each
- * assembly (once for the linear schemes), whatever the precision option; tetrahedral meshes of one instance. */
+ * the C++ adapters keep. LU: banded factorization without pivoting of the assembled (symmetric) operator in FP64,
+ * factored after each assembly (once for the linear schemes, whose dense compliance is then kept when n^2 doubles
+ * fit in 1 GiB and applied on every step), whatever the precision option; tetrahedral meshes of one instance. */
 enum {
     FEM_LINEAR_SOLVER_LU = 0,   /* LINEARSOLVER_LU (LinearSparseSolveAndInverseLU, .cpp:31-97) */
     FEM_LINEAR_SOLVER_CG = 1    /* LINEARSOLVER_CONJUGATEGRADIENT (LinearSparseSolveAndInverseCG, .cpp:99-129) */
