@@ -821,12 +821,20 @@ __device__ __forceinline__ void block_sums7(const double* v, double* lds, double
 
 // The 7 sums of the nb partials of the last launch (nb = its grid size, at most kCgBlocks; rows of kCgBlocks):
 // wave j reduces sum j in a fixed order; broadcast through LDS (lds[112 ..], clear of the rows block_sums7
-// writes).
+// writes). A lane's kCgBlocks / 64 loads all go out before the first add (the partials were written on other
+// XCDs, so each load misses this XCD's L2); the adds keep the order i = lane, lane + 64, ...
 __device__ __forceinline__ void reduce7(const double* __restrict__ part, double* s, double* lds, int nb) {
     const int w = threadIdx.x >> 6;
     if (w < kFusedSums) {
+        const int l = threadIdx.x & 63;
+        const double* pw = part + w * kCgBlocks;
+        double t[kCgBlocks / 64];
+#pragma unroll
+        for (int q = 0; q < kCgBlocks / 64; ++q) t[q] = pw[l + 64 * q < nb ? l + 64 * q : 0];
         double v = 0.0;
-        for (int i = threadIdx.x & 63; i < nb; i += 64) v += part[w * kCgBlocks + i];
+#pragma unroll
+        for (int q = 0; q < kCgBlocks / 64; ++q)
+            if (l + 64 * q < nb) v += t[q];
         v = wave_total(v);
         if ((threadIdx.x & 63) == 0) lds[112 + w] = v;
     }
