@@ -590,10 +590,8 @@ __global__ __launch_bounds__(kCgThreads) void cg_init2_guess_kernel(const double
                                                                     double tol, double* __restrict__ x, int n) {
     __shared__ double lds[64];
     __shared__ int exit_zero;
-    double bb, rz;
-    reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
-    __syncthreads();
-    const double rr = reduce_partials(partial + 2 * kCgBlocks, lds);
+    double bb, rz, rr;
+    reduce_partials3(partial, partial + kCgBlocks, partial + 2 * kCgBlocks, bb, rz, rr, lds);
     if (threadIdx.x == 0) {
         sc->bb = bb;
         sc->rz[0] = rz;
@@ -1325,9 +1323,8 @@ __global__ __launch_bounds__(kCgThreads) void refine_check_init_kernel(const dou
                                                                        int* __restrict__ refine_iters, int nrec) {
     __shared__ double lds[64];
     if (sweep > 0 && outer->done) return;
-    const double rr = reduce_partials(partial + 3 * kCgBlocks, lds);
-    double bb, rz;
-    reduce_partials2(partial, partial + kCgBlocks, bb, rz, lds);
+    double rr, bb, rz;
+    reduce_partials3(partial + 3 * kCgBlocks, partial, partial + kCgBlocks, rr, bb, rz, lds);
     if (threadIdx.x == 0) {
         if (sweep == 0) {
             for (int s = 0; s < nrec; ++s) refine_iters[s] = -1;

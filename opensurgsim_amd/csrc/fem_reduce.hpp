@@ -101,26 +101,50 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return t;
 }
 
-// Two sums of kCgBlocks partials each, formed by wave 0 in a fixed order and broadcast through LDS.
-__device__ __forceinline__ void reduce_partials2(const double* pa, const double* pb, double& a, double& b,
-                                                 double* lds) {
+// Three sums of kCgBlocks partials each (pb, pc may be null), formed by wave 0 in a fixed order (lane l adds
+// entries l, l + 64, ... in that order, then the wave total) and broadcast through LDS (lds[61..63]). Every load
+// of a lane goes out before the first add: the partials come from workgroups on every XCD, so each load misses
+// this XCD's L2, and a loop that waits for one load at a time pays that latency kCgBlocks / 64 times.
+__device__ __forceinline__ void reduce_partials3(const double* pa, const double* pb, const double* pc, double& a,
+                                                 double& b, double& c, double* lds) {
+    constexpr int kPer = kCgBlocks / 64;
     if (threadIdx.x < 64) {
-        double va = 0.0, vb = 0.0;
-        for (int i = threadIdx.x; i < kCgBlocks; i += 64) {
-            va += pa[i];
-            if (pb) vb += pb[i];
+        double ta[kPer], tb[kPer], tc[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = threadIdx.x + 64 * q;
+            ta[q] = pa[i];
+            tb[q] = pb ? pb[i] : 0.0;
+            tc[q] = pc ? pc[i] : 0.0;
+        }
+        double va = 0.0, vb = 0.0, vc = 0.0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            va += ta[q];
+            vb += tb[q];
+            vc += tc[q];
         }
         va = wave_total(va);
         vb = wave_total(vb);
+        vc = wave_total(vc);
         if (threadIdx.x == 0) {
-            lds[62] = va;
-            lds[63] = vb;
+            lds[61] = va;
+            lds[62] = vb;
+            lds[63] = vc;
         }
     }
     __syncthreads();
-    a = lds[62];
-    b = lds[63];
+    a = lds[61];
+    b = lds[62];
+    c = lds[63];
     __syncthreads();
+}
+
+// Two sums of kCgBlocks partials each (pb may be null), as reduce_partials3.
+__device__ __forceinline__ void reduce_partials2(const double* pa, const double* pb, double& a, double& b,
+                                                 double* lds) {
+    double c;
+    reduce_partials3(pa, pb, nullptr, a, b, c, lds);
 }
 
 __device__ __forceinline__ double reduce_partials(const double* part, double* lds) {
