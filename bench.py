@@ -56,12 +56,24 @@ def make_gpu(cfg, precision):
     return g
 
 
+def phase_pass(g, steps, timer_every):
+    """Per-phase event times from `steps` further steps, one in `timer_every` event-timed, after the timed region
+    (the events' own cost stays out of the measured steps)."""
+    if not timer_every:
+        return {}
+    g.enable_phase_timers(timer_every)
+    g.step(steps)
+    device_sync(g)
+    phases = g.phase_times()
+    g.enable_phase_timers(0)
+    return phases
+
+
 def run_timed(cfg, precision, warmup, steps, timer_every):
     g = make_gpu(cfg, precision)
     if warmup:
         g.step(warmup)
     device_sync(g)
-    g.enable_phase_timers(timer_every)
     it0 = g.stats()["total_cg_iterations"]
     device_sync(g)
     t0 = time.perf_counter()
@@ -71,8 +83,8 @@ def run_timed(cfg, precision, warmup, steps, timer_every):
     if rc != femgpu.FEM_OK:
         raise RuntimeError("GPU state became non-finite")
     st = g.stats()
-    phases = g.phase_times()
     x, v = g.state()
+    phases = phase_pass(g, steps, timer_every)
     op_ms = g.time_operator(100)
     g.close()
     return dict(seconds=t1 - t0, cg_iters=st["total_cg_iterations"] - it0, stats=st, phases=phases, x=x, v=v,
@@ -221,7 +233,6 @@ def bench_hex(args):
             if args.warmup:
                 g.step(args.warmup)
             device_sync(g)
-            g.enable_phase_timers(args.timer_every)
             it0 = g.stats()["total_cg_iterations"]
             t0 = time.perf_counter()
             rc = g.step(steps)
@@ -230,7 +241,7 @@ def bench_hex(args):
             assert rc == femgpu.FEM_OK
             st = g.stats()
             x, v = g.state()
-            ph = g.phase_times()
+            ph = phase_pass(g, steps, args.timer_every)
             distinct = g.hex_distinct_matrices()
             op_ms = g.time_operator(20)
             g.close()
@@ -876,7 +887,8 @@ def main():
     ap.add_argument("--cpu-faithful-steps", type=int, default=1,
                     help="steps of the oracle with the reference's O(#BC x D) boundary zeroing (0: skip)")
     ap.add_argument("--timer-every", type=int, default=10,
-                    help="event-time the kernels of one step in this many (0: off)")
+                    help="phase breakdown: after the timed steps, as many more steps with the kernels of one in this "
+                         "many event-timed (0: off)")
     ap.add_argument("--workload", default="corotational",
                     help="corotational (headline and every config's sub-record), linear, batch, adjoint, hex, "
                          "schemes, compliance")
