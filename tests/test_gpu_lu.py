@@ -108,6 +108,29 @@ def test_compliance_through_lu_and_dense_compliance():
     g.close()
 
 
+@pytest.mark.parametrize("n", [(1, 1, 1), (2, 1, 1), (3, 2, 2)])
+def test_lu_small_meshes_vs_oracle(n):
+    """Meshes below one 64-row block of the solves and one or a few 32-column panels of the factorization
+    (24 to 108 DOF): the linear scheme's trajectory and its kept dense compliance against the oracle's LU."""
+    cfg = mg.config_linear_cpu(n)
+    say("small linear mesh with LU, %d DOF" % cfg.x0.size)
+    o = make_oracle(cfg, scheme=oracle.LINEAR_EULER_IMPLICIT, linsolver=oracle.LU)
+    o.set_compliance_at_first_linear_step(True)
+    g = make_gpu(cfg, scheme=femgpu.LINEAR_EULER_IMPLICIT, linear_solver=femgpu.LINEAR_SOLVER_LU)
+    o.step(cfg.dt, 5)
+    assert g.step(5) == femgpu.FEM_OK
+    xr, vr = o.state()
+    x, v = g.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x - x0, xr - x0), v=rel(v, vr))
+    Cr = o.compliance_matrix()
+    C = g.compliance_matrix()
+    ec = np.abs(C - Cr).max() / np.abs(Cr).max()
+    say("  rel err", e, "compliance %.2e" % ec)
+    assert e["u"] < 1e-11 and e["v"] < 1e-11 and ec <= 1e-12
+    g.close()
+
+
 def test_dense_compliance_of_the_linear_case_with_cg_steps():
     """The dense compliance of the 10^3 linear case (3,993 DOF, 127.6 MB), which the reference's linear scheme
     forms on its first step, after CG steps: the LU is factored on demand."""
