@@ -189,10 +189,11 @@ int fem_apply_compliance_device_noalloc(fem_ctx* ctx, const double* b, int64_t n
 /* The dense compliance matrix of the last step's system matrix: C = A^-1 with the boundary rows and columns
  * zero, their diagonal entries included (ndof x ndof, column-major). Replaces OdeSolver::getComplianceMatrix after
  * OdeSolver::computeComplianceMatrixFromSystemMatrix (Math/OdeSolver.cpp:69-76), which the linear schemes run on
- * their first step (Math/OdeSolverLinearEulerImplicit.cpp:49). Computed on demand from the band LU of the
- * operator (factored here when the step used CG, whose reference getInverse is the dense inverse,
- * Math/LinearSparseSolveAndInverse.cpp:25-29). Tetrahedral meshes of one instance; FEM_ERR_ORDER before the first
- * step; FEM_ERR_INVALID when the band storage does not fit the device. */
+ * their first step (Math/OdeSolverLinearEulerImplicit.cpp:49). The linear implicit scheme with the direct solver
+ * keeps it from its first step (when ndof^2 doubles fit in 1 GiB) and this copies the kept matrix; otherwise it is
+ * computed on demand from the band factors of the operator (factored here when the step used CG, whose reference
+ * getInverse is the dense inverse, Math/LinearSparseSolveAndInverse.cpp:25-29). Tetrahedral meshes of one
+ * instance; FEM_ERR_ORDER before the first step; FEM_ERR_INVALID when the band storage does not fit the device. */
 int fem_get_compliance_matrix(fem_ctx* ctx, double* out);         /* host output */
 int fem_get_compliance_matrix_device(fem_ctx* ctx, double* out);  /* device output, ordered on the context stream */
 /* The compliance terms of a constraint localized in one element, as the constraint builders form them for
