@@ -131,6 +131,25 @@ def test_lu_small_meshes_vs_oracle(n):
     g.close()
 
 
+def test_lu_wide_band_vs_cg():
+    """A 16^3 linear block (14,739 DOF, bandwidth 923: 15 update tiles per panel, 231 solve blocks), implicit Euler
+    refactoring every step; its dense compliance (1.7 GB) is over the kept limit, so every step also solves through
+    the band factors. The direct solve against FP64 CG at 1e-14 (the CG is pinned against the oracle elsewhere)."""
+    cfg = mg.config_linear_cpu((16, 16, 16))
+    say("wide-band LU, %d DOF" % cfg.x0.size)
+    g1 = make_gpu(cfg, linear_solver=femgpu.LINEAR_SOLVER_LU)
+    g2 = make_gpu(cfg, tol=1e-14)
+    assert g1.step(3) == femgpu.FEM_OK and g2.step(3) == femgpu.FEM_OK
+    x1, v1 = g1.state()
+    x2, v2 = g2.state()
+    x0 = cfg.x0.reshape(-1)
+    e = dict(u=rel(x1 - x0, x2 - x0), v=rel(v1, v2))
+    say("  rel err LU vs CG", e)
+    assert e["u"] < 1e-10 and e["v"] < 1e-10
+    g1.close()
+    g2.close()
+
+
 def test_dense_compliance_of_the_linear_case_with_cg_steps():
     """The dense compliance of the 10^3 linear case (3,993 DOF, 127.6 MB), which the reference's linear scheme
     forms on its first step, after CG steps: the LU is factored on demand."""
